@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain WCSPH step on one MI355X.
+
+Workload: the 1,048,576-particle dam break (128x64x128 lattice, spacing
+0.01, h = 0.02, 2x1x2 box, velocities N(0, 0.05), dt = 5e-5).  A step is
+one full pass of the hot path: integrate + hash, stable radix sort,
+reorder + cellStart/cellEnd, density + EOS, force + velocity update.  The
+state is resident in HBM before the timed region; nothing is copied to or
+from the host inside it.
+
+Prints ONE JSON line (rank 0, N = 1):
+  value        particle-updates/s = N_particles * steps / timed seconds
+  roofline     force kernel: algorithmic bytes per launch / mean launch time
+               (HIP events on the context stream) against 8 TB/s HBM
+  cpu_baseline the CPU oracle (FP32 reference order) on the 15,625-particle
+               dam break, all host threads, a bounded number of steps
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from pibiti_amd import scenes, sph  # noqa: E402
+
+METRIC = "particle-updates/s at 1M (plain vs learned, ≤1% ρ err); % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(p) -> dict:
+    """Minimal FP32 traffic per particle of each kernel (pos 12 B, vel 12 B,
+    rho 4 B, p 4 B, uint32 key and index; every array counted once per
+    kernel; cell tables 8 B per cell)."""
+    n = int(p.num_particles)
+    c = int(p.num_cells) / n
+    passes = (int(p.key_bits) + 7) // 8
+    return {
+        "integrate_hash": 12 + 12 + 12 + 12 + 8,
+        "sort": 24 * passes,
+        "reorder": 8 + 24 + 24 + 8 * c,
+        "density": 12 + 8 + 8 * c,
+        "force": 32 + 12 + 8 * c,
+    }
+
+
+def cpu_baseline(budget_s: float) -> dict:
+    import oracle
+    oracle.build()
+    p, pos, vel, _ = scenes.cpu_dam_break_15k()
+    n = int(p.num_particles)
+    threads = oracle.num_threads()
+    # one untimed step, then as many steps as fit the budget (at most 100)
+    oracle.step_f32(p, pos, vel)
+    t0 = time.perf_counter()
+    steps = 0
+    x, v = pos, vel
+    while steps < 100 and (time.perf_counter() - t0) < budget_s:
+        o = oracle.run_f32(p, x, v, 5)
+        x, v = o["pos"], o["vel"]
+        steps += 5
+    dt = time.perf_counter() - t0
+    return {
+        "value": n * steps / dt,
+        "unit": "particle-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"CPU oracle, FP32 reference order, 15,625-particle dam break, {steps} steps "
+                  f"({dt:.1f} s), {threads} OpenMP threads, host {os.uname().nodename}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--prec", choices=["reference", "kahan"], default="reference")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-steps", type=int, default=50)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, not a build)", file=sys.stderr)
+        return 2
+
+    prec = sph.SPH_PREC_KAHAN if args.prec == "kahan" else sph.SPH_PREC_REFERENCE
+    p, pos, vel, info = scenes.gpu_dam_break_1m(precision=prec)
+    n = int(p.num_particles)
+    s = sph.SphSystem(p)
+    s.reset(pos, vel)
+    s.update(args.warmup)
+    s.synchronize()
+
+    t0 = time.perf_counter()
+    s.update(args.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+
+    # per-kernel device times: HIP events on the context stream, one step per call
+    s.set_timing(True)
+    acc = {}
+    for _ in range(args.roofline_steps):
+        s.update(1)
+        for k, v in s.timing().items():
+            acc[k] = acc.get(k, 0.0) + v
+    s.set_timing(False)
+    ms = {k: v / args.roofline_steps for k, v in acc.items()}
+    st = s.stats()
+    s.close()
+
+    ab = algorithmic_bytes(p)
+    kern_gbs = {k: ab[k] * n / (ms[k] * 1e-3) / 1e9 for k in ab if ms.get(k, 0) > 0}
+    dom = max(("density", "force"), key=lambda k: ms[k])
+    roof = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(kern_gbs[dom], 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(kern_gbs[dom] / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "algorithmic_bytes_per_particle": round(ab[dom], 3),
+        "mean_launch_ms": round(ms[dom], 5),
+    }
+    gather_ms = ms["density"] + ms["force"]
+    gather_bytes = (ab["density"] + ab["force"]) * n
+    step_bytes = sum(ab.values()) * n
+
+    out = {
+        "metric": METRIC,
+        "value": n * args.steps / secs,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded jittered lattice, N(0,0.05) velocities)",
+        "config": {
+            "workload": "WCSPH dam break, 1,048,576 particles (128x64x128, spacing 0.01, h 0.02, "
+                        "2x1x2 box, dt 5e-5), plain step",
+            "particles": n,
+            "grid": list(info["grid"]),
+            "key_bits": int(p.key_bits),
+            "precision": args.prec,
+            "eos": "linear",
+        },
+        "roofline": roof,
+        "kernels_ms": {k: round(v, 5) for k, v in ms.items()},
+        "kernels_gbs": {k: round(v, 1) for k, v in kern_gbs.items()},
+        "gather_hbm_frac": round(gather_bytes / (gather_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "step_hbm_frac": round(step_bytes / (ms["step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "end_state": {"sim_time": st["sim_time"], "max_speed": st["max_speed"],
+                      "cap_cells": st["cap_cells"], "max_cell_count": st["max_cell_count"]},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

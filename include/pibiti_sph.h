@@ -1,0 +1,244 @@
+/*
+ * pibiti_sph.h - C ABI of the MI355X-native WCSPH step.
+ *
+ * The reference exposes its device layer through an extern "C" seam
+ * (source/CUDA/System.cuh:1-38, source/CUDA/radixsort.cuh:30-32) that the host
+ * class cSPH (source/SPH/SPH.h:9-50) drives once per frame from
+ * cSPH::Update (source/SPH/SPH_Update.cpp:12-81).  This header is the
+ * replacement seam: plain scalars and pointers only (no vector types, no
+ * torch or HIP types), every call returns an sph_status and never exits the
+ * process.  Host buffers are caller-owned and copied before the call returns.
+ *
+ * Threading: one context per host thread; calls on a context are serialised
+ * on the context's own HIP stream.  sph_step() is asynchronous, every
+ * sph_get_*() synchronises the context stream before it returns.
+ */
+#ifndef PIBITI_SPH_H
+#define PIBITI_SPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_ABI_VERSION 1u
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.  The reference exits on any CUDA error
+ * (source/external/cutil.h:772-829); here every failure is a return value. */
+typedef enum sph_status {
+    SPH_OK = 0,
+    SPH_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad enum, range error      */
+    SPH_ERR_INVALID_PARAMS = 2,   /* h<=0, cell<h, N==0, key bits > 32 ...     */
+    SPH_ERR_NO_DEVICE = 3,        /* no HIP device visible                     */
+    SPH_ERR_OUT_OF_MEMORY = 4,    /* device allocation failed                  */
+    SPH_ERR_DEVICE = 5,           /* a HIP runtime call failed                 */
+    SPH_ERR_NOT_INITIALISED = 6,  /* step/readback before the state was set    */
+    SPH_ERR_UNSUPPORTED = 7,      /* feature not built (e.g. other bnd types)  */
+    SPH_ERR_CAPACITY = 8          /* size above a fixed capacity               */
+} sph_status;
+
+/* Boundary types (source/CUDA/Params.cuh:24).  Only the box is on the path. */
+enum { SPH_BND_BOX = 0 };
+/* Boundary effects along z (source/CUDA/Params.cuh:28).  Only NONE.        */
+enum { SPH_BND_EFF_NONE = 0 };
+
+/* Equation of state.  LINEAR is the reference's p = k (rho - rho0)
+ * (source/CUDA/Kernel_Cell.cui:195).  TAIT is an opt-in extension:
+ * p = (k rho0 / 7) ((rho/rho0)^7 - 1), whose slope at rho0 equals k.        */
+enum { SPH_EOS_LINEAR = 0, SPH_EOS_TAIT = 1 };
+
+/* Accumulation mode of the density and force sums.
+ * REFERENCE: per-cell partial sums added in (z,y,x) cell order, exactly the
+ *   loop structure of computeDensityD/computeForceD
+ *   (source/CUDA/Kernel_Cell.cui:142-199, source/CUDA/System.cu:228-250).
+ * KAHAN: one compensated FP32 accumulator per sum over the same candidate
+ *   order (extension).                                                      */
+enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1 };
+
+/* Arrays exchanged with the host.  POSITION/VELOCITY keep the reference's
+ * 0/1 flag values of cSPH::setArray (source/SPH/SPH_Util.cpp:59-71) and are
+ * float4 (x,y,z,w) per particle; DENSITY/PRESSURE are float per particle;
+ * DVEL is the last step's force velocity increment as float4 (w = 0).
+ * All arrays are indexed by the ORIGINAL particle index.                   */
+enum {
+    SPH_ARRAY_POSITION = 0,
+    SPH_ARRAY_VELOCITY = 1,
+    SPH_ARRAY_DENSITY = 2,
+    SPH_ARRAY_PRESSURE = 3,
+    SPH_ARRAY_DVEL = 4
+};
+
+/* Context flags (sph_create). */
+enum {
+    SPH_FLAG_KEEP_DVEL = 1u /* force kernel also stores dv for readback */
+};
+
+/* ------------------------------------------------------------------------ */
+/* User-level scene description: the raw values that Scene holds before
+ * Scene::_UpdatePar/_UpdateGrid derive the kernel constants
+ * (source/SPH/Scene.cpp:7-66 for the defaults, :76-113 for the derivation). */
+typedef struct sph_scene {
+    uint32_t num_particles;
+    uint32_t max_par_in_cell;   /* reference default 16                      */
+    float time_step;
+    float gravity[3];
+    float global_damping;
+    float world_min[3];
+    float world_max[3];
+    float cell_size;            /* uniform cell edge; must be >= h           */
+    float particle_r;           /* the reference's particleR                 */
+    float h;
+    float rest_density;
+    float stiffness;
+    float viscosity;
+    float min_dens;             /* factor: minDens <- 1/(min_dens*rho0)^2    */
+    float min_dist;             /* factor of particle_r                      */
+    float particle_mass;        /* > 0: explicit; 0: rho0*4/3*PI*r^3          */
+    float dist_bnd_soft;        /* factor of particle_r                      */
+    float dist_bnd_hard;        /* factor of particle_r                      */
+    float bnd_stiff;
+    float bnd_damp;
+    float bnd_damp_c;
+    uint32_t eos;               /* SPH_EOS_*                                 */
+    uint32_t precision;         /* SPH_PREC_*                                */
+    float cfl;                  /* 0: fixed dt; > 0: dt = min(time_step, cfl*h/(c+vmax)) */
+} sph_scene;
+
+/* Derived kernel constants: the plain-scalar equivalent of SimParams
+ * (source/CUDA/Params.cuh:53-83) restricted to the hot path, plus the
+ * values this build derives once on the host.                              */
+typedef struct sph_params {
+    float time_step;
+    uint32_t num_particles;
+    uint32_t max_par_in_cell;
+    float gravity[3];
+    float global_damping;
+    uint32_t grid_size[3];
+    float cell_size[3];
+    uint32_t grid_size_yx;
+    uint32_t num_cells;
+    float world_min[3];
+    float world_max[3];
+    float particle_r, h, h2;
+    float spiky_kern, lap_kern, poly6_kern;
+    float particle_mass, rest_density, stiffness, viscosity;
+    float min_dens;             /* 1/(min_dens*rho0)^2                       */
+    float min_dist;             /* min_dist * particle_r                     */
+    float dist_bnd_hard, dist_bnd_soft;
+    float bnd_damp, bnd_stiff, bnd_damp_c;
+    uint32_t bnd_type;          /* SPH_BND_BOX                               */
+    uint32_t bnd_eff_z;         /* SPH_BND_EFF_NONE                          */
+    uint32_t eos;
+    uint32_t precision;
+    float cfl;
+    float sound_speed;          /* sqrt(k): dp/drho at rho0 for both EOS     */
+    uint32_t key_bits;          /* bits of the cell key the sort orders      */
+    /* Largest float r2 with max(min_dist, sqrtf(r2)) < h, i.e. the force
+     * predicate of compForcePair (source/CUDA/Kernel_Cell.cui:214-218)
+     * rewritten as one comparison r2 <= force_r2_max; -1 when min_dist >= h. */
+    float force_r2_max;
+    uint32_t reserved[8];
+} sph_params;
+
+/* Global FP64 diagnostics reduced on the device (extension). */
+typedef struct sph_stats {
+    double mass;                /* sum m                                     */
+    double kinetic_energy;      /* sum 1/2 m |v|^2                           */
+    double potential_energy;    /* sum -m g.x                                */
+    double max_speed;           /* max |v|                                   */
+    double dt_cfl;              /* time step the CFL rule picked (== dt when cfl == 0) */
+    double sim_time;            /* simulated seconds advanced so far         */
+    uint64_t steps;             /* steps advanced so far                     */
+    uint32_t cap_cells;         /* cells holding more than max_par_in_cell   */
+    uint32_t out_of_grid;       /* particles whose cell had to be clamped    */
+    uint32_t max_cell_count;    /* largest particle count of one cell        */
+    uint32_t reserved[7];
+} sph_stats;
+
+typedef struct sph_ctx sph_ctx;
+
+/* ------------------------------------------------------------------------ */
+/* Host-only helpers (no device needed). */
+
+/* ABI version of the loaded library (compare with SPH_ABI_VERSION). */
+uint32_t sph_abi_version(void);
+
+/* Human-readable text of a status code (never NULL). */
+const char* sph_strerror(int status);
+
+/* Scene::InitDefault (source/SPH/Scene.cpp:7-66), hot-path fields only. */
+int sph_scene_default(sph_scene* scene);
+
+/* Scene::_UpdatePar + Scene::_UpdateGrid (source/SPH/Scene.cpp:76-113).
+ * Unlike the reference (which scales minDist/bnd distances in place and so
+ * must run once), this is a pure function of the scene.                   */
+int sph_params_from_scene(const sph_scene* scene, sph_params* params);
+
+/* Lattice initialiser in the spirit of cSPH::Reset(0)
+ * (source/SPH/SPH_Init.cpp:40-76): particle n = (iy*nz + iz)*nx + ix sits at
+ * region_min + (i + 1/2)*spacing on each axis, jittered by
+ * U(-jitter, +jitter)*spacing from the seeded counter-based generator, with
+ * velocity N(0, vel_sigma^2) per axis plus vel_offset.  pos4 and vel4 hold
+ * nx*ny*nz float4 each (w = 1 for positions, 0 for velocities).           */
+int sph_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float region_min[3],
+                      float spacing, float jitter, float vel_sigma, const float vel_offset[3],
+                      uint64_t seed, uint64_t stream, float* pos4, float* vel4);
+
+/* ------------------------------------------------------------------------ */
+/* Device context (replaces cSPH::_InitMem/_FreeMem + setParameters,
+ * source/SPH/SPH_Mem.cpp:11-82, source/CUDA/System.cu:634-637).            */
+
+/* Allocate every device buffer for params->num_particles particles on the
+ * given HIP device.  flags: SPH_FLAG_*.                                    */
+int sph_create(const sph_params* params, int device, uint32_t flags, sph_ctx** out);
+
+/* Free the context (NULL is accepted). */
+int sph_destroy(sph_ctx* ctx);
+
+/* setParameters (source/CUDA/System.cu:634-637).  num_particles and the
+ * grid shape must not change; every other constant may.                   */
+int sph_set_params(sph_ctx* ctx, const sph_params* params);
+
+/* cSPH::setArray (source/SPH/SPH_Util.cpp:59-71): copy `count` records
+ * starting at original index `first` into the device state.  which is
+ * SPH_ARRAY_POSITION or SPH_ARRAY_VELOCITY, data is float4 per particle.  */
+int sph_set_array(sph_ctx* ctx, int which, const float* data, uint32_t first, uint32_t count);
+
+/* cSPH::getArray (source/SPH/SPH_Util.cpp:44-56) plus density, pressure
+ * and dv: all num_particles records in original index order.              */
+int sph_get_array(sph_ctx* ctx, int which, float* out);
+
+/* cSPH::Update (source/SPH/SPH_Update.cpp:12-81) run nsteps times:
+ * integrate (boundary, gravity, x) -> hash -> stable sort -> reorder with
+ * cellStart/cellEnd -> density + EOS -> force + velocity update.          */
+int sph_step(sph_ctx* ctx, uint32_t nsteps);
+
+/* Grid of the last step: sorted cell keys and permutation (sorted slot ->
+ * original index) as the reference's dParHash after RadixSort
+ * (source/SPH/SPH_Update.cpp:56-58), and cellStart/cellEnd per cell
+ * (0xFFFFFFFF for an empty cell, as source/CUDA/System.cu:694).  Any
+ * pointer may be NULL to skip that array.                                  */
+int sph_get_grid(sph_ctx* ctx, uint32_t* sorted_hash, uint32_t* perm, uint32_t* cell_start,
+                 uint32_t* cell_end);
+
+/* FP64 reductions of the current state plus counters of the last step.   */
+int sph_get_stats(sph_ctx* ctx, sph_stats* out);
+
+/* Block until every queued step has finished. */
+int sph_synchronize(sph_ctx* ctx);
+
+/* Device-side time (ms) of the kernels of the last sph_step call, measured
+ * with HIP events on the context stream: [0] integrate+hash, [1] sort,
+ * [2] reorder, [3] density, [4] force, [5] whole call.  Timing is only
+ * recorded while enabled.                                                  */
+int sph_set_timing(sph_ctx* ctx, int enable);
+int sph_get_timing(sph_ctx* ctx, float* ms6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIBITI_SPH_H */

@@ -1,0 +1,520 @@
+// sph_oracle.cpp - CPU oracle of the WCSPH step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker the parity tests, the
+// smoke check and the CPU-baseline leg of bench.py compare the HIP path with.
+// The product library never links, loads or calls it.
+//
+// It restates, in plain C++, the arithmetic of the reference's CUDA step
+// (the reference has no CPU path of its own):
+//   integrate + box boundary ... source/CUDA/System.cu:41-77 (box, no z effect), :165-205
+//   cell hash ................... source/CUDA/Kernel_Cell.cui:5-34
+//   stable key/value sort ....... source/CUDA/radixsort_kernel.cu:445-472 (LSD, stable)
+//   cellStart + reorder ......... source/CUDA/Kernel_Cell.cui:40-67, source/CUDA/System.cu:689-713
+//   density + EOS ............... source/CUDA/Kernel_Cell.cui:142-199
+//   force + velocity update ..... source/CUDA/Kernel_Cell.cui:210-261, source/CUDA/System.cu:228-250,402
+//   step order .................. source/SPH/SPH_Update.cpp:39-80
+//   vector helpers .............. source/external/cutil_math.h:392-407 (dot left to right)
+//
+// Modes:
+//   float, ORACLE_PREC_REFERENCE: same operation order as the reference
+//     kernels (per-cell partial sums, (c*c)*c for pow(c,3)), no FMA
+//     contraction (built with -ffp-contract=off).
+//   float, ORACLE_PREC_KAHAN: one compensated accumulator per sum over the
+//     same candidate order.
+//   double: the FP64 twin.  Takes the same float inputs and float-valued
+//     constants, computes every step in double, and also returns per-particle
+//     error scales (the sums re-evaluated with absolute values) that the
+//     tests use to normalise FP32-vs-FP64 differences.
+//
+// Deviations from the reference that are shared with the HIP path:
+//   - neighbour cells outside the grid are skipped (the reference reads
+//     aliased or out-of-range cells whose particles cannot pass r < h);
+//   - a particle's own cell index is clamped into the grid and counted.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../include/pibiti_sph.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr float kBndEps = 0.00001f;  // EPS of boundary(), source/CUDA/System.cu:51
+
+struct Counters {
+    uint32_t cap_cells;
+    uint32_t out_of_grid;
+    uint32_t max_cell_count;
+    uint32_t reserved;
+};
+
+// ---------------------------------------------------------------- integrate
+template <typename T>
+inline void bnd_wall(T diff, T nx, T ny, T nz, T stiff, T damp, T dt, T v[3]) {
+    // addB()/addC(): accBnd = stiff*diff - damp*dot(norm, vel); vel += accBnd*norm*dt
+    if (diff > (T)kBndEps) {
+        T dot = nx * v[0] + ny * v[1] + nz * v[2];
+        T acc = stiff * diff - damp * dot;
+        v[0] = v[0] + (acc * nx) * dt;
+        v[1] = v[1] + (acc * ny) * dt;
+        v[2] = v[2] + (acc * nz) * dt;
+    }
+}
+
+// integrateD + boundary() for one particle (source/CUDA/System.cu:63-77,165-205).
+template <typename T>
+inline void integrate_one(const sph_params& p, T x[3], T v[3]) {
+    const T b = (T)p.dist_bnd_soft, stiff = (T)p.bnd_stiff, damp = (T)p.bnd_damp,
+            damp2 = (T)p.bnd_damp_c, dt = (T)p.time_step;
+    const T wmin[3] = {(T)p.world_min[0], (T)p.world_min[1], (T)p.world_min[2]};
+    const T wmax[3] = {(T)p.world_max[0], (T)p.world_max[1], (T)p.world_max[2]};
+    const T z = 0, one = 1;
+    // z walls use bndDampC, x and y walls bndDamp, in this order.
+    bnd_wall<T>((b - x[2]) + wmin[2], z, z, one, stiff, damp2, dt, v);
+    bnd_wall<T>((b + x[2]) - wmax[2], z, z, -one, stiff, damp2, dt, v);
+    bnd_wall<T>((b - x[0]) + wmin[0], one, z, z, stiff, damp, dt, v);
+    bnd_wall<T>((b + x[0]) - wmax[0], -one, z, z, stiff, damp, dt, v);
+    bnd_wall<T>((b - x[1]) + wmin[1], z, one, z, stiff, damp, dt, v);
+    bnd_wall<T>((b + x[1]) - wmax[1], z, -one, z, stiff, damp, dt, v);
+    for (int c = 0; c < 3; ++c) v[c] = v[c] + (T)p.gravity[c] * dt;
+    for (int c = 0; c < 3; ++c) v[c] = v[c] * (T)p.global_damping;
+    for (int c = 0; c < 3; ++c) x[c] = x[c] + v[c] * dt;
+    const T bh = (T)p.dist_bnd_hard;
+    for (int c = 0; c < 3; ++c) {
+        T hi = wmax[c] - bh, lo = wmin[c] + bh;
+        if (x[c] > hi) x[c] = hi;
+        if (x[c] < lo) x[c] = lo;
+    }
+}
+
+// ------------------------------------------------------------------ hashing
+// calcGridPos (source/CUDA/Kernel_Cell.cui:5-13) with the index clamped.
+template <typename T>
+inline void grid_pos(const sph_params& p, const T x[3], int g[3], uint32_t* oob) {
+    bool out = false;
+    for (int c = 0; c < 3; ++c) {
+        T q = (x[c] - (T)p.world_min[c]) / (T)p.cell_size[c];
+        T f = std::floor(q);
+        T hi = (T)(p.grid_size[c] - 1);
+        if (!(f >= (T)0 && f <= hi)) {
+            out = true;
+            f = std::isnan(f) ? (T)0 : std::min(std::max(f, (T)0), hi);
+        }
+        g[c] = (int)f;
+    }
+    if (out && oob) ++*oob;
+}
+
+// calcGridHash (source/CUDA/Kernel_Cell.cui:15-19).
+inline uint32_t grid_hash(const sph_params& p, int gx, int gy, int gz) {
+    return (uint32_t)gz * p.grid_size_yx + (uint32_t)gy * p.grid_size[0] + (uint32_t)gx;
+}
+
+// -------------------------------------------------------------- accumulators
+template <typename T>
+struct Plain {
+    T s = 0;
+    void add(T v) { s = s + v; }
+};
+template <typename T>
+struct Kahan {
+    T s = 0, c = 0;
+    void add(T v) {
+        T y = v - c;
+        T t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+template <typename T>
+struct Grid {
+    std::vector<uint32_t> cell_start, cell_end;  // cellEnd valid for non-empty cells
+};
+
+// ---------------------------------------------------------------- density
+// computeDensityD/compDensCell (source/CUDA/Kernel_Cell.cui:142-199) for the
+// particle in sorted slot i.  Returns the raw kernel sum (before Poly6*m).
+template <typename T, bool kKahan>
+T density_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
+              const std::vector<uint32_t>& cs, const std::vector<uint32_t>& ce, T h2) {
+    const T xi[3] = {spos[4 * i], spos[4 * i + 1], spos[4 * i + 2]};
+    int g[3];
+    grid_pos<T>(p, xi, g, nullptr);
+    Plain<T> total;
+    Kahan<T> ktotal;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int cx = g[0] + dx, cy = g[1] + dy, cz = g[2] + dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)p.grid_size[0] ||
+                    cy >= (int)p.grid_size[1] || cz >= (int)p.grid_size[2])
+                    continue;
+                uint32_t c = grid_hash(p, cx, cy, cz);
+                uint32_t s = cs[c];
+                if (s == kEmpty) continue;
+                uint32_t e = std::min(ce[c], s + p.max_par_in_cell);
+                Plain<T> cell;
+                for (uint32_t j = s; j < e; ++j) {
+                    if (j == i) continue;
+                    T rx = xi[0] - spos[4 * j], ry = xi[1] - spos[4 * j + 1],
+                      rz = xi[2] - spos[4 * j + 2];
+                    T r2 = rx * rx + ry * ry + rz * rz;
+                    if (r2 < h2) {
+                        T c3 = h2 - r2;
+                        T w = (c3 * c3) * c3;
+                        if (kKahan)
+                            ktotal.add(w);
+                        else
+                            cell.add(w);
+                    }
+                }
+                if (!kKahan) total.add(cell.s);
+            }
+    return kKahan ? ktotal.s : total.s;
+}
+
+template <typename T>
+inline T eos(const sph_params& p, T rho) {
+    if (p.eos == SPH_EOS_TAIT) {
+        // p = (k rho0 / 7) * ((rho/rho0)^7 - 1), powers as explicit products.
+        T x = rho / (T)p.rest_density;
+        T x2 = x * x;
+        T x4 = x2 * x2;
+        T x7 = (x4 * x2) * x;
+        T bcoef = ((T)p.stiffness * (T)p.rest_density) / (T)7;
+        return bcoef * (x7 - (T)1);
+    }
+    return (rho - (T)p.rest_density) * (T)p.stiffness;
+}
+
+// ------------------------------------------------------------------ force
+// computeForceD/compForceCell/compForcePair (source/CUDA/Kernel_Cell.cui:210-261,
+// source/CUDA/System.cu:228-250) for sorted slot i.  Writes the velocity
+// increment addVel = F*m*dt.  For double also returns the absolute-value
+// evaluation of the same sum (error scale).
+template <typename T, bool kKahan>
+void force_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
+               const std::vector<T>& svel, const std::vector<T>& dens, const std::vector<T>& pres,
+               const std::vector<uint32_t>& cs, const std::vector<uint32_t>& ce, T out[3],
+               T* scale) {
+    const T xi[3] = {spos[4 * i], spos[4 * i + 1], spos[4 * i + 2]};
+    const T vi[3] = {svel[4 * i], svel[4 * i + 1], svel[4 * i + 2]};
+    const T di = dens[i], pi = pres[i];
+    const bool dbl = sizeof(T) == 8;
+    // |p| evaluated with absolute values: the EOS terms before they cancel.
+    auto pabs = [&](T rho) -> T {
+        if (p.eos == SPH_EOS_TAIT) {
+            T x = rho / (T)p.rest_density;
+            T x2 = x * x, x4 = x2 * x2, x7 = (x4 * x2) * x;
+            return ((T)p.stiffness * (T)p.rest_density / (T)7) * (x7 + (T)1);
+        }
+        return (std::fabs(rho) + (T)p.rest_density) * (T)p.stiffness;
+    };
+    const T pai = dbl ? pabs(di) : (T)0;
+    int g[3];
+    grid_pos<T>(p, xi, g, nullptr);
+    const T h = (T)p.h, spiky = (T)p.spiky_kern, lap = (T)p.lap_kern, visc = (T)p.viscosity,
+            min_dist = (T)p.min_dist, min_dens = (T)p.min_dens;
+    Plain<T> tot[3];
+    Kahan<T> ktot[3];
+    T sc = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int cx = g[0] + dx, cy = g[1] + dy, cz = g[2] + dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)p.grid_size[0] ||
+                    cy >= (int)p.grid_size[1] || cz >= (int)p.grid_size[2])
+                    continue;
+                uint32_t c = grid_hash(p, cx, cy, cz);
+                uint32_t s = cs[c];
+                if (s == kEmpty) continue;
+                uint32_t e = std::min(ce[c], s + p.max_par_in_cell);
+                Plain<T> cell[3];
+                for (uint32_t j = s; j < e; ++j) {
+                    if (j == i) continue;
+                    T rel[3] = {xi[0] - spos[4 * j], xi[1] - spos[4 * j + 1],
+                                xi[2] - spos[4 * j + 2]};
+                    T rv[3] = {svel[4 * j] - vi[0], svel[4 * j + 1] - vi[1],
+                               svel[4 * j + 2] - vi[2]};
+                    T len = std::sqrt(rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2]);
+                    T r = std::fmax(min_dist, len);
+                    if (r < h) {
+                        T d12 = std::fmin(min_dens, (T)1 / (di * dens[j]));
+                        T cc = h - r;
+                        T pterm = ((cc * spiky) * (pi + pres[j])) / r;
+                        T vterm = lap * visc;
+                        T cd = cc * d12;
+                        for (int k = 0; k < 3; ++k) {
+                            T f = (pterm * rel[k] + vterm * rv[k]) * cd;
+                            if (kKahan)
+                                ktot[k].add(f);
+                            else
+                                cell[k].add(f);
+                        }
+                        if (dbl) {
+                            T pa = ((cc * spiky) * (pai + pabs(dens[j]))) / r;
+                            T vn = std::sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+                            sc += (pa * len + std::fabs(vterm) * vn) * std::fabs(cd);
+                        }
+                    }
+                }
+                if (!kKahan)
+                    for (int k = 0; k < 3; ++k) tot[k].add(cell[k].s);
+            }
+    const T mdt = (T)p.particle_mass * (T)p.time_step;
+    for (int k = 0; k < 3; ++k) out[k] = (kKahan ? ktot[k].s : tot[k].s) * mdt;
+    if (scale) *scale = sc * std::fabs(mdt);
+}
+
+// ------------------------------------------------------------- whole step
+template <typename T>
+struct StepOut {
+    // all in original index order
+    T* pos4;
+    T* vel4;
+    T* dens;
+    T* pres;
+    T* dvel4;
+    T* dvel_scale;  // double only
+    uint32_t* sorted_hash;
+    uint32_t* perm;
+    uint32_t* cell_start;
+    uint32_t* cell_end;
+    Counters* cnt;
+};
+
+template <typename T>
+void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const T* vel_in,
+               const StepOut<T>& o) {
+    const uint32_t ncell = p.num_cells;
+    std::vector<T> pos(pos_in, pos_in + 4 * (size_t)n), vel(vel_in, vel_in + 4 * (size_t)n);
+    Counters cnt{};
+
+    // 1. integrate in original order (SPH_Update.cpp:39-47).
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        T x[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+        T v[3] = {vel[4 * i], vel[4 * i + 1], vel[4 * i + 2]};
+        integrate_one<T>(p, x, v);
+        for (int c = 0; c < 3; ++c) {
+            pos[4 * i + c] = x[c];
+            vel[4 * i + c] = v[c];
+        }
+    }
+    // 2. hash in original order (SPH_Update.cpp:56).
+    std::vector<uint32_t> hash(n);
+    uint32_t oob = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        T x[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+        int g[3];
+        grid_pos<T>(p, x, g, &oob);
+        hash[i] = grid_hash(p, g[0], g[1], g[2]);
+    }
+    cnt.out_of_grid = oob;
+    // 3. stable sort of (hash, index) by hash (SPH_Update.cpp:57-58).
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](uint32_t a, uint32_t b) { return hash[a] < hash[b]; });
+    // 4. cellStart/cellEnd + reorder (Kernel_Cell.cui:40-67).
+    std::vector<uint32_t> cs(ncell, kEmpty), ce(ncell, kEmpty);
+    std::vector<T> spos(4 * (size_t)n), svel(4 * (size_t)n);
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t hk = hash[perm[k]];
+        if (k == 0 || hk != hash[perm[k - 1]]) cs[hk] = k;
+        if (k + 1 == n || hk != hash[perm[k + 1]]) ce[hk] = k + 1;
+        for (int c = 0; c < 4; ++c) {
+            spos[4 * (size_t)k + c] = pos[4 * (size_t)perm[k] + c];
+            svel[4 * (size_t)k + c] = vel[4 * (size_t)perm[k] + c];
+        }
+    }
+    for (uint32_t c = 0; c < ncell; ++c)
+        if (cs[c] != kEmpty) {
+            uint32_t cnt_c = ce[c] - cs[c];
+            if (cnt_c > p.max_par_in_cell) ++cnt.cap_cells;
+            cnt.max_cell_count = std::max(cnt.max_cell_count, cnt_c);
+        }
+    // 5. density + EOS in sorted order (Kernel_Cell.cui:178-199).
+    const bool dbl = sizeof(T) == 8;
+    const T h2 = dbl ? (T)p.h * (T)p.h : (T)p.h2;
+    std::vector<T> dens(n), pres(n);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        T s = prec == SPH_PREC_KAHAN ? density_sum<T, true>(p, (uint32_t)i, spos, cs, ce, h2)
+                                     : density_sum<T, false>(p, (uint32_t)i, spos, cs, ce, h2);
+        T rho = (s * (T)p.poly6_kern) * (T)p.particle_mass;
+        dens[i] = rho;
+        pres[i] = eos<T>(p, rho);
+    }
+    // 6. force + velocity update, scattered to original order (System.cu:228-250,402).
+    std::vector<T> dv(4 * (size_t)n, (T)0), dsc(n, (T)0);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        T f[3];
+        T* scp = dbl ? &dsc[i] : nullptr;
+        if (prec == SPH_PREC_KAHAN)
+            force_sum<T, true>(p, (uint32_t)i, spos, svel, dens, pres, cs, ce, f, scp);
+        else
+            force_sum<T, false>(p, (uint32_t)i, spos, svel, dens, pres, cs, ce, f, scp);
+        for (int c = 0; c < 3; ++c) dv[4 * i + c] = f[c];
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t si = perm[k];
+        for (int c = 0; c < 3; ++c) vel[4 * (size_t)si + c] = svel[4 * (size_t)k + c] + dv[4 * (size_t)k + c];
+        vel[4 * (size_t)si + 3] = svel[4 * (size_t)k + 3] + (T)0;
+        if (o.dens) o.dens[si] = dens[k];
+        if (o.pres) o.pres[si] = pres[k];
+        if (o.dvel4)
+            for (int c = 0; c < 4; ++c) o.dvel4[4 * (size_t)si + c] = c < 3 ? dv[4 * (size_t)k + c] : (T)0;
+        if (o.dvel_scale) o.dvel_scale[si] = dsc[k];
+        if (o.sorted_hash) o.sorted_hash[k] = hash[si];
+        if (o.perm) o.perm[k] = si;
+    }
+    if (o.pos4) std::copy(pos.begin(), pos.end(), o.pos4);
+    if (o.vel4) std::copy(vel.begin(), vel.end(), o.vel4);
+    if (o.cell_start) std::copy(cs.begin(), cs.end(), o.cell_start);
+    if (o.cell_end) std::copy(ce.begin(), ce.end(), o.cell_end);
+    if (o.cnt) *o.cnt = cnt;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Number of OpenMP threads the oracle uses (1 when built without OpenMP).
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+// One FP32 step.  pos4/vel4 are updated in place (original order); every
+// other output pointer may be NULL.  prec is SPH_PREC_REFERENCE or
+// SPH_PREC_KAHAN.  counters receives {cap_cells, out_of_grid, max_cell_count, 0}.
+int oracle_step_f32(const sph_params* p, int prec, uint32_t n, float* pos4, float* vel4,
+                    float* dens, float* pres, float* dvel4, uint32_t* sorted_hash, uint32_t* perm,
+                    uint32_t* cell_start, uint32_t* cell_end, uint32_t* counters) {
+    if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
+    StepOut<float> o{pos4, vel4, dens, pres, dvel4, nullptr, sorted_hash, perm, cell_start,
+                     cell_end, reinterpret_cast<Counters*>(counters)};
+    step_impl<float>(*p, prec, n, pos4, vel4, o);
+    return 0;
+}
+
+// Several FP32 steps in a row (outputs of the last step only).
+int oracle_run_f32(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, float* pos4,
+                   float* vel4, float* dens, float* pres) {
+    if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
+    for (uint32_t s = 0; s < nsteps; ++s) {
+        StepOut<float> o{pos4, vel4, dens, pres, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, nullptr};
+        step_impl<float>(*p, prec, n, pos4, vel4, o);
+    }
+    return 0;
+}
+
+// One FP64 step from FP32 inputs.  Outputs are double, original order.
+// dvel_scale[i] = m*dt * sum_j |c d12| (|pterm|_abs r + |vterm| |dv_ij|): the
+// absolute-value evaluation of the force sum, the scale FP32 rounding error
+// of that sum is proportional to.
+int oracle_step_f64(const sph_params* p, uint32_t n, const float* pos4_in, const float* vel4_in,
+                    double* pos4, double* vel4, double* dens, double* pres, double* dvel4,
+                    double* dvel_scale) {
+    if (!p || !pos4_in || !vel4_in || n != p->num_particles) return 1;
+    std::vector<double> pin(pos4_in, pos4_in + 4 * (size_t)n), vin(vel4_in, vel4_in + 4 * (size_t)n);
+    StepOut<double> o{pos4, vel4, dens, pres, dvel4, dvel_scale, nullptr, nullptr, nullptr,
+                      nullptr, nullptr};
+    step_impl<double>(*p, SPH_PREC_REFERENCE, n, pin.data(), vin.data(), o);
+    return 0;
+}
+
+// ---- stage functions for unit tests -------------------------------------
+
+// integrateD on n particles (original order), in place.
+void oracle_integrate_f32(const sph_params* p, uint32_t n, float* pos4, float* vel4) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float x[3] = {pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]};
+        float v[3] = {vel4[4 * i], vel4[4 * i + 1], vel4[4 * i + 2]};
+        integrate_one<float>(*p, x, v);
+        for (int c = 0; c < 3; ++c) {
+            pos4[4 * i + c] = x[c];
+            vel4[4 * i + c] = v[c];
+        }
+    }
+}
+
+// calcHashD on n positions; returns the number of clamped cells.
+uint32_t oracle_hash_f32(const sph_params* p, uint32_t n, const float* pos4, uint32_t* hash) {
+    uint32_t oob = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        float x[3] = {pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]};
+        int g[3];
+        grid_pos<float>(*p, x, g, &oob);
+        hash[i] = grid_hash(*p, g[0], g[1], g[2]);
+    }
+    return oob;
+}
+
+// Density of sorted particles given sorted positions and cell ranges
+// (float, chosen accumulation mode).  dens/pres in sorted order.
+void oracle_density_f32(const sph_params* p, int prec, uint32_t n, const float* spos4,
+                        const uint32_t* cell_start, const uint32_t* cell_end, float* dens,
+                        float* pres) {
+    std::vector<float> sp(spos4, spos4 + 4 * (size_t)n);
+    std::vector<uint32_t> cs(cell_start, cell_start + p->num_cells),
+        ce(cell_end, cell_end + p->num_cells);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        float s = prec == SPH_PREC_KAHAN ? density_sum<float, true>(*p, (uint32_t)i, sp, cs, ce, p->h2)
+                                         : density_sum<float, false>(*p, (uint32_t)i, sp, cs, ce, p->h2);
+        float rho = (s * p->poly6_kern) * p->particle_mass;
+        dens[i] = rho;
+        pres[i] = eos<float>(*p, rho);
+    }
+}
+
+// Brute-force O(n^2) density in double: all pairs j != i with r2 < h^2,
+// no grid.  The reference for neighbour-set completeness when cell >= h and
+// no cell exceeds max_par_in_cell.
+void oracle_density_bruteforce_f64(const sph_params* p, uint32_t n, const float* pos4,
+                                   double* dens, uint32_t* nnb) {
+    const double h2 = (double)p->h * (double)p->h;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        double s = 0;
+        uint32_t cnt = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            if (j == (uint32_t)i) continue;
+            double rx = (double)pos4[4 * i] - pos4[4 * j], ry = (double)pos4[4 * i + 1] - pos4[4 * j + 1],
+                   rz = (double)pos4[4 * i + 2] - pos4[4 * j + 2];
+            double r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 < h2) {
+                double c = h2 - r2;
+                s += c * c * c;
+                ++cnt;
+            }
+        }
+        dens[i] = s * (double)p->poly6_kern * (double)p->particle_mass;
+        if (nnb) nnb[i] = cnt;
+    }
+}
+
+}  // extern "C"

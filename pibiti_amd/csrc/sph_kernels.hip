@@ -1,0 +1,528 @@
+// sph_kernels.hip - HIP kernels of one WCSPH step for gfx950 (MI355X).
+//
+// Data layout in HBM: the particle state lives in SORTED order (the order of
+// the last stable sort by (cell key, original index)), one float4 per
+// particle for position (x,y,z,w) and velocity (vx,vy,vz,w), a uint32
+// original index per slot, and a float2 (density, pressure) per slot.  The
+// reference keeps positions in original order and scatters velocities back
+// every step (source/CUDA/System.cu:402); keeping the sorted order instead
+// makes integrate, reorder and force streams coalesced, and the original
+// order is only materialised on readback.
+//
+// Arithmetic follows the reference kernels operation by operation (no FMA
+// contraction: the library is built with -ffp-contract=off and correctly
+// rounded fp32 division/sqrt), so one thread per particle reproduces the
+// reference's FP32 result bit for bit:
+//   integrateD + boundary (box) .... source/CUDA/System.cu:41-77,165-205
+//   calcGridPos/calcGridHash ....... source/CUDA/Kernel_Cell.cui:5-34
+//   reorderD ........................ source/CUDA/Kernel_Cell.cui:40-67
+//   computeDensityD ................. source/CUDA/Kernel_Cell.cui:142-199
+//   computeForceD ................... source/CUDA/Kernel_Cell.cui:210-261,
+//                                     source/CUDA/System.cu:228-250,402
+#include "sph_internal.hpp"
+
+namespace sph {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr float kBndEps = 0.00001f;  // EPS of boundary()
+
+__device__ __forceinline__ void bnd_wall(float diff, float nx, float ny, float nz, float stiff,
+                                         float damp, float dt, float& vx, float& vy, float& vz) {
+    if (diff > kBndEps) {
+        float dot = nx * vx + ny * vy + nz * vz;
+        float acc = stiff * diff - damp * dot;
+        vx = vx + (acc * nx) * dt;
+        vy = vy + (acc * ny) * dt;
+        vz = vz + (acc * nz) * dt;
+    }
+}
+
+// Cell coordinate of one axis: floor((x - min)/cell), clamped into the grid.
+__device__ __forceinline__ int cell_coord(float x, float wmin, float cs, uint32_t gsize, bool& out) {
+    float f = floorf((x - wmin) / cs);
+    float hi = (float)(gsize - 1);
+    if (!(f >= 0.0f && f <= hi)) {
+        out = true;
+        f = (f != f) ? 0.0f : fminf(fmaxf(f, 0.0f), hi);
+    }
+    return (int)f;
+}
+
+__device__ __forceinline__ void grid_pos(const sph_params& p, float x, float y, float z, int& gx,
+                                         int& gy, int& gz, bool& out) {
+    gx = cell_coord(x, p.world_min[0], p.cell_size[0], p.grid_size[0], out);
+    gy = cell_coord(y, p.world_min[1], p.cell_size[1], p.grid_size[1], out);
+    gz = cell_coord(z, p.world_min[2], p.cell_size[2], p.grid_size[2], out);
+}
+
+__device__ __forceinline__ uint32_t grid_hash(const sph_params& p, int gx, int gy, int gz) {
+    return (uint32_t)gz * p.grid_size_yx + (uint32_t)gy * p.grid_size[0] + (uint32_t)gx;
+}
+
+__device__ __forceinline__ float eos(const sph_params& p, float rho) {
+    if (p.eos == SPH_EOS_TAIT) {
+        float x = rho / p.rest_density;
+        float x2 = x * x;
+        float x4 = x2 * x2;
+        float x7 = (x4 * x2) * x;
+        float bcoef = (p.stiffness * p.rest_density) / 7.0f;
+        return bcoef * (x7 - 1.0f);
+    }
+    return (rho - p.rest_density) * p.stiffness;
+}
+
+struct KahanF {
+    float s, c;
+    __device__ __forceinline__ void add(float v) {
+        float y = v - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// ------------------------------------------------------------ integrate+hash
+__global__ __launch_bounds__(kBlock) void k_integrate_hash(
+    sph_params p, uint32_t n, float4* __restrict__ pos, float4* __restrict__ vel,
+    const uint32_t* __restrict__ pid, uint32_t* __restrict__ key_pid,
+    uint32_t* __restrict__ val_pid, uint32_t* __restrict__ cell_start,
+    uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    // cellStart of the previous step is dead from here on: clear it.
+    for (uint32_t c = tid; c < p.num_cells; c += nthreads) cell_start[c] = kEmptyCell;
+    if (tid >= n) return;
+
+    const float dt = *dt_dev;
+    float4 x4 = pos[tid];
+    float4 v4 = vel[tid];
+    float x = x4.x, y = x4.y, z = x4.z;
+    float vx = v4.x, vy = v4.y, vz = v4.z;
+    const float b = p.dist_bnd_soft, stiff = p.bnd_stiff, damp = p.bnd_damp, damp2 = p.bnd_damp_c;
+    bnd_wall((b - z) + p.world_min[2], 0.0f, 0.0f, 1.0f, stiff, damp2, dt, vx, vy, vz);
+    bnd_wall((b + z) - p.world_max[2], 0.0f, 0.0f, -1.0f, stiff, damp2, dt, vx, vy, vz);
+    bnd_wall((b - x) + p.world_min[0], 1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+    bnd_wall((b + x) - p.world_max[0], -1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+    bnd_wall((b - y) + p.world_min[1], 0.0f, 1.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+    bnd_wall((b + y) - p.world_max[1], 0.0f, -1.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+    vx = vx + p.gravity[0] * dt;
+    vy = vy + p.gravity[1] * dt;
+    vz = vz + p.gravity[2] * dt;
+    vx = vx * p.global_damping;
+    vy = vy * p.global_damping;
+    vz = vz * p.global_damping;
+    x = x + vx * dt;
+    y = y + vy * dt;
+    z = z + vz * dt;
+    const float bh = p.dist_bnd_hard;
+    {
+        float hi = p.world_max[0] - bh, lo = p.world_min[0] + bh;
+        if (x > hi) x = hi;
+        if (x < lo) x = lo;
+    }
+    {
+        float hi = p.world_max[1] - bh, lo = p.world_min[1] + bh;
+        if (y > hi) y = hi;
+        if (y < lo) y = lo;
+    }
+    {
+        float hi = p.world_max[2] - bh, lo = p.world_min[2] + bh;
+        if (z > hi) z = hi;
+        if (z < lo) z = lo;
+    }
+    pos[tid] = make_float4(x, y, z, x4.w);
+    vel[tid] = make_float4(vx, vy, vz, v4.w);
+
+    bool out = false;
+    int gx, gy, gz;
+    grid_pos(p, x, y, z, gx, gy, gz, out);
+    if (out) atomicAdd(&counters[kCntOutOfGrid], 1u);
+    const uint32_t id = pid[tid];
+    key_pid[id] = grid_hash(p, gx, gy, gz);
+    val_pid[id] = tid;
+}
+
+// ------------------------------------------------------------------ reorder
+__global__ __launch_bounds__(kBlock) void k_reorder(
+    uint32_t n, uint32_t cap, const uint32_t* __restrict__ skey, const uint32_t* __restrict__ old_slot,
+    const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
+    const uint32_t* __restrict__ pid_in, float4* __restrict__ pos_out, float4* __restrict__ vel_out,
+    uint32_t* __restrict__ pid_out, uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_end,
+    uint32_t* __restrict__ counters) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t s = old_slot[k];
+    pos_out[k] = pos_in[s];
+    vel_out[k] = vel_in[s];
+    pid_out[k] = pid_in[s];
+    const uint32_t h = skey[k];
+    if (k == 0 || skey[k - 1] != h) cell_start[h] = k;
+    if (k + 1 == n || skey[k + 1] != h) {
+        cell_end[h] = k + 1;
+        if (k >= cap && skey[k - cap] == h) atomicAdd(&counters[kCntCapCells], 1u);
+    }
+}
+
+// ------------------------------------------------------------------ density
+template <int kPrec>
+__global__ __launch_bounds__(kBlock) void k_density(sph_params p, uint32_t n,
+                                                    const float4* __restrict__ pos,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    const uint32_t* __restrict__ cell_end,
+                                                    float2* __restrict__ rhop) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xi = pos[i];
+    bool out = false;
+    int gx, gy, gz;
+    grid_pos(p, xi.x, xi.y, xi.z, gx, gy, gz, out);
+    const float h2 = p.h2;
+    float sum = 0.0f;
+    KahanF ks{0.0f, 0.0f};
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int cz = gz + dz;
+        if (cz < 0 || cz >= (int)p.grid_size[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int cy = gy + dy;
+            if (cy < 0 || cy >= (int)p.grid_size[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = gx + dx;
+                if (cx < 0 || cx >= (int)p.grid_size[0]) continue;
+                const uint32_t c = grid_hash(p, cx, cy, cz);
+                const uint32_t s = cell_start[c];
+                if (s == kEmptyCell) continue;
+                const uint32_t e = min(cell_end[c], s + p.max_par_in_cell);
+                float cell = 0.0f;
+                for (uint32_t j = s; j < e; ++j) {
+                    if (j == i) continue;
+                    const float4 xj = pos[j];
+                    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    if (r2 < h2) {
+                        const float c3 = h2 - r2;
+                        const float w = (c3 * c3) * c3;
+                        if (kPrec == SPH_PREC_KAHAN)
+                            ks.add(w);
+                        else
+                            cell = cell + w;
+                    }
+                }
+                if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
+            }
+        }
+    }
+    if (kPrec == SPH_PREC_KAHAN) sum = ks.s;
+    const float rho = (sum * p.poly6_kern) * p.particle_mass;
+    rhop[i] = make_float2(rho, eos(p, rho));
+}
+
+// -------------------------------------------------------------------- force
+template <int kPrec, bool kKeepDvel>
+__global__ __launch_bounds__(kBlock) void k_force(
+    sph_params p, uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xi = pos[i];
+    const float4 vi = vel[i];
+    const float2 ri = rhop[i];
+    bool out = false;
+    int gx, gy, gz;
+    grid_pos(p, xi.x, xi.y, xi.z, gx, gy, gz, out);
+    const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
+    const float vterm = p.lap_kern * p.viscosity;
+    const float r2max = p.force_r2_max;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    KahanF kx{0.0f, 0.0f}, ky{0.0f, 0.0f}, kz{0.0f, 0.0f};
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int cz = gz + dz;
+        if (cz < 0 || cz >= (int)p.grid_size[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int cy = gy + dy;
+            if (cy < 0 || cy >= (int)p.grid_size[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = gx + dx;
+                if (cx < 0 || cx >= (int)p.grid_size[0]) continue;
+                const uint32_t c = grid_hash(p, cx, cy, cz);
+                const uint32_t s = cell_start[c];
+                if (s == kEmptyCell) continue;
+                const uint32_t e = min(cell_end[c], s + p.max_par_in_cell);
+                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                for (uint32_t j = s; j < e; ++j) {
+                    if (j == i) continue;
+                    const float4 xj = pos[j];
+                    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max
+                    if (r2 <= r2max) {
+                        const float4 vj = vel[j];
+                        const float2 rj = rhop[j];
+                        const float r = fmaxf(min_dist, sqrtf(r2));
+                        const float d12 = fminf(min_dens, 1.0f / (ri.x * rj.x));
+                        const float cc = h - r;
+                        const float pterm = ((cc * spiky) * (ri.y + rj.y)) / r;
+                        const float cd = cc * d12;
+                        const float gxv = (pterm * rx + vterm * (vj.x - vi.x)) * cd;
+                        const float gyv = (pterm * ry + vterm * (vj.y - vi.y)) * cd;
+                        const float gzv = (pterm * rz + vterm * (vj.z - vi.z)) * cd;
+                        if (kPrec == SPH_PREC_KAHAN) {
+                            kx.add(gxv);
+                            ky.add(gyv);
+                            kz.add(gzv);
+                        } else {
+                            fx = fx + gxv;
+                            fy = fy + gyv;
+                            fz = fz + gzv;
+                        }
+                    }
+                }
+                if (kPrec != SPH_PREC_KAHAN) {
+                    ax = ax + fx;
+                    ay = ay + fy;
+                    az = az + fz;
+                }
+            }
+        }
+    }
+    if (kPrec == SPH_PREC_KAHAN) {
+        ax = kx.s;
+        ay = ky.s;
+        az = kz.s;
+    }
+    const float mdt = p.particle_mass * *dt_dev;
+    ax = ax * mdt;
+    ay = ay * mdt;
+    az = az * mdt;
+    vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
+    if (kKeepDvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// -------------------------------------------------------------------- stats
+constexpr int kStatBlock = 256;
+constexpr uint32_t kMaxStatBlocks = 1024;
+
+__device__ __forceinline__ void block_reduce4(double v[4], double* sh) {
+    // sh: 4 * kStatBlock doubles.  Sums for 0..2, max for 3; fixed order.
+    const int t = threadIdx.x;
+    for (int k = 0; k < 4; ++k) sh[k * kStatBlock + t] = v[k];
+    __syncthreads();
+    for (int w = kStatBlock / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 3; ++k) sh[k * kStatBlock + t] += sh[k * kStatBlock + t + w];
+            sh[3 * kStatBlock + t] = fmax(sh[3 * kStatBlock + t], sh[3 * kStatBlock + t + w]);
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < 4; ++k) v[k] = sh[k * kStatBlock];
+}
+
+__global__ __launch_bounds__(kStatBlock) void k_stats_partials(sph_params p, uint32_t n,
+                                                              const float4* __restrict__ pos,
+                                                              const float4* __restrict__ vel,
+                                                              double* __restrict__ partials) {
+    __shared__ double sh[4 * kStatBlock];
+    const double m = (double)p.particle_mass;
+    const double g[3] = {(double)p.gravity[0], (double)p.gravity[1], (double)p.gravity[2]};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = blockIdx.x * kStatBlock + threadIdx.x; i < n; i += gridDim.x * kStatBlock) {
+        const float4 x = pos[i];
+        const float4 v = vel[i];
+        const double v2 = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
+        acc[0] += m;
+        acc[1] += 0.5 * m * v2;
+        acc[2] += -m * (g[0] * x.x + g[1] * x.y + g[2] * x.z);
+        acc[3] = fmax(acc[3], v2);
+    }
+    block_reduce4(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) partials[blockIdx.x * kPNum + k] = acc[k];
+}
+
+__global__ __launch_bounds__(kStatBlock) void k_stats_finalize(sph_params p, uint32_t nblocks,
+                                                              const double* __restrict__ partials,
+                                                              double* __restrict__ stats,
+                                                              float* __restrict__ dt_dev,
+                                                              int advance_clock) {
+    __shared__ double sh[4 * kStatBlock];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < nblocks; b += kStatBlock) {
+        for (int k = 0; k < 3; ++k) acc[k] += partials[b * kPNum + k];
+        acc[3] = fmax(acc[3], partials[b * kPNum + 3]);
+    }
+    block_reduce4(acc, sh);
+    if (threadIdx.x == 0) {
+        const double vmax = sqrt(acc[3]);
+        stats[kPMass] = acc[0];
+        stats[kPKin] = acc[1];
+        stats[kPPot] = acc[2];
+        stats[kPVmax] = vmax;
+        double dt = (double)p.time_step;
+        if (p.cfl > 0.0f) {
+            const double dcfl = (double)p.cfl * (double)p.h / ((double)p.sound_speed + vmax);
+            if (dcfl < dt) dt = dcfl;
+        }
+        stats[kPNum] = dt;
+        *dt_dev = (float)dt;
+        if (advance_clock) stats[kPNum + 1] += (double)(float)dt;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_stats(uint32_t num_cells,
+                                                      const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ cell_end,
+                                                      uint32_t* __restrict__ max_count) {
+    uint32_t m = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < num_cells;
+         c += gridDim.x * blockDim.x) {
+        const uint32_t s = cell_start[c];
+        if (s != kEmptyCell) m = max(m, cell_end[c] - s);
+    }
+    if (m) atomicMax(max_count, m);
+}
+
+// ---------------------------------------------------------------- transfers
+__global__ __launch_bounds__(kBlock) void k_scatter_to_original(uint32_t n,
+                                                               const uint32_t* __restrict__ pid,
+                                                               const float* __restrict__ src,
+                                                               float* __restrict__ dst, int rec,
+                                                               int stride) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t id = pid[k];
+    for (int c = 0; c < rec; ++c) dst[(size_t)id * rec + c] = src[(size_t)k * stride + c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_from_original(uint32_t n,
+                                                                const uint32_t* __restrict__ pid,
+                                                                const float4* __restrict__ staging,
+                                                                uint32_t first, uint32_t count,
+                                                                float4* __restrict__ dst) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t id = pid[k];
+    if (id >= first && id - first < count) dst[k] = staging[id - first];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ dst, uint32_t v,
+                                                    size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota_u32(uint32_t* __restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = i;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t block) {
+    return (uint32_t)((n + block - 1) / block);
+}
+
+}  // namespace
+
+hipError_t launch_integrate_hash(const sph_params& p, const StepBuffers& b, hipStream_t s) {
+    const uint32_t n = p.num_particles;
+    uint32_t nb = blocks_for(n, kBlock);
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_integrate_hash, dim3(nb), dim3(kBlock), 0, s, p, n, b.pos_cur, b.vel_cur,
+                       b.pid_cur, b.key_pid, b.val_pid, b.cell_start, b.counters, b.dt_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder(const sph_params& p, const StepBuffers& b, const uint32_t* sorted_key,
+                          const uint32_t* old_slot, hipStream_t s) {
+    const uint32_t n = p.num_particles;
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, n,
+                       p.max_par_in_cell, sorted_key, old_slot, b.pos_cur, b.vel_cur, b.pid_cur,
+                       b.pos_alt, b.vel_alt, b.pid_alt, b.cell_start, b.cell_end, b.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const sph_params& p, const StepBuffers& b, hipStream_t s) {
+    const uint32_t n = p.num_particles;
+    if (p.precision == SPH_PREC_KAHAN)
+        hipLaunchKernelGGL(k_density<SPH_PREC_KAHAN>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0,
+                           s, p, n, b.pos_cur, b.cell_start, b.cell_end, b.rhop);
+    else
+        hipLaunchKernelGGL(k_density<SPH_PREC_REFERENCE>, dim3(blocks_for(n, kBlock)), dim3(kBlock),
+                           0, s, p, n, b.pos_cur, b.cell_start, b.cell_end, b.rhop);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(const sph_params& p, const StepBuffers& b, float4* vel_out, bool keep_dvel,
+                        hipStream_t s) {
+    const uint32_t n = p.num_particles;
+    const dim3 g(blocks_for(n, kBlock)), t(kBlock);
+    const bool kahan = p.precision == SPH_PREC_KAHAN;
+    if (kahan && keep_dvel)
+        hipLaunchKernelGGL((k_force<SPH_PREC_KAHAN, true>), g, t, 0, s, p, n, b.pos_cur, b.vel_cur,
+                           b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
+    else if (kahan)
+        hipLaunchKernelGGL((k_force<SPH_PREC_KAHAN, false>), g, t, 0, s, p, n, b.pos_cur, b.vel_cur,
+                           b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
+    else if (keep_dvel)
+        hipLaunchKernelGGL((k_force<SPH_PREC_REFERENCE, true>), g, t, 0, s, p, n, b.pos_cur,
+                           b.vel_cur, b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
+    else
+        hipLaunchKernelGGL((k_force<SPH_PREC_REFERENCE, false>), g, t, 0, s, p, n, b.pos_cur,
+                           b.vel_cur, b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
+    return hipGetLastError();
+}
+
+uint32_t stats_num_blocks(uint32_t n) {
+    uint32_t nb = blocks_for(n, kStatBlock);
+    if (nb > kMaxStatBlocks) nb = kMaxStatBlocks;
+    return nb == 0 ? 1 : nb;
+}
+
+hipError_t launch_stats(const sph_params& p, const StepBuffers& b, bool advance_clock,
+                        hipStream_t s) {
+    const uint32_t nb = stats_num_blocks(p.num_particles);
+    hipLaunchKernelGGL(k_stats_partials, dim3(nb), dim3(kStatBlock), 0, s, p, p.num_particles,
+                       b.pos_cur, b.vel_cur, b.partials);
+    hipLaunchKernelGGL(k_stats_finalize, dim3(1), dim3(kStatBlock), 0, s, p, nb, b.partials,
+                       b.stats, b.dt_dev, advance_clock ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_stats(const sph_params& p, const StepBuffers& b, uint32_t* max_count,
+                             hipStream_t s) {
+    uint32_t nb = blocks_for(p.num_cells, kBlock);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_cell_stats, dim3(nb), dim3(kBlock), 0, s, p.num_cells, b.cell_start,
+                       b.cell_end, max_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_to_original(uint32_t n, const uint32_t* pid, const void* src, void* dst,
+                                      int record_floats, int src_stride_floats, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_to_original, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, n,
+                       pid, (const float*)src, (float*)dst, record_floats, src_stride_floats);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_from_original(uint32_t n, const uint32_t* pid, const float4* staging,
+                                       uint32_t first, uint32_t count, float4* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_from_original, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, n,
+                       pid, staging, first, count, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, size_t count, hipStream_t s) {
+    uint32_t nb = blocks_for(count, kBlock);
+    if (nb > 4096) nb = 4096;
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(kBlock), 0, s, dst, value, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_iota_u32, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace sph

@@ -1,0 +1,302 @@
+"""ctypes binding of the C ABI in include/pibiti_sph.h and a host-side mirror
+of the reference's cSPH class (source/SPH/SPH.h:9-50).
+
+The binding loads the in-tree ``libpibiti_sph.so``.  There is no fallback: if
+the library is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libpibiti_sph.so"
+
+# status codes (sph_status)
+SPH_OK = 0
+# enums
+SPH_EOS_LINEAR, SPH_EOS_TAIT = 0, 1
+SPH_PREC_REFERENCE, SPH_PREC_KAHAN = 0, 1
+SPH_ARRAY_POSITION, SPH_ARRAY_VELOCITY, SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_DVEL = range(5)
+SPH_FLAG_KEEP_DVEL = 1
+EMPTY_CELL = 0xFFFFFFFF
+
+f3 = C.c_float * 3
+u3 = C.c_uint32 * 3
+
+
+class Scene(C.Structure):
+    """sph_scene: raw scene values before Scene::_UpdatePar/_UpdateGrid."""
+
+    _fields_ = [
+        ("num_particles", C.c_uint32),
+        ("max_par_in_cell", C.c_uint32),
+        ("time_step", C.c_float),
+        ("gravity", f3),
+        ("global_damping", C.c_float),
+        ("world_min", f3),
+        ("world_max", f3),
+        ("cell_size", C.c_float),
+        ("particle_r", C.c_float),
+        ("h", C.c_float),
+        ("rest_density", C.c_float),
+        ("stiffness", C.c_float),
+        ("viscosity", C.c_float),
+        ("min_dens", C.c_float),
+        ("min_dist", C.c_float),
+        ("particle_mass", C.c_float),
+        ("dist_bnd_soft", C.c_float),
+        ("dist_bnd_hard", C.c_float),
+        ("bnd_stiff", C.c_float),
+        ("bnd_damp", C.c_float),
+        ("bnd_damp_c", C.c_float),
+        ("eos", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("cfl", C.c_float),
+    ]
+
+
+class Params(C.Structure):
+    """sph_params: derived kernel constants (SimParams equivalent)."""
+
+    _fields_ = [
+        ("time_step", C.c_float),
+        ("num_particles", C.c_uint32),
+        ("max_par_in_cell", C.c_uint32),
+        ("gravity", f3),
+        ("global_damping", C.c_float),
+        ("grid_size", u3),
+        ("cell_size", f3),
+        ("grid_size_yx", C.c_uint32),
+        ("num_cells", C.c_uint32),
+        ("world_min", f3),
+        ("world_max", f3),
+        ("particle_r", C.c_float),
+        ("h", C.c_float),
+        ("h2", C.c_float),
+        ("spiky_kern", C.c_float),
+        ("lap_kern", C.c_float),
+        ("poly6_kern", C.c_float),
+        ("particle_mass", C.c_float),
+        ("rest_density", C.c_float),
+        ("stiffness", C.c_float),
+        ("viscosity", C.c_float),
+        ("min_dens", C.c_float),
+        ("min_dist", C.c_float),
+        ("dist_bnd_hard", C.c_float),
+        ("dist_bnd_soft", C.c_float),
+        ("bnd_damp", C.c_float),
+        ("bnd_stiff", C.c_float),
+        ("bnd_damp_c", C.c_float),
+        ("bnd_type", C.c_uint32),
+        ("bnd_eff_z", C.c_uint32),
+        ("eos", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("cfl", C.c_float),
+        ("sound_speed", C.c_float),
+        ("key_bits", C.c_uint32),
+        ("force_r2_max", C.c_float),
+        ("reserved", C.c_uint32 * 8),
+    ]
+
+    def copy(self) -> "Params":
+        out = Params()
+        C.memmove(C.byref(out), C.byref(self), C.sizeof(Params))
+        return out
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("mass", C.c_double),
+        ("kinetic_energy", C.c_double),
+        ("potential_energy", C.c_double),
+        ("max_speed", C.c_double),
+        ("dt_cfl", C.c_double),
+        ("sim_time", C.c_double),
+        ("steps", C.c_uint64),
+        ("cap_cells", C.c_uint32),
+        ("out_of_grid", C.c_uint32),
+        ("max_cell_count", C.c_uint32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class SphError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {strerror(status)} (status {status})")
+        self.status = status
+
+
+_lib_handle = None
+
+_P = C.c_void_p
+_fp = C.POINTER(C.c_float)
+_up = C.POINTER(C.c_uint32)
+
+# exported symbol -> (restype, argtypes); every symbol declared in pibiti_sph.h
+SIGNATURES = {
+    "sph_abi_version": (C.c_uint32, []),
+    "sph_strerror": (C.c_char_p, [C.c_int]),
+    "sph_scene_default": (C.c_int, [C.POINTER(Scene)]),
+    "sph_params_from_scene": (C.c_int, [C.POINTER(Scene), C.POINTER(Params)]),
+    "sph_scene_lattice": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, _fp, C.c_float, C.c_float,
+                                    C.c_float, _fp, C.c_uint64, C.c_uint64, _fp, _fp]),
+    "sph_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_uint32, C.POINTER(_P)]),
+    "sph_destroy": (C.c_int, [_P]),
+    "sph_set_params": (C.c_int, [_P, C.POINTER(Params)]),
+    "sph_set_array": (C.c_int, [_P, C.c_int, _fp, C.c_uint32, C.c_uint32]),
+    "sph_get_array": (C.c_int, [_P, C.c_int, _fp]),
+    "sph_step": (C.c_int, [_P, C.c_uint32]),
+    "sph_get_grid": (C.c_int, [_P, _up, _up, _up, _up]),
+    "sph_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "sph_synchronize": (C.c_int, [_P]),
+    "sph_set_timing": (C.c_int, [_P, C.c_int]),
+    "sph_get_timing": (C.c_int, [_P, _fp]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pibiti_amd.build`")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib_handle = h
+    return _lib_handle
+
+
+def strerror(status: int) -> str:
+    return lib().sph_strerror(status).decode()
+
+
+def _check(status: int, what: str) -> None:
+    if status != SPH_OK:
+        raise SphError(status, what)
+
+
+def _f(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+def _u(a: np.ndarray):
+    return a.ctypes.data_as(_up)
+
+
+def scene_default() -> Scene:
+    s = Scene()
+    _check(lib().sph_scene_default(C.byref(s)), "sph_scene_default")
+    return s
+
+
+def params_from_scene(s: Scene) -> Params:
+    p = Params()
+    _check(lib().sph_params_from_scene(C.byref(s), C.byref(p)), "sph_params_from_scene")
+    return p
+
+
+def scene_lattice(nx: int, ny: int, nz: int, region_min, spacing: float, jitter: float,
+                  vel_sigma: float = 0.0, vel_offset=(0.0, 0.0, 0.0), seed: int = 0,
+                  stream: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """Seeded jittered lattice (float4 positions, float4 velocities)."""
+    n = nx * ny * nz
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    rmin = np.asarray(region_min, np.float32)
+    voff = np.asarray(vel_offset, np.float32)
+    _check(lib().sph_scene_lattice(nx, ny, nz, _f(rmin), spacing, jitter, vel_sigma, _f(voff),
+                                   seed, stream, _f(pos), _f(vel)), "sph_scene_lattice")
+    return pos, vel
+
+
+class SphSystem:
+    """Mirror of cSPH (source/SPH/SPH.h:9-50) over the C ABI.
+
+    init with params ~ cSPH::InitScene/_InitMem, set_array ~ setArray,
+    get_array ~ getArray, update ~ Update.
+    """
+
+    def __init__(self, params: Params, device: int = 0, keep_dvel: bool = False):
+        self.params = params.copy()
+        self.n = int(params.num_particles)
+        self._ctx = C.c_void_p()
+        flags = SPH_FLAG_KEEP_DVEL if keep_dvel else 0
+        _check(lib().sph_create(C.byref(self.params), device, flags, C.byref(self._ctx)), "sph_create")
+
+    # context management -------------------------------------------------
+    def close(self) -> None:
+        if self._ctx:
+            lib().sph_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference interface ------------------------------------------------
+    def set_params(self, params: Params) -> None:
+        _check(lib().sph_set_params(self._ctx, C.byref(params)), "sph_set_params")
+        self.params = params.copy()
+
+    def set_array(self, which: int, data: np.ndarray, first: int = 0) -> None:
+        a = np.ascontiguousarray(data, np.float32).reshape(-1, 4)
+        _check(lib().sph_set_array(self._ctx, which, _f(a), first, a.shape[0]), "sph_set_array")
+
+    def reset(self, pos4: np.ndarray, vel4: np.ndarray) -> None:
+        self.set_array(SPH_ARRAY_POSITION, pos4)
+        self.set_array(SPH_ARRAY_VELOCITY, vel4)
+
+    def get_array(self, which: int) -> np.ndarray:
+        shape = (self.n,) if which in (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE) else (self.n, 4)
+        out = np.empty(shape, np.float32)
+        _check(lib().sph_get_array(self._ctx, which, _f(out)), "sph_get_array")
+        return out
+
+    def update(self, nsteps: int = 1) -> None:
+        _check(lib().sph_step(self._ctx, nsteps), "sph_step")
+
+    def synchronize(self) -> None:
+        _check(lib().sph_synchronize(self._ctx), "sph_synchronize")
+
+    def get_grid(self) -> dict:
+        n, nc = self.n, int(self.params.num_cells)
+        out = {
+            "sorted_hash": np.empty(n, np.uint32),
+            "perm": np.empty(n, np.uint32),
+            "cell_start": np.empty(nc, np.uint32),
+            "cell_end": np.empty(nc, np.uint32),
+        }
+        _check(lib().sph_get_grid(self._ctx, _u(out["sorted_hash"]), _u(out["perm"]),
+                                  _u(out["cell_start"]), _u(out["cell_end"])), "sph_get_grid")
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().sph_get_stats(self._ctx, C.byref(s)), "sph_get_stats")
+        return s.as_dict()
+
+    def set_timing(self, enable: bool) -> None:
+        _check(lib().sph_set_timing(self._ctx, int(enable)), "sph_set_timing")
+
+    def timing(self) -> dict:
+        ms = np.zeros(6, np.float32)
+        _check(lib().sph_get_timing(self._ctx, _f(ms)), "sph_get_timing")
+        keys = ["integrate_hash", "sort", "reorder", "density", "force", "step"]
+        return dict(zip(keys, map(float, ms)))

@@ -1,0 +1,104 @@
+"""The C-ABI library loads, exports every entry point the public headers
+declare, and its host-only functions and error paths behave (no device use)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pibiti_amd import sph
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = []
+    for hdr in sorted((ROOT / "include").glob("*.h")):
+        text = hdr.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**\s*(sph_[A-Za-z0-9_]+)\s*\(",
+                             text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_headers_declare_entry_points():
+    names = declared_functions()
+    assert "sph_create" in names and "sph_step" in names and "sph_get_array" in names
+    assert len(names) >= 16
+
+
+def test_library_exports_every_declared_symbol():
+    h = C.CDLL(str(sph.LIB_PATH))
+    missing = [n for n in declared_functions() if not hasattr(h, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert set(declared_functions()) <= set(sph.SIGNATURES)
+
+
+def test_abi_version_and_strerror():
+    assert sph.lib().sph_abi_version() == 1
+    assert sph.strerror(0) == "ok"
+    for code in range(1, 9):
+        assert sph.strerror(code) not in ("ok", "unknown status")
+    assert sph.strerror(999) == "unknown status"
+
+
+def test_scene_default_matches_reference_defaults():
+    s = sph.scene_default()
+    # source/SPH/Scene.cpp:11-31
+    assert s.num_particles == 7 * 8 * 1024 and s.max_par_in_cell == 16
+    assert s.time_step == np.float32(0.0026) and s.h == np.float32(0.01)
+    assert s.particle_r == np.float32(0.004) and s.rest_density == 1000
+    assert s.stiffness == 3 and s.viscosity == np.float32(0.5)
+    assert s.dist_bnd_soft == 8 and s.dist_bnd_hard == 1
+    assert (s.bnd_stiff, s.bnd_damp, s.bnd_damp_c) == (30000, 256, 60)
+    assert s.cell_size == np.float32(0.008)
+
+
+@pytest.mark.parametrize("field,value", [
+    ("num_particles", 0), ("h", 0.0), ("h", -1.0), ("cell_size", 0.0), ("time_step", 0.0),
+    ("rest_density", 0.0), ("max_par_in_cell", 0), ("eos", 7), ("precision", 9), ("cfl", -1.0),
+])
+def test_params_from_scene_rejects_bad_input(field, value):
+    s = sph.scene_default()
+    setattr(s, field, value)
+    p = sph.Params()
+    assert sph.lib().sph_params_from_scene(C.byref(s), C.byref(p)) == 2  # SPH_ERR_INVALID_PARAMS
+
+
+def test_params_from_scene_rejects_inverted_world():
+    s = sph.scene_default()
+    s.world_max[1] = s.world_min[1]
+    p = sph.Params()
+    assert sph.lib().sph_params_from_scene(C.byref(s), C.byref(p)) == 2
+
+
+def test_null_arguments_are_errors_not_crashes():
+    L = sph.lib()
+    assert L.sph_scene_default(None) == 1
+    assert L.sph_params_from_scene(None, None) == 1
+    assert L.sph_create(None, 0, 0, None) == 1
+    assert L.sph_destroy(None) == 0
+    assert L.sph_step(None, 1) == 1
+    assert L.sph_get_array(None, 0, None) == 1
+    assert L.sph_set_array(None, 0, None, 0, 0) == 1
+    assert L.sph_get_stats(None, None) == 1
+    assert L.sph_get_grid(None, None, None, None, None) == 1
+    assert L.sph_synchronize(None) == 1
+
+
+def test_create_rejects_invalid_params_before_touching_a_device():
+    p = sph.params_from_scene(sph.scene_default())
+    p.num_particles = 0
+    ctx = C.c_void_p()
+    assert sph.lib().sph_create(C.byref(p), 0, 0, C.byref(ctx)) == 2
+    p = sph.params_from_scene(sph.scene_default())
+    p.key_bits = 2  # fewer bits than the cell count needs
+    assert sph.lib().sph_create(C.byref(p), 0, 0, C.byref(ctx)) == 2
+    p = sph.params_from_scene(sph.scene_default())
+    p.bnd_type = 3  # only the box boundary is built
+    assert sph.lib().sph_create(C.byref(p), 0, 0, C.byref(ctx)) == 2
