@@ -140,7 +140,12 @@ typedef struct sph_params {
      * predicate of compForcePair (source/CUDA/Kernel_Cell.cui:214-218)
      * rewritten as one comparison r2 <= force_r2_max; -1 when min_dist >= h. */
     float force_r2_max;
-    uint32_t reserved[8];
+    /* Launch tuning (0 = library default); any legal value gives the same
+     * results.  tune[0]: force kernel variant, 1 = fused candidate and pair
+     * loop, 16/24/32/48 = neighbour-list entries per lane; tune[1..3]:
+     * reserved.                                                            */
+    uint32_t tune[4];
+    uint32_t reserved[4];
 } sph_params;
 
 /* Global FP64 diagnostics reduced on the device (extension). */
@@ -155,7 +160,8 @@ typedef struct sph_stats {
     uint32_t cap_cells;         /* cells holding more than max_par_in_cell   */
     uint32_t out_of_grid;       /* particles whose cell had to be clamped    */
     uint32_t max_cell_count;    /* largest particle count of one cell        */
-    uint32_t reserved[7];
+    uint32_t sort_errors;       /* nonzero: a sort look-back wait hit its bound (results invalid) */
+    uint32_t reserved[6];
 } sph_stats;
 
 typedef struct sph_ctx sph_ctx;

@@ -58,8 +58,12 @@ def lib() -> C.CDLL:
                                       _up, _up, _up, _up, _up]
         h.oracle_run_f32.restype = C.c_int
         h.oracle_run_f32.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, _fp, _fp, _fp, _fp]
+        h.oracle_step_f32_ex.restype = C.c_int
+        h.oracle_step_f32_ex.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _fp, _fp, _fp, _fp, _fp, _fp,
+                                         _up, _up, _up, _up, _up]
         h.oracle_step_f64.restype = C.c_int
-        h.oracle_step_f64.argtypes = [C.c_void_p, C.c_uint32, _fp, _fp, _dp, _dp, _dp, _dp, _dp, _dp]
+        h.oracle_step_f64.argtypes = [C.c_void_p, C.c_uint32, _fp, _fp, C.c_int, _dp, _dp, _dp, _dp,
+                                      _dp, _dp]
         h.oracle_integrate_f32.argtypes = [C.c_void_p, C.c_uint32, _fp, _fp]
         h.oracle_hash_f32.restype = C.c_uint32
         h.oracle_hash_f32.argtypes = [C.c_void_p, C.c_uint32, _fp, _up]
@@ -101,12 +105,14 @@ def step_f32(params, pos4, vel4, prec: int = 0) -> dict:
     vel = np.array(vel4, np.float32, copy=True).reshape(n, 4)
     out = dict(
         dens=np.empty(n, np.float32), pres=np.empty(n, np.float32), dvel=np.empty((n, 4), np.float32),
+        vel_int=np.empty((n, 4), np.float32),
         sorted_hash=np.empty(n, np.uint32), perm=np.empty(n, np.uint32),
         cell_start=np.empty(nc, np.uint32), cell_end=np.empty(nc, np.uint32),
         counters=np.zeros(4, np.uint32))
-    st = lib().oracle_step_f32(_pp(params), prec, n, _f(pos), _f(vel), _f(out["dens"]), _f(out["pres"]),
-                               _f(out["dvel"]), _u(out["sorted_hash"]), _u(out["perm"]),
-                               _u(out["cell_start"]), _u(out["cell_end"]), _u(out["counters"]))
+    st = lib().oracle_step_f32_ex(_pp(params), prec, n, _f(pos), _f(vel), _f(out["vel_int"]),
+                                  _f(out["dens"]), _f(out["pres"]), _f(out["dvel"]),
+                                  _u(out["sorted_hash"]), _u(out["perm"]), _u(out["cell_start"]),
+                                  _u(out["cell_end"]), _u(out["counters"]))
     assert st == 0
     out["pos"] = pos
     out["vel"] = vel
@@ -124,16 +130,25 @@ def run_f32(params, pos4, vel4, nsteps: int, prec: int = 0) -> dict:
     return dict(pos=pos, vel=vel, dens=dens, pres=pres)
 
 
-def step_f64(params, pos4, vel4) -> dict:
+def step_f64(params, pos4, vel4, skip_integrate: bool = False) -> dict:
+    """One FP64 step from float inputs.  skip_integrate: the inputs already are
+    the post-integrate state (compare density/force sums on identical inputs)."""
     n = int(params.num_particles)
     pin = np.ascontiguousarray(pos4, np.float32).reshape(n, 4)
     vin = np.ascontiguousarray(vel4, np.float32).reshape(n, 4)
     out = dict(pos=np.empty((n, 4)), vel=np.empty((n, 4)), dens=np.empty(n), pres=np.empty(n),
                dvel=np.empty((n, 4)), dvel_scale=np.empty(n))
-    st = lib().oracle_step_f64(_pp(params), n, _f(pin), _f(vin), _d(out["pos"]), _d(out["vel"]),
-                               _d(out["dens"]), _d(out["pres"]), _d(out["dvel"]), _d(out["dvel_scale"]))
+    st = lib().oracle_step_f64(_pp(params), n, _f(pin), _f(vin), int(skip_integrate), _d(out["pos"]),
+                               _d(out["vel"]), _d(out["dens"]), _d(out["pres"]), _d(out["dvel"]),
+                               _d(out["dvel_scale"]))
     assert st == 0
     return out
+
+
+def kernels_f64_on(step32: dict, params) -> dict:
+    """Density/pressure/force in FP64 on the exact post-integrate FP32 state of
+    an FP32 step (its positions and integrated velocities)."""
+    return step_f64(params, step32["pos"], step32["vel_int"], skip_integrate=True)
 
 
 def integrate_f32(params, pos4, vel4):

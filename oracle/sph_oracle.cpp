@@ -280,6 +280,7 @@ struct StepOut {
     // all in original index order
     T* pos4;
     T* vel4;
+    T* vel_int4;    // velocity after integrate, before the force update
     T* dens;
     T* pres;
     T* dvel4;
@@ -293,7 +294,7 @@ struct StepOut {
 
 template <typename T>
 void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const T* vel_in,
-               const StepOut<T>& o) {
+               bool skip_integrate, const StepOut<T>& o) {
     const uint32_t ncell = p.num_cells;
     std::vector<T> pos(pos_in, pos_in + 4 * (size_t)n), vel(vel_in, vel_in + 4 * (size_t)n);
     Counters cnt{};
@@ -301,6 +302,7 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
     // 1. integrate in original order (SPH_Update.cpp:39-47).
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; ++i) {
+        if (skip_integrate) continue;
         T x[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
         T v[3] = {vel[4 * i], vel[4 * i + 1], vel[4 * i + 2]};
         integrate_one<T>(p, x, v);
@@ -309,6 +311,7 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
             vel[4 * i + c] = v[c];
         }
     }
+    if (o.vel_int4) std::copy(vel.begin(), vel.end(), o.vel_int4);
     // 2. hash in original order (SPH_Update.cpp:56).
     std::vector<uint32_t> hash(n);
     uint32_t oob = 0;
@@ -413,9 +416,22 @@ int oracle_step_f32(const sph_params* p, int prec, uint32_t n, float* pos4, floa
                     float* dens, float* pres, float* dvel4, uint32_t* sorted_hash, uint32_t* perm,
                     uint32_t* cell_start, uint32_t* cell_end, uint32_t* counters) {
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
-    StepOut<float> o{pos4, vel4, dens, pres, dvel4, nullptr, sorted_hash, perm, cell_start,
+    StepOut<float> o{pos4, vel4, nullptr, dens, pres, dvel4, nullptr, sorted_hash, perm, cell_start,
                      cell_end, reinterpret_cast<Counters*>(counters)};
-    step_impl<float>(*p, prec, n, pos4, vel4, o);
+    step_impl<float>(*p, prec, n, pos4, vel4, false, o);
+    return 0;
+}
+
+// The same step, also returning the velocity after integrate (before the
+// force update) in vel_int4 (original order, float4).
+int oracle_step_f32_ex(const sph_params* p, int prec, uint32_t n, float* pos4, float* vel4,
+                       float* vel_int4, float* dens, float* pres, float* dvel4,
+                       uint32_t* sorted_hash, uint32_t* perm, uint32_t* cell_start,
+                       uint32_t* cell_end, uint32_t* counters) {
+    if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
+    StepOut<float> o{pos4, vel4, vel_int4, dens, pres, dvel4, nullptr, sorted_hash, perm,
+                     cell_start, cell_end, reinterpret_cast<Counters*>(counters)};
+    step_impl<float>(*p, prec, n, pos4, vel4, false, o);
     return 0;
 }
 
@@ -424,9 +440,9 @@ int oracle_run_f32(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, f
                    float* vel4, float* dens, float* pres) {
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
     for (uint32_t s = 0; s < nsteps; ++s) {
-        StepOut<float> o{pos4, vel4, dens, pres, nullptr, nullptr, nullptr, nullptr, nullptr,
-                         nullptr, nullptr};
-        step_impl<float>(*p, prec, n, pos4, vel4, o);
+        StepOut<float> o{pos4, vel4, nullptr, dens, pres, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, nullptr};
+        step_impl<float>(*p, prec, n, pos4, vel4, false, o);
     }
     return 0;
 }
@@ -434,15 +450,18 @@ int oracle_run_f32(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, f
 // One FP64 step from FP32 inputs.  Outputs are double, original order.
 // dvel_scale[i] = m*dt * sum_j |c d12| (|pterm|_abs r + |vterm| |dv_ij|): the
 // absolute-value evaluation of the force sum, the scale FP32 rounding error
-// of that sum is proportional to.
+// of that sum is proportional to.  skip_integrate != 0 treats the inputs as
+// the state after integrate (hash, sort, density and force only), so the
+// density and force sums can be compared on the exact inputs the FP32 path
+// used.
 int oracle_step_f64(const sph_params* p, uint32_t n, const float* pos4_in, const float* vel4_in,
-                    double* pos4, double* vel4, double* dens, double* pres, double* dvel4,
-                    double* dvel_scale) {
+                    int skip_integrate, double* pos4, double* vel4, double* dens, double* pres,
+                    double* dvel4, double* dvel_scale) {
     if (!p || !pos4_in || !vel4_in || n != p->num_particles) return 1;
     std::vector<double> pin(pos4_in, pos4_in + 4 * (size_t)n), vin(vel4_in, vel4_in + 4 * (size_t)n);
-    StepOut<double> o{pos4, vel4, dens, pres, dvel4, dvel_scale, nullptr, nullptr, nullptr,
-                      nullptr, nullptr};
-    step_impl<double>(*p, SPH_PREC_REFERENCE, n, pin.data(), vin.data(), o);
+    StepOut<double> o{pos4, vel4, nullptr, dens, pres, dvel4, dvel_scale, nullptr, nullptr,
+                      nullptr, nullptr, nullptr};
+    step_impl<double>(*p, SPH_PREC_REFERENCE, n, pin.data(), vin.data(), skip_integrate != 0, o);
     return 0;
 }
 

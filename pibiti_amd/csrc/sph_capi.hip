@@ -32,6 +32,7 @@ struct sph_ctx {
     double* stats = nullptr;
     float* dt_dev = nullptr;
     float4* staging = nullptr;
+    void* sort_scratch = nullptr;
     sph::RadixSortPlan sort{};
 
     int cur_pos = 0, cur_vel = 0, cur_pid = 0;
@@ -110,7 +111,7 @@ void free_all(sph_ctx* c) {
     }
     void* ptrs[] = {c->rhop,      c->dvel,       c->key_pid,  c->val_pid,  c->key_tmp,
                     c->val_tmp,   c->cell_start, c->cell_end, c->counters, c->cell_max,
-                    c->partials,  c->stats,      c->dt_dev,   c->staging,  c->sort.block_hist};
+                    c->partials,  c->stats,      c->dt_dev,   c->staging,  c->sort_scratch};
     for (void* q : ptrs) (void)hipFree(q);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -177,11 +178,8 @@ int sph_create(const sph_params* params, int device, uint32_t flags, sph_ctx** o
     if (e == hipSuccess) e = dalloc(&c->stats, (size_t)kNumStats);
     if (e == hipSuccess) e = dalloc(&c->dt_dev, (size_t)1);
     if (e == hipSuccess) e = dalloc(&c->staging, n);
-    if (e == hipSuccess) {
-        c->sort.n = (uint32_t)n;
-        c->sort.num_tiles = (uint32_t)(sph::radix_scratch_bytes((uint32_t)n) / (256 * sizeof(uint32_t)));
-        e = hipMalloc(reinterpret_cast<void**>(&c->sort.block_hist), sph::radix_scratch_bytes((uint32_t)n));
-    }
+    if (e == hipSuccess) e = hipMalloc(&c->sort_scratch, sph::radix_scratch_bytes((uint32_t)n));
+    if (e == hipSuccess) e = sph::radix_plan_init(c->sort, (uint32_t)n, c->sort_scratch, c->stream);
     for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e == hipSuccess) e = sph::launch_iota_u32(c->pid[0], (uint32_t)n, c->stream);
     if (e == hipSuccess) e = sph::launch_fill_u32(c->cell_start, sph::kEmptyCell, ncell, c->stream);
@@ -377,6 +375,8 @@ int sph_get_stats(sph_ctx* c, sph_stats* out) {
     SPH_TRY(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     SPH_TRY(hipMemcpyAsync(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     SPH_TRY(hipMemcpyAsync(&cmax, c->cell_max, sizeof(cmax), hipMemcpyDeviceToHost, c->stream));
+    uint32_t sort_err = 0;
+    SPH_TRY(hipMemcpyAsync(&sort_err, c->sort.err, sizeof(sort_err), hipMemcpyDeviceToHost, c->stream));
     // Restore the time step the kernels use (the finalisation rewrote it from
     // the current state, which is what the next CFL step computes anyway).
     if (c->p.cfl <= 0.0f)
@@ -393,6 +393,7 @@ int sph_get_stats(sph_ctx* c, sph_stats* out) {
     out->cap_cells = cnt[sph::kCntCapCells];
     out->out_of_grid = cnt[sph::kCntOutOfGrid];
     out->max_cell_count = cmax;
+    out->sort_errors = sort_err;
     return SPH_OK;
 }
 

@@ -26,11 +26,18 @@ enum Partial : int { kPMass = 0, kPKin = 1, kPPot = 2, kPVmax = 3, kPNum = 4 };
 struct RadixSortPlan {
     uint32_t n = 0;
     uint32_t num_tiles = 0;
-    uint32_t* block_hist = nullptr;  // 256 * num_tiles counters
+    uint32_t* block_hist = nullptr;  // per-block digit histograms of every pass
+    uint32_t* digit_off = nullptr;   // exclusive digit offsets per pass
+    uint64_t* status = nullptr;      // look-back words, one per (tile, digit)
+    uint64_t* ticket = nullptr;      // tile ticket counter (monotone)
+    uint32_t* err = nullptr;         // set when a look-back wait hit its limit
 };
 
-// Bytes of scratch the sort of n keys needs for its per-tile histograms.
+// Bytes of device scratch the sort of n keys needs, and the plan carved out
+// of such a buffer (clears the look-back state; not graph-capturable).
 size_t radix_scratch_bytes(uint32_t n);
+uint32_t radix_num_tiles(uint32_t n);
+hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipStream_t stream);
 
 // Stable LSD sort of (keys, vals) by the low `bits` bits of keys.  keys0/vals0
 // hold the input; keys1/vals1 are the ping-pong buffers.  On return

@@ -164,6 +164,62 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     }
 }
 
+// ------------------------------------------------------- neighbour-cell walk
+// The 27 cells around a particle's cell in the reference's (dz, dy, dx)
+// order (source/CUDA/Kernel_Cell.cui:188-192).  A cell whose box is farther
+// than h from the particle is skipped: none of its particles can pass
+// r2 < h2 (or the force predicate, which is stricter), so its partial sum
+// would be +0 and adding +0 leaves the total unchanged.  The test is
+// conservative: each face distance is shrunk by a fuzz covering the
+// rounding of cell assignment, and the bound carries a 1e-5 relative margin.
+struct AxisWalk {
+    int g;           // own cell coordinate
+    int gmax;        // grid size - 1
+    float lo2, hi2;  // squared distance to the own cell's lower/upper face (minus fuzz)
+
+    __device__ __forceinline__ void init(float x, float wl, float wh, float cs, uint32_t gsize) {
+        bool out = false;
+        g = cell_coord(x, wl, cs, gsize, out);
+        gmax = (int)gsize - 1;
+        const float fuzz = 1e-6f * fmaxf(fabsf(wl), fabsf(wh)) + 1e-30f;
+        const float clo = wl + (float)g * cs;
+        const float chi = clo + cs;
+        const float lo = fmaxf(x - clo - fuzz, 0.0f);
+        const float hi = fmaxf(chi - x - fuzz, 0.0f);
+        lo2 = lo * lo;
+        hi2 = hi * hi;
+    }
+    // squared distance component of a neighbour offset d in {-1, 0, 1}
+    __device__ __forceinline__ float d2(int d) const { return d < 0 ? lo2 : (d > 0 ? hi2 : 0.0f); }
+    __device__ __forceinline__ bool ok(int d) const { return g + d >= 0 && g + d <= gmax; }
+};
+
+struct CellWalk {
+    AxisWalk ax, ay, az;
+    float cull2;  // squared cull radius
+
+    __device__ __forceinline__ void init(const sph_params& p, float x, float y, float z) {
+        ax.init(x, p.world_min[0], p.world_max[0], p.cell_size[0], p.grid_size[0]);
+        ay.init(y, p.world_min[1], p.world_max[1], p.cell_size[1], p.grid_size[1]);
+        az.init(z, p.world_min[2], p.world_max[2], p.cell_size[2], p.grid_size[2]);
+        cull2 = p.h2 * 1.00001f;
+    }
+    // Slot range [s, e) of neighbour cell (dx, dy, dz), capped at max_par_in_cell;
+    // false if the cell is outside the grid, culled or empty.
+    __device__ __forceinline__ bool open(const sph_params& p, int dx, int dy, int dz,
+                                         const uint32_t* __restrict__ cell_start,
+                                         const uint32_t* __restrict__ cell_end, uint32_t& s,
+                                         uint32_t& e) const {
+        if (!ax.ok(dx) || !ay.ok(dy) || !az.ok(dz)) return false;
+        if (az.d2(dz) + ay.d2(dy) + ax.d2(dx) > cull2) return false;
+        const uint32_t c = grid_hash(p, ax.g + dx, ay.g + dy, az.g + dz);
+        s = cell_start[c];
+        if (s == kEmptyCell) return false;
+        e = min(cell_end[c], s + p.max_par_in_cell);
+        return true;
+    }
+};
+
 // ------------------------------------------------------------------ density
 template <int kPrec>
 __global__ __launch_bounds__(kBlock) void k_density(sph_params p, uint32_t n,
@@ -174,43 +230,34 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params p, uint32_t n,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 xi = pos[i];
-    bool out = false;
-    int gx, gy, gz;
-    grid_pos(p, xi.x, xi.y, xi.z, gx, gy, gz, out);
+    CellWalk w;
+    w.init(p, xi.x, xi.y, xi.z);
     const float h2 = p.h2;
     float sum = 0.0f;
     KahanF ks{0.0f, 0.0f};
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int cz = gz + dz;
-        if (cz < 0 || cz >= (int)p.grid_size[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int cy = gy + dy;
-            if (cy < 0 || cy >= (int)p.grid_size[1]) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int cx = gx + dx;
-                if (cx < 0 || cx >= (int)p.grid_size[0]) continue;
-                const uint32_t c = grid_hash(p, cx, cy, cz);
-                const uint32_t s = cell_start[c];
-                if (s == kEmptyCell) continue;
-                const uint32_t e = min(cell_end[c], s + p.max_par_in_cell);
-                float cell = 0.0f;
-                for (uint32_t j = s; j < e; ++j) {
-                    if (j == i) continue;
-                    const float4 xj = pos[j];
-                    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                    const float r2 = rx * rx + ry * ry + rz * rz;
-                    if (r2 < h2) {
-                        const float c3 = h2 - r2;
-                        const float w = (c3 * c3) * c3;
-                        if (kPrec == SPH_PREC_KAHAN)
-                            ks.add(w);
-                        else
-                            cell = cell + w;
-                    }
-                }
-                if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
+    for (int ord = 0; ord < 27; ++ord) {
+        uint32_t s, e;
+        if (!w.open(p, ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
+            continue;
+        float cell = 0.0f;
+        // next candidate's position loaded one iteration ahead
+        float4 xn = pos[s];
+        for (uint32_t j = s; j < e; ++j) {
+            const float4 xj = xn;
+            if (j + 1 < e) xn = pos[j + 1];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const bool hit = (r2 < h2) && (j != i);
+            const float c3 = h2 - r2;
+            const float wgt = (c3 * c3) * c3;
+            if (kPrec == SPH_PREC_KAHAN) {
+                if (hit) ks.add(wgt);
+            } else {
+                // the partial sum is >= +0, so adding +0 for a miss is exact
+                cell = cell + (hit ? wgt : 0.0f);
             }
         }
+        if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
     }
     if (kPrec == SPH_PREC_KAHAN) sum = ks.s;
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
@@ -218,8 +265,142 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params p, uint32_t n,
 }
 
 // -------------------------------------------------------------------- force
-template <int kPrec, bool kKeepDvel>
+// Candidate tests and pair terms are split so that the expensive pair body
+// runs with the lanes of a wave mostly active (with ~15% of candidates
+// passing, a fused loop would run the body at ~15% lane occupancy):
+//   test    - walk the candidates in reference order and append every slot
+//             j that passes the force predicate to the lane's LDS list
+//             (entry = j | neighbour-cell ordinal << 27);
+//   compute - evaluate the pair terms of the listed neighbours in list
+//             order.  The per-cell partial sum is flushed into the total
+//             when the cell ordinal changes, which reproduces the
+//             reference's per-cell sums (cells without a passing neighbour
+//             contribute +0 and are not visited).
+// The list is drained by the whole wave before a cell whose candidates
+// could overflow some lane's list.  The list size (entries per lane) trades
+// LDS per wave (occupancy) against drains at partial lane occupancy.
+constexpr int kForceListDefault = 16;
+constexpr uint32_t kSlotBits = 27;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+
+struct ForceAcc {
+    float ax, ay, az;  // total
+    float fx, fy, fz;  // current cell's partial sum
+    KahanF kx, ky, kz;
+    uint32_t prev_ord;
+};
+
+template <int kPrec>
+__device__ __forceinline__ void force_drain(const sph_params& p, const uint32_t* list, int t, int cnt,
+                                            const float4& xi, const float4& vi, const float2& ri,
+                                            const float4* __restrict__ pos,
+                                            const float4* __restrict__ vel,
+                                            const float2* __restrict__ rhop, ForceAcc& a) {
+    const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
+    const float vterm = p.lap_kern * p.viscosity;
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t ent = list[k * kBlock + t];
+        const uint32_t jj = ent & kSlotMask;
+        const uint32_t o = ent >> kSlotBits;
+        if (kPrec != SPH_PREC_KAHAN && o != a.prev_ord) {
+            a.ax = a.ax + a.fx;
+            a.ay = a.ay + a.fy;
+            a.az = a.az + a.fz;
+            a.fx = a.fy = a.fz = 0.0f;
+            a.prev_ord = o;
+        }
+        const float4 xj = pos[jj];
+        const float4 vj = vel[jj];
+        const float2 rj = rhop[jj];
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r = fmaxf(min_dist, sqrtf(r2));
+        const float d12 = fminf(min_dens, 1.0f / (ri.x * rj.x));
+        const float cc = h - r;
+        const float pterm = ((cc * spiky) * (ri.y + rj.y)) / r;
+        const float cd = cc * d12;
+        const float gxv = (pterm * rx + vterm * (vj.x - vi.x)) * cd;
+        const float gyv = (pterm * ry + vterm * (vj.y - vi.y)) * cd;
+        const float gzv = (pterm * rz + vterm * (vj.z - vi.z)) * cd;
+        if (kPrec == SPH_PREC_KAHAN) {
+            a.kx.add(gxv);
+            a.ky.add(gyv);
+            a.kz.add(gzv);
+        } else {
+            a.fx = a.fx + gxv;
+            a.fy = a.fy + gyv;
+            a.fz = a.fz + gzv;
+        }
+    }
+}
+
+template <int kPrec, int kList>
 __global__ __launch_bounds__(kBlock) void k_force(
+    sph_params p, uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    __shared__ uint32_t list[kList * kBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = threadIdx.x;
+    const float4 xi = pos[i];
+    const float4 vi = vel[i];
+    const float2 ri = rhop[i];
+    CellWalk w;
+    w.init(p, xi.x, xi.y, xi.z);
+    const float r2max = p.force_r2_max;
+
+    ForceAcc a{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f},
+               0xFFFFFFFFu};
+    int cnt = 0;
+    for (int ord = 0; ord < 27; ++ord) {
+        uint32_t s = 0, e = 0;
+        const bool open =
+            w.open(p, ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e);
+        // every candidate of this cell could pass: make room first
+        if (__any(cnt + (int)(e - s) > kList)) {
+            force_drain<kPrec>(p, list, t, cnt, xi, vi, ri, pos, vel, rhop, a);
+            cnt = 0;
+        }
+        if (!open) continue;
+        const uint32_t tag = (uint32_t)ord << kSlotBits;
+        float4 xn = pos[s];
+        for (uint32_t j = s; j < e; ++j) {
+            const float4 xj = xn;
+            if (j + 1 < e) xn = pos[j + 1];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max.  The store
+            // is unconditional (cnt < list size holds for the whole cell);
+            // only a hit advances cnt.
+            list[cnt * kBlock + t] = j | tag;
+            cnt += (r2 <= r2max && j != i) ? 1 : 0;
+        }
+    }
+    force_drain<kPrec>(p, list, t, cnt, xi, vi, ri, pos, vel, rhop, a);
+    float ax, ay, az;
+    if (kPrec == SPH_PREC_KAHAN) {
+        ax = a.kx.s;
+        ay = a.ky.s;
+        az = a.kz.s;
+    } else {
+        ax = a.ax + a.fx;
+        ay = a.ay + a.fy;
+        az = a.az + a.fz;
+    }
+    const float mdt = p.particle_mass * *dt_dev;
+    ax = ax * mdt;
+    ay = ay * mdt;
+    az = az * mdt;
+    vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
+    if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// Force kernel for max_par_in_cell above the list capacity: the pair body
+// sits directly in the candidate loop (same order, same sums).
+template <int kPrec>
+__global__ __launch_bounds__(kBlock) void k_force_fused(
     sph_params p, uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
@@ -229,62 +410,49 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const float4 xi = pos[i];
     const float4 vi = vel[i];
     const float2 ri = rhop[i];
-    bool out = false;
-    int gx, gy, gz;
-    grid_pos(p, xi.x, xi.y, xi.z, gx, gy, gz, out);
+    CellWalk w;
+    w.init(p, xi.x, xi.y, xi.z);
     const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
     const float vterm = p.lap_kern * p.viscosity;
     const float r2max = p.force_r2_max;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     KahanF kx{0.0f, 0.0f}, ky{0.0f, 0.0f}, kz{0.0f, 0.0f};
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int cz = gz + dz;
-        if (cz < 0 || cz >= (int)p.grid_size[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int cy = gy + dy;
-            if (cy < 0 || cy >= (int)p.grid_size[1]) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int cx = gx + dx;
-                if (cx < 0 || cx >= (int)p.grid_size[0]) continue;
-                const uint32_t c = grid_hash(p, cx, cy, cz);
-                const uint32_t s = cell_start[c];
-                if (s == kEmptyCell) continue;
-                const uint32_t e = min(cell_end[c], s + p.max_par_in_cell);
-                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-                for (uint32_t j = s; j < e; ++j) {
-                    if (j == i) continue;
-                    const float4 xj = pos[j];
-                    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                    const float r2 = rx * rx + ry * ry + rz * rz;
-                    // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max
-                    if (r2 <= r2max) {
-                        const float4 vj = vel[j];
-                        const float2 rj = rhop[j];
-                        const float r = fmaxf(min_dist, sqrtf(r2));
-                        const float d12 = fminf(min_dens, 1.0f / (ri.x * rj.x));
-                        const float cc = h - r;
-                        const float pterm = ((cc * spiky) * (ri.y + rj.y)) / r;
-                        const float cd = cc * d12;
-                        const float gxv = (pterm * rx + vterm * (vj.x - vi.x)) * cd;
-                        const float gyv = (pterm * ry + vterm * (vj.y - vi.y)) * cd;
-                        const float gzv = (pterm * rz + vterm * (vj.z - vi.z)) * cd;
-                        if (kPrec == SPH_PREC_KAHAN) {
-                            kx.add(gxv);
-                            ky.add(gyv);
-                            kz.add(gzv);
-                        } else {
-                            fx = fx + gxv;
-                            fy = fy + gyv;
-                            fz = fz + gzv;
-                        }
-                    }
-                }
-                if (kPrec != SPH_PREC_KAHAN) {
-                    ax = ax + fx;
-                    ay = ay + fy;
-                    az = az + fz;
+    for (int ord = 0; ord < 27; ++ord) {
+        uint32_t s, e;
+        if (!w.open(p, ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
+            continue;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        for (uint32_t j = s; j < e; ++j) {
+            if (j == i) continue;
+            const float4 xj = pos[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 <= r2max) {
+                const float4 vj = vel[j];
+                const float2 rj = rhop[j];
+                const float r = fmaxf(min_dist, sqrtf(r2));
+                const float d12 = fminf(min_dens, 1.0f / (ri.x * rj.x));
+                const float cc = h - r;
+                const float pterm = ((cc * spiky) * (ri.y + rj.y)) / r;
+                const float cd = cc * d12;
+                const float gxv = (pterm * rx + vterm * (vj.x - vi.x)) * cd;
+                const float gyv = (pterm * ry + vterm * (vj.y - vi.y)) * cd;
+                const float gzv = (pterm * rz + vterm * (vj.z - vi.z)) * cd;
+                if (kPrec == SPH_PREC_KAHAN) {
+                    kx.add(gxv);
+                    ky.add(gyv);
+                    kz.add(gzv);
+                } else {
+                    fx = fx + gxv;
+                    fy = fy + gyv;
+                    fz = fz + gzv;
                 }
             }
+        }
+        if (kPrec != SPH_PREC_KAHAN) {
+            ax = ax + fx;
+            ay = ay + fy;
+            az = az + fz;
         }
     }
     if (kPrec == SPH_PREC_KAHAN) {
@@ -297,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
     ay = ay * mdt;
     az = az * mdt;
     vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
-    if (kKeepDvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
+    if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
 
 // -------------------------------------------------------------------- stats
@@ -453,23 +621,43 @@ hipError_t launch_density(const sph_params& p, const StepBuffers& b, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_force(const sph_params& p, const StepBuffers& b, float4* vel_out, bool keep_dvel,
-                        hipStream_t s) {
+template <int kPrec, int kList>
+void launch_force_list(const sph_params& p, const StepBuffers& b, float4* vel_out, float4* dvel,
+                       dim3 g, dim3 t, hipStream_t s) {
+    hipLaunchKernelGGL((k_force<kPrec, kList>), g, t, 0, s, p, p.num_particles, b.pos_cur, b.vel_cur,
+                       b.rhop, b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+}
+
+template <int kPrec>
+void launch_force_prec(const sph_params& p, const StepBuffers& b, float4* vel_out, float4* dvel,
+                       hipStream_t s) {
     const uint32_t n = p.num_particles;
     const dim3 g(blocks_for(n, kBlock)), t(kBlock);
-    const bool kahan = p.precision == SPH_PREC_KAHAN;
-    if (kahan && keep_dvel)
-        hipLaunchKernelGGL((k_force<SPH_PREC_KAHAN, true>), g, t, 0, s, p, n, b.pos_cur, b.vel_cur,
-                           b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
-    else if (kahan)
-        hipLaunchKernelGGL((k_force<SPH_PREC_KAHAN, false>), g, t, 0, s, p, n, b.pos_cur, b.vel_cur,
-                           b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
-    else if (keep_dvel)
-        hipLaunchKernelGGL((k_force<SPH_PREC_REFERENCE, true>), g, t, 0, s, p, n, b.pos_cur,
-                           b.vel_cur, b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
+    uint32_t list = p.tune[0] ? p.tune[0] : (uint32_t)kForceListDefault;
+    if (list != 16 && list != 24 && list != 32 && list != 48) list = kForceListDefault;
+    if (p.max_par_in_cell > list) list = 48;
+    if (p.max_par_in_cell > 48 || p.tune[0] == 1) {
+        // fused test + pair loop: chosen by tune[0] == 1, or forced when one
+        // cell could overflow any neighbour list
+        hipLaunchKernelGGL((k_force_fused<kPrec>), g, t, 0, s, p, n, b.pos_cur, b.vel_cur, b.rhop,
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+        return;
+    }
+    switch (list) {
+        case 16: launch_force_list<kPrec, 16>(p, b, vel_out, dvel, g, t, s); break;
+        case 24: launch_force_list<kPrec, 24>(p, b, vel_out, dvel, g, t, s); break;
+        case 32: launch_force_list<kPrec, 32>(p, b, vel_out, dvel, g, t, s); break;
+        default: launch_force_list<kPrec, 48>(p, b, vel_out, dvel, g, t, s); break;
+    }
+}
+
+hipError_t launch_force(const sph_params& p, const StepBuffers& b, float4* vel_out, bool keep_dvel,
+                        hipStream_t s) {
+    float4* dvel = keep_dvel ? b.dvel : nullptr;
+    if (p.precision == SPH_PREC_KAHAN)
+        launch_force_prec<SPH_PREC_KAHAN>(p, b, vel_out, dvel, s);
     else
-        hipLaunchKernelGGL((k_force<SPH_PREC_REFERENCE, false>), g, t, 0, s, p, n, b.pos_cur,
-                           b.vel_cur, b.rhop, b.cell_start, b.cell_end, vel_out, b.dvel, b.dt_dev);
+        launch_force_prec<SPH_PREC_REFERENCE>(p, b, vel_out, dvel, s);
     return hipGetLastError();
 }
 

@@ -8,13 +8,20 @@
 // asks for (ceil(log2(numCells)) bits) instead of the reference's fixed 16
 // or 32 bits.
 //
-// Per pass, three kernels:
-//   upsweep   - per-tile digit histograms (wave-private LDS counters),
-//   scan      - exclusive scan of the digit-major [digit][tile] table,
-//   downsweep - stable local ranking with 64-lane ballots (peer mask per
-//               digit, rank = popcount of lower peers), local reorder in LDS,
-//               then coalesced runs written to the digit's global range.
-// No spin waits: every kernel finishes independently of the others.
+// Structure (one read of the keys for all digit histograms, then one kernel
+// per pass):
+//   k_hist        per-block digit histograms of every pass in one read,
+//   k_hist_scan   sum over blocks + exclusive scan -> global digit offsets,
+//   k_pass        per tile: stable local ranking with 64-lane ballots (peer
+//                 mask per digit, rank = popcount of lower peers), tile digit
+//                 counts published and prefixed across tiles with a
+//                 decoupled look-back, local reorder in LDS, coalesced runs
+//                 written to each digit's global range.
+// Tiles are numbered by an atomic ticket so a tile only ever waits for tiles
+// that started before it; every wait is bounded by a spin limit that records
+// an error instead of hanging.  Ticket counter and look-back status words
+// carry the launch number, so nothing needs clearing between launches and
+// replays of a captured graph stay correct.
 #include "sph_internal.hpp"
 
 namespace sph {
@@ -23,77 +30,92 @@ namespace {
 constexpr int kSortBlock = 256;
 constexpr int kWave = 64;
 constexpr int kWaves = kSortBlock / kWave;
-constexpr int kItems = 16;
-constexpr int kTile = kSortBlock * kItems;  // keys per tile (block)
+constexpr int kItems = 8;
+constexpr int kTile = kSortBlock * kItems;  // keys per tile
 constexpr int kRadix = 256;
-constexpr int kScanBlock = 1024;
+constexpr int kMaxPasses = 4;
+constexpr int kHistBlocks = 64;
+constexpr uint32_t kSpinLimit = 1u << 22;
 
-static_assert(kSortBlock == kRadix, "one thread per digit in the downsweep bookkeeping");
+constexpr uint64_t kFlagAgg = 1ull << 30;
+constexpr uint64_t kFlagInc = 2ull << 30;
+constexpr uint64_t kValMask = (1ull << 30) - 1;
 
-__global__ __launch_bounds__(kSortBlock) void k_upsweep(const uint32_t* __restrict__ keys,
-                                                       uint32_t n, uint32_t shift, uint32_t mask,
-                                                       uint32_t* __restrict__ hist_out,
-                                                       uint32_t num_tiles) {
-    __shared__ uint32_t hist[kWaves][kRadix];
+static_assert(kSortBlock == kRadix, "one thread per digit in the tile bookkeeping");
+
+__device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                    uint32_t bits, uint32_t* __restrict__ block_hist) {
+    __shared__ uint32_t hist[kMaxPasses][kRadix];
     const int t = threadIdx.x;
-    const int wave = t / kWave;
-    for (int w = 0; w < kWaves; ++w) hist[w][t] = 0;
+    for (int p = 0; p < kMaxPasses; ++p) hist[p][t] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kTile;
-#pragma unroll 4
-    for (int it = 0; it < kItems; ++it) {
-        const uint64_t e = base + (uint64_t)it * kSortBlock + t;
-        if (e < n) atomicAdd(&hist[wave][(keys[e] >> shift) & mask], 1u);
+    const uint32_t passes = (bits + 7) / 8;
+    for (uint32_t i = blockIdx.x * kSortBlock + t; i < n; i += gridDim.x * kSortBlock) {
+        const uint32_t k = keys[i];
+        for (uint32_t p = 0; p < passes; ++p) atomicAdd(&hist[p][(k >> (8 * p)) & 0xFFu], 1u);
     }
     __syncthreads();
-    uint32_t s = 0;
-    for (int w = 0; w < kWaves; ++w) s += hist[w][t];
-    hist_out[(size_t)t * num_tiles + blockIdx.x] = s;
+    for (uint32_t p = 0; p < passes; ++p)
+        block_hist[((size_t)blockIdx.x * kMaxPasses + p) * kRadix + t] = hist[p][t];
 }
 
-// Exclusive scan of `total` counters in place, one block.
-__global__ __launch_bounds__(kScanBlock) void k_scan(uint32_t* __restrict__ data, uint32_t total) {
-    __shared__ uint32_t sums[kScanBlock];
-    const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (total + kScanBlock - 1) / kScanBlock;
-    const uint32_t b = t * chunk;
-    const uint32_t e = min(total, b + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += data[i];
-    sums[t] = s;
+// One block per pass: digit totals over the histogram blocks, exclusive scan.
+__global__ __launch_bounds__(kRadix) void k_hist_scan(const uint32_t* __restrict__ block_hist,
+                                                     uint32_t nblocks, uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t s[kRadix];
+    const int t = threadIdx.x;
+    const int p = blockIdx.x;
+    uint32_t part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // independent loads in flight: eight running sums
+    for (uint32_t b = 0; b < nblocks; b += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (b + u < nblocks) part[u] += block_hist[((size_t)(b + u) * kMaxPasses + p) * kRadix + t];
+    }
+    uint32_t tot = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) tot += part[u];
+    s[t] = tot;
     __syncthreads();
-    // Hillis-Steele inclusive scan of the per-thread sums.
-    for (uint32_t off = 1; off < kScanBlock; off <<= 1) {
-        uint32_t v = t >= off ? sums[t - off] : 0u;
+    for (int off = 1; off < kRadix; off <<= 1) {
+        const uint32_t add = t >= off ? s[t - off] : 0u;
         __syncthreads();
-        sums[t] += v;
+        s[t] += add;
         __syncthreads();
     }
-    uint32_t run = t == 0 ? 0u : sums[t - 1];
-    for (uint32_t i = b; i < e; ++i) {
-        uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
+    offsets[p * kRadix + t] = s[t] - tot;
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_downsweep(
+__global__ __launch_bounds__(kSortBlock) void k_pass(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
-    uint32_t mask, uint32_t pass_bits, const uint32_t* __restrict__ tile_off, uint32_t num_tiles) {
+    uint32_t pass_bits, const uint32_t* __restrict__ digit_off, uint64_t* __restrict__ status,
+    uint64_t* __restrict__ ticket, uint32_t num_tiles, uint32_t* __restrict__ err) {
     __shared__ uint32_t wcount[kWaves][kRadix];
     __shared__ uint32_t dig_off[kRadix];
     __shared__ uint32_t gbase[kRadix];
     __shared__ uint32_t skeys[kTile];
     __shared__ uint32_t svals[kTile];
+    __shared__ uint64_t s_ticket;
 
     const int t = threadIdx.x;
     const int wave = t / kWave;
     const int lane = t % kWave;
+    if (t == 0) s_ticket = atomicAdd((unsigned long long*)ticket, 1ull);
     for (int w = 0; w < kWaves; ++w) wcount[w][t] = 0;
     __syncthreads();
+    const uint32_t tile = (uint32_t)(s_ticket % num_tiles);
+    const uint64_t epoch = (s_ticket / num_tiles) & 0xFFFFFFFFull;
+    const uint32_t mask = (1u << pass_bits) - 1u;
 
-    const uint64_t tile_base = (uint64_t)blockIdx.x * kTile;
+    const uint64_t tile_base = (uint64_t)tile * kTile;
     const uint64_t wbase = tile_base + (uint64_t)wave * (kWave * kItems);
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     uint32_t k[kItems], v[kItems], rank[kItems];
@@ -124,16 +146,42 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(
     __syncthreads();
 
     // Per digit (thread t == digit): exclusive prefix over waves, tile total.
-    uint32_t run = 0;
+    uint32_t cnt = 0;
     for (int w = 0; w < kWaves; ++w) {
         const uint32_t c = wcount[w][t];
-        wcount[w][t] = run;
-        run += c;
+        wcount[w][t] = cnt;
+        cnt += c;
     }
-    dig_off[t] = run;
-    gbase[t] = tile_off[(size_t)t * num_tiles + blockIdx.x];
+    // Publish the tile's count, then look back over earlier tiles.
+    uint64_t* my = status + (size_t)tile * kRadix + t;
+    const uint64_t tag = epoch << 32;
+    if (tile == 0) {
+        st_status(my, tag | kFlagInc | cnt);
+        gbase[t] = digit_off[t];
+    } else {
+        st_status(my, tag | kFlagAgg | cnt);
+        uint32_t prefix = 0;
+        int64_t kt = (int64_t)tile - 1;
+        uint32_t spins = 0;
+        while (kt >= 0) {
+            const uint64_t s = ld_status(status + (size_t)kt * kRadix + t);
+            if ((s >> 32) != epoch || (s & (kFlagAgg | kFlagInc)) == 0) {
+                if (++spins > kSpinLimit) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            prefix += (uint32_t)(s & kValMask);
+            if (s & kFlagInc) break;
+            --kt;
+        }
+        st_status(my, tag | kFlagInc | (uint64_t)(prefix + cnt));
+        gbase[t] = digit_off[t] + prefix;
+    }
+    dig_off[t] = cnt;
     __syncthreads();
-    // Exclusive scan of the digit totals (Hillis-Steele, inclusive then shift).
     for (int off = 1; off < kRadix; off <<= 1) {
         const uint32_t add = t >= off ? dig_off[t - off] : 0u;
         __syncthreads();
@@ -142,7 +190,7 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(
     }
     const uint32_t incl = dig_off[t];
     __syncthreads();
-    dig_off[t] = incl - run;
+    dig_off[t] = incl - cnt;
     __syncthreads();
 
 #pragma unroll
@@ -168,9 +216,31 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(
 
 }  // namespace
 
-size_t radix_scratch_bytes(uint32_t n) {
+uint32_t radix_num_tiles(uint32_t n) {
     const uint32_t tiles = (n + kTile - 1) / kTile;
-    return (size_t)kRadix * (tiles == 0 ? 1 : tiles) * sizeof(uint32_t);
+    return tiles == 0 ? 1 : tiles;
+}
+
+size_t radix_scratch_bytes(uint32_t n) {
+    // block histograms | digit offsets | status words | ticket | error word
+    return (size_t)kHistBlocks * kMaxPasses * kRadix * 4 + (size_t)kMaxPasses * kRadix * 4 +
+           (size_t)radix_num_tiles(n) * kRadix * 8 + 64;
+}
+
+hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipStream_t stream) {
+    plan.n = n;
+    plan.num_tiles = radix_num_tiles(n);
+    char* b = static_cast<char*>(scratch);
+    plan.block_hist = reinterpret_cast<uint32_t*>(b);
+    b += (size_t)kHistBlocks * kMaxPasses * kRadix * 4;
+    plan.digit_off = reinterpret_cast<uint32_t*>(b);
+    b += (size_t)kMaxPasses * kRadix * 4;
+    plan.status = reinterpret_cast<uint64_t*>(b);
+    b += (size_t)plan.num_tiles * kRadix * 8;
+    plan.ticket = reinterpret_cast<uint64_t*>(b);
+    plan.err = reinterpret_cast<uint32_t*>(b + 8);
+    // status words start at epoch 0 / no flag; the ticket and error word at 0
+    return hipMemsetAsync(plan.status, 0, (size_t)plan.num_tiles * kRadix * 8 + 64, stream);
 }
 
 hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* keys0,
@@ -181,18 +251,20 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
     uint32_t* kout = keys1;
     uint32_t* vout = vals1;
     const uint32_t n = plan.n;
-    const uint32_t tiles = plan.num_tiles;
-    for (uint32_t shift = 0; shift < bits; shift += 8) {
+    const uint32_t passes = (bits + 7) / 8;
+    if (passes > (uint32_t)kMaxPasses) return hipErrorInvalidValue;
+    if (passes > 0) {
+        hipLaunchKernelGGL(k_hist, dim3(kHistBlocks), dim3(kSortBlock), 0, stream, kin, n, bits,
+                           plan.block_hist);
+        hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(kRadix), 0, stream, plan.block_hist,
+                           (uint32_t)kHistBlocks, plan.digit_off);
+    }
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t shift = 8 * p;
         const uint32_t pb = (bits - shift) < 8 ? (bits - shift) : 8;
-        const uint32_t mask = (1u << pb) - 1u;
-        hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(kSortBlock), 0, stream, kin, n, shift, mask,
-                           plan.block_hist, tiles);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, stream, plan.block_hist,
-                           (uint32_t)(kRadix * tiles));
-        hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kSortBlock), 0, stream, kin, vin, kout,
-                           vout, n, shift, mask, pb, plan.block_hist, tiles);
-        hipError_t err = hipGetLastError();
-        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(k_pass, dim3(plan.num_tiles), dim3(kSortBlock), 0, stream, kin, vin, kout,
+                           vout, n, shift, pb, plan.digit_off + p * kRadix, plan.status, plan.ticket,
+                           plan.num_tiles, plan.err);
         uint32_t* tk = kin;
         kin = kout;
         kout = tk;
@@ -202,7 +274,7 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
     }
     *keys_out = kin;
     *vals_out = vin;
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 }  // namespace sph

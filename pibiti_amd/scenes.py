@@ -9,7 +9,13 @@ Choices that the workloads leave open, fixed here once:
   * particle radius r = spacing / 2; soft wall band 8 r, hard clamp 1 r,
     wall stiffness 30000, wall damping 256 / 60, min_dist 1 r, min_dens 1
     (the reference defaults of source/SPH/Scene.cpp:16-31, in units of r);
-  * particle mass m = rho0 * spacing^3 (the lattice's own mass per site);
+  * the fluid block starts (soft - r) inside the world's min corner, the
+    shift the reference applies to initMin (source/SPH/Scene.cpp:101-102),
+    so no particle starts inside a soft wall band;
+  * particle mass m = rho0 / (Poly6 * S) with S the self-excluded kernel sum
+    over the unjittered lattice, so an interior particle starts exactly at
+    the rest density (m = rho0 * spacing^3 would start the fluid at ~0.8
+    rho0 because the density sum excludes the self term);
   * cell size = h, so the 27-cell search finds every pair with r < h.
 """
 from __future__ import annotations
@@ -19,6 +25,26 @@ import math
 import numpy as np
 
 from . import sph
+
+
+def lattice_rest_mass(spacing: float, h: float, rest_density: float, poly6: float) -> float:
+    """Mass that puts an interior site of the unjittered cubic lattice at rest
+    density: rho0 / (Poly6 * sum_{0 < |d| < h} (h^2 - |d|^2)^3)."""
+    s = float(np.float32(spacing))
+    hh = float(np.float32(h)) ** 2
+    k = int(math.ceil(float(np.float32(h)) / s)) + 1
+    r = np.arange(-k, k + 1) * s
+    d2 = r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2
+    sel = (d2 > 0) & (d2 < hh)
+    total = float(np.sum((hh - d2[sel]) ** 3))
+    return rest_density / (poly6 * total)
+
+
+def fluid_origin(spacing: float, world_min=(0.0, 0.0, 0.0)) -> tuple:
+    """Min corner of the fluid block: world_min + (soft - r) with soft = 8 r,
+    r = spacing / 2."""
+    b = 8.0 * (spacing / 2) - spacing / 2
+    return tuple(float(w) + b for w in world_min)
 
 
 def _scene(n: int, h: float, spacing: float, world_min, world_max, dt: float, *,
@@ -43,7 +69,7 @@ def _scene(n: int, h: float, spacing: float, world_min, world_max, dt: float, *,
     s.viscosity = viscosity
     s.min_dens = 1.0
     s.min_dist = 1.0
-    s.particle_mass = rest_density * spacing ** 3
+    s.particle_mass = 0.0
     s.dist_bnd_soft = 8.0
     s.dist_bnd_hard = 1.0
     s.bnd_stiff = 30000.0
@@ -52,6 +78,8 @@ def _scene(n: int, h: float, spacing: float, world_min, world_max, dt: float, *,
     s.eos = eos
     s.precision = precision
     s.cfl = cfl
+    poly6 = float(sph.params_from_scene(s).poly6_kern)
+    s.particle_mass = lattice_rest_mass(spacing, h, rest_density, poly6)
     return s
 
 
@@ -61,9 +89,10 @@ def dam_break(nx: int, ny: int, nz: int, *, spacing: float, h: float, world_max,
     n = nx * ny * nz
     s = _scene(n, h, spacing, (0.0, 0.0, 0.0), world_max, dt, viscosity=viscosity, **kw)
     p = sph.params_from_scene(s)
-    pos, vel = sph.scene_lattice(nx, ny, nz, (0.0, 0.0, 0.0), spacing, 0.1, vel_sigma,
+    origin = fluid_origin(spacing)
+    pos, vel = sph.scene_lattice(nx, ny, nz, origin, spacing, 0.1, vel_sigma,
                                  (0.0, 0.0, 0.0), seed, 0)
-    info = dict(kind="dam_break", n=n, lattice=(nx, ny, nz), spacing=spacing, h=h,
+    info = dict(kind="dam_break", n=n, lattice=(nx, ny, nz), spacing=spacing, h=h, origin=origin,
                 world_max=tuple(world_max), dt=dt, viscosity=viscosity, vel_sigma=vel_sigma,
                 seed=seed, particle_r=spacing / 2, mass=float(p.particle_mass),
                 grid=tuple(p.grid_size), num_cells=int(p.num_cells), key_bits=int(p.key_bits))
@@ -71,9 +100,10 @@ def dam_break(nx: int, ny: int, nz: int, *, spacing: float, h: float, world_max,
 
 
 def cpu_dam_break_15k(seed: int = 1, **kw):
-    """15,625 particles on a 25^3 lattice (spacing 0.02) filling [0,0.5]^3 of a
+    """15,625 particles on a 25^3 lattice (spacing 0.02, a 0.5^3 block) in a
     unit box, jitter +-0.1 spacing, zero velocity, h = 0.04, rho0 = 1000, k = 3,
-    viscosity 0.1, dt = 1e-4 (100 steps)."""
+    viscosity 0.1, dt = 1e-4 (100 steps).  The block sits at
+    fluid_origin() = (0.07, 0.07, 0.07), just inside the soft wall band."""
     return dam_break(25, 25, 25, spacing=0.02, h=0.04, world_max=(1.0, 1.0, 1.0), dt=1e-4,
                      viscosity=0.1, seed=seed, **kw)
 
@@ -87,10 +117,12 @@ def gpu_dam_break_1m(seed: int = 2, **kw):
 
 def scaled_dam_break(lattice, *, seed: int = 3, **kw):
     """A smaller dam break with the 1M scene's resolution (spacing 0.01,
-    h = 0.02, dt = 5e-5) and a box twice the block on every axis."""
+    h = 0.02, dt = 5e-5) and a box twice the block (plus the wall band) on
+    every axis."""
     nx, ny, nz = lattice
     s = 0.01
-    wm = (2 * nx * s, 2 * ny * s, 2 * nz * s)
+    b = fluid_origin(s)[0]
+    wm = (2 * nx * s + 2 * b, 2 * ny * s + 2 * b, 2 * nz * s + 2 * b)
     return dam_break(nx, ny, nz, spacing=s, h=0.02, world_max=wm, dt=5e-5, viscosity=0.1,
                      vel_sigma=0.05, seed=seed, **kw)
 
@@ -143,6 +175,7 @@ def batch_scene(count: int, index: int, *, seed: int = 5, **kw):
     (the 15k scene's fill fraction of 1/8)."""
     nx, ny, nz = batch_lattice(count)
     s = 0.02
-    wm = (2 * nx * s, 2 * ny * s, 2 * nz * s)
+    b = fluid_origin(s)[0]
+    wm = (2 * nx * s + 2 * b, 2 * ny * s + 2 * b, 2 * nz * s + 2 * b)
     return dam_break(nx, ny, nz, spacing=s, h=0.04, world_max=wm, dt=1e-4, viscosity=0.1,
                      seed=seed * 1000 + index, **kw)

@@ -98,7 +98,8 @@ class Params(C.Structure):
         ("sound_speed", C.c_float),
         ("key_bits", C.c_uint32),
         ("force_r2_max", C.c_float),
-        ("reserved", C.c_uint32 * 8),
+        ("tune", C.c_uint32 * 4),
+        ("reserved", C.c_uint32 * 4),
     ]
 
     def copy(self) -> "Params":
@@ -119,7 +120,8 @@ class Stats(C.Structure):
         ("cap_cells", C.c_uint32),
         ("out_of_grid", C.c_uint32),
         ("max_cell_count", C.c_uint32),
-        ("reserved", C.c_uint32 * 7),
+        ("sort_errors", C.c_uint32),
+        ("reserved", C.c_uint32 * 6),
     ]
 
     def as_dict(self) -> dict:
