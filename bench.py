@@ -51,29 +51,39 @@ def algorithmic_bytes(p) -> dict:
     }
 
 
-def cpu_baseline(budget_s: float) -> dict:
+def cpu_baseline(budget_s: float, p, pos, vel) -> dict:
+    """The CPU oracle (FP32 reference order, OpenMP over all host threads) on
+    the first steps of the same 1M-particle workload, bounded by budget_s.
+    The single-thread rate on the 15,625-particle dam break rides along."""
     import oracle
     oracle.build()
-    p, pos, vel, _ = scenes.cpu_dam_break_15k()
     n = int(p.num_particles)
     threads = oracle.num_threads()
-    # one untimed step, then as many steps as fit the budget (at most 100)
-    oracle.step_f32(p, pos, vel)
     t0 = time.perf_counter()
     steps = 0
     x, v = pos, vel
     while steps < 100 and (time.perf_counter() - t0) < budget_s:
-        o = oracle.run_f32(p, x, v, 5)
+        o = oracle.run_f32(p, x, v, 1)
         x, v = o["pos"], o["vel"]
-        steps += 5
+        steps += 1
     dt = time.perf_counter() - t0
+
+    q, qx, qv, _ = scenes.cpu_dam_break_15k()
+    oracle.set_threads(1)
+    t1 = time.perf_counter()
+    oracle.run_f32(q, qx, qv, 20)
+    dt1 = time.perf_counter() - t1
+    oracle.set_threads(threads)
     return {
         "value": n * steps / dt,
         "unit": "particle-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle, FP32 reference order, 15,625-particle dam break, {steps} steps "
-                  f"({dt:.1f} s), {threads} OpenMP threads, host {os.uname().nodename}",
+        "sample": f"CPU oracle (FP32 reference order) on the first {steps} steps of the same "
+                  f"1,048,576-particle dam break ({dt:.1f} s), {threads} OpenMP threads, "
+                  f"host {os.uname().nodename}, {os.cpu_count()} logical CPUs visible",
+        "single_thread_15k": {"value": int(q.num_particles) * 20 / dt1,
+                              "sample": "15,625-particle dam break, 20 steps, 1 thread"},
     }
 
 
@@ -168,7 +178,7 @@ def main() -> int:
                       "cap_cells": st["cap_cells"], "max_cell_count": st["max_cell_count"]},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget, p, pos, vel)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -63,7 +63,7 @@ def lib() -> C.CDLL:
                                          _up, _up, _up, _up, _up]
         h.oracle_step_f64.restype = C.c_int
         h.oracle_step_f64.argtypes = [C.c_void_p, C.c_uint32, _fp, _fp, C.c_int, _dp, _dp, _dp, _dp,
-                                      _dp, _dp]
+                                      _dp, _dp, _dp, _dp]
         h.oracle_integrate_f32.argtypes = [C.c_void_p, C.c_uint32, _fp, _fp]
         h.oracle_hash_f32.restype = C.c_uint32
         h.oracle_hash_f32.argtypes = [C.c_void_p, C.c_uint32, _fp, _up]
@@ -137,10 +137,11 @@ def step_f64(params, pos4, vel4, skip_integrate: bool = False) -> dict:
     pin = np.ascontiguousarray(pos4, np.float32).reshape(n, 4)
     vin = np.ascontiguousarray(vel4, np.float32).reshape(n, 4)
     out = dict(pos=np.empty((n, 4)), vel=np.empty((n, 4)), dens=np.empty(n), pres=np.empty(n),
-               dvel=np.empty((n, 4)), dvel_scale=np.empty(n))
+               dvel=np.empty((n, 4)), dvel_scale=np.empty(n), dens_scale=np.empty(n),
+               pres_scale=np.empty(n))
     st = lib().oracle_step_f64(_pp(params), n, _f(pin), _f(vin), int(skip_integrate), _d(out["pos"]),
                                _d(out["vel"]), _d(out["dens"]), _d(out["pres"]), _d(out["dvel"]),
-                               _d(out["dvel_scale"]))
+                               _d(out["dvel_scale"]), _d(out["dens_scale"]), _d(out["pres_scale"]))
     assert st == 0
     return out
 

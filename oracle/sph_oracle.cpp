@@ -142,9 +142,14 @@ struct Grid {
 // ---------------------------------------------------------------- density
 // computeDensityD/compDensCell (source/CUDA/Kernel_Cell.cui:142-199) for the
 // particle in sorted slot i.  Returns the raw kernel sum (before Poly6*m).
+// scale (if given) receives sum_j [c^3 + 3 c^2 (h2 + r2)]: each term plus
+// its first-order sensitivity to a relative perturbation of the operands of
+// c = h2 - r2, the error scale of the sum in floating point.
 template <typename T, bool kKahan>
 T density_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
-              const std::vector<uint32_t>& cs, const std::vector<uint32_t>& ce, T h2) {
+              const std::vector<uint32_t>& cs, const std::vector<uint32_t>& ce, T h2,
+              T* scale = nullptr) {
+    T sc = 0;
     const T xi[3] = {spos[4 * i], spos[4 * i + 1], spos[4 * i + 2]};
     int g[3];
     grid_pos<T>(p, xi, g, nullptr);
@@ -174,11 +179,34 @@ T density_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
                             ktotal.add(w);
                         else
                             cell.add(w);
+                        if (scale) sc += w + (T)3 * (c3 * c3) * (h2 + r2);
                     }
                 }
                 if (!kKahan) total.add(cell.s);
             }
+    if (scale) *scale = sc;
     return kKahan ? ktotal.s : total.s;
+}
+
+// |p| evaluated with absolute values (the EOS terms before they cancel) and
+// |dp/drho|: together the error scale of a pressure computed from a density.
+template <typename T>
+inline T eos_abs(const sph_params& p, T rho) {
+    if (p.eos == SPH_EOS_TAIT) {
+        T x = std::fabs(rho) / (T)p.rest_density;
+        T x2 = x * x, x4 = x2 * x2, x7 = (x4 * x2) * x;
+        return ((T)p.stiffness * (T)p.rest_density / (T)7) * (x7 + (T)1);
+    }
+    return (std::fabs(rho) + (T)p.rest_density) * (T)p.stiffness;
+}
+template <typename T>
+inline T eos_slope(const sph_params& p, T rho) {
+    if (p.eos == SPH_EOS_TAIT) {
+        T x = std::fabs(rho) / (T)p.rest_density;
+        T x2 = x * x;
+        return (T)p.stiffness * (x2 * x2 * x2);
+    }
+    return (T)p.stiffness;
 }
 
 template <typename T>
@@ -203,22 +231,15 @@ inline T eos(const sph_params& p, T rho) {
 template <typename T, bool kKahan>
 void force_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
                const std::vector<T>& svel, const std::vector<T>& dens, const std::vector<T>& pres,
-               const std::vector<uint32_t>& cs, const std::vector<uint32_t>& ce, T out[3],
-               T* scale) {
+               const std::vector<T>& pscale, const std::vector<uint32_t>& cs,
+               const std::vector<uint32_t>& ce, T out[3], T* scale) {
     const T xi[3] = {spos[4 * i], spos[4 * i + 1], spos[4 * i + 2]};
     const T vi[3] = {svel[4 * i], svel[4 * i + 1], svel[4 * i + 2]};
     const T di = dens[i], pi = pres[i];
     const bool dbl = sizeof(T) == 8;
-    // |p| evaluated with absolute values: the EOS terms before they cancel.
-    auto pabs = [&](T rho) -> T {
-        if (p.eos == SPH_EOS_TAIT) {
-            T x = rho / (T)p.rest_density;
-            T x2 = x * x, x4 = x2 * x2, x7 = (x4 * x2) * x;
-            return ((T)p.stiffness * (T)p.rest_density / (T)7) * (x7 + (T)1);
-        }
-        return (std::fabs(rho) + (T)p.rest_density) * (T)p.stiffness;
-    };
-    const T pai = dbl ? pabs(di) : (T)0;
+    // error scale of p_i + p_j: the pressures' own error scales
+    auto pabs = [&](uint32_t k) -> T { return pscale[k]; };
+    const T pai = dbl ? pabs(i) : (T)0;
     int g[3];
     grid_pos<T>(p, xi, g, nullptr);
     const T h = (T)p.h, spiky = (T)p.spiky_kern, lap = (T)p.lap_kern, visc = (T)p.viscosity,
@@ -260,7 +281,7 @@ void force_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
                                 cell[k].add(f);
                         }
                         if (dbl) {
-                            T pa = ((cc * spiky) * (pai + pabs(dens[j]))) / r;
+                            T pa = ((cc * spiky) * (pai + pabs(j))) / r;
                             T vn = std::sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
                             sc += (pa * len + std::fabs(vterm) * vn) * std::fabs(cd);
                         }
@@ -285,6 +306,8 @@ struct StepOut {
     T* pres;
     T* dvel4;
     T* dvel_scale;  // double only
+    T* dens_scale;  // double only
+    T* pres_scale;  // double only
     uint32_t* sorted_hash;
     uint32_t* perm;
     uint32_t* cell_start;
@@ -348,14 +371,21 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
     // 5. density + EOS in sorted order (Kernel_Cell.cui:178-199).
     const bool dbl = sizeof(T) == 8;
     const T h2 = dbl ? (T)p.h * (T)p.h : (T)p.h2;
-    std::vector<T> dens(n), pres(n);
+    std::vector<T> dens(n), pres(n), rsc(n, (T)0), psc(n, (T)0);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < (int64_t)n; ++i) {
-        T s = prec == SPH_PREC_KAHAN ? density_sum<T, true>(p, (uint32_t)i, spos, cs, ce, h2)
-                                     : density_sum<T, false>(p, (uint32_t)i, spos, cs, ce, h2);
+        T sc = 0;
+        T* scp = dbl ? &sc : nullptr;
+        T s = prec == SPH_PREC_KAHAN ? density_sum<T, true>(p, (uint32_t)i, spos, cs, ce, h2, scp)
+                                     : density_sum<T, false>(p, (uint32_t)i, spos, cs, ce, h2, scp);
         T rho = (s * (T)p.poly6_kern) * (T)p.particle_mass;
         dens[i] = rho;
         pres[i] = eos<T>(p, rho);
+        if (dbl) {
+            // error scales: density sum, and the EOS terms plus dp/drho times that
+            rsc[i] = sc * (T)p.poly6_kern * (T)p.particle_mass;
+            psc[i] = eos_abs<T>(p, rho) + eos_slope<T>(p, rho) * rsc[i];
+        }
     }
     // 6. force + velocity update, scattered to original order (System.cu:228-250,402).
     std::vector<T> dv(4 * (size_t)n, (T)0), dsc(n, (T)0);
@@ -364,9 +394,9 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
         T f[3];
         T* scp = dbl ? &dsc[i] : nullptr;
         if (prec == SPH_PREC_KAHAN)
-            force_sum<T, true>(p, (uint32_t)i, spos, svel, dens, pres, cs, ce, f, scp);
+            force_sum<T, true>(p, (uint32_t)i, spos, svel, dens, pres, psc, cs, ce, f, scp);
         else
-            force_sum<T, false>(p, (uint32_t)i, spos, svel, dens, pres, cs, ce, f, scp);
+            force_sum<T, false>(p, (uint32_t)i, spos, svel, dens, pres, psc, cs, ce, f, scp);
         for (int c = 0; c < 3; ++c) dv[4 * i + c] = f[c];
     }
     for (uint32_t k = 0; k < n; ++k) {
@@ -378,6 +408,8 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
         if (o.dvel4)
             for (int c = 0; c < 4; ++c) o.dvel4[4 * (size_t)si + c] = c < 3 ? dv[4 * (size_t)k + c] : (T)0;
         if (o.dvel_scale) o.dvel_scale[si] = dsc[k];
+        if (o.dens_scale) o.dens_scale[si] = rsc[k];
+        if (o.pres_scale) o.pres_scale[si] = psc[k];
         if (o.sorted_hash) o.sorted_hash[k] = hash[si];
         if (o.perm) o.perm[k] = si;
     }
@@ -416,8 +448,8 @@ int oracle_step_f32(const sph_params* p, int prec, uint32_t n, float* pos4, floa
                     float* dens, float* pres, float* dvel4, uint32_t* sorted_hash, uint32_t* perm,
                     uint32_t* cell_start, uint32_t* cell_end, uint32_t* counters) {
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
-    StepOut<float> o{pos4, vel4, nullptr, dens, pres, dvel4, nullptr, sorted_hash, perm, cell_start,
-                     cell_end, reinterpret_cast<Counters*>(counters)};
+    StepOut<float> o{pos4, vel4, nullptr, dens, pres, dvel4, nullptr, nullptr, nullptr, sorted_hash,
+                     perm, cell_start, cell_end, reinterpret_cast<Counters*>(counters)};
     step_impl<float>(*p, prec, n, pos4, vel4, false, o);
     return 0;
 }
@@ -429,8 +461,8 @@ int oracle_step_f32_ex(const sph_params* p, int prec, uint32_t n, float* pos4, f
                        uint32_t* sorted_hash, uint32_t* perm, uint32_t* cell_start,
                        uint32_t* cell_end, uint32_t* counters) {
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
-    StepOut<float> o{pos4, vel4, vel_int4, dens, pres, dvel4, nullptr, sorted_hash, perm,
-                     cell_start, cell_end, reinterpret_cast<Counters*>(counters)};
+    StepOut<float> o{pos4, vel4, vel_int4, dens, pres, dvel4, nullptr, nullptr, nullptr,
+                     sorted_hash, perm, cell_start, cell_end, reinterpret_cast<Counters*>(counters)};
     step_impl<float>(*p, prec, n, pos4, vel4, false, o);
     return 0;
 }
@@ -441,7 +473,7 @@ int oracle_run_f32(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, f
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
     for (uint32_t s = 0; s < nsteps; ++s) {
         StepOut<float> o{pos4, vel4, nullptr, dens, pres, nullptr, nullptr, nullptr, nullptr,
-                         nullptr, nullptr, nullptr};
+                         nullptr, nullptr, nullptr, nullptr, nullptr};
         step_impl<float>(*p, prec, n, pos4, vel4, false, o);
     }
     return 0;
@@ -450,17 +482,19 @@ int oracle_run_f32(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, f
 // One FP64 step from FP32 inputs.  Outputs are double, original order.
 // dvel_scale[i] = m*dt * sum_j |c d12| (|pterm|_abs r + |vterm| |dv_ij|): the
 // absolute-value evaluation of the force sum, the scale FP32 rounding error
-// of that sum is proportional to.  skip_integrate != 0 treats the inputs as
+// of that sum is proportional to; dens_scale[i] = m Poly6 sum_j [c^3 +
+// 3 c^2 (h2 + r2)] and pres_scale[i] = |p|_abs + |dp/drho| dens_scale[i] are
+// the matching scales of the density and pressure.  skip_integrate != 0 treats the inputs as
 // the state after integrate (hash, sort, density and force only), so the
 // density and force sums can be compared on the exact inputs the FP32 path
 // used.
 int oracle_step_f64(const sph_params* p, uint32_t n, const float* pos4_in, const float* vel4_in,
                     int skip_integrate, double* pos4, double* vel4, double* dens, double* pres,
-                    double* dvel4, double* dvel_scale) {
+                    double* dvel4, double* dvel_scale, double* dens_scale, double* pres_scale) {
     if (!p || !pos4_in || !vel4_in || n != p->num_particles) return 1;
     std::vector<double> pin(pos4_in, pos4_in + 4 * (size_t)n), vin(vel4_in, vel4_in + 4 * (size_t)n);
-    StepOut<double> o{pos4, vel4, nullptr, dens, pres, dvel4, dvel_scale, nullptr, nullptr,
-                      nullptr, nullptr, nullptr};
+    StepOut<double> o{pos4, vel4, nullptr, dens, pres, dvel4, dvel_scale, dens_scale, pres_scale,
+                      nullptr, nullptr, nullptr, nullptr, nullptr};
     step_impl<double>(*p, SPH_PREC_REFERENCE, n, pin.data(), vin.data(), skip_integrate != 0, o);
     return 0;
 }
