@@ -142,10 +142,18 @@ typedef struct sph_params {
     float force_r2_max;
     /* Launch tuning (0 = library default); any legal value gives the same
      * results.  tune[0]: force kernel variant, 1 = fused candidate and pair
-     * loop, 16/24/32/48 = neighbour-list entries per lane; tune[1..3]:
+     * loop, 16/24/32/48 = neighbour-list entries per lane; tune[1]: gather
+     * grid, 0 = one thread per particle, else a persistent grid of tune[1]
+     * blocks of 256 threads pulling 256-particle tiles; tune[2..3]:
      * reserved.                                                            */
     uint32_t tune[4];
-    uint32_t reserved[4];
+    /* First global cell key of this scene: 0 for a single-scene context; a
+     * batch gives each scene a disjoint key range (set by sph_batch_create). */
+    uint32_t cell_base;
+    /* Simulated end time (s): the time step is clipped so the scene's clock
+     * stops at t_end, after which the scene is frozen.  0 = no end time.    */
+    float t_end;
+    uint32_t reserved[2];
 } sph_params;
 
 /* Global FP64 diagnostics reduced on the device (extension). */
@@ -242,6 +250,48 @@ int sph_synchronize(sph_ctx* ctx);
  * recorded while enabled.                                                  */
 int sph_set_timing(sph_ctx* ctx, int enable);
 int sph_get_timing(sph_ctx* ctx, float* ms6);
+
+/* Record nsteps steps (even: the ping-pong buffers end where they began)
+ * into a hipGraph, then replay it: each replay advances nsteps steps with one
+ * graph launch.  No counterpart in the reference (extension).  Any later
+ * parameter change discards the recording.                                 */
+int sph_capture(sph_ctx* ctx, uint32_t nsteps);
+int sph_replay(sph_ctx* ctx, uint32_t times);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many independent scenes of uneven particle counts advanced by
+ * the same launches (extension; the reference runs one scene per process).
+ * Scenes are packed one after another; each keeps its own parameters, grid,
+ * time step (fixed, CFL or clipped at t_end) and clock, and its results are
+ * those of the same scene run alone.  All scenes share one precision mode.
+ * Array and stats calls address one scene with scene-local indices.        */
+typedef struct sph_batch sph_batch;
+
+int sph_batch_create(const sph_params* scenes, uint32_t nscenes, int device, uint32_t flags,
+                     sph_batch** out);
+int sph_batch_destroy(sph_batch* batch);
+/* Parameters of a scene as the batch holds them (cell_base, key_bits set). */
+int sph_batch_get_params(sph_batch* batch, uint32_t scene, sph_params* out);
+int sph_batch_set_params(sph_batch* batch, uint32_t scene, const sph_params* params);
+int sph_batch_set_array(sph_batch* batch, uint32_t scene, int which, const float* data,
+                        uint32_t first, uint32_t count);
+int sph_batch_get_array(sph_batch* batch, uint32_t scene, int which, float* out);
+int sph_batch_step(sph_batch* batch, uint32_t nsteps);
+int sph_batch_capture(sph_batch* batch, uint32_t nsteps);
+int sph_batch_replay(sph_batch* batch, uint32_t times);
+int sph_batch_get_stats(sph_batch* batch, uint32_t scene, sph_stats* out);
+int sph_batch_synchronize(sph_batch* batch);
+/* Launch tuning of every scene (sph_params::tune). */
+int sph_batch_set_tune(sph_batch* batch, const uint32_t tune[4]);
+/* Time each launch configuration (force list size x gather grid) over a few
+ * steps of the live state, restore positions, velocities and clocks, keep
+ * the fastest and append it to cache_path (JSON lines; NULL = no cache).  A
+ * cached entry for the same device and batch shape is used without timing.
+ * tune_out receives the choice, best_ms its time per step.                */
+int sph_batch_autotune(sph_batch* batch, const char* cache_path, uint32_t tune_out[4],
+                       float* best_ms);
+int sph_batch_info(sph_batch* batch, uint32_t* nscenes, uint32_t* total_particles,
+                   uint32_t* total_cells, uint32_t* key_bits);
 
 #ifdef __cplusplus
 }

@@ -281,9 +281,14 @@ void force_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
                                 cell[k].add(f);
                         }
                         if (dbl) {
+                            // |term| with absolute values, plus its first-order
+                            // sensitivity to c = h - r (which cancels near r = h):
+                            // term = (c A' + B) c d12 -> d/dc = (2 A + B) d12,
+                            // and c carries an error proportional to h + r.
                             T pa = ((cc * spiky) * (pai + pabs(j))) / r;
                             T vn = std::sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
-                            sc += (pa * len + std::fabs(vterm) * vn) * std::fabs(cd);
+                            T A = pa * len, B = std::fabs(vterm) * vn;
+                            sc += (A + B) * std::fabs(cd) + ((T)2 * A + B) * std::fabs(d12) * (h + r);
                         }
                     }
                 }

@@ -19,8 +19,17 @@ enum Counter : int {
     kCntNum = 4
 };
 
-// Per-block FP64 partials of the diagnostics reduction.
-enum Partial : int { kPMass = 0, kPKin = 1, kPPot = 2, kPVmax = 3, kPNum = 4 };
+// Per-scene FP64 diagnostics: stats[scene * kNumStats + k].
+enum Stat : int {
+    kPMass = 0,
+    kPKin = 1,
+    kPPot = 2,
+    kPVmax = 3,
+    kPNum = 4,     // number of reduced quantities
+    kSDt = 4,      // time step picked for the next step
+    kSClock = 5,   // simulated clock
+    kNumStats = 8
+};
 
 // ---- radix sort (sph_sort.hip) --------------------------------------------
 struct RadixSortPlan {
@@ -42,6 +51,7 @@ hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipSt
 // Stable LSD sort of (keys, vals) by the low `bits` bits of keys.  keys0/vals0
 // hold the input; keys1/vals1 are the ping-pong buffers.  On return
 // *keys_out/*vals_out point at whichever pair holds the sorted result.
+// Graph-capturable (no host-side state changes between launches).
 hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* keys0,
                             uint32_t* vals0, uint32_t* keys1, uint32_t* vals1,
                             uint32_t** keys_out, uint32_t** vals_out, hipStream_t stream);
@@ -51,9 +61,11 @@ struct StepBuffers {
     float4* pos_cur;      // sorted-order state after the last reorder
     float4* vel_cur;
     uint32_t* pid_cur;    // original index of each slot
+    uint32_t* sid_cur;    // scene of each slot (batch only, else null)
     float4* pos_alt;
     float4* vel_alt;
     uint32_t* pid_alt;
+    uint32_t* sid_alt;
     float2* rhop;         // (density, pressure) per sorted slot
     float4* dvel;         // optional force increment per sorted slot
     uint32_t* key_pid;    // cell key in original-index order (sort input)
@@ -63,36 +75,49 @@ struct StepBuffers {
     uint32_t* cell_start;
     uint32_t* cell_end;
     uint32_t* counters;
-    double* partials;     // kPNum * max partial blocks
-    double* stats;        // kPNum finalised values + dt at [kPNum]
-    float* dt_dev;        // time step the integrate/force kernels use
+    uint32_t* work;       // persistent-kernel tickets: [ticket, done] per gather kernel
+    double* partials;     // per (scene, chunk) FP64 partials
+    double* stats;        // kNumStats doubles per scene
+    float* dt_dev;        // time step of each scene
 };
 
-// integrate + boundary + hash on the current slots; writes the sort input in
-// original-index order and clears cellStart.
-hipError_t launch_integrate_hash(const sph_params& p, const StepBuffers& b, hipStream_t s);
+// What a launch covers: one scene (params by value) or a batch of scenes
+// (device parameter table + per-slot scene ids + static slot ranges).
+struct LaunchCfg {
+    uint32_t n = 0;             // particles over all scenes
+    uint32_t num_cells = 0;     // cells over all scenes
+    uint32_t key_bits = 0;
+    uint32_t nscenes = 1;
+    uint32_t max_cap = 16;      // largest max_par_in_cell of any scene
+    uint32_t max_scene_n = 0;   // largest scene
+    uint32_t precision = 0;     // shared by every scene
+    uint32_t tune[4] = {0, 0, 0, 0};
+    const sph_params* tab = nullptr;      // device table (batch) or null
+    const uint32_t* scene_off = nullptr;  // device: first slot of each scene, nscenes + 1
+    sph_params one{};                     // the scene (one-scene launches)
+};
 
-// reorder into the alternate buffers + cellStart/cellEnd from the sorted keys.
-hipError_t launch_reorder(const sph_params& p, const StepBuffers& b, const uint32_t* sorted_key,
+// Launch tuning words (LaunchCfg::tune, from sph_params::tune):
+//   tune[0]  force kernel: 1 = fused candidate/pair loop, 16/24/32/48 =
+//            neighbour-list entries per lane (default 16)
+//   tune[1]  gather grid: 0 = one thread per slot over all slots, else a
+//            persistent grid of tune[1] blocks pulling 256-slot tiles
+uint32_t stats_chunks(const LaunchCfg& c);
+size_t stats_partials_count(const LaunchCfg& c);
+
+hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
+hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
                           const uint32_t* old_slot, hipStream_t s);
-
-// density + EOS on the sorted slots of pos_cur.
-hipError_t launch_density(const sph_params& p, const StepBuffers& b, hipStream_t s);
-
-// force + velocity update: reads vel_cur, writes vel_out.
-hipError_t launch_force(const sph_params& p, const StepBuffers& b, float4* vel_out, bool keep_dvel,
+hipError_t launch_density(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
+hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, bool keep_dvel,
                         hipStream_t s);
-
-// FP64 partials of (mass, kinetic, potential, max speed) over pos_cur/vel_cur,
-// then the single-block finalisation into stats[0..4] and dt_dev.
-// stats[kPNum] receives the time step, stats[kPNum + 1] the simulated clock
-// (advanced by that step when advance_clock is set).
-uint32_t stats_num_blocks(uint32_t n);
-hipError_t launch_stats(const sph_params& p, const StepBuffers& b, bool advance_clock,
+// FP64 per-scene reductions of (mass, kinetic, potential, max speed) over
+// pos_cur/vel_cur, then per-scene finalisation: stats, the next time step
+// (fixed, CFL, clipped at t_end) into dt_dev, and the clock when advance_clock.
+hipError_t launch_stats(const LaunchCfg& c, const StepBuffers& b, bool advance_clock,
                         hipStream_t s);
-
 // Largest particle count of one cell (atomicMax into *max_count).
-hipError_t launch_cell_stats(const sph_params& p, const StepBuffers& b, uint32_t* max_count,
+hipError_t launch_cell_stats(const LaunchCfg& c, const StepBuffers& b, uint32_t* max_count,
                              hipStream_t s);
 
 // Host <-> device record transfer through the pid permutation.
@@ -103,5 +128,9 @@ hipError_t launch_update_from_original(uint32_t n, const uint32_t* pid, const fl
                                        uint32_t first, uint32_t count, float4* dst, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t s);
+// sid[k] = scene of original index pid[k]; scene_off holds the first
+// original index of each scene (scenes are contiguous in original order).
+hipError_t launch_scene_ids(uint32_t n, uint32_t nscenes, const uint32_t* scene_off,
+                            const uint32_t* pid, uint32_t* sid, hipStream_t s);
 
 }  // namespace sph

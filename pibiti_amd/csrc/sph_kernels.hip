@@ -9,6 +9,12 @@
 // makes integrate, reorder and force streams coalesced, and the original
 // order is only materialised on readback.
 //
+// Every kernel serves one scene (parameters passed by value) or a batch of
+// independent scenes packed into one particle array: each scene owns a
+// disjoint range of global cell keys (cell_base + local hash), so one sort
+// and one launch per kernel advance every scene, and a slot finds its
+// scene's parameters through a per-slot scene id.
+//
 // Arithmetic follows the reference kernels operation by operation (no FMA
 // contraction: the library is built with -ffp-contract=off and correctly
 // rounded fp32 division/sqrt), so one thread per particle reproduces the
@@ -26,6 +32,22 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr float kBndEps = 0.00001f;  // EPS of boundary()
+
+// Parameters of the scene a slot belongs to: the by-value parameters of a
+// one-scene launch, or the batch table entry of the slot's scene.
+template <bool kBatch>
+__device__ __forceinline__ const sph_params& scene_params(const sph_params& one,
+                                                         const sph_params* __restrict__ tab,
+                                                         const uint32_t* __restrict__ sid,
+                                                         uint32_t slot, uint32_t& scene) {
+    if constexpr (kBatch) {
+        scene = sid[slot];
+        return tab[scene];
+    } else {
+        scene = 0;
+        return one;
+    }
+}
 
 __device__ __forceinline__ void bnd_wall(float diff, float nx, float ny, float nz, float stiff,
                                          float damp, float dt, float& vx, float& vy, float& vz) {
@@ -49,15 +71,8 @@ __device__ __forceinline__ int cell_coord(float x, float wmin, float cs, uint32_
     return (int)f;
 }
 
-__device__ __forceinline__ void grid_pos(const sph_params& p, float x, float y, float z, int& gx,
-                                         int& gy, int& gz, bool& out) {
-    gx = cell_coord(x, p.world_min[0], p.cell_size[0], p.grid_size[0], out);
-    gy = cell_coord(y, p.world_min[1], p.cell_size[1], p.grid_size[1], out);
-    gz = cell_coord(z, p.world_min[2], p.cell_size[2], p.grid_size[2], out);
-}
-
 __device__ __forceinline__ uint32_t grid_hash(const sph_params& p, int gx, int gy, int gz) {
-    return (uint32_t)gz * p.grid_size_yx + (uint32_t)gy * p.grid_size[0] + (uint32_t)gx;
+    return p.cell_base + (uint32_t)gz * p.grid_size_yx + (uint32_t)gy * p.grid_size[0] + (uint32_t)gx;
 }
 
 __device__ __forceinline__ float eos(const sph_params& p, float rho) {
@@ -83,18 +98,22 @@ struct KahanF {
 };
 
 // ------------------------------------------------------------ integrate+hash
+template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_integrate_hash(
-    sph_params p, uint32_t n, float4* __restrict__ pos, float4* __restrict__ vel,
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid,
+    uint32_t n, uint32_t num_cells, float4* __restrict__ pos, float4* __restrict__ vel,
     const uint32_t* __restrict__ pid, uint32_t* __restrict__ key_pid,
     uint32_t* __restrict__ val_pid, uint32_t* __restrict__ cell_start,
     uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     // cellStart of the previous step is dead from here on: clear it.
-    for (uint32_t c = tid; c < p.num_cells; c += nthreads) cell_start[c] = kEmptyCell;
+    for (uint32_t c = tid; c < num_cells; c += nthreads) cell_start[c] = kEmptyCell;
     if (tid >= n) return;
 
-    const float dt = *dt_dev;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
+    const float dt = dt_dev[scene];
     float4 x4 = pos[tid];
     float4 v4 = vel[tid];
     float x = x4.x, y = x4.y, z = x4.z;
@@ -135,8 +154,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     vel[tid] = make_float4(vx, vy, vz, v4.w);
 
     bool out = false;
-    int gx, gy, gz;
-    grid_pos(p, x, y, z, gx, gy, gz, out);
+    const int gx = cell_coord(x, p.world_min[0], p.cell_size[0], p.grid_size[0], out);
+    const int gy = cell_coord(y, p.world_min[1], p.cell_size[1], p.grid_size[1], out);
+    const int gz = cell_coord(z, p.world_min[2], p.cell_size[2], p.grid_size[2], out);
     if (out) atomicAdd(&counters[kCntOutOfGrid], 1u);
     const uint32_t id = pid[tid];
     key_pid[id] = grid_hash(p, gx, gy, gz);
@@ -144,18 +164,27 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
 }
 
 // ------------------------------------------------------------------ reorder
+template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_reorder(
-    uint32_t n, uint32_t cap, const uint32_t* __restrict__ skey, const uint32_t* __restrict__ old_slot,
+    sph_params one, const sph_params* __restrict__ tab, uint32_t n,
+    const uint32_t* __restrict__ skey, const uint32_t* __restrict__ old_slot,
     const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
-    const uint32_t* __restrict__ pid_in, float4* __restrict__ pos_out, float4* __restrict__ vel_out,
-    uint32_t* __restrict__ pid_out, uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_end,
-    uint32_t* __restrict__ counters) {
+    const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in,
+    float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out,
+    uint32_t* __restrict__ sid_out, uint32_t* __restrict__ cell_start,
+    uint32_t* __restrict__ cell_end, uint32_t* __restrict__ counters) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t s = old_slot[k];
     pos_out[k] = pos_in[s];
     vel_out[k] = vel_in[s];
     pid_out[k] = pid_in[s];
+    uint32_t cap = one.max_par_in_cell;
+    if constexpr (kBatch) {
+        const uint32_t sc = sid_in[s];
+        sid_out[k] = sc;
+        cap = tab[sc].max_par_in_cell;
+    }
     const uint32_t h = skey[k];
     if (k == 0 || skey[k - 1] != h) cell_start[h] = k;
     if (k + 1 == n || skey[k + 1] != h) {
@@ -196,39 +225,49 @@ struct AxisWalk {
 
 struct CellWalk {
     AxisWalk ax, ay, az;
-    float cull2;  // squared cull radius
+    float cull2;        // squared cull radius
+    uint32_t base;      // cell key of (0, 0, 0) of this scene
+    uint32_t gyx, gx;   // key strides
+    uint32_t cap;       // max_par_in_cell
 
     __device__ __forceinline__ void init(const sph_params& p, float x, float y, float z) {
         ax.init(x, p.world_min[0], p.world_max[0], p.cell_size[0], p.grid_size[0]);
         ay.init(y, p.world_min[1], p.world_max[1], p.cell_size[1], p.grid_size[1]);
         az.init(z, p.world_min[2], p.world_max[2], p.cell_size[2], p.grid_size[2]);
         cull2 = p.h2 * 1.00001f;
+        base = p.cell_base;
+        gyx = p.grid_size_yx;
+        gx = p.grid_size[0];
+        cap = p.max_par_in_cell;
     }
     // Slot range [s, e) of neighbour cell (dx, dy, dz), capped at max_par_in_cell;
     // false if the cell is outside the grid, culled or empty.
-    __device__ __forceinline__ bool open(const sph_params& p, int dx, int dy, int dz,
+    __device__ __forceinline__ bool open(int dx, int dy, int dz,
                                          const uint32_t* __restrict__ cell_start,
                                          const uint32_t* __restrict__ cell_end, uint32_t& s,
                                          uint32_t& e) const {
         if (!ax.ok(dx) || !ay.ok(dy) || !az.ok(dz)) return false;
         if (az.d2(dz) + ay.d2(dy) + ax.d2(dx) > cull2) return false;
-        const uint32_t c = grid_hash(p, ax.g + dx, ay.g + dy, az.g + dz);
+        const uint32_t c = base + (uint32_t)(az.g + dz) * gyx + (uint32_t)(ay.g + dy) * gx +
+                           (uint32_t)(ax.g + dx);
         s = cell_start[c];
         if (s == kEmptyCell) return false;
-        e = min(cell_end[c], s + p.max_par_in_cell);
+        e = min(cell_end[c], s + cap);
         return true;
     }
 };
 
 // ------------------------------------------------------------------ density
-template <int kPrec>
-__global__ __launch_bounds__(kBlock) void k_density(sph_params p, uint32_t n,
-                                                    const float4* __restrict__ pos,
-                                                    const uint32_t* __restrict__ cell_start,
-                                                    const uint32_t* __restrict__ cell_end,
-                                                    float2* __restrict__ rhop) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+template <bool kBatch, int kPrec>
+__device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
+                                            const sph_params* __restrict__ tab,
+                                            const uint32_t* __restrict__ sid,
+                                            const float4* __restrict__ pos,
+                                            const uint32_t* __restrict__ cell_start,
+                                            const uint32_t* __restrict__ cell_end,
+                                            float2* __restrict__ rhop) {
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
     const float4 xi = pos[i];
     CellWalk w;
     w.init(p, xi.x, xi.y, xi.z);
@@ -237,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params p, uint32_t n,
     KahanF ks{0.0f, 0.0f};
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
-        if (!w.open(p, ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
+        if (!w.open(ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
             continue;
         float cell = 0.0f;
         // next candidate's position loaded one iteration ahead
@@ -262,6 +301,56 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params p, uint32_t n,
     if (kPrec == SPH_PREC_KAHAN) sum = ks.s;
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
     rhop[i] = make_float2(rho, eos(p, rho));
+}
+
+template <bool kBatch, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_density(sph_params one, const sph_params* __restrict__ tab,
+                                                    const uint32_t* __restrict__ sid, uint32_t n,
+                                                    const float4* __restrict__ pos,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    const uint32_t* __restrict__ cell_end,
+                                                    float2* __restrict__ rhop) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    density_one<kBatch, kPrec>(i, one, tab, sid, pos, cell_start, cell_end, rhop);
+}
+
+// Persistent form: a fixed grid pulls 256-slot tiles from a ticket counter.
+// Every block ends on exactly one ticket past the last tile; the last block
+// to finish resets both counters, so consecutive launches (and graph
+// replays) start from zero without a separate clear.
+__device__ __forceinline__ uint32_t next_tile(uint32_t* work, uint32_t* s_tile) {
+    if (threadIdx.x == 0) *s_tile = atomicAdd(&work[0], 1u);
+    __syncthreads();
+    const uint32_t t = *s_tile;
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void retire_block(uint32_t* work) {
+    if (threadIdx.x == 0) {
+        const uint32_t d = atomicAdd(&work[1], 1u);
+        if (d == gridDim.x - 1) {
+            atomicExch(&work[0], 0u);
+            atomicExch(&work[1], 0u);
+        }
+    }
+}
+
+template <bool kBatch, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_density_persistent(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ work) {
+    __shared__ uint32_t s_tile;
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    for (;;) {
+        const uint32_t tile = next_tile(work, &s_tile);
+        if (tile >= ntiles) break;
+        const uint32_t i = tile * kBlock + threadIdx.x;
+        if (i < n) density_one<kBatch, kPrec>(i, one, tab, sid, pos, cell_start, cell_end, rhop);
+    }
+    retire_block(work);
 }
 
 // -------------------------------------------------------------------- force
@@ -334,19 +423,26 @@ __device__ __forceinline__ void force_drain(const sph_params& p, const uint32_t*
     }
 }
 
-template <int kPrec, int kList>
-__global__ __launch_bounds__(kBlock) void k_force(
-    sph_params p, uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ vel,
-    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
-    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
-    __shared__ uint32_t list[kList * kBlock];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// One slot per lane; every lane of the wave must call this (inactive lanes
+// with active = false) because the list drains are wave-wide decisions.
+template <bool kBatch, int kPrec, int kList>
+__device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* list,
+                                          const sph_params& one, const sph_params* __restrict__ tab,
+                                          const uint32_t* __restrict__ sid,
+                                          const float4* __restrict__ pos,
+                                          const float4* __restrict__ vel,
+                                          const float2* __restrict__ rhop,
+                                          const uint32_t* __restrict__ cell_start,
+                                          const uint32_t* __restrict__ cell_end,
+                                          float4* __restrict__ vel_out, float4* __restrict__ dvel,
+                                          const float* __restrict__ dt_dev) {
     const int t = threadIdx.x;
-    const float4 xi = pos[i];
-    const float4 vi = vel[i];
-    const float2 ri = rhop[i];
+    const uint32_t ii = active ? i : 0u;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, ii, scene);
+    const float4 xi = pos[ii];
+    const float4 vi = vel[ii];
+    const float2 ri = rhop[ii];
     CellWalk w;
     w.init(p, xi.x, xi.y, xi.z);
     const float r2max = p.force_r2_max;
@@ -356,8 +452,8 @@ __global__ __launch_bounds__(kBlock) void k_force(
     int cnt = 0;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s = 0, e = 0;
-        const bool open =
-            w.open(p, ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e);
+        const bool open = active && w.open(ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start,
+                                           cell_end, s, e);
         // every candidate of this cell could pass: make room first
         if (__any(cnt + (int)(e - s) > kList)) {
             force_drain<kPrec>(p, list, t, cnt, xi, vi, ri, pos, vel, rhop, a);
@@ -379,6 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
         }
     }
     force_drain<kPrec>(p, list, t, cnt, xi, vi, ri, pos, vel, rhop, a);
+    if (!active) return;
     float ax, ay, az;
     if (kPrec == SPH_PREC_KAHAN) {
         ax = a.kx.s;
@@ -389,7 +486,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
         ay = a.ay + a.fy;
         az = a.az + a.fz;
     }
-    const float mdt = p.particle_mass * *dt_dev;
+    const float mdt = p.particle_mass * dt_dev[scene];
     ax = ax * mdt;
     ay = ay * mdt;
     az = az * mdt;
@@ -397,16 +494,52 @@ __global__ __launch_bounds__(kBlock) void k_force(
     if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
 
+template <bool kBatch, int kPrec, int kList>
+__global__ __launch_bounds__(kBlock) void k_force(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    __shared__ uint32_t list[kList * kBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
+                                    cell_end, vel_out, dvel, dt_dev);
+}
+
+template <bool kBatch, int kPrec, int kList>
+__global__ __launch_bounds__(kBlock) void k_force_persistent(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev, uint32_t* __restrict__ work) {
+    __shared__ uint32_t list[kList * kBlock];
+    __shared__ uint32_t s_tile;
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    for (;;) {
+        const uint32_t tile = next_tile(work, &s_tile);
+        if (tile >= ntiles) break;
+        const uint32_t i = tile * kBlock + threadIdx.x;
+        force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
+                                        cell_end, vel_out, dvel, dt_dev);
+    }
+    retire_block(work);
+}
+
 // Force kernel for max_par_in_cell above the list capacity: the pair body
 // sits directly in the candidate loop (same order, same sums).
-template <int kPrec>
+template <bool kBatch, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_force_fused(
-    sph_params p, uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ vel,
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
     float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
     const float4 xi = pos[i];
     const float4 vi = vel[i];
     const float2 ri = rhop[i];
@@ -419,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
     KahanF kx{0.0f, 0.0f}, ky{0.0f, 0.0f}, kz{0.0f, 0.0f};
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
-        if (!w.open(p, ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
+        if (!w.open(ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
             continue;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (uint32_t j = s; j < e; ++j) {
@@ -460,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
         ay = ky.s;
         az = kz.s;
     }
-    const float mdt = p.particle_mass * *dt_dev;
+    const float mdt = p.particle_mass * dt_dev[scene];
     ax = ax * mdt;
     ay = ay * mdt;
     az = az * mdt;
@@ -470,7 +603,6 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
 
 // -------------------------------------------------------------------- stats
 constexpr int kStatBlock = 256;
-constexpr uint32_t kMaxStatBlocks = 1024;
 
 __device__ __forceinline__ void block_reduce4(double v[4], double* sh) {
     // sh: 4 * kStatBlock doubles.  Sums for 0..2, max for 3; fixed order.
@@ -487,15 +619,20 @@ __device__ __forceinline__ void block_reduce4(double v[4], double* sh) {
     for (int k = 0; k < 4; ++k) v[k] = sh[k * kStatBlock];
 }
 
-__global__ __launch_bounds__(kStatBlock) void k_stats_partials(sph_params p, uint32_t n,
-                                                              const float4* __restrict__ pos,
-                                                              const float4* __restrict__ vel,
-                                                              double* __restrict__ partials) {
+// grid (chunks, nscenes): block (b, s) reduces chunk b of scene s's slots.
+template <bool kBatch>
+__global__ __launch_bounds__(kStatBlock) void k_stats_partials(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ scene_off,
+    const float4* __restrict__ pos, const float4* __restrict__ vel, double* __restrict__ partials) {
     __shared__ double sh[4 * kStatBlock];
+    const uint32_t s = blockIdx.y;
+    uint32_t unused;
+    const sph_params& p = scene_params<kBatch>(one, tab, &s, 0, unused);
+    const uint32_t lo = scene_off[s], hi = scene_off[s + 1];
     const double m = (double)p.particle_mass;
     const double g[3] = {(double)p.gravity[0], (double)p.gravity[1], (double)p.gravity[2]};
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t i = blockIdx.x * kStatBlock + threadIdx.x; i < n; i += gridDim.x * kStatBlock) {
+    for (uint32_t i = lo + blockIdx.x * kStatBlock + threadIdx.x; i < hi; i += gridDim.x * kStatBlock) {
         const float4 x = pos[i];
         const float4 v = vel[i];
         const double v2 = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
@@ -506,35 +643,46 @@ __global__ __launch_bounds__(kStatBlock) void k_stats_partials(sph_params p, uin
     }
     block_reduce4(acc, sh);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 4; ++k) partials[blockIdx.x * kPNum + k] = acc[k];
+        for (int k = 0; k < 4; ++k) partials[((size_t)s * gridDim.x + blockIdx.x) * kPNum + k] = acc[k];
 }
 
-__global__ __launch_bounds__(kStatBlock) void k_stats_finalize(sph_params p, uint32_t nblocks,
-                                                              const double* __restrict__ partials,
-                                                              double* __restrict__ stats,
-                                                              float* __restrict__ dt_dev,
-                                                              int advance_clock) {
+// grid (nscenes): one block per scene.
+template <bool kBatch>
+__global__ __launch_bounds__(kStatBlock) void k_stats_finalize(
+    sph_params one, const sph_params* __restrict__ tab, uint32_t chunks,
+    const double* __restrict__ partials, double* __restrict__ stats, float* __restrict__ dt_dev,
+    int advance_clock) {
     __shared__ double sh[4 * kStatBlock];
+    const uint32_t s = blockIdx.x;
+    uint32_t unused;
+    const sph_params& p = scene_params<kBatch>(one, tab, &s, 0, unused);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t b = threadIdx.x; b < nblocks; b += kStatBlock) {
-        for (int k = 0; k < 3; ++k) acc[k] += partials[b * kPNum + k];
-        acc[3] = fmax(acc[3], partials[b * kPNum + 3]);
+    for (uint32_t b = threadIdx.x; b < chunks; b += kStatBlock) {
+        const double* q = partials + ((size_t)s * chunks + b) * kPNum;
+        for (int k = 0; k < 3; ++k) acc[k] += q[k];
+        acc[3] = fmax(acc[3], q[3]);
     }
     block_reduce4(acc, sh);
     if (threadIdx.x == 0) {
+        double* st = stats + (size_t)s * kNumStats;
         const double vmax = sqrt(acc[3]);
-        stats[kPMass] = acc[0];
-        stats[kPKin] = acc[1];
-        stats[kPPot] = acc[2];
-        stats[kPVmax] = vmax;
+        st[kPMass] = acc[0];
+        st[kPKin] = acc[1];
+        st[kPPot] = acc[2];
+        st[kPVmax] = vmax;
         double dt = (double)p.time_step;
         if (p.cfl > 0.0f) {
             const double dcfl = (double)p.cfl * (double)p.h / ((double)p.sound_speed + vmax);
             if (dcfl < dt) dt = dcfl;
         }
-        stats[kPNum] = dt;
-        *dt_dev = (float)dt;
-        if (advance_clock) stats[kPNum + 1] += (double)(float)dt;
+        if (p.t_end > 0.0f) {
+            const double rem = (double)p.t_end - st[kSClock];
+            // a remainder below 1e-4 of a nominal step ends the run instead
+            if (rem < dt) dt = rem > 1e-4 * (double)p.time_step ? rem : 0.0;
+        }
+        st[kSDt] = dt;
+        dt_dev[s] = (float)dt;
+        if (advance_clock) st[kSClock] += (double)(float)dt;
     }
 }
 
@@ -586,102 +734,182 @@ __global__ __launch_bounds__(kBlock) void k_iota_u32(uint32_t* __restrict__ dst,
     if (i < n) dst[i] = i;
 }
 
+__global__ __launch_bounds__(kBlock) void k_scene_ids(uint32_t n, uint32_t nscenes,
+                                                     const uint32_t* __restrict__ scene_off,
+                                                     const uint32_t* __restrict__ pid,
+                                                     uint32_t* __restrict__ sid) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t id = pid[k];
+    uint32_t lo = 0, hi = nscenes;  // last s with scene_off[s] <= id
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (scene_off[mid] <= id)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    sid[k] = lo;
+}
+
 inline uint32_t blocks_for(uint64_t n, uint32_t block) {
-    return (uint32_t)((n + block - 1) / block);
+    const uint32_t b = (uint32_t)((n + block - 1) / block);
+    return b == 0 ? 1 : b;
 }
 
-}  // namespace
-
-hipError_t launch_integrate_hash(const sph_params& p, const StepBuffers& b, hipStream_t s) {
-    const uint32_t n = p.num_particles;
-    uint32_t nb = blocks_for(n, kBlock);
-    if (nb == 0) nb = 1;
-    hipLaunchKernelGGL(k_integrate_hash, dim3(nb), dim3(kBlock), 0, s, p, n, b.pos_cur, b.vel_cur,
-                       b.pid_cur, b.key_pid, b.val_pid, b.cell_start, b.counters, b.dt_dev);
-    return hipGetLastError();
+// ------------------------------------------------------------- dispatchers
+template <bool kBatch>
+void integrate_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL((k_integrate_hash<kBatch>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
+                       c.one, c.tab, b.sid_cur, c.n, c.num_cells, b.pos_cur, b.vel_cur, b.pid_cur,
+                       b.key_pid, b.val_pid, b.cell_start, b.counters, b.dt_dev);
 }
 
-hipError_t launch_reorder(const sph_params& p, const StepBuffers& b, const uint32_t* sorted_key,
-                          const uint32_t* old_slot, hipStream_t s) {
-    const uint32_t n = p.num_particles;
-    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, n,
-                       p.max_par_in_cell, sorted_key, old_slot, b.pos_cur, b.vel_cur, b.pid_cur,
-                       b.pos_alt, b.vel_alt, b.pid_alt, b.cell_start, b.cell_end, b.counters);
-    return hipGetLastError();
+template <bool kBatch>
+void reorder_impl(const LaunchCfg& c, const StepBuffers& b, const uint32_t* skey,
+                  const uint32_t* oslot, hipStream_t s) {
+    hipLaunchKernelGGL((k_reorder<kBatch>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, c.one,
+                       c.tab, c.n, skey, oslot, b.pos_cur, b.vel_cur, b.pid_cur, b.sid_cur,
+                       b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.cell_start, b.cell_end,
+                       b.counters);
 }
 
-hipError_t launch_density(const sph_params& p, const StepBuffers& b, hipStream_t s) {
-    const uint32_t n = p.num_particles;
-    if (p.precision == SPH_PREC_KAHAN)
-        hipLaunchKernelGGL(k_density<SPH_PREC_KAHAN>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0,
-                           s, p, n, b.pos_cur, b.cell_start, b.cell_end, b.rhop);
+template <bool kBatch, int kPrec>
+void density_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
+    const uint32_t grid = c.tune[1];
+    if (grid)
+        hipLaunchKernelGGL((k_density_persistent<kBatch, kPrec>), dim3(grid), dim3(kBlock), 0, s,
+                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop,
+                           b.work);
     else
-        hipLaunchKernelGGL(k_density<SPH_PREC_REFERENCE>, dim3(blocks_for(n, kBlock)), dim3(kBlock),
-                           0, s, p, n, b.pos_cur, b.cell_start, b.cell_end, b.rhop);
-    return hipGetLastError();
+        hipLaunchKernelGGL((k_density<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0,
+                           s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end,
+                           b.rhop);
 }
 
-template <int kPrec, int kList>
-void launch_force_list(const sph_params& p, const StepBuffers& b, float4* vel_out, float4* dvel,
-                       dim3 g, dim3 t, hipStream_t s) {
-    hipLaunchKernelGGL((k_force<kPrec, kList>), g, t, 0, s, p, p.num_particles, b.pos_cur, b.vel_cur,
-                       b.rhop, b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+template <bool kBatch, int kPrec, int kList>
+void force_list_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float4* dvel,
+                     hipStream_t s) {
+    const uint32_t grid = c.tune[1];
+    if (grid)
+        hipLaunchKernelGGL((k_force_persistent<kBatch, kPrec, kList>), dim3(grid), dim3(kBlock), 0, s,
+                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop, b.cell_start,
+                           b.cell_end, vel_out, dvel, b.dt_dev, b.work + 2);
+    else
+        hipLaunchKernelGGL((k_force<kBatch, kPrec, kList>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock),
+                           0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
 }
 
-template <int kPrec>
-void launch_force_prec(const sph_params& p, const StepBuffers& b, float4* vel_out, float4* dvel,
-                       hipStream_t s) {
-    const uint32_t n = p.num_particles;
-    const dim3 g(blocks_for(n, kBlock)), t(kBlock);
-    uint32_t list = p.tune[0] ? p.tune[0] : (uint32_t)kForceListDefault;
-    if (list != 16 && list != 24 && list != 32 && list != 48) list = kForceListDefault;
-    if (p.max_par_in_cell > list) list = 48;
-    if (p.max_par_in_cell > 48 || p.tune[0] == 1) {
+template <bool kBatch, int kPrec>
+void force_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float4* dvel,
+                hipStream_t s) {
+    uint32_t list = c.tune[0] ? c.tune[0] : (uint32_t)kForceListDefault;
+    if (list != 1 && list != 16 && list != 24 && list != 32 && list != 48) list = kForceListDefault;
+    if (list != 1 && c.max_cap > list) list = 48;
+    if (list == 1 || c.max_cap > 48) {
         // fused test + pair loop: chosen by tune[0] == 1, or forced when one
         // cell could overflow any neighbour list
-        hipLaunchKernelGGL((k_force_fused<kPrec>), g, t, 0, s, p, n, b.pos_cur, b.vel_cur, b.rhop,
+        hipLaunchKernelGGL((k_force_fused<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock),
+                           0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
                            b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
         return;
     }
     switch (list) {
-        case 16: launch_force_list<kPrec, 16>(p, b, vel_out, dvel, g, t, s); break;
-        case 24: launch_force_list<kPrec, 24>(p, b, vel_out, dvel, g, t, s); break;
-        case 32: launch_force_list<kPrec, 32>(p, b, vel_out, dvel, g, t, s); break;
-        default: launch_force_list<kPrec, 48>(p, b, vel_out, dvel, g, t, s); break;
+        case 16: force_list_impl<kBatch, kPrec, 16>(c, b, vel_out, dvel, s); break;
+        case 24: force_list_impl<kBatch, kPrec, 24>(c, b, vel_out, dvel, s); break;
+        case 32: force_list_impl<kBatch, kPrec, 32>(c, b, vel_out, dvel, s); break;
+        default: force_list_impl<kBatch, kPrec, 48>(c, b, vel_out, dvel, s); break;
     }
 }
 
-hipError_t launch_force(const sph_params& p, const StepBuffers& b, float4* vel_out, bool keep_dvel,
+template <bool kBatch>
+void stats_impl(const LaunchCfg& c, const StepBuffers& b, bool advance, hipStream_t s) {
+    const uint32_t chunks = stats_chunks(c);
+    hipLaunchKernelGGL((k_stats_partials<kBatch>), dim3(chunks, c.nscenes), dim3(kStatBlock), 0, s,
+                       c.one, c.tab, c.scene_off, b.pos_cur, b.vel_cur, b.partials);
+    hipLaunchKernelGGL((k_stats_finalize<kBatch>), dim3(c.nscenes), dim3(kStatBlock), 0, s, c.one,
+                       c.tab, chunks, b.partials, b.stats, b.dt_dev, advance ? 1 : 0);
+}
+
+}  // namespace
+
+uint32_t stats_chunks(const LaunchCfg& c) {
+    uint32_t ch = (c.max_scene_n + 2047) / 2048;
+    if (ch < 1) ch = 1;
+    if (ch > 1024) ch = 1024;
+    return ch;
+}
+
+size_t stats_partials_count(const LaunchCfg& c) {
+    return (size_t)stats_chunks(c) * c.nscenes * kPNum;
+}
+
+hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
+    if (c.tab)
+        integrate_impl<true>(c, b, s);
+    else
+        integrate_impl<false>(c, b, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
+                          const uint32_t* old_slot, hipStream_t s) {
+    if (c.tab)
+        reorder_impl<true>(c, b, sorted_key, old_slot, s);
+    else
+        reorder_impl<false>(c, b, sorted_key, old_slot, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
+    const bool kahan = c.precision == SPH_PREC_KAHAN;
+    if (c.tab) {
+        if (kahan)
+            density_impl<true, SPH_PREC_KAHAN>(c, b, s);
+        else
+            density_impl<true, SPH_PREC_REFERENCE>(c, b, s);
+    } else {
+        if (kahan)
+            density_impl<false, SPH_PREC_KAHAN>(c, b, s);
+        else
+            density_impl<false, SPH_PREC_REFERENCE>(c, b, s);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, bool keep_dvel,
                         hipStream_t s) {
     float4* dvel = keep_dvel ? b.dvel : nullptr;
-    if (p.precision == SPH_PREC_KAHAN)
-        launch_force_prec<SPH_PREC_KAHAN>(p, b, vel_out, dvel, s);
-    else
-        launch_force_prec<SPH_PREC_REFERENCE>(p, b, vel_out, dvel, s);
+    const bool kahan = c.precision == SPH_PREC_KAHAN;
+    if (c.tab) {
+        if (kahan)
+            force_impl<true, SPH_PREC_KAHAN>(c, b, vel_out, dvel, s);
+        else
+            force_impl<true, SPH_PREC_REFERENCE>(c, b, vel_out, dvel, s);
+    } else {
+        if (kahan)
+            force_impl<false, SPH_PREC_KAHAN>(c, b, vel_out, dvel, s);
+        else
+            force_impl<false, SPH_PREC_REFERENCE>(c, b, vel_out, dvel, s);
+    }
     return hipGetLastError();
 }
 
-uint32_t stats_num_blocks(uint32_t n) {
-    uint32_t nb = blocks_for(n, kStatBlock);
-    if (nb > kMaxStatBlocks) nb = kMaxStatBlocks;
-    return nb == 0 ? 1 : nb;
-}
-
-hipError_t launch_stats(const sph_params& p, const StepBuffers& b, bool advance_clock,
+hipError_t launch_stats(const LaunchCfg& c, const StepBuffers& b, bool advance_clock,
                         hipStream_t s) {
-    const uint32_t nb = stats_num_blocks(p.num_particles);
-    hipLaunchKernelGGL(k_stats_partials, dim3(nb), dim3(kStatBlock), 0, s, p, p.num_particles,
-                       b.pos_cur, b.vel_cur, b.partials);
-    hipLaunchKernelGGL(k_stats_finalize, dim3(1), dim3(kStatBlock), 0, s, p, nb, b.partials,
-                       b.stats, b.dt_dev, advance_clock ? 1 : 0);
+    if (c.tab)
+        stats_impl<true>(c, b, advance_clock, s);
+    else
+        stats_impl<false>(c, b, advance_clock, s);
     return hipGetLastError();
 }
 
-hipError_t launch_cell_stats(const sph_params& p, const StepBuffers& b, uint32_t* max_count,
+hipError_t launch_cell_stats(const LaunchCfg& c, const StepBuffers& b, uint32_t* max_count,
                              hipStream_t s) {
-    uint32_t nb = blocks_for(p.num_cells, kBlock);
+    uint32_t nb = blocks_for(c.num_cells, kBlock);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(k_cell_stats, dim3(nb), dim3(kBlock), 0, s, p.num_cells, b.cell_start,
+    hipLaunchKernelGGL(k_cell_stats, dim3(nb), dim3(kBlock), 0, s, c.num_cells, b.cell_start,
                        b.cell_end, max_count);
     return hipGetLastError();
 }
@@ -703,13 +931,19 @@ hipError_t launch_update_from_original(uint32_t n, const uint32_t* pid, const fl
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, size_t count, hipStream_t s) {
     uint32_t nb = blocks_for(count, kBlock);
     if (nb > 4096) nb = 4096;
-    if (nb == 0) nb = 1;
     hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(kBlock), 0, s, dst, value, count);
     return hipGetLastError();
 }
 
 hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_iota_u32, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scene_ids(uint32_t n, uint32_t nscenes, const uint32_t* scene_off,
+                            const uint32_t* pid, uint32_t* sid, hipStream_t s) {
+    hipLaunchKernelGGL(k_scene_ids, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, n, nscenes,
+                       scene_off, pid, sid);
     return hipGetLastError();
 }
 

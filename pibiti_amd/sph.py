@@ -99,7 +99,9 @@ class Params(C.Structure):
         ("key_bits", C.c_uint32),
         ("force_r2_max", C.c_float),
         ("tune", C.c_uint32 * 4),
-        ("reserved", C.c_uint32 * 4),
+        ("cell_base", C.c_uint32),
+        ("t_end", C.c_float),
+        ("reserved", C.c_uint32 * 2),
     ]
 
     def copy(self) -> "Params":
@@ -159,6 +161,23 @@ SIGNATURES = {
     "sph_synchronize": (C.c_int, [_P]),
     "sph_set_timing": (C.c_int, [_P, C.c_int]),
     "sph_get_timing": (C.c_int, [_P, _fp]),
+    "sph_capture": (C.c_int, [_P, C.c_uint32]),
+    "sph_replay": (C.c_int, [_P, C.c_uint32]),
+    "sph_batch_create": (C.c_int, [C.POINTER(Params), C.c_uint32, C.c_int, C.c_uint32,
+                                   C.POINTER(_P)]),
+    "sph_batch_destroy": (C.c_int, [_P]),
+    "sph_batch_get_params": (C.c_int, [_P, C.c_uint32, C.POINTER(Params)]),
+    "sph_batch_set_params": (C.c_int, [_P, C.c_uint32, C.POINTER(Params)]),
+    "sph_batch_set_array": (C.c_int, [_P, C.c_uint32, C.c_int, _fp, C.c_uint32, C.c_uint32]),
+    "sph_batch_get_array": (C.c_int, [_P, C.c_uint32, C.c_int, _fp]),
+    "sph_batch_step": (C.c_int, [_P, C.c_uint32]),
+    "sph_batch_capture": (C.c_int, [_P, C.c_uint32]),
+    "sph_batch_replay": (C.c_int, [_P, C.c_uint32]),
+    "sph_batch_get_stats": (C.c_int, [_P, C.c_uint32, C.POINTER(Stats)]),
+    "sph_batch_synchronize": (C.c_int, [_P]),
+    "sph_batch_set_tune": (C.c_int, [_P, _up]),
+    "sph_batch_autotune": (C.c_int, [_P, C.c_char_p, _up, _fp]),
+    "sph_batch_info": (C.c_int, [_P, _up, _up, _up, _up]),
 }
 
 
@@ -302,3 +321,98 @@ class SphSystem:
         _check(lib().sph_get_timing(self._ctx, _f(ms)), "sph_get_timing")
         keys = ["integrate_hash", "sort", "reorder", "density", "force", "step"]
         return dict(zip(keys, map(float, ms)))
+
+    def capture(self, nsteps: int) -> None:
+        """Record nsteps (even) steps into a hipGraph."""
+        _check(lib().sph_capture(self._ctx, nsteps), "sph_capture")
+
+    def replay(self, times: int = 1) -> None:
+        _check(lib().sph_replay(self._ctx, times), "sph_replay")
+
+
+class SphBatch:
+    """Many independent scenes advanced by the same launches (sph_batch_*)."""
+
+    def __init__(self, scene_params: list, device: int = 0, keep_dvel: bool = False):
+        arr = (Params * len(scene_params))(*[p.copy() for p in scene_params])
+        self._b = C.c_void_p()
+        flags = SPH_FLAG_KEEP_DVEL if keep_dvel else 0
+        _check(lib().sph_batch_create(arr, len(scene_params), device, flags, C.byref(self._b)),
+               "sph_batch_create")
+        self.nscenes = len(scene_params)
+        self.counts = [int(p.num_particles) for p in scene_params]
+
+    def close(self) -> None:
+        if self._b:
+            lib().sph_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self, scene: int) -> Params:
+        p = Params()
+        _check(lib().sph_batch_get_params(self._b, scene, C.byref(p)), "sph_batch_get_params")
+        return p
+
+    def set_params(self, scene: int, p: Params) -> None:
+        _check(lib().sph_batch_set_params(self._b, scene, C.byref(p)), "sph_batch_set_params")
+
+    def set_array(self, scene: int, which: int, data: np.ndarray, first: int = 0) -> None:
+        a = np.ascontiguousarray(data, np.float32).reshape(-1, 4)
+        _check(lib().sph_batch_set_array(self._b, scene, which, _f(a), first, a.shape[0]),
+               "sph_batch_set_array")
+
+    def reset(self, scene: int, pos4: np.ndarray, vel4: np.ndarray) -> None:
+        self.set_array(scene, SPH_ARRAY_POSITION, pos4)
+        self.set_array(scene, SPH_ARRAY_VELOCITY, vel4)
+
+    def get_array(self, scene: int, which: int) -> np.ndarray:
+        n = self.counts[scene]
+        shape = (n,) if which in (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE) else (n, 4)
+        out = np.empty(shape, np.float32)
+        _check(lib().sph_batch_get_array(self._b, scene, which, _f(out)), "sph_batch_get_array")
+        return out
+
+    def update(self, nsteps: int = 1) -> None:
+        _check(lib().sph_batch_step(self._b, nsteps), "sph_batch_step")
+
+    def capture(self, nsteps: int) -> None:
+        _check(lib().sph_batch_capture(self._b, nsteps), "sph_batch_capture")
+
+    def replay(self, times: int = 1) -> None:
+        _check(lib().sph_batch_replay(self._b, times), "sph_batch_replay")
+
+    def synchronize(self) -> None:
+        _check(lib().sph_batch_synchronize(self._b), "sph_batch_synchronize")
+
+    def stats(self, scene: int) -> dict:
+        s = Stats()
+        _check(lib().sph_batch_get_stats(self._b, scene, C.byref(s)), "sph_batch_get_stats")
+        return s.as_dict()
+
+    def set_tune(self, tune) -> None:
+        t = np.asarray(list(tune) + [0] * (4 - len(tune)), np.uint32)
+        _check(lib().sph_batch_set_tune(self._b, _u(t)), "sph_batch_set_tune")
+
+    def autotune(self, cache_path: str | None = None) -> tuple[list[int], float]:
+        t = np.zeros(4, np.uint32)
+        ms = np.zeros(1, np.float32)
+        path = cache_path.encode() if cache_path else None
+        _check(lib().sph_batch_autotune(self._b, path, _u(t), _f(ms)), "sph_batch_autotune")
+        return [int(x) for x in t], float(ms[0])
+
+    def info(self) -> dict:
+        v = np.zeros(4, np.uint32)
+        _check(lib().sph_batch_info(self._b, _u(v[0:1]), _u(v[1:2]), _u(v[2:3]), _u(v[3:4])),
+               "sph_batch_info")
+        return dict(nscenes=int(v[0]), particles=int(v[1]), cells=int(v[2]), key_bits=int(v[3]))

@@ -102,3 +102,20 @@ def test_create_rejects_invalid_params_before_touching_a_device():
     p = sph.params_from_scene(sph.scene_default())
     p.bnd_type = 3  # only the box boundary is built
     assert sph.lib().sph_create(C.byref(p), 0, 0, C.byref(ctx)) == 2
+
+
+def test_batch_create_rejects_bad_scenes_and_null():
+    L = sph.lib()
+    good = sph.params_from_scene(sph.scene_default())
+    bad = good.copy()
+    bad.num_particles = 0
+    arr = (sph.Params * 2)(good, bad)
+    out = C.c_void_p()
+    assert L.sph_batch_create(arr, 2, 0, 0, C.byref(out)) == 2
+    assert L.sph_batch_create(None, 2, 0, 0, C.byref(out)) == 1
+    assert L.sph_batch_create(arr, 0, 0, 0, C.byref(out)) == 1
+    assert L.sph_batch_destroy(None) == 0
+    assert L.sph_batch_step(None, 1) == 1
+    assert L.sph_batch_autotune(None, None, None, None) == 1
+    assert L.sph_capture(None, 2) == 1
+    assert L.sph_replay(None, 1) == 1

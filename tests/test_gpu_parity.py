@@ -14,9 +14,11 @@ sensitivities where a subtraction cancels):
             (c = h2 - r2 cancels for neighbours near h); and plainly
             |rho32 - rho64| / rho64 for the bulk fluid (rho64 >= rho0 / 2)
   pressure  |p32 - p64| / (|p|_abs + |dp/drho| S_rho)  (p = k rho - k rho0 cancels)
-  dv        |dv32 - dv64|_inf / S, S = m dt sum_j |c d12| (|pterm|_abs r +
-            |vterm| |v_j - v_i|): the absolute-value evaluation of the pair
-            sum, since pressure and viscosity terms cancel between neighbours
+  dv        |dv32 - dv64|_inf / S, S = m dt sum_j [(A + B) |c d12| +
+            (2A + B) |d12| (h + r)] with A = |pterm|_abs r, B = |vterm| |v_j - v_i|:
+            the absolute-value evaluation of the pair sum (pressure and
+            viscosity terms cancel between neighbours) plus the sensitivity
+            to c = h - r, which cancels for neighbours near h
   position  |x32 - x64|_inf / |x64|_inf after the whole step
 """
 import numpy as np
