@@ -293,6 +293,13 @@ int sph_batch_autotune(sph_batch* batch, const char* cache_path, uint32_t tune_o
 int sph_batch_info(sph_batch* batch, uint32_t* nscenes, uint32_t* total_particles,
                    uint32_t* total_cells, uint32_t* key_bits);
 
+/* ------------------------------------------------------------------------ */
+/* Self-test of the BF16 matrix-core operand layouts the learned surrogate
+ * uses (no counterpart in the reference).  variant 0: A 32x16, B 16x32,
+ * C 32x32; variant 1: A 16x32, B 32x16, C 16x16.  A, B row-major bf16 bit
+ * patterns; C row-major float.  Synchronous, uses device 0.               */
+int sph_mfma_selftest(int variant, const uint16_t* a, const uint16_t* b, float* c);
+
 #ifdef __cplusplus
 }
 #endif

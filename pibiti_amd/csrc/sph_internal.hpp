@@ -132,5 +132,59 @@ hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t s);
 // original index of each scene (scenes are contiguous in original order).
 hipError_t launch_scene_ids(uint32_t n, uint32_t nscenes, const uint32_t* scene_off,
                             const uint32_t* pid, uint32_t* sid, hipStream_t s);
+// grid + cell ranges + density of the current state without integrating.
+hipError_t launch_gather_to_sorted(uint32_t n, const uint32_t* pid, const float* src_orig,
+                                   float* dst_sorted, int record_floats, hipStream_t s);
+
+// ---- learned pair-force surrogate (sph_mlp.hip) --------------------------
+namespace mlp {
+constexpr int kIn = 9;       // edge features
+constexpr int kInPad = 16;   // padded to one MFMA k-block
+constexpr int kHid = 64;
+constexpr int kNode = 3;     // node features
+constexpr int kOut = 3;
+constexpr int kOffW1 = 0;                                // [kHid][kInPad]
+constexpr int kOffB1 = kOffW1 + kHid * kInPad;           // [kHid]
+constexpr int kOffW2 = kOffB1 + kHid;                    // [kHid out][kHid in]
+constexpr int kOffB2 = kOffW2 + kHid * kHid;             // [kHid]
+constexpr int kOffW3 = kOffB2 + kHid;                    // [kOut][kHid + kNode]
+constexpr int kOffB3 = kOffW3 + kOut * (kHid + kNode);   // [kOut]
+constexpr int kNumParams = kOffB3 + kOut;
+constexpr int kPairParams = kOffW3;                      // W1, b1, W2, b2
+}  // namespace mlp
+
+struct MlpNorm {
+    float inv_h, inv_v, inv_rho0, inv_p, inv_nnb, a_scale;
+};
+
+// Device state of one model.
+struct MlpModel {
+    int device = 0;
+    float* params = nullptr;   // FP32 master weights
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* grad = nullptr;
+    __bf16* w1b = nullptr;     // [kHid][kInPad]
+    __bf16* w2b = nullptr;     // [kHid out][kHid in]
+    __bf16* w2tb = nullptr;    // transposed
+    MlpNorm norm{};
+    uint64_t adam_t = 0;
+    // training scratch, sized for `cap` particles
+    uint32_t cap = 0;
+    float *h = nullptr, *out = nullptr, *node = nullptr, *dh = nullptr, *dout = nullptr,
+          *tgt = nullptr, *part_pair = nullptr, *part_node = nullptr, *loss_part = nullptr;
+};
+
+hipError_t mlp_pack(const MlpModel& m, hipStream_t s);  // bf16 copies of W1, W2
+// Pair MLP + node layer on the prepared state: velocity update v += a dt.
+hipError_t mlp_infer(const MlpModel& m, const sph_params& p, const StepBuffers& b, float4* vel_out,
+                     float dt, hipStream_t s);
+// Forward (H, out, node per sorted slot into m's scratch), loss/gradient vs
+// targets given in sorted order (m.tgt), gradient into m.grad.  loss_dev: 1 double.
+hipError_t mlp_gradient(MlpModel& m, const sph_params& p, const StepBuffers& b, double* loss_dev,
+                        hipStream_t s);
+hipError_t mlp_adam(MlpModel& m, float lr, float beta1, float beta2, float eps, hipStream_t s);
+uint32_t mlp_pair_blocks(uint32_t n);
+uint32_t mlp_node_blocks(uint32_t n);
 
 }  // namespace sph

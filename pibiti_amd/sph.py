@@ -178,6 +178,7 @@ SIGNATURES = {
     "sph_batch_set_tune": (C.c_int, [_P, _up]),
     "sph_batch_autotune": (C.c_int, [_P, C.c_char_p, _up, _fp]),
     "sph_batch_info": (C.c_int, [_P, _up, _up, _up, _up]),
+    "sph_mfma_selftest": (C.c_int, [C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), _fp]),
 }
 
 
