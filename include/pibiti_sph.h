@@ -61,14 +61,17 @@ enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1 };
 /* Arrays exchanged with the host.  POSITION/VELOCITY keep the reference's
  * 0/1 flag values of cSPH::setArray (source/SPH/SPH_Util.cpp:59-71) and are
  * float4 (x,y,z,w) per particle; DENSITY/PRESSURE are float per particle;
- * DVEL is the last step's force velocity increment as float4 (w = 0).
+ * DVEL is the last step's force velocity increment as float4 (w = 0);
+ * TARGET is the window-mean pair-force acceleration of a kept training
+ * window (float4, w = 0; sph_learn_get_window only).
  * All arrays are indexed by the ORIGINAL particle index.                   */
 enum {
     SPH_ARRAY_POSITION = 0,
     SPH_ARRAY_VELOCITY = 1,
     SPH_ARRAY_DENSITY = 2,
     SPH_ARRAY_PRESSURE = 3,
-    SPH_ARRAY_DVEL = 4
+    SPH_ARRAY_DVEL = 4,
+    SPH_ARRAY_TARGET = 5
 };
 
 /* Context flags (sph_create). */
@@ -299,6 +302,94 @@ int sph_batch_info(sph_batch* batch, uint32_t* nscenes, uint32_t* total_particle
  * C 32x32; variant 1: A 16x32, B 32x16, C 16x16.  A, B row-major bf16 bit
  * patterns; C row-major float.  Synchronous, uses device 0.               */
 int sph_mfma_selftest(int variant, const uint16_t* a, const uint16_t* b, float* c);
+
+/* ------------------------------------------------------------------------ */
+/* Learned pair-force surrogate (extension; no counterpart in the reference).
+ * For every neighbour pair of particle i (|x_i - x_j| < h, j != i, the
+ * density kernel's pairs) the pair network sees 16 features
+ *   e = [(x_i - x_j) inv_h (3), (v_j - v_i) inv_v (3), |x_i - x_j| inv_h,
+ *        rho_i inv_rho0 - 1, rho_j inv_rho0 - 1, 1, 0 (6)]
+ *   m_ij = relu(W2 relu(W1 e) + b2)       W1: 64 x 16 (column 9 = bias)
+ * summed over the neighbours, H_i = sum_j m_ij.  With the node features
+ *   q_i = [rho_i inv_rho0 - 1, p_i inv_p, nnb_i inv_nnb]
+ * the acceleration is a_i = a_scale (W3 [agg_scale H_i ; q_i] + b3), W3: 3 x 67.
+ * The pair network runs on BF16 matrix cores (BF16 weights and features,
+ * FP32 accumulation); the node layer and all sums are FP32.
+ * Parameter vector (floats): W1 [64][16] at 0, W2 [64][64] at 1024,
+ * b2 [64] at 5120, W3 [3][67] at 5184, b3 [3] at 5385.                     */
+#define SPH_MLP_NUM_PARAMS 5388
+
+typedef struct sph_mlp_norm {
+    float inv_h;      /* relative position unit (1/m)                       */
+    float inv_v;      /* relative velocity unit (s/m)                       */
+    float inv_rho0;   /* 1 / rest density                                   */
+    float inv_p;      /* pressure unit                                      */
+    float inv_nnb;    /* neighbour-count unit                               */
+    float agg_scale;  /* factor on the neighbour sum H                      */
+    float a_scale;    /* output unit (m/s^2)                                */
+    float reserved;
+} sph_mlp_norm;
+
+typedef struct sph_model sph_model;
+
+/* Weights drawn from the seed (He-normal W1, W2; small W3; zero biases). */
+int sph_model_create(int device, const sph_mlp_norm* norm, uint64_t seed, sph_model** out);
+int sph_model_destroy(sph_model* model);
+int sph_model_get_norm(sph_model* model, sph_mlp_norm* out);
+int sph_model_get_weights(sph_model* model, float* out);     /* SPH_MLP_NUM_PARAMS */
+int sph_model_set_weights(sph_model* model, const float* in); /* clears Adam state */
+
+/* Training windows (contexts created with SPH_FLAG_KEEP_DVEL).  Called after
+ * a step, sph_learn_begin keeps the state the step's force kernel saw:
+ * positions, velocities before the force update, densities and cell lists.
+ * The target is the sum of the pair-force velocity increments of that step
+ * and of every later step followed by sph_learn_add, divided by
+ * (steps in the window x time_step): the mean pair-force acceleration over
+ * the window.  Loss: mean over particles of |a / a_scale - target / a_scale|^2
+ * summed over the three components.                                        */
+int sph_learn_begin(sph_ctx* ctx, sph_model* model);
+int sph_learn_add(sph_ctx* ctx, sph_model* model);
+/* iters Adam steps (beta 0.9/0.999, eps 1e-8) on the kept window;
+ * loss_out (may be NULL) receives the loss of the last forward pass.      */
+int sph_learn_train(sph_ctx* ctx, sph_model* model, uint32_t iters, float lr, double* loss_out);
+/* Loss and gradient (SPH_MLP_NUM_PARAMS floats) on the kept window. */
+int sph_learn_gradient(sph_ctx* ctx, sph_model* model, double* loss_out, float* grad_out);
+/* Surrogate acceleration a_i on the kept window (float4, original order). */
+int sph_learn_predict(sph_ctx* ctx, sph_model* model, float* accel4_out);
+/* Arrays of the kept window in original order: POSITION, VELOCITY (before
+ * the force update), DENSITY, PRESSURE, TARGET.                            */
+int sph_learn_get_window(sph_ctx* ctx, int which, float* out);
+
+int sph_batch_learn_begin(sph_batch* batch, sph_model* model);
+int sph_batch_learn_add(sph_batch* batch, sph_model* model);
+int sph_batch_learn_train(sph_batch* batch, sph_model* model, uint32_t iters, float lr,
+                          double* loss_out);
+int sph_batch_learn_gradient(sph_batch* batch, sph_model* model, double* loss_out,
+                             float* grad_out);
+int sph_batch_learn_get_window(sph_batch* batch, uint32_t scene, int which, float* out);
+
+typedef struct sph_learned_report {
+    uint32_t coarse_steps;  /* surrogate steps kept                         */
+    uint32_t fallbacks;     /* surrogate steps undone and replaced          */
+    uint32_t plain_steps;   /* plain steps run in their place               */
+    uint32_t reserved0;
+    double drift_max;       /* largest |D - D_anchor| of a kept step        */
+    double drift_last;      /* |D - D_anchor| of the last surrogate step    */
+    double anchor;          /* D of the last plain step                     */
+    double sim_time;        /* clock after the call                         */
+    double reserved[4];
+} sph_learned_report;
+
+/* Advance ncoarse surrogate steps of factor x time_step each: the plain
+ * integrate (coarse step), sort, reorder and density kernels, then
+ * v += a dt_coarse from the surrogate.  D = mean_i |rho_i / rho0 - 1| of
+ * each surrogate step is compared with D of the last plain step (the
+ * anchor); a step whose drift |D - anchor| exceeds tol is undone and
+ * replaced by `factor` plain steps, which set a new anchor.  The current
+ * state must come from a plain step.  Single-scene contexts with a fixed
+ * time step (cfl = 0, t_end = 0).                                          */
+int sph_step_learned(sph_ctx* ctx, sph_model* model, uint32_t ncoarse, uint32_t factor, float tol,
+                     sph_learned_report* out);
 
 #ifdef __cplusplus
 }

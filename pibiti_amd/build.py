@@ -19,7 +19,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpibiti_sph.so"
 
-HIP_SOURCES = ["sph_kernels.hip", "sph_sort.hip", "sph_capi.hip", "sph_mlp.hip"]
+HIP_SOURCES = ["sph_kernels.hip", "sph_sort.hip", "sph_capi.hip", "sph_mlp.hip", "sph_learn.hip"]
 CPP_SOURCES = ["sph_scene.cpp"]
 
 # -packed-fp32-ops keeps the compiler from emitting packed fp32 ops.

@@ -1,731 +1,8 @@
-// sph_capi.hip - device contexts behind the C ABI of include/pibiti_sph.h.
-//
-// An Engine owns every device buffer of one particle array and the stream
-// its kernels run on.  A single-scene context (sph_ctx) is an Engine with
-// one scene whose parameters travel by value; a batch (sph_batch) is an
-// Engine whose scenes are packed one after the other in original-index
-// order, each with a disjoint cell-key range, parameters in a device table
-// and a per-slot scene id.
-//
-// The Engine replaces what cSPH::_InitMem/_FreeMem (source/SPH/SPH_Mem.cpp:11-82)
-// allocate and what cSPH::Update (source/SPH/SPH_Update.cpp:12-81) launches,
-// without GL buffers and without a device-wide sync after every kernel: a
-// step is a chain of kernels on the engine's own stream, and a run of steps
-// can be recorded once into a hipGraph and replayed.
-#include <algorithm>
-#include <cstdio>
-#include <cstring>
-#include <new>
-#include <string>
-#include <vector>
+// sph_capi.hip - extern "C" entry points of include/pibiti_sph.h for single
+// scenes and batches; the device context is sph::Engine (sph_engine.hpp).
+#include "sph_engine.hpp"
 
-#include "sph_internal.hpp"
-
-namespace {
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
-
-int to_status(hipError_t e) {
-    if (e == hipSuccess) return SPH_OK;
-    if (e == hipErrorOutOfMemory) return SPH_ERR_OUT_OF_MEMORY;
-    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SPH_ERR_NO_DEVICE;
-    return SPH_ERR_DEVICE;
-}
-
-#define SPH_TRY(expr)                               \
-    do {                                            \
-        hipError_t e_ = (expr);                     \
-        if (e_ != hipSuccess) return to_status(e_); \
-    } while (0)
-
-#define SPH_TRY_ST(expr)                 \
-    do {                                 \
-        int st_ = (expr);                \
-        if (st_ != SPH_OK) return st_;   \
-    } while (0)
-
-template <typename T>
-hipError_t dalloc(T** ptr, size_t count) {
-    return hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 16);
-}
-
-// Slot ranges of the sort's 27-bit slot field bound the particle count.
-constexpr uint64_t kMaxParticles = 1ull << 27;
-
-int validate(const sph_params* p) {
-    if (!p) return SPH_ERR_INVALID_ARGUMENT;
-    if (p->num_particles == 0 || p->num_particles > kMaxParticles || p->num_cells == 0 ||
-        p->max_par_in_cell == 0 || !(p->h > 0.0f) || !(p->time_step > 0.0f) ||
-        p->bnd_type != SPH_BND_BOX || p->bnd_eff_z != SPH_BND_EFF_NONE || p->eos > SPH_EOS_TAIT ||
-        p->precision > SPH_PREC_KAHAN || !(p->t_end >= 0.0f))
-        return SPH_ERR_INVALID_PARAMS;
-    if ((uint64_t)p->grid_size[0] * p->grid_size[1] * p->grid_size[2] != p->num_cells ||
-        p->grid_size_yx != p->grid_size[0] * p->grid_size[1])
-        return SPH_ERR_INVALID_PARAMS;
-    return SPH_OK;
-}
-
-uint32_t bits_for(uint64_t cells) {
-    uint32_t b = 0;
-    while ((1ull << b) < cells) ++b;
-    return b;
-}
-
-}  // namespace
-
-namespace sph {
-
-struct Engine {
-    int device = 0;
-    uint32_t flags = 0;
-    bool batch = false;
-    hipStream_t stream = nullptr;
-    LaunchCfg cfg{};
-    std::vector<sph_params> scenes;     // host copies (cell_base assigned)
-    std::vector<uint32_t> scene_off;    // first original index (= slot) of each scene
-    sph_params* d_tab = nullptr;
-    uint32_t* d_scene_off = nullptr;
-
-    float4* pos[2] = {nullptr, nullptr};
-    float4* vel[2] = {nullptr, nullptr};
-    uint32_t* pid[2] = {nullptr, nullptr};
-    uint32_t* sid[2] = {nullptr, nullptr};
-    float2* rhop = nullptr;
-    float4* dvel = nullptr;
-    uint32_t *key_pid = nullptr, *val_pid = nullptr, *key_tmp = nullptr, *val_tmp = nullptr;
-    uint32_t *cell_start = nullptr, *cell_end = nullptr, *counters = nullptr, *cell_max = nullptr;
-    uint32_t* work = nullptr;
-    double *partials = nullptr, *stats = nullptr;
-    float* dt_dev = nullptr;
-    float4* staging = nullptr;
-    void* sort_scratch = nullptr;
-    RadixSortPlan sort{};
-
-    int cur_pos = 0, cur_vel = 0, cur_pid = 0;
-    const uint32_t* last_skey = nullptr;
-    std::vector<uint8_t> have_pos, have_vel;  // per scene: full upload seen
-    bool stepped = false;
-    uint64_t steps = 0;
-    double host_clock = 0.0;
-    bool device_clock = false;  // clock and time step advanced on the device each step
-
-    bool timing = false;
-    hipEvent_t ev[6] = {};
-    float last_ms[6] = {0, 0, 0, 0, 0, 0};
-
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    uint32_t graph_steps = 0;
-
-    StepBuffers buffers() const {
-        StepBuffers b{};
-        b.pos_cur = pos[cur_pos];
-        b.vel_cur = vel[cur_vel];
-        b.pid_cur = pid[cur_pid];
-        b.sid_cur = sid[cur_pid];
-        b.pos_alt = pos[cur_pos ^ 1];
-        b.vel_alt = vel[cur_vel ^ 1];
-        b.pid_alt = pid[cur_pid ^ 1];
-        b.sid_alt = sid[cur_pid ^ 1];
-        b.rhop = rhop;
-        b.dvel = dvel;
-        b.key_pid = key_pid;
-        b.val_pid = val_pid;
-        b.key_tmp = key_tmp;
-        b.val_tmp = val_tmp;
-        b.cell_start = cell_start;
-        b.cell_end = cell_end;
-        b.counters = counters;
-        b.work = work;
-        b.partials = partials;
-        b.stats = stats;
-        b.dt_dev = dt_dev;
-        return b;
-    }
-
-    void drop_graph() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
-        graph_steps = 0;
-    }
-
-    void release() {
-        drop_graph();
-        for (int k = 0; k < 2; ++k) {
-            (void)hipFree(pos[k]);
-            (void)hipFree(vel[k]);
-            (void)hipFree(pid[k]);
-            (void)hipFree(sid[k]);
-        }
-        void* ptrs[] = {rhop,     dvel,     key_pid,  val_pid,    key_tmp, val_tmp,
-                        cell_start, cell_end, counters, cell_max, work,    partials,
-                        stats,    dt_dev,   staging,  sort_scratch, d_tab, d_scene_off};
-        for (void* q : ptrs) (void)hipFree(q);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    // Scenes must be validated; cell_base / key_bits are assigned here.
-    int init(const sph_params* in, uint32_t nscenes, int dev, uint32_t fl, bool as_batch) {
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SPH_ERR_NO_DEVICE;
-        if (dev < 0 || dev >= ndev) return SPH_ERR_NO_DEVICE;
-        device = dev;
-        flags = fl;
-        batch = as_batch;
-        scenes.assign(in, in + nscenes);
-        scene_off.assign(nscenes + 1, 0);
-        uint64_t n = 0, cells = 0;
-        uint32_t max_cap = 0, max_n = 0;
-        for (uint32_t s = 0; s < nscenes; ++s) {
-            scene_off[s] = (uint32_t)n;
-            scenes[s].cell_base = (uint32_t)cells;
-            n += scenes[s].num_particles;
-            cells += scenes[s].num_cells;
-            max_cap = std::max(max_cap, scenes[s].max_par_in_cell);
-            max_n = std::max(max_n, scenes[s].num_particles);
-            if (scenes[s].precision != scenes[0].precision) return SPH_ERR_INVALID_PARAMS;
-            if (scenes[s].cfl > 0.0f || scenes[s].t_end > 0.0f) device_clock = true;
-        }
-        if (n > kMaxParticles || cells >= (1ull << 31)) return SPH_ERR_CAPACITY;
-        scene_off[nscenes] = (uint32_t)n;
-        const uint32_t kb = bits_for(cells);
-        if (!as_batch) {
-            // a single scene keeps its own key width (it may ask for more bits)
-            if ((1ull << scenes[0].key_bits) < cells || scenes[0].key_bits > 31)
-                return SPH_ERR_INVALID_PARAMS;
-        } else {
-            for (auto& s : scenes) s.key_bits = kb;
-        }
-        cfg.n = (uint32_t)n;
-        cfg.num_cells = (uint32_t)cells;
-        cfg.key_bits = as_batch ? kb : scenes[0].key_bits;
-        cfg.nscenes = nscenes;
-        cfg.max_cap = max_cap;
-        cfg.max_scene_n = max_n;
-        cfg.precision = scenes[0].precision;
-        for (int k = 0; k < 4; ++k) cfg.tune[k] = scenes[0].tune[k];
-        cfg.one = scenes[0];
-        have_pos.assign(nscenes, 0);
-        have_vel.assign(nscenes, 0);
-        batch = as_batch;
-        device_clock = device_clock || as_batch;
-
-        DeviceGuard guard(device);
-        const size_t N = n, NC = cells;
-        hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-            e = dalloc(&pos[k], N);
-            if (e == hipSuccess) e = dalloc(&vel[k], N);
-            if (e == hipSuccess) e = dalloc(&pid[k], N);
-            if (e == hipSuccess && as_batch) e = dalloc(&sid[k], N);
-        }
-        if (e == hipSuccess) e = dalloc(&rhop, N);
-        if (e == hipSuccess && (flags & SPH_FLAG_KEEP_DVEL)) e = dalloc(&dvel, N);
-        if (e == hipSuccess) e = dalloc(&key_pid, N);
-        if (e == hipSuccess) e = dalloc(&val_pid, N);
-        if (e == hipSuccess) e = dalloc(&key_tmp, N);
-        if (e == hipSuccess) e = dalloc(&val_tmp, N);
-        if (e == hipSuccess) e = dalloc(&cell_start, NC);
-        if (e == hipSuccess) e = dalloc(&cell_end, NC);
-        if (e == hipSuccess) e = dalloc(&counters, (size_t)kCntNum);
-        if (e == hipSuccess) e = dalloc(&cell_max, (size_t)1);
-        if (e == hipSuccess) e = dalloc(&work, (size_t)4);
-        if (e == hipSuccess) e = dalloc(&partials, stats_partials_count(cfg));
-        if (e == hipSuccess) e = dalloc(&stats, (size_t)kNumStats * nscenes);
-        if (e == hipSuccess) e = dalloc(&dt_dev, (size_t)nscenes);
-        if (e == hipSuccess) e = dalloc(&staging, N);
-        if (e == hipSuccess) e = dalloc(&d_scene_off, (size_t)nscenes + 1);
-        if (e == hipSuccess && as_batch) e = dalloc(&d_tab, (size_t)nscenes);
-        if (e == hipSuccess) e = hipMalloc(&sort_scratch, radix_scratch_bytes((uint32_t)N));
-        if (e == hipSuccess) e = radix_plan_init(sort, (uint32_t)N, sort_scratch, stream);
-        for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(d_scene_off, scene_off.data(), (nscenes + 1) * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess && as_batch)
-            e = hipMemcpyAsync(d_tab, scenes.data(), nscenes * sizeof(sph_params),
-                               hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = launch_iota_u32(pid[0], (uint32_t)N, stream);
-        if (e == hipSuccess && as_batch)
-            e = launch_scene_ids((uint32_t)N, nscenes, d_scene_off, pid[0], sid[0], stream);
-        if (e == hipSuccess) e = launch_fill_u32(cell_start, kEmptyCell, NC, stream);
-        if (e == hipSuccess) e = launch_fill_u32(cell_end, kEmptyCell, NC, stream);
-        if (e == hipSuccess) e = launch_fill_u32(counters, 0u, kCntNum, stream);
-        if (e == hipSuccess) e = launch_fill_u32(work, 0u, 4, stream);
-        if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(rhop), 0u, 2 * N, stream);
-        if (e == hipSuccess)
-            e = launch_fill_u32(reinterpret_cast<uint32_t*>(stats), 0u, 2 * (size_t)kNumStats * nscenes,
-                                stream);
-        if (e == hipSuccess && dvel)
-            e = launch_fill_u32(reinterpret_cast<uint32_t*>(dvel), 0u, 4 * N, stream);
-        if (e == hipSuccess) {
-            std::vector<float> dts(nscenes);
-            for (uint32_t s = 0; s < nscenes; ++s) dts[s] = scenes[s].time_step;
-            e = hipMemcpyAsync(dt_dev, dts.data(), nscenes * sizeof(float), hipMemcpyHostToDevice,
-                               stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        }
-        return to_status(e);
-    }
-
-    LaunchCfg launch_cfg() const {
-        LaunchCfg c = cfg;
-        c.tab = batch ? d_tab : nullptr;
-        c.scene_off = d_scene_off;
-        return c;
-    }
-
-    // One step's kernels (timed: record events between the stages).
-    int enqueue_step(bool timed) {
-        const LaunchCfg c = launch_cfg();
-        const bool keep = (flags & SPH_FLAG_KEEP_DVEL) != 0;
-        if (timed) SPH_TRY(hipEventRecord(ev[0], stream));
-        if (device_clock) SPH_TRY(launch_stats(c, buffers(), true, stream));
-        SPH_TRY(launch_fill_u32(counters, 0u, kCntNum, stream));
-        SPH_TRY(launch_integrate_hash(c, buffers(), stream));
-        if (timed) SPH_TRY(hipEventRecord(ev[1], stream));
-        uint32_t *skey = nullptr, *oslot = nullptr;
-        SPH_TRY(radix_sort_pairs(sort, c.key_bits, key_pid, val_pid, key_tmp, val_tmp, &skey, &oslot,
-                                 stream));
-        if (timed) SPH_TRY(hipEventRecord(ev[2], stream));
-        SPH_TRY(launch_reorder(c, buffers(), skey, oslot, stream));
-        cur_pos ^= 1;
-        cur_vel ^= 1;
-        cur_pid ^= 1;
-        last_skey = skey;
-        if (timed) SPH_TRY(hipEventRecord(ev[3], stream));
-        SPH_TRY(launch_density(c, buffers(), stream));
-        if (timed) SPH_TRY(hipEventRecord(ev[4], stream));
-        StepBuffers b = buffers();
-        SPH_TRY(launch_force(c, b, b.vel_alt, keep, stream));
-        cur_vel ^= 1;
-        if (timed) SPH_TRY(hipEventRecord(ev[5], stream));
-        return SPH_OK;
-    }
-
-    bool ready() const {
-        for (size_t s = 0; s < have_pos.size(); ++s)
-            if (!have_pos[s] || !have_vel[s]) return false;
-        return true;
-    }
-
-    void after_steps(uint32_t k) {
-        steps += k;
-        if (!device_clock) host_clock += (double)scenes[0].time_step * k;
-        stepped = true;
-    }
-
-    int step(uint32_t nsteps) {
-        if (!ready()) return SPH_ERR_NOT_INITIALISED;
-        DeviceGuard guard(device);
-        for (uint32_t s = 0; s < nsteps; ++s) {
-            const bool tm = timing && s + 1 == nsteps;
-            SPH_TRY_ST(enqueue_step(tm));
-            after_steps(1);
-            if (tm) {
-                SPH_TRY(hipEventSynchronize(ev[5]));
-                for (int k = 0; k < 5; ++k) SPH_TRY(hipEventElapsedTime(&last_ms[k], ev[k], ev[k + 1]));
-                SPH_TRY(hipEventElapsedTime(&last_ms[5], ev[0], ev[5]));
-            }
-        }
-        return SPH_OK;
-    }
-
-    // Record `nsteps` (even, so the ping-pong buffers end where they began)
-    // steps into a graph.
-    int capture(uint32_t nsteps) {
-        if (nsteps == 0 || (nsteps & 1u)) return SPH_ERR_INVALID_ARGUMENT;
-        if (!ready()) return SPH_ERR_NOT_INITIALISED;
-        DeviceGuard guard(device);
-        drop_graph();
-        SPH_TRY(hipStreamSynchronize(stream));
-        const int p0 = cur_pos, v0 = cur_vel, q0 = cur_pid;
-        SPH_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int st = SPH_OK;
-        for (uint32_t s = 0; s < nsteps && st == SPH_OK; ++s) st = enqueue_step(false);
-        hipGraph_t g = nullptr;
-        const hipError_t ee = hipStreamEndCapture(stream, &g);
-        cur_pos = p0;
-        cur_vel = v0;
-        cur_pid = q0;
-        if (st != SPH_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return st;
-        }
-        SPH_TRY(ee);
-        graph = g;
-        SPH_TRY(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-        graph_steps = nsteps;
-        return SPH_OK;
-    }
-
-    int replay(uint32_t times) {
-        if (!graph_exec) return SPH_ERR_NOT_INITIALISED;
-        DeviceGuard guard(device);
-        for (uint32_t t = 0; t < times; ++t) {
-            SPH_TRY(hipGraphLaunch(graph_exec, stream));
-            after_steps(graph_steps);
-        }
-        return SPH_OK;
-    }
-
-    int set_array(uint32_t scene, int which, const float* data, uint32_t first, uint32_t count) {
-        if (!data || scene >= cfg.nscenes) return SPH_ERR_INVALID_ARGUMENT;
-        if (which != SPH_ARRAY_POSITION && which != SPH_ARRAY_VELOCITY) return SPH_ERR_INVALID_ARGUMENT;
-        const uint32_t ns = scenes[scene].num_particles;
-        if (first > ns || count > ns - first) return SPH_ERR_INVALID_ARGUMENT;
-        if (count == 0) return SPH_OK;
-        DeviceGuard guard(device);
-        SPH_TRY(hipMemcpyAsync(staging, data, (size_t)count * sizeof(float4), hipMemcpyHostToDevice,
-                               stream));
-        float4* dst = which == SPH_ARRAY_POSITION ? pos[cur_pos] : vel[cur_vel];
-        SPH_TRY(launch_update_from_original(cfg.n, pid[cur_pid], staging, scene_off[scene] + first,
-                                            count, dst, stream));
-        SPH_TRY(hipStreamSynchronize(stream));
-        if (first == 0 && count == ns) (which == SPH_ARRAY_POSITION ? have_pos : have_vel)[scene] = 1;
-        return SPH_OK;
-    }
-
-    int get_array(uint32_t scene, int which, float* out) {
-        if (!out || scene >= cfg.nscenes) return SPH_ERR_INVALID_ARGUMENT;
-        DeviceGuard guard(device);
-        const uint32_t n = cfg.n;
-        const uint32_t* p = pid[cur_pid];
-        int rec = 4;
-        switch (which) {
-            case SPH_ARRAY_POSITION:
-                SPH_TRY(launch_scatter_to_original(n, p, pos[cur_pos], staging, 4, 4, stream));
-                break;
-            case SPH_ARRAY_VELOCITY:
-                SPH_TRY(launch_scatter_to_original(n, p, vel[cur_vel], staging, 4, 4, stream));
-                break;
-            case SPH_ARRAY_DENSITY:
-                SPH_TRY(launch_scatter_to_original(n, p, rhop, staging, 1, 2, stream));
-                rec = 1;
-                break;
-            case SPH_ARRAY_PRESSURE:
-                SPH_TRY(launch_scatter_to_original(n, p, reinterpret_cast<float*>(rhop) + 1, staging, 1,
-                                                   2, stream));
-                rec = 1;
-                break;
-            case SPH_ARRAY_DVEL:
-                if (!dvel) return SPH_ERR_UNSUPPORTED;
-                SPH_TRY(launch_scatter_to_original(n, p, dvel, staging, 4, 4, stream));
-                break;
-            default:
-                return SPH_ERR_INVALID_ARGUMENT;
-        }
-        const float* src = reinterpret_cast<const float*>(staging) + (size_t)scene_off[scene] * rec;
-        SPH_TRY(hipMemcpyAsync(out, src, (size_t)scenes[scene].num_particles * rec * sizeof(float),
-                               hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipStreamSynchronize(stream));
-        return SPH_OK;
-    }
-
-    int get_stats(uint32_t scene, sph_stats* out) {
-        if (!out || scene >= cfg.nscenes) return SPH_ERR_INVALID_ARGUMENT;
-        DeviceGuard guard(device);
-        std::memset(out, 0, sizeof(*out));
-        const uint32_t ns = cfg.nscenes;
-        std::vector<double> keep(2 * (size_t)ns), st((size_t)kNumStats * ns);
-        std::vector<float> dts(ns);
-        // keep each scene's time step and clock: recompute only the reductions
-        for (uint32_t s = 0; s < ns; ++s)
-            SPH_TRY(hipMemcpyAsync(&keep[2 * s], stats + (size_t)s * kNumStats + kSDt, 2 * sizeof(double),
-                                   hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipMemcpyAsync(dts.data(), dt_dev, ns * sizeof(float), hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipStreamSynchronize(stream));
-        const LaunchCfg c = launch_cfg();
-        SPH_TRY(launch_stats(c, buffers(), false, stream));
-        SPH_TRY(launch_fill_u32(cell_max, 0u, 1, stream));
-        if (stepped) SPH_TRY(launch_cell_stats(c, buffers(), cell_max, stream));
-        uint32_t cnt[kCntNum], cmax = 0, sort_err = 0;
-        SPH_TRY(hipMemcpyAsync(st.data(), stats, st.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipMemcpyAsync(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipMemcpyAsync(&cmax, cell_max, sizeof(cmax), hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipMemcpyAsync(&sort_err, sort.err, sizeof(sort_err), hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipStreamSynchronize(stream));
-        // the next step's time step (as the next step will compute it) for reporting
-        const double dt_next = st[(size_t)scene * kNumStats + kSDt];
-        for (uint32_t s = 0; s < ns; ++s)
-            SPH_TRY(hipMemcpyAsync(stats + (size_t)s * kNumStats + kSDt, &keep[2 * s], 2 * sizeof(double),
-                                   hipMemcpyHostToDevice, stream));
-        SPH_TRY(hipMemcpyAsync(dt_dev, dts.data(), ns * sizeof(float), hipMemcpyHostToDevice, stream));
-        SPH_TRY(hipStreamSynchronize(stream));
-        const double* q = &st[(size_t)scene * kNumStats];
-        out->mass = q[kPMass];
-        out->kinetic_energy = q[kPKin];
-        out->potential_energy = q[kPPot];
-        out->max_speed = q[kPVmax];
-        out->dt_cfl = dt_next;
-        out->sim_time = device_clock ? keep[2 * scene + 1] : host_clock;
-        out->steps = steps;
-        out->cap_cells = cnt[kCntCapCells];
-        out->out_of_grid = cnt[kCntOutOfGrid];
-        out->max_cell_count = cmax;
-        out->sort_errors = sort_err;
-        return SPH_OK;
-    }
-
-    int get_grid(uint32_t* sorted_hash, uint32_t* perm, uint32_t* cs, uint32_t* ce) {
-        if (!stepped) return SPH_ERR_NOT_INITIALISED;
-        DeviceGuard guard(device);
-        const size_t n = cfg.n, ncell = cfg.num_cells;
-        if (sorted_hash) SPH_TRY(hipMemcpyAsync(sorted_hash, last_skey, n * 4, hipMemcpyDeviceToHost, stream));
-        if (perm) SPH_TRY(hipMemcpyAsync(perm, pid[cur_pid], n * 4, hipMemcpyDeviceToHost, stream));
-        std::vector<uint32_t> tmp;
-        uint32_t* csp = cs;
-        if (ce && !cs) {
-            tmp.resize(ncell);
-            csp = tmp.data();
-        }
-        if (csp) SPH_TRY(hipMemcpyAsync(csp, cell_start, ncell * 4, hipMemcpyDeviceToHost, stream));
-        if (ce) SPH_TRY(hipMemcpyAsync(ce, cell_end, ncell * 4, hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipStreamSynchronize(stream));
-        // cellEnd is only maintained for occupied cells; report empty ones as
-        // the reference marks an empty cellStart.
-        if (ce)
-            for (size_t k = 0; k < ncell; ++k)
-                if (csp[k] == kEmptyCell) ce[k] = kEmptyCell;
-        return SPH_OK;
-    }
-
-    int set_tune(const uint32_t t[4]) {
-        for (int k = 0; k < 4; ++k) cfg.tune[k] = t[k];
-        for (auto& s : scenes)
-            for (int k = 0; k < 4; ++k) s.tune[k] = t[k];
-        cfg.one = scenes[0];
-        drop_graph();
-        return SPH_OK;
-    }
-
-    int set_params(uint32_t scene, const sph_params* p) {
-        int st = validate(p);
-        if (st != SPH_OK) return st;
-        sph_params& cur = scenes[scene];
-        if (p->num_particles != cur.num_particles || p->num_cells != cur.num_cells ||
-            p->grid_size[0] != cur.grid_size[0] || p->grid_size[1] != cur.grid_size[1] ||
-            p->grid_size[2] != cur.grid_size[2] || (cfg.nscenes > 1 && p->precision != cfg.precision))
-            return SPH_ERR_INVALID_PARAMS;
-        if (!batch && ((1ull << p->key_bits) < cfg.num_cells || p->key_bits > 31))
-            return SPH_ERR_INVALID_PARAMS;
-        DeviceGuard guard(device);
-        if ((p->cfl > 0.0f || p->t_end > 0.0f) && !device_clock) {
-            // from here on the clock and time step advance on the device
-            SPH_TRY(hipMemcpyAsync(stats + kSClock, &host_clock, sizeof(double), hipMemcpyHostToDevice,
-                                   stream));
-            device_clock = true;
-        }
-        const uint32_t base = cur.cell_base, kb = cur.key_bits;
-        cur = *p;
-        cur.cell_base = base;
-        if (batch) cur.key_bits = kb;
-        if (!batch) cfg.key_bits = cur.key_bits;
-        cfg.precision = cur.precision;
-        cfg.max_cap = 0;
-        for (auto& s : scenes) cfg.max_cap = std::max(cfg.max_cap, s.max_par_in_cell);
-        for (int k = 0; k < 4; ++k) cfg.tune[k] = scenes[0].tune[k];
-        cfg.one = scenes[0];
-        if (batch)
-            SPH_TRY(hipMemcpyAsync(d_tab + scene, &cur, sizeof(sph_params), hipMemcpyHostToDevice, stream));
-        SPH_TRY(hipMemcpyAsync(dt_dev + scene, &cur.time_step, sizeof(float), hipMemcpyHostToDevice, stream));
-        SPH_TRY(hipStreamSynchronize(stream));
-        drop_graph();
-        return SPH_OK;
-    }
-
-    // ---- autotuner --------------------------------------------------------
-    std::string tune_key() const {
-        hipDeviceProp_t prop{};
-        (void)hipGetDeviceProperties(&prop, device);
-        char buf[512];
-        std::snprintf(buf, sizeof(buf), "%s|cu%d|n%u|s%u|c%u|cap%u|p%u|nb%u", prop.gcnArchName,
-                      prop.multiProcessorCount, cfg.n, cfg.nscenes, cfg.num_cells, cfg.max_cap,
-                      cfg.precision, cfg.max_scene_n);
-        return buf;
-    }
-
-    static bool cache_lookup(const char* path, const std::string& key, uint32_t t[4], float* ms) {
-        if (!path) return false;
-        FILE* f = std::fopen(path, "r");
-        if (!f) return false;
-        char line[1024];
-        bool found = false;
-        while (std::fgets(line, sizeof(line), f)) {
-            char k[512];
-            unsigned a = 0, b = 0;
-            float m = 0;
-            if (std::sscanf(line, "{\"key\": \"%511[^\"]\", \"tune0\": %u, \"tune1\": %u, \"ms\": %f}", k, &a,
-                            &b, &m) == 4 &&
-                key == k) {
-                t[0] = a;
-                t[1] = b;
-                t[2] = t[3] = 0;
-                if (ms) *ms = m;
-                found = true;  // the last matching line wins
-            }
-        }
-        std::fclose(f);
-        return found;
-    }
-
-    static void cache_store(const char* path, const std::string& key, const uint32_t t[4], float ms) {
-        if (!path) return;
-        FILE* f = std::fopen(path, "a");
-        if (!f) return;
-        std::fprintf(f, "{\"key\": \"%s\", \"tune0\": %u, \"tune1\": %u, \"ms\": %.6f}\n", key.c_str(), t[0],
-                     t[1], ms);
-        std::fclose(f);
-    }
-
-    // Time each candidate launch configuration over a few steps of the live
-    // state, restore the state, keep the fastest.  Every candidate computes
-    // identical results, so the choice only moves time.
-    int autotune(const char* cache_path, uint32_t out[4], float* best_ms_out) {
-        if (!ready()) return SPH_ERR_NOT_INITIALISED;
-        const std::string key = tune_key();
-        uint32_t t[4] = {0, 0, 0, 0};
-        float ms = 0.0f;
-        if (cache_lookup(cache_path, key, t, &ms)) {
-            set_tune(t);
-            for (int k = 0; k < 4; ++k) out[k] = t[k];
-            if (best_ms_out) *best_ms_out = ms;
-            return SPH_OK;
-        }
-        DeviceGuard guard(device);
-        hipDeviceProp_t prop{};
-        SPH_TRY(hipGetDeviceProperties(&prop, device));
-        const uint32_t cus = (uint32_t)std::max(1, prop.multiProcessorCount);
-        // snapshot of everything a step reads or writes that outlives it
-        const size_t N = cfg.n;
-        float4 *sp = nullptr, *sv = nullptr;
-        uint32_t *spid = nullptr, *ssid = nullptr;
-        double* sst = nullptr;
-        float* sdt = nullptr;
-        hipError_t e = dalloc(&sp, N);
-        if (e == hipSuccess) e = dalloc(&sv, N);
-        if (e == hipSuccess) e = dalloc(&spid, N);
-        if (e == hipSuccess && batch) e = dalloc(&ssid, N);
-        if (e == hipSuccess) e = dalloc(&sst, (size_t)kNumStats * cfg.nscenes);
-        if (e == hipSuccess) e = dalloc(&sdt, (size_t)cfg.nscenes);
-        auto cleanup = [&]() {
-            (void)hipFree(sp);
-            (void)hipFree(sv);
-            (void)hipFree(spid);
-            (void)hipFree(ssid);
-            (void)hipFree(sst);
-            (void)hipFree(sdt);
-        };
-        if (e != hipSuccess) {
-            cleanup();
-            return to_status(e);
-        }
-        const int p0 = cur_pos, v0 = cur_vel, q0 = cur_pid;
-        const uint64_t steps0 = steps;
-        const double clock0 = host_clock;
-        const bool stepped0 = stepped;
-        auto snap = [&](bool save) -> hipError_t {
-            hipError_t r = hipSuccess;
-            const hipMemcpyKind k = hipMemcpyDeviceToDevice;
-            float4* P = pos[cur_pos];
-            float4* V = vel[cur_vel];
-            uint32_t* Q = pid[cur_pid];
-            uint32_t* S = sid[cur_pid];
-            if (r == hipSuccess) r = hipMemcpyAsync(save ? sp : P, save ? P : sp, N * 16, k, stream);
-            if (r == hipSuccess) r = hipMemcpyAsync(save ? sv : V, save ? V : sv, N * 16, k, stream);
-            if (r == hipSuccess) r = hipMemcpyAsync(save ? spid : Q, save ? Q : spid, N * 4, k, stream);
-            if (r == hipSuccess && batch) r = hipMemcpyAsync(save ? ssid : S, save ? S : ssid, N * 4, k, stream);
-            if (r == hipSuccess)
-                r = hipMemcpyAsync(save ? sst : stats, save ? stats : sst,
-                                   (size_t)kNumStats * cfg.nscenes * 8, k, stream);
-            if (r == hipSuccess)
-                r = hipMemcpyAsync(save ? sdt : dt_dev, save ? dt_dev : sdt, (size_t)cfg.nscenes * 4, k,
-                                   stream);
-            return r;
-        };
-        e = snap(true);
-        const uint32_t lists[] = {16, 24, 32};
-        const uint32_t grids[] = {0, 2 * cus, 4 * cus, 8 * cus};
-        float best = 1e30f;
-        uint32_t best_t[4] = {cfg.tune[0], cfg.tune[1], 0, 0};
-        const uint32_t saved_tune[4] = {cfg.tune[0], cfg.tune[1], cfg.tune[2], cfg.tune[3]};
-        int st = SPH_OK;
-        for (uint32_t li : lists) {
-            if (cfg.max_cap > li) continue;
-            for (uint32_t g : grids) {
-                if (e != hipSuccess || st != SPH_OK) break;
-                const uint32_t cand[4] = {li, g, 0, 0};
-                set_tune(cand);
-                cur_pos = p0;
-                cur_vel = v0;
-                cur_pid = q0;
-                e = snap(false);
-                if (e != hipSuccess) break;
-                st = enqueue_step(false);  // warm-up
-                if (st != SPH_OK) break;
-                e = hipEventRecord(ev[0], stream);
-                const int reps = 4;
-                for (int r = 0; r < reps && st == SPH_OK; ++r) st = enqueue_step(false);
-                if (e == hipSuccess) e = hipEventRecord(ev[5], stream);
-                if (e == hipSuccess) e = hipEventSynchronize(ev[5]);
-                float m = 0.0f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&m, ev[0], ev[5]);
-                m /= reps;
-                if (e == hipSuccess && st == SPH_OK && m < best) {
-                    best = m;
-                    best_t[0] = li;
-                    best_t[1] = g;
-                }
-            }
-        }
-        cur_pos = p0;
-        cur_vel = v0;
-        cur_pid = q0;
-        if (e == hipSuccess) e = snap(false);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        cleanup();
-        steps = steps0;
-        host_clock = clock0;
-        stepped = stepped0;
-        if (st != SPH_OK) {
-            set_tune(saved_tune);
-            return st;
-        }
-        if (e != hipSuccess) {
-            set_tune(saved_tune);
-            return to_status(e);
-        }
-        set_tune(best_t);
-        cache_store(cache_path, key, best_t, best);
-        for (int k = 0; k < 4; ++k) out[k] = best_t[k];
-        if (best_ms_out) *best_ms_out = best;
-        return SPH_OK;
-    }
-};
-
-}  // namespace sph
-
-struct sph_ctx {
-    sph::Engine eng;
-};
-
-struct sph_batch {
-    sph::Engine eng;
-};
+using namespace sph;
 
 extern "C" {
 

@@ -11,11 +11,14 @@
 namespace sph {
 
 constexpr uint32_t kEmptyCell = 0xFFFFFFFFu;
+// Neighbour-list capacity per particle (learned surrogate).
+constexpr uint32_t kNbrCap = 128;
 
 // Counters written by the step kernels (device memory, uint32 each).
 enum Counter : int {
     kCntCapCells = 0,   // cells with more than max_par_in_cell particles
     kCntOutOfGrid = 1,  // particles whose own cell index was clamped
+    kCntNbrOverflow = 2,  // particles with more than kNbrCap listed neighbours
     kCntNum = 4
 };
 
@@ -109,6 +112,10 @@ hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipSt
 hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
                           const uint32_t* old_slot, hipStream_t s);
 hipError_t launch_density(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
+// Density plus each slot's neighbour list: nbr[c * n + k] (c < kNbrCap),
+// nnb[k] = neighbour count (may exceed kNbrCap: counted in kCntNbrOverflow).
+hipError_t launch_density_lists(const LaunchCfg& c, const StepBuffers& b, uint32_t* nbr, uint32_t* nnb,
+                                hipStream_t s);
 hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, bool keep_dvel,
                         hipStream_t s);
 // FP64 per-scene reductions of (mass, kinetic, potential, max speed) over
@@ -132,59 +139,4 @@ hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t s);
 // original index of each scene (scenes are contiguous in original order).
 hipError_t launch_scene_ids(uint32_t n, uint32_t nscenes, const uint32_t* scene_off,
                             const uint32_t* pid, uint32_t* sid, hipStream_t s);
-// grid + cell ranges + density of the current state without integrating.
-hipError_t launch_gather_to_sorted(uint32_t n, const uint32_t* pid, const float* src_orig,
-                                   float* dst_sorted, int record_floats, hipStream_t s);
-
-// ---- learned pair-force surrogate (sph_mlp.hip) --------------------------
-namespace mlp {
-constexpr int kIn = 9;       // edge features
-constexpr int kInPad = 16;   // padded to one MFMA k-block
-constexpr int kHid = 64;
-constexpr int kNode = 3;     // node features
-constexpr int kOut = 3;
-constexpr int kOffW1 = 0;                                // [kHid][kInPad]
-constexpr int kOffB1 = kOffW1 + kHid * kInPad;           // [kHid]
-constexpr int kOffW2 = kOffB1 + kHid;                    // [kHid out][kHid in]
-constexpr int kOffB2 = kOffW2 + kHid * kHid;             // [kHid]
-constexpr int kOffW3 = kOffB2 + kHid;                    // [kOut][kHid + kNode]
-constexpr int kOffB3 = kOffW3 + kOut * (kHid + kNode);   // [kOut]
-constexpr int kNumParams = kOffB3 + kOut;
-constexpr int kPairParams = kOffW3;                      // W1, b1, W2, b2
-}  // namespace mlp
-
-struct MlpNorm {
-    float inv_h, inv_v, inv_rho0, inv_p, inv_nnb, a_scale;
-};
-
-// Device state of one model.
-struct MlpModel {
-    int device = 0;
-    float* params = nullptr;   // FP32 master weights
-    float* adam_m = nullptr;
-    float* adam_v = nullptr;
-    float* grad = nullptr;
-    __bf16* w1b = nullptr;     // [kHid][kInPad]
-    __bf16* w2b = nullptr;     // [kHid out][kHid in]
-    __bf16* w2tb = nullptr;    // transposed
-    MlpNorm norm{};
-    uint64_t adam_t = 0;
-    // training scratch, sized for `cap` particles
-    uint32_t cap = 0;
-    float *h = nullptr, *out = nullptr, *node = nullptr, *dh = nullptr, *dout = nullptr,
-          *tgt = nullptr, *part_pair = nullptr, *part_node = nullptr, *loss_part = nullptr;
-};
-
-hipError_t mlp_pack(const MlpModel& m, hipStream_t s);  // bf16 copies of W1, W2
-// Pair MLP + node layer on the prepared state: velocity update v += a dt.
-hipError_t mlp_infer(const MlpModel& m, const sph_params& p, const StepBuffers& b, float4* vel_out,
-                     float dt, hipStream_t s);
-// Forward (H, out, node per sorted slot into m's scratch), loss/gradient vs
-// targets given in sorted order (m.tgt), gradient into m.grad.  loss_dev: 1 double.
-hipError_t mlp_gradient(MlpModel& m, const sph_params& p, const StepBuffers& b, double* loss_dev,
-                        hipStream_t s);
-hipError_t mlp_adam(MlpModel& m, float lr, float beta1, float beta2, float eps, hipStream_t s);
-uint32_t mlp_pair_blocks(uint32_t n);
-uint32_t mlp_node_blocks(uint32_t n);
-
 }  // namespace sph

@@ -25,29 +25,15 @@
 //   computeDensityD ................. source/CUDA/Kernel_Cell.cui:142-199
 //   computeForceD ................... source/CUDA/Kernel_Cell.cui:210-261,
 //                                     source/CUDA/System.cu:228-250,402
-#include "sph_internal.hpp"
+#include "sph_device.hpp"
 
 namespace sph {
 namespace {
 
+using namespace dev;
+
 constexpr int kBlock = 256;
 constexpr float kBndEps = 0.00001f;  // EPS of boundary()
-
-// Parameters of the scene a slot belongs to: the by-value parameters of a
-// one-scene launch, or the batch table entry of the slot's scene.
-template <bool kBatch>
-__device__ __forceinline__ const sph_params& scene_params(const sph_params& one,
-                                                         const sph_params* __restrict__ tab,
-                                                         const uint32_t* __restrict__ sid,
-                                                         uint32_t slot, uint32_t& scene) {
-    if constexpr (kBatch) {
-        scene = sid[slot];
-        return tab[scene];
-    } else {
-        scene = 0;
-        return one;
-    }
-}
 
 __device__ __forceinline__ void bnd_wall(float diff, float nx, float ny, float nz, float stiff,
                                          float damp, float dt, float& vx, float& vy, float& vz) {
@@ -59,43 +45,6 @@ __device__ __forceinline__ void bnd_wall(float diff, float nx, float ny, float n
         vz = vz + (acc * nz) * dt;
     }
 }
-
-// Cell coordinate of one axis: floor((x - min)/cell), clamped into the grid.
-__device__ __forceinline__ int cell_coord(float x, float wmin, float cs, uint32_t gsize, bool& out) {
-    float f = floorf((x - wmin) / cs);
-    float hi = (float)(gsize - 1);
-    if (!(f >= 0.0f && f <= hi)) {
-        out = true;
-        f = (f != f) ? 0.0f : fminf(fmaxf(f, 0.0f), hi);
-    }
-    return (int)f;
-}
-
-__device__ __forceinline__ uint32_t grid_hash(const sph_params& p, int gx, int gy, int gz) {
-    return p.cell_base + (uint32_t)gz * p.grid_size_yx + (uint32_t)gy * p.grid_size[0] + (uint32_t)gx;
-}
-
-__device__ __forceinline__ float eos(const sph_params& p, float rho) {
-    if (p.eos == SPH_EOS_TAIT) {
-        float x = rho / p.rest_density;
-        float x2 = x * x;
-        float x4 = x2 * x2;
-        float x7 = (x4 * x2) * x;
-        float bcoef = (p.stiffness * p.rest_density) / 7.0f;
-        return bcoef * (x7 - 1.0f);
-    }
-    return (rho - p.rest_density) * p.stiffness;
-}
-
-struct KahanF {
-    float s, c;
-    __device__ __forceinline__ void add(float v) {
-        float y = v - c;
-        float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-};
 
 // ------------------------------------------------------------ integrate+hash
 template <bool kBatch>
@@ -193,79 +142,22 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     }
 }
 
-// ------------------------------------------------------- neighbour-cell walk
-// The 27 cells around a particle's cell in the reference's (dz, dy, dx)
-// order (source/CUDA/Kernel_Cell.cui:188-192).  A cell whose box is farther
-// than h from the particle is skipped: none of its particles can pass
-// r2 < h2 (or the force predicate, which is stricter), so its partial sum
-// would be +0 and adding +0 leaves the total unchanged.  The test is
-// conservative: each face distance is shrunk by a fuzz covering the
-// rounding of cell assignment, and the bound carries a 1e-5 relative margin.
-struct AxisWalk {
-    int g;           // own cell coordinate
-    int gmax;        // grid size - 1
-    float lo2, hi2;  // squared distance to the own cell's lower/upper face (minus fuzz)
-
-    __device__ __forceinline__ void init(float x, float wl, float wh, float cs, uint32_t gsize) {
-        bool out = false;
-        g = cell_coord(x, wl, cs, gsize, out);
-        gmax = (int)gsize - 1;
-        const float fuzz = 1e-6f * fmaxf(fabsf(wl), fabsf(wh)) + 1e-30f;
-        const float clo = wl + (float)g * cs;
-        const float chi = clo + cs;
-        const float lo = fmaxf(x - clo - fuzz, 0.0f);
-        const float hi = fmaxf(chi - x - fuzz, 0.0f);
-        lo2 = lo * lo;
-        hi2 = hi * hi;
-    }
-    // squared distance component of a neighbour offset d in {-1, 0, 1}
-    __device__ __forceinline__ float d2(int d) const { return d < 0 ? lo2 : (d > 0 ? hi2 : 0.0f); }
-    __device__ __forceinline__ bool ok(int d) const { return g + d >= 0 && g + d <= gmax; }
-};
-
-struct CellWalk {
-    AxisWalk ax, ay, az;
-    float cull2;        // squared cull radius
-    uint32_t base;      // cell key of (0, 0, 0) of this scene
-    uint32_t gyx, gx;   // key strides
-    uint32_t cap;       // max_par_in_cell
-
-    __device__ __forceinline__ void init(const sph_params& p, float x, float y, float z) {
-        ax.init(x, p.world_min[0], p.world_max[0], p.cell_size[0], p.grid_size[0]);
-        ay.init(y, p.world_min[1], p.world_max[1], p.cell_size[1], p.grid_size[1]);
-        az.init(z, p.world_min[2], p.world_max[2], p.cell_size[2], p.grid_size[2]);
-        cull2 = p.h2 * 1.00001f;
-        base = p.cell_base;
-        gyx = p.grid_size_yx;
-        gx = p.grid_size[0];
-        cap = p.max_par_in_cell;
-    }
-    // Slot range [s, e) of neighbour cell (dx, dy, dz), capped at max_par_in_cell;
-    // false if the cell is outside the grid, culled or empty.
-    __device__ __forceinline__ bool open(int dx, int dy, int dz,
-                                         const uint32_t* __restrict__ cell_start,
-                                         const uint32_t* __restrict__ cell_end, uint32_t& s,
-                                         uint32_t& e) const {
-        if (!ax.ok(dx) || !ay.ok(dy) || !az.ok(dz)) return false;
-        if (az.d2(dz) + ay.d2(dy) + ax.d2(dx) > cull2) return false;
-        const uint32_t c = base + (uint32_t)(az.g + dz) * gyx + (uint32_t)(ay.g + dy) * gx +
-                           (uint32_t)(ax.g + dx);
-        s = cell_start[c];
-        if (s == kEmptyCell) return false;
-        e = min(cell_end[c], s + cap);
-        return true;
-    }
-};
-
 // ------------------------------------------------------------------ density
-template <bool kBatch, int kPrec>
+// With kLists the density walk also records each particle's neighbours
+// (the slots that pass r2 < h2, j != i, in walk order) entry-major,
+// nbr[c * n + i], c < kNbrCap, and their count in nnb[i]; a count above
+// kNbrCap raises the overflow counter (entries past the cap are dropped).
+template <bool kBatch, int kPrec, bool kLists = false>
 __device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
                                             const sph_params* __restrict__ tab,
                                             const uint32_t* __restrict__ sid,
                                             const float4* __restrict__ pos,
                                             const uint32_t* __restrict__ cell_start,
                                             const uint32_t* __restrict__ cell_end,
-                                            float2* __restrict__ rhop) {
+                                            float2* __restrict__ rhop,
+                                            uint32_t n = 0, uint32_t* __restrict__ nbr = nullptr,
+                                            uint32_t* __restrict__ nnb = nullptr,
+                                            uint32_t* __restrict__ counters = nullptr) {
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
     const float4 xi = pos[i];
@@ -274,6 +166,7 @@ __device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
     const float h2 = p.h2;
     float sum = 0.0f;
     KahanF ks{0.0f, 0.0f};
+    uint32_t count = 0;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
         if (!w.open(ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
@@ -295,12 +188,34 @@ __device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
                 // the partial sum is >= +0, so adding +0 for a miss is exact
                 cell = cell + (hit ? wgt : 0.0f);
             }
+            if constexpr (kLists) {
+                if (hit) {
+                    if (count < kNbrCap) nbr[(size_t)count * n + i] = j;
+                    ++count;
+                }
+            }
         }
         if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
     }
     if (kPrec == SPH_PREC_KAHAN) sum = ks.s;
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
     rhop[i] = make_float2(rho, eos(p, rho));
+    if constexpr (kLists) {
+        nnb[i] = count;
+        if (count > kNbrCap) atomicAdd(&counters[kCntNbrOverflow], 1u);
+    }
+}
+
+template <bool kBatch, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_density_lists(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ nbr,
+    uint32_t* __restrict__ nnb, uint32_t* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    density_one<kBatch, kPrec, true>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n, nbr, nnb,
+                                     counters);
 }
 
 template <bool kBatch, int kPrec>
@@ -859,6 +774,28 @@ hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32
         reorder_impl<true>(c, b, sorted_key, old_slot, s);
     else
         reorder_impl<false>(c, b, sorted_key, old_slot, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_lists(const LaunchCfg& c, const StepBuffers& b, uint32_t* nbr, uint32_t* nnb,
+                                hipStream_t s) {
+    const bool kahan = c.precision == SPH_PREC_KAHAN;
+    const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
+#define SPH_DENS_LISTS(BATCH, PREC)                                                                  \
+    hipLaunchKernelGGL((k_density_lists<BATCH, PREC>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, \
+                       b.pos_cur, b.cell_start, b.cell_end, b.rhop, nbr, nnb, b.counters)
+    if (c.tab) {
+        if (kahan)
+            SPH_DENS_LISTS(true, SPH_PREC_KAHAN);
+        else
+            SPH_DENS_LISTS(true, SPH_PREC_REFERENCE);
+    } else {
+        if (kahan)
+            SPH_DENS_LISTS(false, SPH_PREC_KAHAN);
+        else
+            SPH_DENS_LISTS(false, SPH_PREC_REFERENCE);
+    }
+#undef SPH_DENS_LISTS
     return hipGetLastError();
 }
 

@@ -19,7 +19,9 @@ SPH_OK = 0
 # enums
 SPH_EOS_LINEAR, SPH_EOS_TAIT = 0, 1
 SPH_PREC_REFERENCE, SPH_PREC_KAHAN = 0, 1
-SPH_ARRAY_POSITION, SPH_ARRAY_VELOCITY, SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_DVEL = range(5)
+(SPH_ARRAY_POSITION, SPH_ARRAY_VELOCITY, SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_DVEL,
+ SPH_ARRAY_TARGET) = range(6)
+MLP_NUM_PARAMS = 5388
 SPH_FLAG_KEEP_DVEL = 1
 EMPTY_CELL = 0xFFFFFFFF
 
@@ -130,6 +132,40 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class MlpNorm(C.Structure):
+    """sph_mlp_norm: feature and output units of the learned surrogate."""
+    _fields_ = [
+        ("inv_h", C.c_float),
+        ("inv_v", C.c_float),
+        ("inv_rho0", C.c_float),
+        ("inv_p", C.c_float),
+        ("inv_nnb", C.c_float),
+        ("agg_scale", C.c_float),
+        ("a_scale", C.c_float),
+        ("reserved", C.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class LearnedReport(C.Structure):
+    _fields_ = [
+        ("coarse_steps", C.c_uint32),
+        ("fallbacks", C.c_uint32),
+        ("plain_steps", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("drift_max", C.c_double),
+        ("drift_last", C.c_double),
+        ("anchor", C.c_double),
+        ("sim_time", C.c_double),
+        ("reserved", C.c_double * 4),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
 class SphError(RuntimeError):
     def __init__(self, status: int, what: str):
         super().__init__(f"{what}: {strerror(status)} (status {status})")
@@ -179,6 +215,24 @@ SIGNATURES = {
     "sph_batch_autotune": (C.c_int, [_P, C.c_char_p, _up, _fp]),
     "sph_batch_info": (C.c_int, [_P, _up, _up, _up, _up]),
     "sph_mfma_selftest": (C.c_int, [C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), _fp]),
+    "sph_model_create": (C.c_int, [C.c_int, C.POINTER(MlpNorm), C.c_uint64, C.POINTER(_P)]),
+    "sph_model_destroy": (C.c_int, [_P]),
+    "sph_model_get_norm": (C.c_int, [_P, C.POINTER(MlpNorm)]),
+    "sph_model_get_weights": (C.c_int, [_P, _fp]),
+    "sph_model_set_weights": (C.c_int, [_P, _fp]),
+    "sph_learn_begin": (C.c_int, [_P, _P]),
+    "sph_learn_add": (C.c_int, [_P, _P]),
+    "sph_learn_train": (C.c_int, [_P, _P, C.c_uint32, C.c_float, C.POINTER(C.c_double)]),
+    "sph_learn_gradient": (C.c_int, [_P, _P, C.POINTER(C.c_double), _fp]),
+    "sph_learn_predict": (C.c_int, [_P, _P, _fp]),
+    "sph_learn_get_window": (C.c_int, [_P, C.c_int, _fp]),
+    "sph_batch_learn_begin": (C.c_int, [_P, _P]),
+    "sph_batch_learn_add": (C.c_int, [_P, _P]),
+    "sph_batch_learn_train": (C.c_int, [_P, _P, C.c_uint32, C.c_float, C.POINTER(C.c_double)]),
+    "sph_batch_learn_gradient": (C.c_int, [_P, _P, C.POINTER(C.c_double), _fp]),
+    "sph_batch_learn_get_window": (C.c_int, [_P, C.c_uint32, C.c_int, _fp]),
+    "sph_step_learned": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_float,
+                                   C.POINTER(LearnedReport)]),
 }
 
 
@@ -331,6 +385,110 @@ class SphSystem:
         _check(lib().sph_replay(self._ctx, times), "sph_replay")
 
 
+class Model:
+    """Learned pair-force surrogate on one device (sph_model)."""
+
+    def __init__(self, norm: MlpNorm, device: int = 0, seed: int = 0):
+        self._m = C.c_void_p()
+        self.device = device
+        _check(lib().sph_model_create(device, C.byref(norm), seed, C.byref(self._m)), "sph_model_create")
+
+    def close(self) -> None:
+        if self._m:
+            lib().sph_model_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._m
+
+    def norm(self) -> MlpNorm:
+        n = MlpNorm()
+        _check(lib().sph_model_get_norm(self._m, C.byref(n)), "sph_model_get_norm")
+        return n
+
+    def weights(self) -> np.ndarray:
+        w = np.empty(MLP_NUM_PARAMS, np.float32)
+        _check(lib().sph_model_get_weights(self._m, _f(w)), "sph_model_get_weights")
+        return w
+
+    def set_weights(self, w: np.ndarray) -> None:
+        a = np.ascontiguousarray(w, np.float32).reshape(-1)
+        if a.size != MLP_NUM_PARAMS:
+            raise ValueError(f"expected {MLP_NUM_PARAMS} weights, got {a.size}")
+        _check(lib().sph_model_set_weights(self._m, _f(a)), "sph_model_set_weights")
+
+
+def _window_shape(which: int, n: int) -> tuple:
+    return (n,) if which in (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE) else (n, 4)
+
+
+def _learn_methods(prefix: str):
+    """Training-window methods shared by SphSystem and SphBatch."""
+
+    def learn_begin(self, model: Model) -> None:
+        _check(getattr(lib(), prefix + "learn_begin")(self._h, model.handle), prefix + "learn_begin")
+
+    def learn_add(self, model: Model) -> None:
+        _check(getattr(lib(), prefix + "learn_add")(self._h, model.handle), prefix + "learn_add")
+
+    def learn_train(self, model: Model, iters: int, lr: float) -> float:
+        loss = C.c_double()
+        _check(getattr(lib(), prefix + "learn_train")(self._h, model.handle, iters, lr, C.byref(loss)),
+               prefix + "learn_train")
+        return loss.value
+
+    def learn_gradient(self, model: Model) -> tuple[float, np.ndarray]:
+        loss = C.c_double()
+        g = np.empty(MLP_NUM_PARAMS, np.float32)
+        _check(getattr(lib(), prefix + "learn_gradient")(self._h, model.handle, C.byref(loss), _f(g)),
+               prefix + "learn_gradient")
+        return loss.value, g
+
+    return learn_begin, learn_add, learn_train, learn_gradient
+
+
+(SphSystem.learn_begin, SphSystem.learn_add, SphSystem.learn_train,
+ SphSystem.learn_gradient) = _learn_methods("sph_")
+SphSystem._h = property(lambda self: self._ctx)
+
+
+def _sys_learn_predict(self, model: Model) -> np.ndarray:
+    out = np.empty((self.n, 4), np.float32)
+    _check(lib().sph_learn_predict(self._ctx, model.handle, _f(out)), "sph_learn_predict")
+    return out
+
+
+def _sys_learn_window(self, which: int) -> np.ndarray:
+    out = np.empty(_window_shape(which, self.n), np.float32)
+    _check(lib().sph_learn_get_window(self._ctx, which, _f(out)), "sph_learn_get_window")
+    return out
+
+
+def _sys_step_learned(self, model: Model, ncoarse: int, factor: int = 4, tol: float = 0.01) -> dict:
+    r = LearnedReport()
+    _check(lib().sph_step_learned(self._ctx, model.handle, ncoarse, factor, tol, C.byref(r)),
+           "sph_step_learned")
+    return r.as_dict()
+
+
+SphSystem.learn_predict = _sys_learn_predict
+SphSystem.learn_window = _sys_learn_window
+SphSystem.step_learned = _sys_step_learned
+
+
 class SphBatch:
     """Many independent scenes advanced by the same launches (sph_batch_*)."""
 
@@ -417,3 +575,18 @@ class SphBatch:
         _check(lib().sph_batch_info(self._b, _u(v[0:1]), _u(v[1:2]), _u(v[2:3]), _u(v[3:4])),
                "sph_batch_info")
         return dict(nscenes=int(v[0]), particles=int(v[1]), cells=int(v[2]), key_bits=int(v[3]))
+
+
+(SphBatch.learn_begin, SphBatch.learn_add, SphBatch.learn_train,
+ SphBatch.learn_gradient) = _learn_methods("sph_batch_")
+SphBatch._h = property(lambda self: self._b)
+
+
+def _batch_learn_window(self, scene: int, which: int) -> np.ndarray:
+    n = int(self.params(scene).num_particles)
+    out = np.empty(_window_shape(which, n), np.float32)
+    _check(lib().sph_batch_learn_get_window(self._b, scene, which, _f(out)), "sph_batch_learn_get_window")
+    return out
+
+
+SphBatch.learn_window = _batch_learn_window
