@@ -308,10 +308,10 @@ int sph_mfma_selftest(int variant, const uint16_t* a, const uint16_t* b, float* 
  * For every neighbour pair of particle i (|x_i - x_j| < h, j != i, the
  * density kernel's pairs) the pair network sees 16 features
  *   e = [(x_i - x_j) inv_h (3), (v_j - v_i) inv_v (3), |x_i - x_j| inv_h,
- *        rho_i inv_rho0 - 1, rho_j inv_rho0 - 1, 1, 0 (6)]
+ *        s (rho_i inv_rho0 - 1), s (rho_j inv_rho0 - 1), 1, 0 (6)]   (s = rho_scale)
  *   m_ij = relu(W2 relu(W1 e) + b2)       W1: 64 x 16 (column 9 = bias)
  * summed over the neighbours, H_i = sum_j m_ij.  With the node features
- *   q_i = [rho_i inv_rho0 - 1, p_i inv_p, nnb_i inv_nnb]
+ *   q_i = [s (rho_i inv_rho0 - 1), p_i inv_p, nnb_i inv_nnb]
  * the acceleration is a_i = a_scale (W3 [agg_scale H_i ; q_i] + b3), W3: 3 x 67.
  * The pair network runs on BF16 matrix cores (BF16 weights and features,
  * FP32 accumulation); the node layer and all sums are FP32.
@@ -327,7 +327,7 @@ typedef struct sph_mlp_norm {
     float inv_nnb;    /* neighbour-count unit                               */
     float agg_scale;  /* factor on the neighbour sum H                      */
     float a_scale;    /* output unit (m/s^2)                                */
-    float reserved;
+    float rho_scale;  /* gain on the density features (0 means 1)          */
 } sph_mlp_norm;
 
 typedef struct sph_model sph_model;

@@ -1,0 +1,146 @@
+/* pibiti_sph.hpp - C++ host class over the C ABI of pibiti_sph.h.
+ *
+ * cSPH mirrors the reference's host class (source/SPH/SPH.h:9-50) for the
+ * step path, without GL buffers: a Scene (scn) whose derived constants go
+ * to the device context, InitScene/_InitMem/_FreeMem, Reset, Update,
+ * getArray/setArray with the reference's 0 = position / 1 = velocity flag
+ * values, and a density getter.  Every call goes through the C ABI; errors
+ * throw pibiti::Error instead of exiting the process.
+ */
+#ifndef PIBITI_SPH_HPP
+#define PIBITI_SPH_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pibiti_sph.h"
+
+namespace pibiti {
+
+class Error : public std::runtime_error {
+public:
+    Error(int st, const char* what)
+        : std::runtime_error(std::string(what) + ": " + sph_strerror(st)), status(st) {}
+    int status;
+};
+
+inline void check(int st, const char* what) {
+    if (st != SPH_OK) throw Error(st, what);
+}
+
+/* The block cSPH::Reset(0) fills (source/SPH/SPH_Init.cpp:40-76): a lattice
+ * from `min` in steps of `spacing`, x fastest, then z, then y (the
+ * reference's default initLast), optionally jittered and with random
+ * velocities.                                                               */
+struct InitVolume {
+    uint32_t nx = 0, ny = 0, nz = 0;
+    float min[3] = {0.0f, 0.0f, 0.0f};
+    float spacing = 0.0f;
+    float jitter = 0.0f;     /* fraction of spacing                          */
+    float vel_sigma = 0.0f;  /* m/s per axis                                 */
+    uint64_t seed = 0;
+};
+
+class cSPH {
+public:
+    sph_scene scn{};    /* raw scene values (Scene)                          */
+    sph_params par{};   /* derived constants (Scene::_UpdatePar/_UpdateGrid) */
+    InitVolume vol{};
+    int device = 0;
+    uint32_t flags = 0; /* SPH_FLAG_*                                        */
+    bool bInitialized = false;
+
+    cSPH() { check(sph_scene_default(&scn), "sph_scene_default"); }
+    ~cSPH() { _FreeMem(); }
+    cSPH(const cSPH&) = delete;
+    cSPH& operator=(const cSPH&) = delete;
+
+    /* Derive the constants, allocate, fill the init volume. */
+    void InitScene() {
+        check(sph_params_from_scene(&scn, &par), "sph_params_from_scene");
+        _FreeMem();
+        _InitMem();
+        Reset(0);
+    }
+
+    void _InitMem() {
+        check(sph_create(&par, device, flags, &ctx_), "sph_create");
+        const size_t n = par.num_particles;
+        hPos_.assign(4 * n, 0.0f);
+        hVel_.assign(4 * n, 0.0f);
+        hDens_.assign(n, 0.0f);
+        bInitialized = true;
+    }
+
+    void _FreeMem() {
+        if (ctx_) sph_destroy(ctx_);
+        ctx_ = nullptr;
+        bInitialized = false;
+    }
+
+    /* type 0: the lattice block of `vol` (its size must be the scene's
+     * particle count); other reference fill types are not provided.        */
+    void Reset(int type) {
+        require();
+        if (type != 0) throw Error(SPH_ERR_UNSUPPORTED, "cSPH::Reset");
+        if ((uint64_t)vol.nx * vol.ny * vol.nz != par.num_particles)
+            throw Error(SPH_ERR_INVALID_ARGUMENT, "cSPH::Reset (volume size)");
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        check(sph_scene_lattice(vol.nx, vol.ny, vol.nz, vol.min, vol.spacing, vol.jitter, vol.vel_sigma,
+                                zero, vol.seed, 0, hPos_.data(), hVel_.data()),
+              "sph_scene_lattice");
+        setArray(SPH_ARRAY_POSITION, hPos_.data(), 0, (int)par.num_particles);
+        setArray(SPH_ARRAY_VELOCITY, hVel_.data(), 0, (int)par.num_particles);
+    }
+
+    /* One step (cSPH::Update, source/SPH/SPH_Update.cpp:12-81), or several. */
+    void Update(uint32_t nsteps = 1) {
+        require();
+        check(sph_step(ctx_, nsteps), "sph_step");
+    }
+
+    /* float4 per particle in original order (source/SPH/SPH_Util.cpp:44-57):
+     * which = 0 positions, 1 velocities.                                   */
+    float* getArray(int which) {
+        require();
+        std::vector<float>& h = which == SPH_ARRAY_POSITION ? hPos_ : hVel_;
+        check(sph_get_array(ctx_, which, h.data()), "sph_get_array");
+        return h.data();
+    }
+
+    /* source/SPH/SPH_Util.cpp:59-71: count float4 records from `start`. */
+    void setArray(int which, const float* data, int start, int count) {
+        require();
+        check(sph_set_array(ctx_, which, data, (uint32_t)start, (uint32_t)count), "sph_set_array");
+    }
+
+    /* Density per particle (original order) of the last step. */
+    float* getDensity() {
+        require();
+        check(sph_get_array(ctx_, SPH_ARRAY_DENSITY, hDens_.data()), "sph_get_array");
+        return hDens_.data();
+    }
+
+    sph_stats Stats() {
+        require();
+        sph_stats s{};
+        check(sph_get_stats(ctx_, &s), "sph_get_stats");
+        return s;
+    }
+
+    sph_ctx* context() const { return ctx_; }
+
+private:
+    void require() const {
+        if (!bInitialized) throw Error(SPH_ERR_NOT_INITIALISED, "cSPH");
+    }
+
+    sph_ctx* ctx_ = nullptr;
+    std::vector<float> hPos_, hVel_, hDens_;
+};
+
+}  // namespace pibiti
+
+#endif /* PIBITI_SPH_HPP */

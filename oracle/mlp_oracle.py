@@ -88,6 +88,7 @@ def features(pos, vel, rho, pi, pj, norm: dict, emulate: bool = True) -> np.ndar
     v = np.asarray(vel, f32)[:, :3]
     r = np.asarray(rho, f32)
     inv_h, inv_v, inv_rho0 = f32(norm["inv_h"]), f32(norm["inv_v"]), f32(norm["inv_rho0"])
+    rs = f32(norm.get("rho_scale", 1.0) or 1.0)
     d = (x[pi] - x[pj]).astype(f32)
     dist = np.sqrt(((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]).astype(f32) + d[:, 2] * d[:, 2]).astype(f32))
     dv = (v[pj] - v[pi]).astype(f32)
@@ -95,8 +96,8 @@ def features(pos, vel, rho, pi, pj, norm: dict, emulate: bool = True) -> np.ndar
     F[:, 0:3] = d * inv_h
     F[:, 3:6] = dv * inv_v
     F[:, 6] = dist * inv_h
-    F[:, 7] = (r[pi] * inv_rho0).astype(f32) - f32(1.0)
-    F[:, 8] = (r[pj] * inv_rho0).astype(f32) - f32(1.0)
+    F[:, 7] = ((r[pi] * inv_rho0).astype(f32) - f32(1.0)) * rs
+    F[:, 8] = ((r[pj] * inv_rho0).astype(f32) - f32(1.0)) * rs
     F[:, 9] = 1.0
     return bf16(F) if emulate else F.astype(np.float64)
 
@@ -104,7 +105,8 @@ def features(pos, vel, rho, pi, pj, norm: dict, emulate: bool = True) -> np.ndar
 def node_features(rho, pres, nnb, norm: dict) -> np.ndarray:
     f32 = np.float32
     q = np.zeros((np.asarray(rho).size, NODE), np.float64)
-    q[:, 0] = (np.asarray(rho, f32) * f32(norm["inv_rho0"])).astype(f32) - f32(1.0)
+    rs = f32(norm.get("rho_scale", 1.0) or 1.0)
+    q[:, 0] = ((np.asarray(rho, f32) * f32(norm["inv_rho0"])).astype(f32) - f32(1.0)) * rs
     q[:, 1] = np.asarray(pres, f32) * f32(norm["inv_p"])
     q[:, 2] = np.asarray(nnb, f32) * f32(norm["inv_nnb"])
     return q
@@ -162,7 +164,7 @@ def probe_params(agg: float) -> np.ndarray:
     in both halves of the layer):
       a_x / a_scale = number of neighbours
       a_y / a_scale = sum_j bf16(|x_i - x_j| / h)
-      a_z / a_scale = sum_j bf16(1 + bf16(rho_j / rho0 - 1))"""
+      a_z / a_scale = sum_j bf16(1 + bf16(s (rho_j / rho0 - 1)))"""
     W1 = np.zeros((64, 16))
     W2 = np.zeros((64, 64))
     W3 = np.zeros((3, 67))
@@ -188,7 +190,8 @@ def probe_expected(pos, rho, h: float, norm: dict) -> np.ndarray:
     d = (x[pi] - x[pj]).astype(f32)
     dist = np.sqrt(((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]).astype(f32) + d[:, 2] * d[:, 2]).astype(f32))
     fr = bf16((dist * f32(norm["inv_h"])).astype(f32))
-    frho = bf16(1.0 + bf16((np.asarray(rho, f32)[pj] * f32(norm["inv_rho0"])).astype(f32) - f32(1.0)))
+    rs = f32(norm.get("rho_scale", 1.0) or 1.0)
+    frho = bf16(1.0 + bf16(((np.asarray(rho, f32)[pj] * f32(norm["inv_rho0"])).astype(f32) - f32(1.0)) * rs))
     want = np.zeros((n, 3))
     want[:, 0] = np.bincount(pi, minlength=n)
     want[:, 1] = np.bincount(pi, weights=fr, minlength=n)

@@ -18,6 +18,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpibiti_sph.so"
+RUN_BIN = PKG / "_build" / "sph_run"  # C++ host driver (pibiti_amd/host/sph_run.cpp)
 
 HIP_SOURCES = ["sph_kernels.hip", "sph_sort.hip", "sph_capi.hip", "sph_mlp.hip", "sph_learn.hip"]
 CPP_SOURCES = ["sph_scene.cpp"]
@@ -66,6 +67,9 @@ def build(verbose: bool = True) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB)
+    # the C++ host driver: plain host code over the C ABI
+    _run(["g++", "-O2", "-std=c++17", *inc, str(PKG / "host" / "sph_run.cpp"), "-L", str(PKG),
+          "-lpibiti_sph", "-Wl,-rpath,$ORIGIN/..", "-o", str(RUN_BIN)])
     return LIB
 
 

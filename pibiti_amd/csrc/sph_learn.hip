@@ -409,6 +409,7 @@ int model_create(int device, const sph_mlp_norm* norm, uint64_t seed, sph_model*
     if (!m) return SPH_ERR_OUT_OF_MEMORY;
     m->device = device;
     m->norm = *norm;
+    if (m->norm.rho_scale == 0.0f) m->norm.rho_scale = 1.0f;
     // weights from the seed: SplitMix64 stream, Box-Muller normals
     std::vector<float> w(mlp::kNumParams, 0.0f);
     uint64_t state = seed;

@@ -129,9 +129,9 @@ __device__ __forceinline__ bf16x8 edge_features(int g, bool valid, const PairRaw
         f[4] = bf((r.b.y - vi.y) * nm.inv_v);
         f[5] = bf((r.b.z - vi.z) * nm.inv_v);
         f[6] = bf(d * nm.inv_h);
-        f[7] = bf(rhoi * nm.inv_rho0 - 1.0f);
+        f[7] = bf((rhoi * nm.inv_rho0 - 1.0f) * nm.rho_scale);
     } else {
-        f[0] = bf(r.a.x * nm.inv_rho0 - 1.0f);
+        f[0] = bf((r.a.x * nm.inv_rho0 - 1.0f) * nm.rho_scale);
         f[1] = bf(1.0f);  // feature kBiasFeat
     }
     return f;
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBlk) void k_mlp_fwd(const MlpArgs a) {
         if (g != 0) continue;
         const uint32_t nnb = st.nnb[iT];
         const float2 rp = st.rhop[iT];
-        const float q0 = rp.x * nm.inv_rho0 - 1.0f;
+        const float q0 = (rp.x * nm.inv_rho0 - 1.0f) * nm.rho_scale;
         const float q1 = rp.y * nm.inv_p;
         const float q2 = (float)nnb * nm.inv_nnb;
         float y[kOut];
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kNodeBlk) void k_mlp_node_grad(
             d[k] = diff[k] * two_n;
         }
         const float2 rp = rhop[i];
-        q[0] = rp.x * nm.inv_rho0 - 1.0f;
+        q[0] = (rp.x * nm.inv_rho0 - 1.0f) * nm.rho_scale;
         q[1] = rp.y * nm.inv_p;
         q[2] = o.w * nm.inv_nnb;
 #pragma unroll

@@ -47,7 +47,7 @@ def fluid_origin(spacing: float, world_min=(0.0, 0.0, 0.0)) -> tuple:
     return tuple(float(w) + b for w in world_min)
 
 
-def _scene(n: int, h: float, spacing: float, world_min, world_max, dt: float, *,
+def scene(n: int, h: float, spacing: float, world_min, world_max, dt: float, *,
            viscosity: float, stiffness: float = 3.0, rest_density: float = 1000.0,
            gravity=(0.0, -9.81, 0.0), eos: int = sph.SPH_EOS_LINEAR,
            precision: int = sph.SPH_PREC_REFERENCE, cfl: float = 0.0,
@@ -87,7 +87,7 @@ def dam_break(nx: int, ny: int, nz: int, *, spacing: float, h: float, world_max,
               viscosity: float, vel_sigma: float = 0.0, seed: int = 1, **kw):
     """Jittered lattice block in the corner of a box: a dam break."""
     n = nx * ny * nz
-    s = _scene(n, h, spacing, (0.0, 0.0, 0.0), world_max, dt, viscosity=viscosity, **kw)
+    s = scene(n, h, spacing, (0.0, 0.0, 0.0), world_max, dt, viscosity=viscosity, **kw)
     p = sph.params_from_scene(s)
     origin = fluid_origin(spacing)
     pos, vel = sph.scene_lattice(nx, ny, nz, origin, spacing, 0.1, vel_sigma,
@@ -141,7 +141,7 @@ def colliding_blocks(edge: int = 128, *, spacing: float = 0.01, h: float = 0.02,
     xb = world[0] / 2 + gap / 2
     n = 2 * edge ** 3
     kw.setdefault("gravity", (0.0, 0.0, 0.0))
-    s = _scene(n, h, spacing, (0.0, 0.0, 0.0), world, dt, viscosity=0.1, **kw)
+    s = scene(n, h, spacing, (0.0, 0.0, 0.0), world, dt, viscosity=0.1, **kw)
     p = sph.params_from_scene(s)
     pa, va = sph.scene_lattice(edge, edge, edge, (xa, y0, z0), spacing, 0.1, 0.0,
                                (speed, 0.0, 0.0), seed, 0)

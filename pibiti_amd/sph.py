@@ -142,11 +142,11 @@ class MlpNorm(C.Structure):
         ("inv_nnb", C.c_float),
         ("agg_scale", C.c_float),
         ("a_scale", C.c_float),
-        ("reserved", C.c_float),
+        ("rho_scale", C.c_float),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class LearnedReport(C.Structure):

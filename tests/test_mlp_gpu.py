@@ -27,6 +27,7 @@ def norm_for(p) -> sph.MlpNorm:
     nm.inv_nnb = 1.0 / 32.0
     nm.agg_scale = 1.0 / 32.0
     nm.a_scale = 10.0
+    nm.rho_scale = 10.0
     return nm
 
 

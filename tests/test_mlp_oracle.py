@@ -8,7 +8,7 @@ import torch
 from oracle import mlp_oracle as mo
 
 NORM = dict(inv_h=1.0 / 0.04, inv_v=1.0, inv_rho0=1.0 / 1000.0, inv_p=1.0 / 3000.0, inv_nnb=1.0 / 32.0,
-            agg_scale=1.0 / 32.0, a_scale=10.0)
+            agg_scale=1.0 / 32.0, a_scale=10.0, rho_scale=10.0)
 
 
 def _cloud(n=48, seed=0, h=0.04):
