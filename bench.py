@@ -10,10 +10,16 @@ from the host inside it.
 
 Prints ONE JSON line (rank 0, N = 1):
   value        particle-updates/s = N_particles * steps / timed seconds
-  roofline     force kernel: algorithmic bytes per launch / mean launch time
-               (HIP events on the context stream) against 8 TB/s HBM
-  cpu_baseline the CPU oracle (FP32 reference order) on the 15,625-particle
-               dam break, all host threads, a bounded number of steps
+  roofline     dominant gather kernel: algorithmic bytes per launch / mean
+               launch time (HIP events on the context stream) against 8 TB/s
+  cpu_baseline the CPU oracle (FP32 reference order) on the first steps of the
+               same 1M workload, all host threads, bounded in time (the
+               single-thread rate on the 15,625-particle dam break rides along)
+  learned      the learned surrogate: trained on the device from plain-solver
+               trajectories, then stepping the same workload at 4 x dt from
+               the same state: simulated-time throughput against the plain
+               step, density error against the plain trajectory, and the
+               surrogate kernel's matrix-core roofline (BF16 dense peak)
 """
 from __future__ import annotations
 
@@ -33,6 +39,88 @@ from pibiti_amd import scenes, sph  # noqa: E402
 
 METRIC = "particle-updates/s at 1M (plain vs learned, ≤1% ρ err); % HBM peak"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense
+
+
+def pair_flops(pairs: float) -> float:
+    """Algorithmic FLOPs of one surrogate pass: both pair layers for every
+    listed pair (16 padded inputs -> 64 -> 64, 2 FLOPs per multiply-add)."""
+    return 2.0 * pairs * (16 * 64 + 64 * 64)
+
+
+def learned_section(args, p, snap_pos, snap_vel, rho_plain_end, plain_secs) -> dict:
+    """Train the surrogate, then run the timed workload with it from the
+    state the plain run started from."""
+    from pibiti_amd import learn
+    n = int(p.num_particles)
+    rho0 = float(p.rest_density)
+    factor = 4
+    model = sph.Model(learn.default_norm(p), seed=1)
+    hist = learn.train(model, scenes_count=200, steps=args.train_steps, stride=8, iters=args.train_iters,
+                       lr=args.train_lr, lr_end=1e-4)
+    L = hist["loss"]
+    tail = L[-max(1, len(L) // 10):]
+
+    def fresh():
+        c = sph.SphSystem(p)
+        c.reset(snap_pos, snap_vel)
+        c.update(1)  # the state the timed plain run started from
+        return c
+
+    def err(rho, ref):
+        d = np.abs(np.asarray(rho, np.float64) - ref) / rho0
+        return {"rms": float(np.sqrt(np.mean(d * d))), "max": float(d.max())}
+
+    # one surrogate step against `factor` plain steps from the same state
+    with fresh() as c:
+        c.update(factor)
+        rho_ref = c.get_array(sph.SPH_ARRAY_DENSITY).astype(np.float64)
+    with fresh() as c:
+        c.step_learned(model, 1, factor=factor, tol=1e9)
+        one = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_ref)
+    # the whole timed workload, with the 1 % drift fallback
+    coarse = args.steps // factor
+    with fresh() as c:
+        c.synchronize()
+        t0 = time.perf_counter()
+        rep = c.step_learned(model, coarse, factor=factor, tol=args.tol)
+        c.synchronize()
+        secs = time.perf_counter() - t0
+        run = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_plain_end)
+    # kernel times (HIP events) over a few more surrogate steps
+    with fresh() as c:
+        c.set_timing(True)
+        krep = c.step_learned(model, 40, factor=factor, tol=1e9)
+    sim_steps = factor * rep["coarse_steps"] + rep["plain_steps"]
+    speedup = (sim_steps / secs) / (args.steps / plain_secs)
+    flops = pair_flops(krep["pairs"])
+    achieved = flops / (krep["mlp_ms"] * 1e-3) / 1e12
+    model.close()
+    return {
+        "factor": factor,
+        "value": n * sim_steps / secs,
+        "unit": "particle-updates/s (dt-equivalent)",
+        "sim_time_speedup": round(speedup, 3),
+        "ms_per_coarse_step": secs * 1e3 / max(1, rep["coarse_steps"] + rep["fallbacks"]),
+        "fallbacks": rep["fallbacks"],
+        "coarse_steps": rep["coarse_steps"],
+        "tol": args.tol,
+        "density_error_one_step": one,
+        "density_error_workload": run,
+        "density_error_note": "|rho - rho_plain| / rho0 over particles: one surrogate step vs 4 plain "
+                              "steps from the same state, and the whole timed workload vs the plain run",
+        "roofline": {"bound": "mfma", "kernel": "surrogate pair network",
+                     "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": None,
+                     "pairs_per_launch": krep["pairs"], "flops_per_launch": flops,
+                     "mean_launch_ms": round(krep["mlp_ms"], 5)},
+        "kernels_ms": {"density_lists": round(krep["density_ms"], 5), "surrogate": round(krep["mlp_ms"], 5),
+                       "coarse_step": round(krep["coarse_ms"], 5)},
+        "training": {"seconds": round(hist["seconds"], 2), "trajectories": hist["scenes"],
+                     "particles": hist["particles"], "plain_steps": hist["steps"],
+                     "updates": len(L) * hist["iters"], "final_loss": float(np.mean(tail)),
+                     "first_loss": float(L[0])},
+    }
 
 
 def algorithmic_bytes(p) -> dict:
@@ -96,6 +184,11 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=50)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=1600)
+    ap.add_argument("--train-iters", type=int, default=16)
+    ap.add_argument("--train-lr", type=float, default=3e-3)
+    ap.add_argument("--tol", type=float, default=0.01)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -108,7 +201,10 @@ def main() -> int:
     n = int(p.num_particles)
     s = sph.SphSystem(p)
     s.reset(pos, vel)
-    s.update(args.warmup)
+    s.update(max(0, args.warmup - 1))
+    snap_pos = s.get_array(sph.SPH_ARRAY_POSITION)
+    snap_vel = s.get_array(sph.SPH_ARRAY_VELOCITY)
+    s.update(1 if args.warmup > 0 else 0)
     s.synchronize()
 
     t0 = time.perf_counter()
@@ -116,6 +212,7 @@ def main() -> int:
     s.synchronize()
     t1 = time.perf_counter()
     secs = t1 - t0
+    rho_plain_end = s.get_array(sph.SPH_ARRAY_DENSITY).astype(np.float64)
 
     # per-kernel device times: HIP events on the context stream, one step per call
     s.set_timing(True)
@@ -177,6 +274,8 @@ def main() -> int:
         "end_state": {"sim_time": st["sim_time"], "max_speed": st["max_speed"],
                       "cap_cells": st["cap_cells"], "max_cell_count": st["max_cell_count"]},
     }
+    if not args.no_learned and args.warmup > 0:
+        out["learned"] = learned_section(args, p, snap_pos, snap_vel, rho_plain_end, secs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget, p, pos, vel)
     print(json.dumps(out), flush=True)

@@ -147,8 +147,9 @@ typedef struct sph_params {
      * results.  tune[0]: force kernel variant, 1 = fused candidate and pair
      * loop, 16/24/32/48 = neighbour-list entries per lane; tune[1]: gather
      * grid, 0 = one thread per particle, else a persistent grid of tune[1]
-     * blocks of 256 threads pulling 256-particle tiles; tune[2..3]:
-     * reserved.                                                            */
+     * blocks of 256 threads pulling 256-particle tiles; tune[2]: surrogate
+     * steps per host check in sph_step_learned (0 = 4, at most 64);
+     * tune[3]: reserved.                                                   */
     uint32_t tune[4];
     /* First global cell key of this scene: 0 for a single-scene context; a
      * batch gives each scene a disjoint key range (set by sph_batch_create). */
@@ -307,9 +308,9 @@ int sph_mfma_selftest(int variant, const uint16_t* a, const uint16_t* b, float* 
 /* Learned pair-force surrogate (extension; no counterpart in the reference).
  * For every neighbour pair of particle i (|x_i - x_j| < h, j != i, the
  * density kernel's pairs) the pair network sees 16 features
- *   e = [(x_i - x_j) inv_h (3), (v_j - v_i) inv_v (3), |x_i - x_j| inv_h,
- *        s (rho_i inv_rho0 - 1), s (rho_j inv_rho0 - 1), 1, 0 (6)]   (s = rho_scale)
- *   m_ij = relu(W2 relu(W1 e) + b2)       W1: 64 x 16 (column 9 = bias)
+ *   e = [(x_i - x_j) inv_h (3), |x_i - x_j| inv_h, s (rho_i inv_rho0 - 1), 1, 0, 0,
+ *        (v_j - v_i) inv_v (3), s (rho_j inv_rho0 - 1), 0 (4)]       (s = rho_scale)
+ *   m_ij = relu(W2 relu(W1 e) + b2)       W1: 64 x 16 (column 5 = bias)
  * summed over the neighbours, H_i = sum_j m_ij.  With the node features
  *   q_i = [s (rho_i inv_rho0 - 1), p_i inv_p, nnb_i inv_nnb]
  * the acceleration is a_i = a_scale (W3 [agg_scale H_i ; q_i] + b3), W3: 3 x 67.
@@ -377,17 +378,23 @@ typedef struct sph_learned_report {
     double drift_last;      /* |D - D_anchor| of the last surrogate step    */
     double anchor;          /* D of the last plain step                     */
     double sim_time;        /* clock after the call                         */
-    double reserved[4];
+    /* With sph_set_timing on: mean device time (ms) per kept surrogate step
+     * of the density-and-neighbour-list kernel, of the surrogate kernel and
+     * of the whole step (HIP events on the context stream); else 0.       */
+    double density_ms, mlp_ms, coarse_ms;
+    double pairs;           /* mean neighbour pairs per kept surrogate step  */
 } sph_learned_report;
 
 /* Advance ncoarse surrogate steps of factor x time_step each: the plain
  * integrate (coarse step), sort, reorder and density kernels, then
  * v += a dt_coarse from the surrogate.  D = mean_i |rho_i / rho0 - 1| of
  * each surrogate step is compared with D of the last plain step (the
- * anchor); a step whose drift |D - anchor| exceeds tol is undone and
- * replaced by `factor` plain steps, which set a new anchor.  The current
- * state must come from a plain step.  Single-scene contexts with a fixed
- * time step (cfl = 0, t_end = 0).                                          */
+ * anchor); a step whose drift |D - anchor| exceeds tol (or whose
+ * neighbour lists overflow) is undone and replaced by `factor` plain steps,
+ * which set a new anchor.  D is read back once per tune[2] steps (the
+ * steps are deterministic, so a late check gives the same trajectory as a
+ * check after every step).  The current state must come from a step.
+ * Single-scene contexts with a fixed time step (cfl = 0, t_end = 0).       */
 int sph_step_learned(sph_ctx* ctx, sph_model* model, uint32_t ncoarse, uint32_t factor, float tol,
                      sph_learned_report* out);
 

@@ -30,7 +30,10 @@ from __future__ import annotations
 
 import numpy as np
 
-HID, FEAT, BIAS_FEAT, OUT, NODE = 64, 16, 9, 3, 3
+HID, FEAT, BIAS_FEAT, OUT, NODE = 64, 16, 5, 3, 3
+# W1 columns that carry a feature (the rest of 0..15 besides BIAS_FEAT are padding)
+FEATURE_COLUMNS = (0, 1, 2, 3, 4, 8, 9, 10, 11)
+PAD_COLUMNS = (6, 7, 12, 13, 14, 15)
 W3_COLS = HID + NODE
 OFF_W1 = 0
 OFF_W2 = OFF_W1 + HID * FEAT
@@ -94,11 +97,11 @@ def features(pos, vel, rho, pi, pj, norm: dict, emulate: bool = True) -> np.ndar
     dv = (v[pj] - v[pi]).astype(f32)
     F = np.zeros((pi.size, FEAT), np.float32)
     F[:, 0:3] = d * inv_h
-    F[:, 3:6] = dv * inv_v
-    F[:, 6] = dist * inv_h
-    F[:, 7] = ((r[pi] * inv_rho0).astype(f32) - f32(1.0)) * rs
-    F[:, 8] = ((r[pj] * inv_rho0).astype(f32) - f32(1.0)) * rs
-    F[:, 9] = 1.0
+    F[:, 3] = dist * inv_h
+    F[:, 4] = ((r[pi] * inv_rho0).astype(f32) - f32(1.0)) * rs
+    F[:, BIAS_FEAT] = 1.0
+    F[:, 8:11] = dv * inv_v
+    F[:, 11] = ((r[pj] * inv_rho0).astype(f32) - f32(1.0)) * rs
     return bf16(F) if emulate else F.astype(np.float64)
 
 
@@ -164,18 +167,18 @@ def probe_params(agg: float) -> np.ndarray:
     in both halves of the layer):
       a_x / a_scale = number of neighbours
       a_y / a_scale = sum_j bf16(|x_i - x_j| / h)
-      a_z / a_scale = sum_j bf16(1 + bf16(s (rho_j / rho0 - 1)))"""
+      a_z / a_scale = sum_j bf16(max(0, 1 + bf16(s (rho_j / rho0 - 1))))"""
     W1 = np.zeros((64, 16))
     W2 = np.zeros((64, 64))
     W3 = np.zeros((3, 67))
-    W1[0, 9] = 1.0
+    W1[0, BIAS_FEAT] = 1.0
     W2[5, 0] = 1.0
     W3[0, 5] = 1.0 / agg
-    W1[37, 6] = 1.0
+    W1[37, 3] = 1.0
     W2[50, 37] = 1.0
     W3[1, 50] = 1.0 / agg
-    W1[20, 8] = 1.0
-    W1[20, 9] = 1.0
+    W1[20, 11] = 1.0
+    W1[20, BIAS_FEAT] = 1.0
     W2[60, 20] = 1.0
     W3[2, 60] = 1.0 / agg
     return pack(W1, W2, np.zeros(64), W3, np.zeros(3))
@@ -191,7 +194,8 @@ def probe_expected(pos, rho, h: float, norm: dict) -> np.ndarray:
     dist = np.sqrt(((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]).astype(f32) + d[:, 2] * d[:, 2]).astype(f32))
     fr = bf16((dist * f32(norm["inv_h"])).astype(f32))
     rs = f32(norm.get("rho_scale", 1.0) or 1.0)
-    frho = bf16(1.0 + bf16(((np.asarray(rho, f32)[pj] * f32(norm["inv_rho0"])).astype(f32) - f32(1.0)) * rs))
+    frho = bf16(np.maximum(0.0, 1.0 + bf16(((np.asarray(rho, f32)[pj] * f32(norm["inv_rho0"])).astype(f32)
+                                             - f32(1.0)) * rs)))
     want = np.zeros((n, 3))
     want[:, 0] = np.bincount(pi, minlength=n)
     want[:, 1] = np.bincount(pi, weights=fr, minlength=n)

@@ -12,13 +12,14 @@ namespace sph {
 
 constexpr uint32_t kEmptyCell = 0xFFFFFFFFu;
 // Neighbour-list capacity per particle (learned surrogate).
-constexpr uint32_t kNbrCap = 128;
+constexpr uint32_t kNbrCap = 128;  // a multiple of 4 (rows are written as uint4)
 
 // Counters written by the step kernels (device memory, uint32 each).
 enum Counter : int {
     kCntCapCells = 0,   // cells with more than max_par_in_cell particles
     kCntOutOfGrid = 1,  // particles whose own cell index was clamped
     kCntNbrOverflow = 2,  // particles with more than kNbrCap listed neighbours
+    kCntPairs = 3,        // listed neighbour pairs (min(count, kNbrCap) summed)
     kCntNum = 4
 };
 
@@ -112,7 +113,7 @@ hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipSt
 hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
                           const uint32_t* old_slot, hipStream_t s);
 hipError_t launch_density(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
-// Density plus each slot's neighbour list: nbr[c * n + k] (c < kNbrCap),
+// Density plus each slot's neighbour list: nbr[k * kNbrCap + c] (c < kNbrCap),
 // nnb[k] = neighbour count (may exceed kNbrCap: counted in kCntNbrOverflow).
 hipError_t launch_density_lists(const LaunchCfg& c, const StepBuffers& b, uint32_t* nbr, uint32_t* nnb,
                                 hipStream_t s);

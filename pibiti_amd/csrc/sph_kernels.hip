@@ -144,11 +144,12 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 
 // ------------------------------------------------------------------ density
 // With kLists the density walk also records each particle's neighbours
-// (the slots that pass r2 < h2, j != i, in walk order) entry-major,
-// nbr[c * n + i], c < kNbrCap, and their count in nnb[i]; a count above
-// kNbrCap raises the overflow counter (entries past the cap are dropped).
+// (the slots that pass r2 < h2, j != i, in walk order) in its own row,
+// nbr[i * kNbrCap + c], c < kNbrCap, written four entries at a time, and
+// their count in nnb[i]; a count above kNbrCap raises the overflow counter
+// (entries past the cap are dropped).
 template <bool kBatch, int kPrec, bool kLists = false>
-__device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
+__device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& one,
                                             const sph_params* __restrict__ tab,
                                             const uint32_t* __restrict__ sid,
                                             const float4* __restrict__ pos,
@@ -167,6 +168,8 @@ __device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
     float sum = 0.0f;
     KahanF ks{0.0f, 0.0f};
     uint32_t count = 0;
+    uint4 stage = make_uint4(0u, 0u, 0u, 0u);  // entries (count & ~3) .. (count & ~3) + 3
+    uint32_t* row = kLists ? nbr + (size_t)i * kNbrCap : nullptr;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
         if (!w.open(ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
@@ -190,7 +193,12 @@ __device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
             }
             if constexpr (kLists) {
                 if (hit) {
-                    if (count < kNbrCap) nbr[(size_t)count * n + i] = j;
+                    const uint32_t q = count & 3u;
+                    stage.x = q == 0u ? j : stage.x;
+                    stage.y = q == 1u ? j : stage.y;
+                    stage.z = q == 2u ? j : stage.z;
+                    stage.w = q == 3u ? j : stage.w;
+                    if (q == 3u && count < kNbrCap) *reinterpret_cast<uint4*>(row + (count - 3u)) = stage;
                     ++count;
                 }
             }
@@ -201,9 +209,12 @@ __device__ __forceinline__ void density_one(uint32_t i, const sph_params& one,
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
     rhop[i] = make_float2(rho, eos(p, rho));
     if constexpr (kLists) {
+        // the last, partly filled group of four (its unused entries are never read)
+        if ((count & 3u) != 0u && count < kNbrCap) *reinterpret_cast<uint4*>(row + (count & ~3u)) = stage;
         nnb[i] = count;
         if (count > kNbrCap) atomicAdd(&counters[kCntNbrOverflow], 1u);
     }
+    return count;
 }
 
 template <bool kBatch, int kPrec>
@@ -213,9 +224,14 @@ __global__ __launch_bounds__(kBlock) void k_density_lists(
     const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ nbr,
     uint32_t* __restrict__ nnb, uint32_t* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    density_one<kBatch, kPrec, true>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n, nbr, nnb,
-                                     counters);
+    uint32_t listed = 0;
+    if (i < n)
+        listed = min(density_one<kBatch, kPrec, true>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n,
+                                                      nbr, nnb, counters),
+                     kNbrCap);
+    // pairs the surrogate will process: one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) listed += __shfl_xor(listed, off);
+    if ((threadIdx.x & 63) == 0 && listed) atomicAdd(&counters[kCntPairs], listed);
 }
 
 template <bool kBatch, int kPrec>

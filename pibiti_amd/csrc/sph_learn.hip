@@ -48,11 +48,16 @@ struct LearnState {
     float4* svel = nullptr;
     uint32_t* spid = nullptr;
     double* dev_part = nullptr;
-    double* dev = nullptr;
+    double* dev = nullptr;       // D of each surrogate step of a chunk
+    uint32_t* ovf = nullptr;     // per step: neighbour-list overflow, listed pairs
     float* dt_coarse = nullptr;
-    double* host = nullptr;  // pinned: [deviation, overflow count]
+    double* host = nullptr;      // pinned copies of dev / ovf
     uint32_t* host_cnt = nullptr;
 };
+
+// Surrogate steps between host checks (sph_params::tune[2], 0 = default).
+constexpr uint32_t kDefaultCheck = 4;
+constexpr uint32_t kMaxCheck = 64;
 
 void learn_release(Engine& e) {
     LearnState* L = e.learn;
@@ -61,7 +66,7 @@ void learn_release(Engine& e) {
                     L->nbr,      L->nnb,         L->acc,          L->tb.h,         L->tb.out,
                     L->tb.dh,    L->tb.part_pair, L->tb.part_node, L->tb.loss_part, L->tb.grad,
                     L->tb.loss,  L->lnbr,        L->lnnb,         L->spos,         L->svel,
-                    L->spid,     L->dev_part,    L->dev,          L->dt_coarse};
+                    L->spid,     L->dev_part,    L->dev,          L->dt_coarse,   L->ovf};
     for (void* q : ptrs) (void)hipFree(q);
     if (L->host) (void)hipHostFree(L->host);
     if (L->host_cnt) (void)hipHostFree(L->host_cnt);
@@ -118,10 +123,12 @@ hipError_t ensure_step(LearnState& L, uint32_t nscenes) {
     if (e == hipSuccess) e = dalloc(&L.svel, N);
     if (e == hipSuccess) e = dalloc(&L.spid, N);
     if (e == hipSuccess) e = dalloc(&L.dev_part, (size_t)256);
-    if (e == hipSuccess) e = dalloc(&L.dev, (size_t)1);
+    if (e == hipSuccess) e = dalloc(&L.dev, (size_t)kMaxCheck);
+    if (e == hipSuccess) e = dalloc(&L.ovf, (size_t)2 * kMaxCheck);
     if (e == hipSuccess) e = dalloc(&L.dt_coarse, (size_t)nscenes);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.host), 2 * sizeof(double));
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.host_cnt), 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.host), kMaxCheck * sizeof(double));
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&L.host_cnt), 2 * kMaxCheck * sizeof(uint32_t));
     if (e == hipSuccess) L.step_ready = true;
     return e;
 }
@@ -296,9 +303,12 @@ int learn_get_window(Engine& e, uint32_t scene, int which, float* out) {
 }
 
 // One surrogate step: plain integrate (coarse dt), sort, reorder, density
-// with neighbour lists, density deviation, surrogate velocity update.
-int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m) {
+// with neighbour lists, density deviation into dev_out and the neighbour
+// overflow count into ovf_out (both device), surrogate velocity update.
+int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m, double* dev_out, uint32_t* ovf_out,
+                   bool timed) {
     const LaunchCfg c = e.launch_cfg();
+    if (timed) SPH_TRY(hipEventRecord(e.ev[0], e.stream));
     SPH_TRY(launch_fill_u32(e.counters, 0u, kCntNum, e.stream));
     StepBuffers b = e.buffers();
     b.dt_dev = L.dt_coarse;
@@ -312,8 +322,12 @@ int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m) {
     e.cur_pid ^= 1;
     e.last_skey = skey;
     b = e.buffers();
+    if (timed) SPH_TRY(hipEventRecord(e.ev[1], e.stream));
     SPH_TRY(launch_density_lists(c, b, L.lnbr, L.lnnb, e.stream));
-    SPH_TRY(mlp_density_deviation(c.n, b.rhop, 1.0f / e.scenes[0].rest_density, L.dev_part, L.dev, e.stream));
+    if (timed) SPH_TRY(hipEventRecord(e.ev[2], e.stream));
+    SPH_TRY(mlp_density_deviation(c.n, b.rhop, 1.0f / e.scenes[0].rest_density, L.dev_part, dev_out, e.stream));
+    SPH_TRY(hipMemcpyAsync(ovf_out, e.counters + kCntNbrOverflow, 2 * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, e.stream));
     MlpState st;
     st.n = c.n;
     st.pos = b.pos_cur;
@@ -322,21 +336,29 @@ int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m) {
     st.sid = b.sid_cur;
     st.nbr = L.lnbr;
     st.nnb = L.lnnb;
+    if (timed) SPH_TRY(hipEventRecord(e.ev[3], e.stream));
     SPH_TRY(mlp_infer(c, m.norm, m.params, m.pk, st, b.vel_alt, nullptr, L.dt_coarse, e.stream));
+    if (timed) SPH_TRY(hipEventRecord(e.ev[4], e.stream));
     e.cur_vel ^= 1;
     return SPH_OK;
 }
 
-int read_deviation(Engine& e, LearnState& L, double* dev, uint32_t* overflow) {
+// D of the current state (after a plain step), synchronously.
+int plain_deviation(Engine& e, LearnState& L, double* out) {
+    SPH_TRY(mlp_density_deviation(e.cfg.n, e.rhop, 1.0f / e.scenes[0].rest_density, L.dev_part, L.dev,
+                                  e.stream));
     SPH_TRY(hipMemcpyAsync(L.host, L.dev, sizeof(double), hipMemcpyDeviceToHost, e.stream));
-    SPH_TRY(hipMemcpyAsync(L.host_cnt, e.counters, kCntNum * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           e.stream));
     SPH_TRY(hipStreamSynchronize(e.stream));
-    *dev = L.host[0];
-    if (overflow) *overflow = L.host_cnt[kCntNbrOverflow];
+    *out = L.host[0];
     return SPH_OK;
 }
 
+// Surrogate steps run in chunks of `check` steps between host checks.  A
+// chunk starts from a snapshot of the state; its steps' deviations D are
+// read back once.  The first step whose drift |D - anchor| exceeds tol (or
+// whose neighbour lists overflowed) is replaced: the chunk is undone, the
+// steps before it are taken again (the surrogate step is deterministic, so
+// they reproduce exactly) and `factor` plain steps stand in for it.
 int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, float tol,
                  sph_learned_report* rep) {
     SPH_TRY_ST(check_model(e, m));
@@ -348,53 +370,85 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
     DeviceGuard guard(e.device);
     SPH_TRY(ensure_step(*L, e.cfg.nscenes));
     e.drop_graph();
+    const uint32_t check = std::min<uint32_t>(e.cfg.tune[2] ? e.cfg.tune[2] : kDefaultCheck, kMaxCheck);
     const size_t N = e.cfg.n;
-    const float inv_rho0 = 1.0f / e.scenes[0].rest_density;
     const double dt = (double)e.scenes[0].time_step;
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
     SPH_TRY(mlp_scale_dt(e.cfg.nscenes, e.dt_dev, (float)factor, L->dt_coarse, e.stream));
     double anchor = 0.0;
-    SPH_TRY(mlp_density_deviation(e.cfg.n, e.rhop, inv_rho0, L->dev_part, L->dev, e.stream));
-    SPH_TRY_ST(read_deviation(e, *L, &anchor, nullptr));
+    SPH_TRY_ST(plain_deviation(e, *L, &anchor));
     sph_learned_report r{};
-    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
-    for (uint32_t s = 0; s < ncoarse; ++s) {
+    uint32_t done = 0;
+    double t_dens = 0.0, t_mlp = 0.0, t_step = 0.0, pairs = 0.0;
+    uint32_t timed_steps = 0;
+    while (done < ncoarse) {
+        const uint32_t chunk = std::min(check, ncoarse - done);
         const int p0 = e.cur_pos, v0 = e.cur_vel, q0 = e.cur_pid;
         SPH_TRY(hipMemcpyAsync(L->spos, e.pos[p0], N * 16, k, e.stream));
         SPH_TRY(hipMemcpyAsync(L->svel, e.vel[v0], N * 16, k, e.stream));
         SPH_TRY(hipMemcpyAsync(L->spid, e.pid[q0], N * 4, k, e.stream));
-        SPH_TRY_ST(enqueue_coarse(e, *L, *m));
-        double dev = 0.0;
-        uint32_t overflow = 0;
-        SPH_TRY_ST(read_deviation(e, *L, &dev, &overflow));
-        const double drift = std::fabs(dev - anchor);
-        r.drift_last = drift;
-        if (drift > (double)tol || overflow) {
-            // undo the surrogate step and take `factor` plain steps instead
+        // with timing on, the events bracket the chunk's last step
+        for (uint32_t i = 0; i < chunk; ++i)
+            SPH_TRY_ST(enqueue_coarse(e, *L, *m, L->dev + i, L->ovf + 2 * i, e.timing && i + 1 == chunk));
+        SPH_TRY(hipMemcpyAsync(L->host, L->dev, chunk * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+        SPH_TRY(hipMemcpyAsync(L->host_cnt, L->ovf, 2 * chunk * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               e.stream));
+        SPH_TRY(hipStreamSynchronize(e.stream));
+        if (e.timing) {
+            float a = 0.0f, b = 0.0f, c = 0.0f;
+            SPH_TRY(hipEventElapsedTime(&a, e.ev[1], e.ev[2]));
+            SPH_TRY(hipEventElapsedTime(&b, e.ev[3], e.ev[4]));
+            SPH_TRY(hipEventElapsedTime(&c, e.ev[0], e.ev[4]));
+            t_dens += a;
+            t_mlp += b;
+            t_step += c;
+            timed_steps += 1;
+        }
+        uint32_t bad = chunk;
+        for (uint32_t i = 0; i < chunk; ++i) {
+            const double drift = std::fabs(L->host[i] - anchor);
+            if (drift > (double)tol || L->host_cnt[2 * i] != 0) {
+                bad = i;
+                r.drift_last = drift;
+                break;
+            }
+            r.drift_max = std::max(r.drift_max, drift);
+            r.drift_last = drift;
+            pairs += L->host_cnt[2 * i + 1];
+        }
+        if (bad < chunk) {
             e.cur_pos = p0;
             e.cur_vel = v0;
             e.cur_pid = q0;
             SPH_TRY(hipMemcpyAsync(e.pos[p0], L->spos, N * 16, k, e.stream));
             SPH_TRY(hipMemcpyAsync(e.vel[v0], L->svel, N * 16, k, e.stream));
             SPH_TRY(hipMemcpyAsync(e.pid[q0], L->spid, N * 4, k, e.stream));
+            for (uint32_t i = 0; i < bad; ++i)
+                SPH_TRY_ST(enqueue_coarse(e, *L, *m, L->dev + i, L->ovf + 2 * i, false));
             for (uint32_t f = 0; f < factor; ++f) {
                 SPH_TRY_ST(e.enqueue_step(false));
                 e.after_steps(1);
             }
-            SPH_TRY(mlp_density_deviation(e.cfg.n, e.rhop, inv_rho0, L->dev_part, L->dev, e.stream));
-            SPH_TRY_ST(read_deviation(e, *L, &anchor, nullptr));
+            SPH_TRY_ST(plain_deviation(e, *L, &anchor));
             r.fallbacks += 1;
             r.plain_steps += factor;
-        } else {
-            r.coarse_steps += 1;
-            r.drift_max = std::max(r.drift_max, drift);
-            e.steps += 1;
-            e.host_clock += dt * factor;
-            e.stepped = true;
-            e.last_kind = 2;
         }
+        const uint32_t kept = bad;
+        r.coarse_steps += kept;
+        e.steps += kept;
+        e.host_clock += dt * factor * kept;
+        e.stepped = true;
+        if (bad == chunk) e.last_kind = 2;
+        done += kept + (bad < chunk ? 1 : 0);
     }
     r.anchor = anchor;
     r.sim_time = e.host_clock;
+    if (r.coarse_steps) r.pairs = pairs / r.coarse_steps;
+    if (timed_steps) {
+        r.density_ms = t_dens / timed_steps;
+        r.mlp_ms = t_mlp / timed_steps;
+        r.coarse_ms = t_step / timed_steps;
+    }
     if (rep) *rep = r;
     return SPH_OK;
 }
@@ -424,9 +478,10 @@ int model_create(int device, const sph_mlp_norm* norm, uint64_t seed, sph_model*
         const double u2 = (next() >> 11) * (1.0 / 9007199254740992.0);
         return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
     };
-    const double s1 = std::sqrt(2.0 / mlp::kBiasFeat), s2 = std::sqrt(2.0 / mlp::kHid), s3 = 0.01;
+    const double s1 = std::sqrt(2.0 / 9.0), s2 = std::sqrt(2.0 / mlp::kHid), s3 = 0.01;
     for (int u = 0; u < mlp::kHid; ++u)
-        for (int f = 0; f < mlp::kBiasFeat; ++f) w[mlp::kOffW1 + u * mlp::kFeat + f] = (float)(s1 * normal());
+        for (int f = 0; f < mlp::kFeat; ++f)
+            if (mlp::is_feature_column(f)) w[mlp::kOffW1 + u * mlp::kFeat + f] = (float)(s1 * normal());
     for (int k = 0; k < mlp::kHid * mlp::kHid; ++k) w[mlp::kOffW2 + k] = (float)(s2 * normal());
     for (int k = 0; k < mlp::kOut * mlp::kW3Cols; ++k) w[mlp::kOffW3 + k] = (float)(s3 * normal());
     DeviceGuard guard(device);

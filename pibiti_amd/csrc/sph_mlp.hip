@@ -90,29 +90,31 @@ struct MlpArgs {
     float* part_pair;     // backward
 };
 
-// What one lane gathers for one pair: g = 0 lanes the position and velocity
-// of the neighbour, g = 1 lanes its density.
+// What one lane gathers for one pair: g = 0 lanes the neighbour's position,
+// g = 1 lanes its velocity and density.
 struct PairRaw {
-    float4 a, b;
+    float4 a;
+    float b;
 };
 
 __device__ __forceinline__ PairRaw load_pair(int g, bool valid, uint32_t j, const MlpState& st) {
     PairRaw r;
     r.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    r.b = r.a;
+    r.b = 0.0f;
     if (valid) {
         if (g == 0) {
             r.a = st.pos[j];
-            r.b = st.vel[j];
         } else {
-            r.a.x = st.rhop[j].x;
+            r.a = st.vel[j];
+            r.b = st.rhop[j].x;
         }
     }
     return r;
 }
 
-// Edge features of one pair as this lane's half of the k dimension:
-// g = 0 supplies features 0..7, g = 1 features 8..15.
+// Edge features of one pair as this lane's half of the k dimension.
+//   g = 0 (k 0..7):  (x_i - x_j)/h (3), |x_i - x_j|/h, s(rho_i/rho0 - 1), 1, 0, 0
+//   g = 1 (k 8..15): (v_j - v_i)/v (3), s(rho_j/rho0 - 1), 0 (4)
 __device__ __forceinline__ bf16x8 edge_features(int g, bool valid, const PairRaw& r, const float4& xi,
                                                 const float4& vi, float rhoi, const sph_mlp_norm& nm) {
     bf16x8 f;
@@ -125,16 +127,27 @@ __device__ __forceinline__ bf16x8 edge_features(int g, bool valid, const PairRaw
         f[0] = bf(rx * nm.inv_h);
         f[1] = bf(ry * nm.inv_h);
         f[2] = bf(rz * nm.inv_h);
-        f[3] = bf((r.b.x - vi.x) * nm.inv_v);
-        f[4] = bf((r.b.y - vi.y) * nm.inv_v);
-        f[5] = bf((r.b.z - vi.z) * nm.inv_v);
-        f[6] = bf(d * nm.inv_h);
-        f[7] = bf((rhoi * nm.inv_rho0 - 1.0f) * nm.rho_scale);
+        f[3] = bf(d * nm.inv_h);
+        f[4] = bf((rhoi * nm.inv_rho0 - 1.0f) * nm.rho_scale);
+        f[5] = bf(1.0f);  // feature kBiasFeat
     } else {
-        f[0] = bf((r.a.x * nm.inv_rho0 - 1.0f) * nm.rho_scale);
-        f[1] = bf(1.0f);  // feature kBiasFeat
+        f[0] = bf((r.a.x - vi.x) * nm.inv_v);
+        f[1] = bf((r.a.y - vi.y) * nm.inv_v);
+        f[2] = bf((r.a.z - vi.z) * nm.inv_v);
+        f[3] = bf((r.b * nm.inv_rho0 - 1.0f) * nm.rho_scale);
     }
     return f;
+}
+
+// A target's neighbour row is read four entries at a time: `cur` holds the
+// group of four containing the next tile's entry, `nxt` the group after it.
+__device__ __forceinline__ uint4 nbr_group(const uint32_t* row, int first, int cnt) {
+    return first < cnt ? *reinterpret_cast<const uint4*>(row + first) : make_uint4(0u, 0u, 0u, 0u);
+}
+
+__device__ __forceinline__ uint32_t nbr_pick(const uint4& q, int t) {
+    const int e = t & 3;
+    return e == 0 ? q.x : (e == 1 ? q.y : (e == 2 ? q.z : q.w));
 }
 
 // Layer 1, transposed: the hidden units of pair (lane % 32) as the four
@@ -188,7 +201,6 @@ __global__ __launch_bounds__(kBlk) void k_mlp_fwd(const MlpArgs a) {
     __syncthreads();
     const sph_mlp_norm nm = a.nm;
     const MlpState st = a.st;
-    const size_t ns = st.n;
 
     bf16x8 w1f[2], w2f[2][4];
 #pragma unroll
@@ -197,40 +209,44 @@ __global__ __launch_bounds__(kBlk) void k_mlp_fwd(const MlpArgs a) {
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) w2f[mt][kb] = a.w2f[(mt * 4 + kb) * 64 + lane];
+    // b2 in the transposed accumulator layout: the layer-2 products start from it
+    f32x16 b2t[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) b2t[mt][r] = s_b2[32 * mt + mfma32_row(r, lane)];
 
     const uint32_t nwaves = gridDim.x * kWaves;
     for (uint32_t grp = blockIdx.x * kWaves + wv; grp < a.ngroups; grp += nwaves) {
-        // keeps the LDS-resident W3 and b2 in LDS (hoisting them out of the
-        // loop would take 128 registers per lane)
+        // keeps the LDS-resident W3 in LDS (hoisting it out of the loop would
+        // take 96 registers per lane)
         asm volatile("" ::: "memory");
         const Group G = open_group(grp, p, st);
         f32x16 hacc[2] = {zero16(), zero16()};
-        // two-deep prefetch: neighbour data one tile ahead, its index two
-        uint32_t jn = 0 < G.cT ? st.nbr[G.iT] : 0u;
-        PairRaw rn = load_pair(g, 0 < G.cT, jn, st);
-        jn = 1 < G.cT ? st.nbr[ns + G.iT] : 0u;
+        // neighbour indices four at a time; the neighbour's data one tile ahead
+        const uint32_t* nrow = st.nbr + (size_t)G.iT * kNbrCap;
+        uint4 ncur = nbr_group(nrow, 0, G.cT), nnxt = nbr_group(nrow, 4, G.cT);
+        PairRaw rn = load_pair(g, 0 < G.cT, ncur.x, st);
         for (int t = 0; t < G.m; ++t) {
             const bool valid = t < G.cT;
             const PairRaw rc = rn;
-            rn = load_pair(g, t + 1 < G.cT, jn, st);
-            jn = (t + 2 < G.cT) ? st.nbr[(size_t)(t + 2) * ns + G.iT] : 0u;
+            const int t1 = t + 1;
+            if ((t1 & 3) == 0) {
+                ncur = nnxt;
+                nnxt = nbr_group(nrow, t1 + 4, G.cT);
+            }
+            rn = load_pair(g, t1 < G.cT, nbr_pick(ncur, t1), st);
             const bf16x8 E = edge_features(g, valid, rc, G.xT, G.vT, G.rT, nm);
             bf16x8 h1t[4];
             layer1_t(w1f, E, h1t);
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
-                f32x16 c = zero16();
+                f32x16 c = b2t[mt];  // pre2 = b2 + W2 h1
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) c = mfma32(w2f[mt][kb], h1t[kb], c);
                 if (valid) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float4 b = *reinterpret_cast<const float4*>(s_b2 + 32 * mt + 8 * q + 4 * g);
-                        hacc[mt][4 * q + 0] += relu(c[4 * q + 0] + b.x);
-                        hacc[mt][4 * q + 1] += relu(c[4 * q + 1] + b.y);
-                        hacc[mt][4 * q + 2] += relu(c[4 * q + 2] + b.z);
-                        hacc[mt][4 * q + 3] += relu(c[4 * q + 3] + b.w);
-                    }
+                    for (int r = 0; r < 16; ++r) hacc[mt][r] += relu(c[r]);
                 }
             }
         }
@@ -306,7 +322,6 @@ __global__ __launch_bounds__(kBlk) void k_mlp_bwd(const MlpArgs a) {
     float* D = s_dh + wv * kGroup * kDhStride;
     const sph_mlp_norm nm = a.nm;
     const MlpState st = a.st;
-    const size_t ns = st.n;
 
     bf16x8 w1f[2], w2f[2][4], ifrag;
 #pragma unroll
@@ -357,14 +372,19 @@ __global__ __launch_bounds__(kBlk) void k_mlp_bwd(const MlpArgs a) {
                 dht[mt][4 * q + 3] = v.w;
             }
 
-        uint32_t jn = 0 < G.cT ? st.nbr[G.iT] : 0u;
-        PairRaw rn = load_pair(g, 0 < G.cT, jn, st);
-        jn = 1 < G.cT ? st.nbr[ns + G.iT] : 0u;
+        // neighbour indices four at a time; the neighbour's data one tile ahead
+        const uint32_t* nrow = st.nbr + (size_t)G.iT * kNbrCap;
+        uint4 ncur = nbr_group(nrow, 0, G.cT), nnxt = nbr_group(nrow, 4, G.cT);
+        PairRaw rn = load_pair(g, 0 < G.cT, ncur.x, st);
         for (int t = 0; t < G.m; ++t) {
             const bool valid = t < G.cT;
             const PairRaw rc = rn;
-            rn = load_pair(g, t + 1 < G.cT, jn, st);
-            jn = (t + 2 < G.cT) ? st.nbr[(size_t)(t + 2) * ns + G.iT] : 0u;
+            const int t1 = t + 1;
+            if ((t1 & 3) == 0) {
+                ncur = nnxt;
+                nnxt = nbr_group(nrow, t1 + 4, G.cT);
+            }
+            rn = load_pair(g, t1 < G.cT, nbr_pick(ncur, t1), st);
             const bf16x8 E = edge_features(g, valid, rc, G.xT, G.vT, G.rT, nm);
             const uint32_t vm = (uint32_t)__ballot(valid);  // bit r: pair row r valid
 

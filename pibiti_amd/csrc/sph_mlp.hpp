@@ -9,7 +9,10 @@ namespace mlp {
 
 constexpr int kHid = 64;       // hidden units of both pair layers
 constexpr int kFeat = 16;      // edge features incl. the constant and padding
-constexpr int kBiasFeat = 9;   // feature fixed at 1: column 9 of W1 is its bias
+constexpr int kBiasFeat = 5;   // feature fixed at 1: column 5 of W1 is its bias
+// W1 columns that carry a feature: 0..4 (position, distance, rho_i) and
+// 8..11 (velocity, rho_j); 5 is the bias, the rest are padding.
+__host__ __device__ constexpr bool is_feature_column(int k) { return k < kBiasFeat || (k >= 8 && k < 12); }
 constexpr int kOut = 3;
 constexpr int kNode = 3;
 constexpr int kW3Cols = kHid + kNode;                 // 67
@@ -42,7 +45,7 @@ struct MlpState {
     const float4* vel = nullptr;
     const float2* rhop = nullptr;
     const uint32_t* sid = nullptr;  // batch only
-    const uint32_t* nbr = nullptr;  // neighbour lists, nbr[c * n + k], c < kNbrCap
+    const uint32_t* nbr = nullptr;  // neighbour lists, nbr[k * kNbrCap + c], c < kNbrCap
     const uint32_t* nnb = nullptr;  // neighbour counts
 };
 

@@ -159,7 +159,10 @@ class LearnedReport(C.Structure):
         ("drift_last", C.c_double),
         ("anchor", C.c_double),
         ("sim_time", C.c_double),
-        ("reserved", C.c_double * 4),
+        ("density_ms", C.c_double),
+        ("mlp_ms", C.c_double),
+        ("coarse_ms", C.c_double),
+        ("pairs", C.c_double),
     ]
 
     def as_dict(self) -> dict:

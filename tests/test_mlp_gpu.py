@@ -62,7 +62,7 @@ def oracle_inputs(w, p, nm):
 def random_weights(seed):
     rng = np.random.default_rng(seed)
     W1 = rng.normal(0, 0.5, (64, 16))
-    W1[:, 10:] = 0.0
+    W1[:, list(mo.PAD_COLUMNS)] = 0.0
     W2 = rng.normal(0, 0.2, (64, 64))
     b2 = rng.normal(0, 0.05, 64)
     W3 = rng.normal(0, 0.3, (3, 67))

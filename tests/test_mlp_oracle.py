@@ -40,6 +40,7 @@ def test_parameter_layout():
     u = mo.unpack(w)
     np.testing.assert_array_equal(mo.pack(u["W1"], u["W2"], u["b2"], u["W3"], u["b3"]), w)
     assert u["W1"][2, 9] == 2 * 16 + 9 and u["W3"][1, 0] == mo.OFF_W3 + 67
+    assert sorted(mo.FEATURE_COLUMNS + mo.PAD_COLUMNS + (mo.BIAS_FEAT,)) == list(range(16))
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -52,7 +53,7 @@ def test_gradient_matches_finite_differences(seed):
     q = mo.node_features(rho, pres, nnb, NORM)
     rng = np.random.default_rng(10 + seed)
     W1 = rng.normal(0, 0.6, (64, 16))
-    W1[:, 10:] = 0.0
+    W1[:, list(mo.PAD_COLUMNS)] = 0.0
     W2 = rng.normal(0, 0.25, (64, 64))
     b2 = rng.normal(0, 0.1, 64)
     W3 = rng.normal(0, 0.3, (3, 67))
@@ -66,7 +67,8 @@ def test_gradient_matches_finite_differences(seed):
 
     # every node-layer parameter and a sample of each pair layer
     idx = list(range(mo.OFF_W3, mo.NUM_PARAMS))
-    idx += list(rng.choice(np.arange(mo.OFF_W1, mo.OFF_W2).reshape(64, 16)[:, :10].ravel(), 40, replace=False))
+    w1_idx = np.arange(mo.OFF_W1, mo.OFF_W2).reshape(64, 16)[:, list(mo.FEATURE_COLUMNS) + [mo.BIAS_FEAT]]
+    idx += list(rng.choice(w1_idx.ravel(), 40, replace=False))
     idx += list(rng.choice(np.arange(mo.OFF_W2, mo.OFF_B2), 60, replace=False))
     idx += list(rng.choice(np.arange(mo.OFF_B2, mo.OFF_W3), 20, replace=False))
     eps = 1e-6
@@ -79,9 +81,9 @@ def test_gradient_matches_finite_differences(seed):
         fd = (L(p1) - L(p2)) / (2 * eps)
         worst = max(worst, abs(fd - grad[k]) / (abs(grad[k]) + 1e-4 * abs(loss) + 1e-8))
     assert worst < 1e-4, worst
-    # the unused feature columns get exactly zero gradient
+    # the padding columns get exactly zero gradient
     g1 = grad[mo.OFF_W1:mo.OFF_W2].reshape(64, 16)
-    assert np.all(g1[:, 10:] == 0.0)
+    assert np.all(g1[:, list(mo.PAD_COLUMNS)] == 0.0)
 
 
 def test_probe_network_closed_forms():
