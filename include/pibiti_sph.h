@@ -149,7 +149,10 @@ typedef struct sph_params {
      * grid, 0 = one thread per particle, else a persistent grid of tune[1]
      * blocks of 256 threads pulling 256-particle tiles; tune[2]: surrogate
      * steps per host check in sph_step_learned (0 = 4, at most 64);
-     * tune[3]: reserved.                                                   */
+     * tune[3]: gather variant with tune[1] == 0 (0 = per-lane loads in
+     * XCD-ordered tiles, 1 = candidate positions staged in LDS per tile,
+     * 2 = neighbour-row cell tables staged in LDS, 3 = per-lane loads in
+     * launch order).                                                       */
     uint32_t tune[4];
     /* First global cell key of this scene: 0 for a single-scene context; a
      * batch gives each scene a disjoint key range (set by sph_batch_create). */

@@ -123,6 +123,30 @@ struct CellWalk {
         e = min(cell_end[c], s + cap);
         return true;
     }
+    // Cell key of the particle's own cell.
+    __device__ __forceinline__ uint32_t own_key() const {
+        return base + (uint32_t)az.g * gyx + (uint32_t)ay.g * gx + (uint32_t)ax.g;
+    }
+    // As open(), with the (start, end) pairs of the block's neighbour rows
+    // staged in LDS: row q = 3 (dz + 1) + (dy + 1), entry xbase + dx, where
+    // xbase = own key - (first staged key of the row).
+    __device__ __forceinline__ bool open_staged(int dx, int dy, int dz, const uint2* tab, int cells,
+                                                int xbase, uint32_t& s, uint32_t& e) const {
+        if (!ax.ok(dx) || !ay.ok(dy) || !az.ok(dz)) return false;
+        if (az.d2(dz) + ay.d2(dy) + ax.d2(dx) > cull2) return false;
+        const uint2 r = tab[(3 * (dz + 1) + (dy + 1)) * cells + xbase + dx];
+        s = r.x;
+        if (s == kEmptyCell) return false;
+        e = min(r.y, s + cap);
+        return true;
+    }
+};
+
+// The cell tables of a block's neighbour rows, staged in LDS.
+struct CellView {
+    const uint2* t;   // 9 rows of `cells` (start, end) pairs
+    int cells;
+    uint32_t kmin;    // smallest own-cell key of the block
 };
 
 }  // namespace dev

@@ -150,6 +150,7 @@ struct Engine {
         b.sid_alt = sid[cur_pid ^ 1];
         b.rhop = rhop;
         b.dvel = dvel;
+        b.skey = last_skey;
         b.key_pid = key_pid;
         b.val_pid = val_pid;
         b.key_tmp = key_tmp;
@@ -583,14 +584,15 @@ struct Engine {
         bool found = false;
         while (std::fgets(line, sizeof(line), f)) {
             char k[512];
-            unsigned a = 0, b = 0;
+            unsigned a = 0, b = 0, d = 0;
             float m = 0;
-            if (std::sscanf(line, "{\"key\": \"%511[^\"]\", \"tune0\": %u, \"tune1\": %u, \"ms\": %f}", k, &a,
-                            &b, &m) == 4 &&
+            if (std::sscanf(line, "{\"key\": \"%511[^\"]\", \"tune0\": %u, \"tune1\": %u, \"tune3\": %u, \"ms\": %f}",
+                            k, &a, &b, &d, &m) == 5 &&
                 key == k) {
                 t[0] = a;
                 t[1] = b;
-                t[2] = t[3] = 0;
+                t[2] = 0;
+                t[3] = d;
                 if (ms) *ms = m;
                 found = true;  // the last matching line wins
             }
@@ -603,8 +605,8 @@ struct Engine {
         if (!path) return;
         FILE* f = std::fopen(path, "a");
         if (!f) return;
-        std::fprintf(f, "{\"key\": \"%s\", \"tune0\": %u, \"tune1\": %u, \"ms\": %.6f}\n", key.c_str(), t[0],
-                     t[1], ms);
+        std::fprintf(f, "{\"key\": \"%s\", \"tune0\": %u, \"tune1\": %u, \"tune3\": %u, \"ms\": %.6f}\n",
+                     key.c_str(), t[0], t[1], t[3], ms);
         std::fclose(f);
     }
 
@@ -675,16 +677,19 @@ struct Engine {
         };
         e = snap(true);
         const uint32_t lists[] = {16, 24, 32};
-        const uint32_t grids[] = {0, 2 * cus, 4 * cus, 8 * cus};
+        // gather grid and variant: per-lane loads in XCD order, cell tables in
+        // LDS, or a persistent grid
+        const uint32_t shapes[][2] = {{0, 0}, {0, 2}, {2 * cus, 0}, {4 * cus, 0}, {8 * cus, 0}};
         float best = 1e30f;
-        uint32_t best_t[4] = {cfg.tune[0], cfg.tune[1], 0, 0};
+        uint32_t best_t[4] = {cfg.tune[0], cfg.tune[1], 0, cfg.tune[3]};
         const uint32_t saved_tune[4] = {cfg.tune[0], cfg.tune[1], cfg.tune[2], cfg.tune[3]};
         int st = SPH_OK;
         for (uint32_t li : lists) {
             if (cfg.max_cap > li) continue;
-            for (uint32_t g : grids) {
+            for (const auto& sh : shapes) {
                 if (e != hipSuccess || st != SPH_OK) break;
-                const uint32_t cand[4] = {li, g, 0, 0};
+                const uint32_t g = sh[0];
+                const uint32_t cand[4] = {li, g, 0, sh[1]};
                 set_tune(cand);
                 cur_pos = p0;
                 cur_vel = v0;
@@ -705,6 +710,7 @@ struct Engine {
                     best = m;
                     best_t[0] = li;
                     best_t[1] = g;
+                    best_t[3] = sh[1];
                 }
             }
         }

@@ -72,6 +72,7 @@ struct StepBuffers {
     uint32_t* sid_alt;
     float2* rhop;         // (density, pressure) per sorted slot
     float4* dvel;         // optional force increment per sorted slot
+    const uint32_t* skey; // cell key of each sorted slot (from the last sort)
     uint32_t* key_pid;    // cell key in original-index order (sort input)
     uint32_t* val_pid;    // current slot in original-index order (sort input)
     uint32_t* key_tmp;
@@ -106,6 +107,11 @@ struct LaunchCfg {
 //            neighbour-list entries per lane (default 16)
 //   tune[1]  gather grid: 0 = one thread per slot over all slots, else a
 //            persistent grid of tune[1] blocks pulling 256-slot tiles
+//   tune[3]  gather variant (tune[1] == 0): 0 = cell tables and candidate
+//            positions read per lane, XCD-ordered tiles; 1 = candidate
+//            positions staged in LDS, XCD-ordered; 2 = neighbour-row cell
+//            tables staged in LDS, XCD-ordered; 3 = per lane, tiles in
+//            launch order
 uint32_t stats_chunks(const LaunchCfg& c);
 size_t stats_partials_count(const LaunchCfg& c);
 

@@ -25,6 +25,8 @@
 //   computeDensityD ................. source/CUDA/Kernel_Cell.cui:142-199
 //   computeForceD ................... source/CUDA/Kernel_Cell.cui:210-261,
 //                                     source/CUDA/System.cu:228-250,402
+#include <type_traits>
+
 #include "sph_device.hpp"
 
 namespace sph {
@@ -142,13 +144,102 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     }
 }
 
+// --------------------------------------------------- candidate positions
+// The gathers read candidate positions through a source: straight from the
+// sorted array, or from a block's LDS stage.  Either way the walk visits
+// the same candidates in the same order, so results are identical.
+//
+// Stage: the particles of a block (consecutive sorted slots) have cell keys
+// in [kmin, kmax].  For the neighbour-row offset q = 3 (dz + 1) + (dy + 1),
+// every neighbour cell of every particle lies in the key range
+// [kmin - 1 + off_q, kmax + 1 + off_q] (off_q = dy gx + dz gx gy), whose
+// particles are one contiguous run of sorted slots.  The block finds the 9
+// runs from the cell tables, copies their positions to LDS with coalesced
+// loads, and a candidate j of row q is then read at LDS index j + s_off[q].
+// Blocks whose key span, staged size or scene mix exceed the stage fall
+// back to global loads (a block-uniform decision).
+struct PosGlobal {
+    const float4* __restrict__ p;
+    __device__ __forceinline__ int32_t offset(int) const { return 0; }
+    __device__ __forceinline__ float4 at(uint32_t j, int32_t) const { return p[j]; }
+};
+
+struct PosStaged {
+    const float4* s;
+    const int32_t* s_off;
+    __device__ __forceinline__ int32_t offset(int q) const { return s_off[q]; }
+    __device__ __forceinline__ float4 at(uint32_t j, int32_t off) const { return s[(int32_t)j + off]; }
+};
+
+template <int kCap>
+struct Stage {
+    float4 pos[kCap];
+    int32_t off[9];
+    uint32_t lo[9], hi[9];
+};
+
+// Largest key span (kmax - kmin) a block stages; wider blocks use global loads.
+constexpr uint32_t kStageMaxSpan = 256;
+
+template <bool kBatch, int kCap>
+__device__ bool stage_block(Stage<kCap>& S, uint32_t b0, uint32_t count, const sph_params& one,
+                            const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid,
+                            const uint32_t* __restrict__ skey, const float4* __restrict__ pos,
+                            const uint32_t* __restrict__ cell_start,
+                            const uint32_t* __restrict__ cell_end) {
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    const uint32_t last = b0 + count - 1;
+    if constexpr (kBatch) {
+        if (sid[b0] != sid[last]) return false;
+    }
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, b0, scene);
+    const uint32_t kmin = skey[b0], kmax = skey[last];
+    if (kmax - kmin > kStageMaxSpan) return false;
+    if (t < 9) {
+        S.lo[t] = 0xFFFFFFFFu;
+        S.hi[t] = 0u;
+    }
+    __syncthreads();
+    const uint32_t cells = kmax - kmin + 3;
+    const int64_t klo = p.cell_base, khi = (int64_t)p.cell_base + p.num_cells - 1;
+    for (uint32_t c = t; c < 9 * cells; c += nt) {
+        const uint32_t q = c / cells, x = c - q * cells;
+        const int64_t dz = (int64_t)(q / 3) - 1, dy = (int64_t)(q % 3) - 1;
+        const int64_t key = (int64_t)kmin - 1 + x + dy * p.grid_size[0] + dz * p.grid_size_yx;
+        if (key < klo || key > khi) continue;
+        const uint32_t s = cell_start[key];
+        if (s == kEmptyCell) continue;
+        atomicMin(&S.lo[q], s);
+        atomicMax(&S.hi[q], cell_end[key]);
+    }
+    __syncthreads();
+    uint32_t base[10];
+    base[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const uint32_t lo = S.lo[q], hi = S.hi[q];
+        base[q + 1] = base[q] + (hi > lo ? hi - lo : 0u);
+    }
+    if (base[9] > (uint32_t)kCap) return false;
+    for (uint32_t idx = t; idx < base[9]; idx += nt) {
+        int q = 0;
+#pragma unroll
+        for (int k = 1; k < 9; ++k) q += idx >= base[k] ? 1 : 0;
+        S.pos[idx] = pos[S.lo[q] + (idx - base[q])];
+    }
+    if (t < 9) S.off[t] = (int32_t)base[t] - (int32_t)S.lo[t];
+    __syncthreads();
+    return true;
+}
+
 // ------------------------------------------------------------------ density
 // With kLists the density walk also records each particle's neighbours
 // (the slots that pass r2 < h2, j != i, in walk order) in its own row,
 // nbr[i * kNbrCap + c], c < kNbrCap, written four entries at a time, and
 // their count in nnb[i]; a count above kNbrCap raises the overflow counter
 // (entries past the cap are dropped).
-template <bool kBatch, int kPrec, bool kLists = false>
+template <bool kBatch, int kPrec, bool kLists = false, typename Src = PosGlobal, bool kCells = false>
 __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& one,
                                             const sph_params* __restrict__ tab,
                                             const uint32_t* __restrict__ sid,
@@ -158,10 +249,17 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                                             float2* __restrict__ rhop,
                                             uint32_t n = 0, uint32_t* __restrict__ nbr = nullptr,
                                             uint32_t* __restrict__ nnb = nullptr,
-                                            uint32_t* __restrict__ counters = nullptr) {
+                                            uint32_t* __restrict__ counters = nullptr,
+                                            const Src* src_in = nullptr, const CellView* cv = nullptr) {
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
     const float4 xi = pos[i];
+    Src src{};
+    if constexpr (std::is_same<Src, PosGlobal>::value) {
+        src.p = pos;
+    } else {
+        src = *src_in;
+    }
     CellWalk w;
     w.init(p, xi.x, xi.y, xi.z);
     const float h2 = p.h2;
@@ -170,36 +268,50 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     uint32_t count = 0;
     uint4 stage = make_uint4(0u, 0u, 0u, 0u);  // entries (count & ~3) .. (count & ~3) + 3
     uint32_t* row = kLists ? nbr + (size_t)i * kNbrCap : nullptr;
+    int xbase = 0;
+    if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
-        if (!w.open(ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start, cell_end, s, e))
-            continue;
+        const int dx = ord % 3 - 1, dy = (ord / 3) % 3 - 1, dz = ord / 9 - 1;
+        bool open;
+        if constexpr (kCells)
+            open = w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
+        else
+            open = w.open(dx, dy, dz, cell_start, cell_end, s, e);
+        if (!open) continue;
         float cell = 0.0f;
-        // next candidate's position loaded one iteration ahead
-        float4 xn = pos[s];
-        for (uint32_t j = s; j < e; ++j) {
-            const float4 xj = xn;
-            if (j + 1 < e) xn = pos[j + 1];
-            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const bool hit = (r2 < h2) && (j != i);
-            const float c3 = h2 - r2;
-            const float wgt = (c3 * c3) * c3;
-            if (kPrec == SPH_PREC_KAHAN) {
-                if (hit) ks.add(wgt);
-            } else {
-                // the partial sum is >= +0, so adding +0 for a miss is exact
-                cell = cell + (hit ? wgt : 0.0f);
-            }
-            if constexpr (kLists) {
-                if (hit) {
-                    const uint32_t q = count & 3u;
-                    stage.x = q == 0u ? j : stage.x;
-                    stage.y = q == 1u ? j : stage.y;
-                    stage.z = q == 2u ? j : stage.z;
-                    stage.w = q == 3u ? j : stage.w;
-                    if (q == 3u && count < kNbrCap) *reinterpret_cast<uint4*>(row + (count - 3u)) = stage;
-                    ++count;
+        const int32_t soff = src.offset(ord / 3);
+        // candidates four at a time: the loads go out together (indices past
+        // the cell repeat its last slot and are masked out of the sums)
+        for (uint32_t j0 = s; j0 < e; j0 += 4) {
+            float4 xq[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xq[k] = src.at(min(j0 + k, e - 1), soff);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t j = j0 + k;
+                const float4 xj = xq[k];
+                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const bool hit = (r2 < h2) && (j != i) && (j < e);
+                const float c3 = h2 - r2;
+                const float wgt = (c3 * c3) * c3;
+                if (kPrec == SPH_PREC_KAHAN) {
+                    if (hit) ks.add(wgt);
+                } else {
+                    // the partial sum is >= +0, so adding +0 for a miss is exact
+                    cell = cell + (hit ? wgt : 0.0f);
+                }
+                if constexpr (kLists) {
+                    if (hit) {
+                        const uint32_t q = count & 3u;
+                        stage.x = q == 0u ? j : stage.x;
+                        stage.y = q == 1u ? j : stage.y;
+                        stage.z = q == 2u ? j : stage.z;
+                        stage.w = q == 3u ? j : stage.w;
+                        if (q == 3u && count < kNbrCap) *reinterpret_cast<uint4*>(row + (count - 3u)) = stage;
+                        ++count;
+                    }
                 }
             }
         }
@@ -217,22 +329,6 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     return count;
 }
 
-template <bool kBatch, int kPrec>
-__global__ __launch_bounds__(kBlock) void k_density_lists(
-    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
-    const float4* __restrict__ pos, const uint32_t* __restrict__ cell_start,
-    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ nbr,
-    uint32_t* __restrict__ nnb, uint32_t* __restrict__ counters) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t listed = 0;
-    if (i < n)
-        listed = min(density_one<kBatch, kPrec, true>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n,
-                                                      nbr, nnb, counters),
-                     kNbrCap);
-    // pairs the surrogate will process: one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) listed += __shfl_xor(listed, off);
-    if ((threadIdx.x & 63) == 0 && listed) atomicAdd(&counters[kCntPairs], listed);
-}
 
 template <bool kBatch, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_density(sph_params one, const sph_params* __restrict__ tab,
@@ -310,16 +406,15 @@ struct ForceAcc {
     uint32_t prev_ord;
 };
 
-template <int kPrec>
+template <int kPrec, int kB, typename Src>
 __device__ __forceinline__ void force_drain(const sph_params& p, const uint32_t* list, int t, int cnt,
                                             const float4& xi, const float4& vi, const float2& ri,
-                                            const float4* __restrict__ pos,
-                                            const float4* __restrict__ vel,
+                                            const Src& src, const float4* __restrict__ vel,
                                             const float2* __restrict__ rhop, ForceAcc& a) {
     const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
     const float vterm = p.lap_kern * p.viscosity;
     for (int k = 0; k < cnt; ++k) {
-        const uint32_t ent = list[k * kBlock + t];
+        const uint32_t ent = list[k * kB + t];
         const uint32_t jj = ent & kSlotMask;
         const uint32_t o = ent >> kSlotBits;
         if (kPrec != SPH_PREC_KAHAN && o != a.prev_ord) {
@@ -329,7 +424,7 @@ __device__ __forceinline__ void force_drain(const sph_params& p, const uint32_t*
             a.fx = a.fy = a.fz = 0.0f;
             a.prev_ord = o;
         }
-        const float4 xj = pos[jj];
+        const float4 xj = src.at(jj, src.offset((int)(o / 3)));
         const float4 vj = vel[jj];
         const float2 rj = rhop[jj];
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -356,7 +451,7 @@ __device__ __forceinline__ void force_drain(const sph_params& p, const uint32_t*
 
 // One slot per lane; every lane of the wave must call this (inactive lanes
 // with active = false) because the list drains are wave-wide decisions.
-template <bool kBatch, int kPrec, int kList>
+template <bool kBatch, int kPrec, int kList, int kB = kBlock, typename Src = PosGlobal, bool kCells = false>
 __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* list,
                                           const sph_params& one, const sph_params* __restrict__ tab,
                                           const uint32_t* __restrict__ sid,
@@ -366,8 +461,15 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
                                           const uint32_t* __restrict__ cell_start,
                                           const uint32_t* __restrict__ cell_end,
                                           float4* __restrict__ vel_out, float4* __restrict__ dvel,
-                                          const float* __restrict__ dt_dev) {
+                                          const float* __restrict__ dt_dev, const Src* src_in = nullptr,
+                                          const CellView* cv = nullptr) {
     const int t = threadIdx.x;
+    Src src{};
+    if constexpr (std::is_same<Src, PosGlobal>::value) {
+        src.p = pos;
+    } else {
+        src = *src_in;
+    }
     const uint32_t ii = active ? i : 0u;
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, ii, scene);
@@ -381,31 +483,46 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
     ForceAcc a{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f},
                0xFFFFFFFFu};
     int cnt = 0;
+    int xbase = 0;
+    if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s = 0, e = 0;
-        const bool open = active && w.open(ord % 3 - 1, (ord / 3) % 3 - 1, ord / 9 - 1, cell_start,
-                                           cell_end, s, e);
+        const int dx = ord % 3 - 1, dy = (ord / 3) % 3 - 1, dz = ord / 9 - 1;
+        bool open;
+        if constexpr (kCells)
+            open = active && w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
+        else
+            open = active && w.open(dx, dy, dz, cell_start, cell_end, s, e);
         // every candidate of this cell could pass: make room first
         if (__any(cnt + (int)(e - s) > kList)) {
-            force_drain<kPrec>(p, list, t, cnt, xi, vi, ri, pos, vel, rhop, a);
+            force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
             cnt = 0;
         }
         if (!open) continue;
         const uint32_t tag = (uint32_t)ord << kSlotBits;
-        float4 xn = pos[s];
-        for (uint32_t j = s; j < e; ++j) {
-            const float4 xj = xn;
-            if (j + 1 < e) xn = pos[j + 1];
-            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max.  The store
-            // is unconditional (cnt < list size holds for the whole cell);
-            // only a hit advances cnt.
-            list[cnt * kBlock + t] = j | tag;
-            cnt += (r2 <= r2max && j != i) ? 1 : 0;
+        const int32_t soff = src.offset(ord / 3);
+        // candidates four at a time (see density_one)
+        for (uint32_t j0 = s; j0 < e; j0 += 4) {
+            float4 xq[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xq[k] = src.at(min(j0 + k, e - 1), soff);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t j = j0 + k;
+                const float4 xj = xq[k];
+                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max.  Within the
+                // cell the store is unconditional (cnt < list size holds for the
+                // whole cell); only a hit advances cnt.
+                if (j < e) {
+                    list[cnt * kB + t] = j | tag;
+                    cnt += (r2 <= r2max && j != i) ? 1 : 0;
+                }
+            }
         }
     }
-    force_drain<kPrec>(p, list, t, cnt, xi, vi, ri, pos, vel, rhop, a);
+    force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
     if (!active) return;
     float ax, ay, az;
     if (kPrec == SPH_PREC_KAHAN) {
@@ -456,6 +573,190 @@ __global__ __launch_bounds__(kBlock) void k_force_persistent(
                                         cell_end, vel_out, dvel, dt_dev);
     }
     retire_block(work);
+}
+
+// ------------------------------------------------------ staged gathers
+// XCD-aware tile order: the hardware hands consecutive workgroups to the
+// eight XCDs in turn; renumbering gives each XCD a contiguous run of tiles,
+// so tiles that share neighbour cells also share an L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t g) {
+    const uint32_t x = b % kXcds, local = b / kXcds;
+    const uint32_t per = g / kXcds, rem = g % kXcds;
+    return x * per + min(x, rem) + local;
+}
+
+constexpr int kDensityStage = 2816;  // positions staged per 256-slot tile (44 KB)
+constexpr int kForceTile = 128;      // slots per force tile
+constexpr int kForceStage = 1536;    // positions staged per force tile (24 KB)
+
+// kOrder: 0 = tiles in launch order, 1 = XCD-ordered; kStaged: candidate
+// positions staged in LDS.
+template <bool kBatch, int kPrec, bool kLists, int kOrder, bool kStaged>
+__global__ __launch_bounds__(kBlock) void k_density_tiled(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ skey,
+    const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+    float2* __restrict__ rhop, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nnb,
+    uint32_t* __restrict__ counters) {
+    __shared__ Stage<kStaged ? kDensityStage : 1> S;
+    const uint32_t b0 = (kOrder ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock;
+    const uint32_t count = min((uint32_t)kBlock, n - b0);
+    bool staged = false;
+    if constexpr (kStaged)
+        staged = stage_block<kBatch>(S, b0, count, one, tab, sid, skey, pos, cell_start, cell_end);
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t listed = 0;
+    if (i < n) {
+        uint32_t c;
+        if (kStaged && staged) {
+            const PosStaged src{S.pos, S.off};
+            c = density_one<kBatch, kPrec, kLists, PosStaged>(i, one, tab, sid, pos, cell_start, cell_end,
+                                                              rhop, n, nbr, nnb, counters, &src);
+        } else {
+            c = density_one<kBatch, kPrec, kLists>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n, nbr,
+                                                   nnb, counters);
+        }
+        listed = min(c, kNbrCap);
+    }
+    if constexpr (kLists) {
+        for (int off = 32; off > 0; off >>= 1) listed += __shfl_xor(listed, off);
+        if ((threadIdx.x & 63) == 0 && listed) atomicAdd(&counters[kCntPairs], listed);
+    }
+}
+
+// Cell-table stage: the (start, end) pairs of the block's 9 neighbour rows
+// (the key ranges of stage_block), so that opening a neighbour cell reads
+// LDS instead of two dependent global loads.
+constexpr uint32_t kCellSpanMax = 96;
+constexpr int kCellRow = (int)kCellSpanMax + 3;
+
+struct CellStage {
+    uint2 t[9 * kCellRow];
+};
+
+template <bool kBatch>
+__device__ bool stage_cells(CellStage& S, uint32_t b0, uint32_t count, const sph_params& one,
+                            const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid,
+                            const uint32_t* __restrict__ skey, const uint32_t* __restrict__ cell_start,
+                            const uint32_t* __restrict__ cell_end, CellView& cv) {
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    const uint32_t last = b0 + count - 1;
+    if constexpr (kBatch) {
+        if (sid[b0] != sid[last]) return false;
+    }
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, b0, scene);
+    const uint32_t kmin = skey[b0], kmax = skey[last];
+    if (kmax - kmin > kCellSpanMax) return false;
+    const uint32_t cells = kmax - kmin + 3;
+    const int64_t klo = p.cell_base, khi = (int64_t)p.cell_base + p.num_cells - 1;
+    for (uint32_t c = t; c < 9u * cells; c += nt) {
+        const uint32_t q = c / cells, x = c - q * cells;
+        const int64_t dz = (int64_t)(q / 3) - 1, dy = (int64_t)(q % 3) - 1;
+        const int64_t key = (int64_t)kmin - 1 + x + dy * p.grid_size[0] + dz * p.grid_size_yx;
+        uint2 v = make_uint2(kEmptyCell, 0u);
+        if (key >= klo && key <= khi) {
+            v.x = cell_start[key];
+            v.y = cell_end[key];
+        }
+        S.t[c] = v;
+    }
+    __syncthreads();
+    cv.t = S.t;
+    cv.cells = (int)cells;
+    cv.kmin = kmin;
+    return true;
+}
+
+template <bool kBatch, int kPrec, bool kLists>
+__global__ __launch_bounds__(kBlock) void k_density_cells(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ skey,
+    const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+    float2* __restrict__ rhop, uint32_t* __restrict__ nbr, uint32_t* __restrict__ nnb,
+    uint32_t* __restrict__ counters) {
+    __shared__ CellStage S;
+    const uint32_t b0 = xcd_tile(blockIdx.x, gridDim.x) * kBlock;
+    const uint32_t count = min((uint32_t)kBlock, n - b0);
+    CellView cv{};
+    const bool staged = stage_cells<kBatch>(S, b0, count, one, tab, sid, skey, cell_start, cell_end, cv);
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t listed = 0;
+    if (i < n) {
+        uint32_t c;
+        if (staged)
+            c = density_one<kBatch, kPrec, kLists, PosGlobal, true>(i, one, tab, sid, pos, cell_start, cell_end,
+                                                                    rhop, n, nbr, nnb, counters, nullptr, &cv);
+        else
+            c = density_one<kBatch, kPrec, kLists>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n, nbr,
+                                                   nnb, counters);
+        listed = min(c, kNbrCap);
+    }
+    if constexpr (kLists) {
+        for (int off = 32; off > 0; off >>= 1) listed += __shfl_xor(listed, off);
+        if ((threadIdx.x & 63) == 0 && listed) atomicAdd(&counters[kCntPairs], listed);
+    }
+}
+
+template <bool kBatch, int kPrec, int kList>
+__global__ __launch_bounds__(kBlock) void k_force_cells(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ skey, const float4* __restrict__ vel,
+    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    __shared__ uint32_t list[kList * kBlock];
+    __shared__ CellStage S;
+    const uint32_t b0 = xcd_tile(blockIdx.x, gridDim.x) * kBlock;
+    const uint32_t count = min((uint32_t)kBlock, n - b0);
+    CellView cv{};
+    const bool staged = stage_cells<kBatch>(S, b0, count, one, tab, sid, skey, cell_start, cell_end, cv);
+    const uint32_t i = b0 + threadIdx.x;
+    if (staged)
+        force_one<kBatch, kPrec, kList, kBlock, PosGlobal, true>(i, i < n, list, one, tab, sid, pos, vel, rhop,
+                                                                 cell_start, cell_end, vel_out, dvel, dt_dev,
+                                                                 nullptr, &cv);
+    else
+        force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end,
+                                        vel_out, dvel, dt_dev);
+}
+
+template <bool kBatch, int kPrec, int kList, int kOrder>
+__global__ __launch_bounds__(kBlock) void k_force_ordered(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    __shared__ uint32_t list[kList * kBlock];
+    const uint32_t i = (kOrder ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x;
+    force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end,
+                                    vel_out, dvel, dt_dev);
+}
+
+template <bool kBatch, int kPrec, int kList>
+__global__ __launch_bounds__(kForceTile) void k_force_tiled(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ skey, const float4* __restrict__ vel,
+    const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    __shared__ uint32_t list[kList * kForceTile];
+    __shared__ Stage<kForceStage> S;
+    const uint32_t b0 = xcd_tile(blockIdx.x, gridDim.x) * kForceTile;
+    const uint32_t count = min((uint32_t)kForceTile, n - b0);
+    const bool staged = stage_block<kBatch>(S, b0, count, one, tab, sid, skey, pos, cell_start, cell_end);
+    const uint32_t i = b0 + threadIdx.x;
+    if (staged) {
+        const PosStaged src{S.pos, S.off};
+        force_one<kBatch, kPrec, kList, kForceTile, PosStaged>(i, i < n, list, one, tab, sid, pos, vel, rhop,
+                                                               cell_start, cell_end, vel_out, dvel, dt_dev,
+                                                               &src);
+    } else {
+        force_one<kBatch, kPrec, kList, kForceTile>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
+                                                    cell_end, vel_out, dvel, dt_dev);
+    }
 }
 
 // Force kernel for max_par_in_cell above the list capacity: the pair body
@@ -708,7 +1009,19 @@ void reorder_impl(const LaunchCfg& c, const StepBuffers& b, const uint32_t* skey
 template <bool kBatch, int kPrec>
 void density_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
     const uint32_t grid = c.tune[1];
-    if (grid)
+    if (!grid && c.tune[3] == 2 && b.skey)
+        hipLaunchKernelGGL((k_density_cells<kBatch, kPrec, false>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock),
+                           0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start, b.cell_end,
+                           b.rhop, nullptr, nullptr, b.counters);
+    else if (!grid && c.tune[3] == 1 && b.skey)
+        hipLaunchKernelGGL((k_density_tiled<kBatch, kPrec, false, 1, true>), dim3(blocks_for(c.n, kBlock)),
+                           dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start,
+                           b.cell_end, b.rhop, nullptr, nullptr, b.counters);
+    else if (!grid && c.tune[3] == 0)
+        hipLaunchKernelGGL((k_density_tiled<kBatch, kPrec, false, 1, false>), dim3(blocks_for(c.n, kBlock)),
+                           dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start,
+                           b.cell_end, b.rhop, nullptr, nullptr, b.counters);
+    else if (grid)
         hipLaunchKernelGGL((k_density_persistent<kBatch, kPrec>), dim3(grid), dim3(kBlock), 0, s,
                            c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop,
                            b.work);
@@ -722,7 +1035,19 @@ template <bool kBatch, int kPrec, int kList>
 void force_list_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float4* dvel,
                      hipStream_t s) {
     const uint32_t grid = c.tune[1];
-    if (grid)
+    if (!grid && c.tune[3] == 2 && b.skey)
+        hipLaunchKernelGGL((k_force_cells<kBatch, kPrec, kList>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0,
+                           s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.vel_cur, b.rhop, b.cell_start,
+                           b.cell_end, vel_out, dvel, b.dt_dev);
+    else if (!grid && c.tune[3] == 1 && b.skey)
+        hipLaunchKernelGGL((k_force_tiled<kBatch, kPrec, kList>), dim3(blocks_for(c.n, kForceTile)),
+                           dim3(kForceTile), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.vel_cur,
+                           b.rhop, b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+    else if (!grid && c.tune[3] == 0)
+        hipLaunchKernelGGL((k_force_ordered<kBatch, kPrec, kList, 1>), dim3(blocks_for(c.n, kBlock)),
+                           dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+    else if (grid)
         hipLaunchKernelGGL((k_force_persistent<kBatch, kPrec, kList>), dim3(grid), dim3(kBlock), 0, s,
                            c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop, b.cell_start,
                            b.cell_end, vel_out, dvel, b.dt_dev, b.work + 2);
@@ -795,11 +1120,26 @@ hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32
 
 hipError_t launch_density_lists(const LaunchCfg& c, const StepBuffers& b, uint32_t* nbr, uint32_t* nnb,
                                 hipStream_t s) {
+    if (!b.skey) return hipErrorInvalidValue;
     const bool kahan = c.precision == SPH_PREC_KAHAN;
     const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
-#define SPH_DENS_LISTS(BATCH, PREC)                                                                  \
-    hipLaunchKernelGGL((k_density_lists<BATCH, PREC>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, \
-                       b.pos_cur, b.cell_start, b.cell_end, b.rhop, nbr, nnb, b.counters)
+#define SPH_DENS_LISTS(BATCH, PREC)                                                                   \
+    if (c.tune[3] == 2)                                                                               \
+        hipLaunchKernelGGL((k_density_cells<BATCH, PREC, true>), grid, block, 0, s, c.one, c.tab,          \
+                           b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start, b.cell_end, b.rhop, nbr, nnb,  \
+                           b.counters);                                                                   \
+    else if (c.tune[3] == 1)                                                                          \
+        hipLaunchKernelGGL((k_density_tiled<BATCH, PREC, true, 1, true>), grid, block, 0, s, c.one, c.tab, \
+                           b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start, b.cell_end, b.rhop, nbr, nnb,  \
+                           b.counters);                                                                   \
+    else if (c.tune[3] == 0)                                                                          \
+        hipLaunchKernelGGL((k_density_tiled<BATCH, PREC, true, 1, false>), grid, block, 0, s, c.one,       \
+                           c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start, b.cell_end, b.rhop, nbr, \
+                           nnb, b.counters);                                                              \
+    else                                                                                              \
+        hipLaunchKernelGGL((k_density_tiled<BATCH, PREC, true, 0, false>), grid, block, 0, s, c.one,       \
+                           c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start, b.cell_end, b.rhop, nbr, \
+                           nnb, b.counters)
     if (c.tab) {
         if (kahan)
             SPH_DENS_LISTS(true, SPH_PREC_KAHAN);

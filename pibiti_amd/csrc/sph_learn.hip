@@ -65,7 +65,7 @@ void learn_release(Engine& e) {
     void* ptrs[] = {L->pos,      L->vel,         L->rhop,         L->pid,          L->sid,
                     L->nbr,      L->nnb,         L->acc,          L->tb.h,         L->tb.out,
                     L->tb.dh,    L->tb.part_pair, L->tb.part_node, L->tb.loss_part, L->tb.grad,
-                    L->tb.loss,  L->lnbr,        L->lnnb,         L->spos,         L->svel,
+                    L->tb.loss,  L->tb.chunks,   L->lnbr,         L->lnnb,         L->spos,         L->svel,
                     L->spid,     L->dev_part,    L->dev,          L->dt_coarse,   L->ovf};
     for (void* q : ptrs) (void)hipFree(q);
     if (L->host) (void)hipHostFree(L->host);
@@ -108,6 +108,7 @@ hipError_t ensure_train(LearnState& L) {
     if (e == hipSuccess) e = dalloc(&t.part_pair, (size_t)mlp_pair_grad_waves(L.n) * mlp::kPairParams);
     if (e == hipSuccess) e = dalloc(&t.part_node, (size_t)mlp_node_blocks(L.n) * mlp::kNodeParams);
     if (e == hipSuccess) e = dalloc(&t.loss_part, (size_t)mlp_node_blocks(L.n));
+    if (e == hipSuccess) e = dalloc(&t.chunks, (size_t)mlp::kRedChunks * (mlp::kNumParams + 1));
     if (e == hipSuccess) e = dalloc(&t.grad, (size_t)mlp::kNumParams);
     if (e == hipSuccess) e = dalloc(&t.loss, (size_t)1);
     if (e == hipSuccess) L.train_ready = true;

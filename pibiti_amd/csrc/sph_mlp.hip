@@ -569,27 +569,38 @@ __global__ __launch_bounds__(kNodeBlk) void k_mlp_node_grad(
     }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_reduce(uint32_t rows_pair, uint32_t rows_node, uint32_t n,
-                                                    const float* __restrict__ part_pair,
-                                                    const float* __restrict__ part_node,
-                                                    const double* __restrict__ loss_part,
-                                                    float* __restrict__ grad, double* __restrict__ loss) {
+// Gradient reduction in two deterministic stages: block (x, y) sums rows
+// y, y + kRedChunks, ... of parameter columns 256 x .. 256 x + 255, then one
+// thread per parameter adds the chunk sums in order.
+__global__ __launch_bounds__(256) void k_mlp_reduce_chunks(uint32_t rows_pair, uint32_t rows_node,
+                                                           const float* __restrict__ part_pair,
+                                                           const float* __restrict__ part_node,
+                                                           const double* __restrict__ loss_part,
+                                                           double* __restrict__ chunks) {
     const int k = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t y = blockIdx.y;
+    double s = 0.0;
     if (k < kPairParams) {
-        double s = 0.0;
-        for (uint32_t r = 0; r < rows_pair; ++r) s += (double)part_pair[(size_t)r * kPairParams + k];
-        grad[k] = (float)s;
+        for (uint32_t r = y; r < rows_pair; r += kRedChunks) s += (double)part_pair[(size_t)r * kPairParams + k];
     } else if (k < kNumParams) {
         const int kk = k - kPairParams;
-        double s = 0.0;
-        for (uint32_t r = 0; r < rows_node; ++r) s += (double)part_node[(size_t)r * kNodeParams + kk];
+        for (uint32_t r = y; r < rows_node; r += kRedChunks) s += (double)part_node[(size_t)r * kNodeParams + kk];
+    } else if (k == kNumParams) {
+        for (uint32_t r = y; r < rows_node; r += kRedChunks) s += loss_part[r];
+    }
+    if (k <= kNumParams) chunks[(size_t)y * (kNumParams + 1) + k] = s;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_reduce_final(uint32_t n, const double* __restrict__ chunks,
+                                                          float* __restrict__ grad, double* __restrict__ loss) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k > kNumParams) return;
+    double s = 0.0;
+    for (uint32_t y = 0; y < kRedChunks; ++y) s += chunks[(size_t)y * (kNumParams + 1) + k];
+    if (k < kNumParams)
         grad[k] = (float)s;
-    }
-    if (k == 0) {
-        double s = 0.0;
-        for (uint32_t r = 0; r < rows_node; ++r) s += loss_part[r];
+    else
         loss[0] = s / (double)n;
-    }
 }
 
 __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ prm, float* __restrict__ m,
@@ -802,8 +813,10 @@ hipError_t mlp_gradient(const LaunchCfg& c, const sph_mlp_norm& nm, const float*
         hipLaunchKernelGGL((k_mlp_bwd<false>), dim3(waves / kWaves), dim3(kBlk), 0, s, a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((kNumParams + 255) / 256), dim3(256), 0, s, waves, nb, st.n,
-                       tb.part_pair, tb.part_node, tb.loss_part, tb.grad, tb.loss);
+    hipLaunchKernelGGL(k_mlp_reduce_chunks, dim3((kNumParams + 1 + 255) / 256, kRedChunks), dim3(256), 0, s,
+                       waves, nb, tb.part_pair, tb.part_node, tb.loss_part, tb.chunks);
+    hipLaunchKernelGGL(k_mlp_reduce_final, dim3((kNumParams + 1 + 255) / 256), dim3(256), 0, s, st.n, tb.chunks,
+                       tb.grad, tb.loss);
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ constexpr int kNumParams = kOffB3 + kOut;
 constexpr int kPairParams = kOffW3;                   // W1, W2, b2
 constexpr int kNodeParams = kNumParams - kPairParams; // W3, b3
 static_assert(kNumParams == SPH_MLP_NUM_PARAMS, "parameter layout");
+constexpr uint32_t kRedChunks = 32;  // row chunks of the gradient reduction
 
 }  // namespace mlp
 
@@ -56,6 +57,7 @@ struct MlpTrainBuffers {
     float* part_pair = nullptr;  // [mlp_pair_grad_waves(n)][kPairParams]
     float* part_node = nullptr;  // [mlp_node_blocks(n)][kNodeParams]
     double* loss_part = nullptr; // [mlp_node_blocks(n)]
+    double* chunks = nullptr;    // [kRedChunks][kNumParams + 1] reduction stage
     float* grad = nullptr;       // [kNumParams]
     double* loss = nullptr;      // [1]
 };

@@ -95,7 +95,8 @@ def test_single_scene_graph_replay_and_persistent_grid():
         assert np.array_equal(ref[k], got[k]), k
 
 
-@pytest.mark.parametrize("tune", [(1, 0), (24, 0), (32, 256), (16, 1024)])
+@pytest.mark.parametrize("tune", [(1, 0), (24, 0), (32, 256), (16, 1024), (16, 0, 0, 1), (24, 0, 0, 2),
+                                  (16, 0, 0, 3), (48, 0, 0, 1)])
 def test_launch_variants_identical(tune):
     sc = small_batch([4000, 9000], seed=11)
     base = None
