@@ -314,7 +314,8 @@ int sph_mfma_selftest(int variant, const uint16_t* a, const uint16_t* b, float* 
  *   e = [(x_i - x_j) inv_h (3), |x_i - x_j| inv_h, s (rho_i inv_rho0 - 1), 1, 0, 0,
  *        (v_j - v_i) inv_v (3), s (rho_j inv_rho0 - 1), 0 (4)]       (s = rho_scale)
  *   m_ij = relu(W2 relu(W1 e) + b2)       W1: 64 x 16 (column 5 = bias)
- * summed over the neighbours, H_i = sum_j m_ij.  With the node features
+ * summed over the neighbours, H_i = sum_j m_ij (m rounded to BF16, the sum
+ * accumulated in FP32 on the matrix cores).  With the node features
  *   q_i = [s (rho_i inv_rho0 - 1), p_i inv_p, nnb_i inv_nnb]
  * the acceleration is a_i = a_scale (W3 [agg_scale H_i ; q_i] + b3), W3: 3 x 67.
  * The pair network runs on BF16 matrix cores (BF16 weights and features,

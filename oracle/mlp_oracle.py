@@ -14,10 +14,10 @@ restates its mathematics in float64 numpy:
   loss_grad(...)     loss and analytic gradient by reverse-mode chain rule
 
 With ``emulate=True`` the BF16 rounding points of the device kernels are
-reproduced (weights, features, hidden activations and the backward
-products' inputs); everything else is float64, so the device result differs
-only by FP32 accumulation order (and rare BF16 rounding-boundary flips that
-order can cause).
+reproduced (weights, features, both layers' activations, b2 as a BF16
+high/low pair, and the backward products' inputs); everything else is
+float64, so the device result differs only by FP32 accumulation order (and
+rare BF16 rounding-boundary flips that order can cause).
 
 Parity pinning status (tests/test_mlp_oracle.py, tests/test_mlp_gpu.py):
   loss_grad ... central finite differences of the float64 loss (emulate=False)
@@ -121,10 +121,18 @@ def forward(params, E, pi, n, q, norm: dict, emulate: bool = True) -> dict:
     w = unpack(params)
     rnd = bf16 if emulate else (lambda a: np.asarray(a, np.float64))
     W1, W2 = rnd(w["W1"]), rnd(w["W2"])
+    b2 = w["b2"]
+    if emulate:
+        # the kernel adds b2 as two BF16 parts, hi = bf16(b2), lo = bf16(b2 - hi)
+        b32 = np.asarray(b2, np.float32)
+        hi = bf16(b32)
+        lo = bf16((b32 - hi.astype(np.float32)).astype(np.float32))
+        b2 = hi + lo
     pre1 = E @ W1.T
     h1 = rnd(np.maximum(pre1, 0.0))
-    pre2 = h1 @ W2.T + w["b2"][None, :]
-    m = np.maximum(pre2, 0.0)
+    pre2 = h1 @ W2.T + b2[None, :]
+    # pair outputs enter the neighbour sum as BF16 (FP32 accumulation)
+    m = rnd(np.maximum(pre2, 0.0))
     H = np.zeros((n, HID))
     np.add.at(H, pi, m)
     agg = float(np.float32(norm["agg_scale"]))

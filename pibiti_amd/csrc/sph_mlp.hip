@@ -152,16 +152,60 @@ __device__ __forceinline__ uint32_t nbr_pick(const uint4& q, int t) {
 
 // Layer 1, transposed: the hidden units of pair (lane % 32) as the four
 // k-blocks of layer 2's B operand.
+typedef short short8 __attribute__((ext_vector_type(8)));
+
+// relu on packed BF16: a negative BF16 has its sign bit set, so as a 16-bit
+// integer it is negative, and max(x, 0) over int16 lanes gives
+// bf16(relu(v)) == relu(bf16(v)) two values per instruction.
+__device__ __forceinline__ bf16x8 relu_bf16(const bf16x8& v) {
+    const short8 s = __builtin_elementwise_max(__builtin_bit_cast(short8, v), (short8)0);
+    return __builtin_bit_cast(bf16x8, s);
+}
+
+__device__ __forceinline__ bf16x8 keep_if(const bf16x8& v, bool keep) {
+    const short8 s = keep ? __builtin_bit_cast(short8, v) : (short8)0;
+    return __builtin_bit_cast(bf16x8, s);
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short short2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Two floats to a relu'd BF16 pair: one packed conversion, one packed
+// 16-bit max.
+__device__ __forceinline__ uint32_t cvt_relu2(float a, float b) {
+    const f32x2 v = {a, b};
+    const short2v s = __builtin_bit_cast(short2v, __builtin_convertvector(v, bf16x2));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(s, (short2v)0));
+}
+
+// 16 accumulator values as two k-blocks of a B operand (BF16, relu applied).
+__device__ __forceinline__ void pack_relu(const f32x16& c, bf16x8& lo, bf16x8& hi) {
+    const u32x4 a = {cvt_relu2(c[0], c[1]), cvt_relu2(c[2], c[3]), cvt_relu2(c[4], c[5]), cvt_relu2(c[6], c[7])};
+    const u32x4 b = {cvt_relu2(c[8], c[9]), cvt_relu2(c[10], c[11]), cvt_relu2(c[12], c[13]),
+                     cvt_relu2(c[14], c[15])};
+    lo = __builtin_bit_cast(bf16x8, a);
+    hi = __builtin_bit_cast(bf16x8, b);
+}
+
 __device__ __forceinline__ void layer1_t(const bf16x8 (&w1f)[2], const bf16x8& E, bf16x8 (&h1t)[4]) {
 #pragma unroll
     for (int t1 = 0; t1 < 2; ++t1) {
         const f32x16 c = mfma32(w1f[t1], E, zero16());
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            h1t[2 * t1][e] = bf(relu(c[e]));
-            h1t[2 * t1 + 1][e] = bf(relu(c[8 + e]));
-        }
+        pack_relu(c, h1t[2 * t1], h1t[2 * t1 + 1]);
     }
+}
+
+// Identity fragments that move a B operand in the permuted hidden order
+// (rows of the transposed accumulator) back into the accumulator layout:
+// I_par[e] = 1 where hidden unit 16 par + 8 (e / 4) + 4 g + e % 4 is the
+// lane's row p within its 32-unit tile.
+__device__ __forceinline__ bf16x8 unit_identity(int par, int g, int p) {
+    bf16x8 f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) f[e] = bf((16 * par + 8 * (e >> 2) + 4 * g + (e & 3)) == p ? 1.0f : 0.0f);
+    return f;
 }
 
 // Per-wave target group: slots [32 grp, 32 grp + 32), one target per lane
@@ -209,12 +253,29 @@ __global__ __launch_bounds__(kBlk) void k_mlp_fwd(const MlpArgs a) {
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) w2f[mt][kb] = a.w2f[(mt * 4 + kb) * 64 + lane];
-    // b2 in the transposed accumulator layout: the layer-2 products start from it
-    f32x16 b2t[2];
+    // b2 enters layer 2 as one more k-block: A holds b2 split into two BF16
+    // parts (k slots 0 and 1 of lane half 0), B is 1 in those slots for a
+    // valid pair, so an invalid pair gets pre2 = 0 and adds nothing to H.
+    bf16x8 b2a[2];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
+    for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) b2t[mt][r] = s_b2[32 * mt + mfma32_row(r, lane)];
+        for (int e = 0; e < 8; ++e) b2a[mt][e] = bf(0.0f);
+        if (g == 0) {
+            const float b = s_b2[32 * mt + p];
+            const __bf16 hi = bf(b);
+            b2a[mt][0] = hi;
+            b2a[mt][1] = bf(b - (float)hi);
+        }
+    }
+    bf16x8 one_valid, zero8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        one_valid[e] = bf((g == 0 && e < 2) ? 1.0f : 0.0f);
+        zero8[e] = bf(0.0f);
+    }
+
+    const bf16x8 ident[2] = {unit_identity(0, g, p), unit_identity(1, g, p)};
 
     const uint32_t nwaves = gridDim.x * kWaves;
     for (uint32_t grp = blockIdx.x * kWaves + wv; grp < a.ngroups; grp += nwaves) {
@@ -239,15 +300,17 @@ __global__ __launch_bounds__(kBlk) void k_mlp_fwd(const MlpArgs a) {
             const bf16x8 E = edge_features(g, valid, rc, G.xT, G.vT, G.rT, nm);
             bf16x8 h1t[4];
             layer1_t(w1f, E, h1t);
+            const bf16x8 bias_b = valid ? one_valid : zero8;
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
-                f32x16 c = b2t[mt];  // pre2 = b2 + W2 h1
+                f32x16 c = mfma32(b2a[mt], bias_b, zero16());  // pre2 = b2 + W2 h1
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) c = mfma32(w2f[mt][kb], h1t[kb], c);
-                if (valid) {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) hacc[mt][r] += relu(c[r]);
-                }
+                // H += bf16(relu(pre2)) on the matrix cores (FP32 accumulation)
+                bf16x8 m0, m1;
+                pack_relu(c, m0, m1);
+                hacc[mt] = mfma32(ident[0], m0, hacc[mt]);
+                hacc[mt] = mfma32(ident[1], m1, hacc[mt]);
             }
         }
 
