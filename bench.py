@@ -61,8 +61,11 @@ def learned_section(args, p, snap_pos, snap_vel, rho_plain_end, plain_secs) -> d
     L = hist["loss"]
     tail = L[-max(1, len(L) // 10):]
 
-    def fresh():
-        c = sph.SphSystem(p)
+    q = p.copy()
+    q.tune[2] = args.check  # surrogate steps per host check and per list build
+
+    def fresh(params=q):
+        c = sph.SphSystem(params)
         c.reset(snap_pos, snap_vel)
         c.update(1)  # the state the timed plain run started from
         return c
@@ -76,21 +79,27 @@ def learned_section(args, p, snap_pos, snap_vel, rho_plain_end, plain_secs) -> d
         c.update(factor)
         rho_ref = c.get_array(sph.SPH_ARRAY_DENSITY).astype(np.float64)
     with fresh() as c:
-        c.step_learned(model, 1, factor=factor, tol=1e9)
+        c.step_learned(model, 1, factor=factor, tol=1e9, skin=args.skin)
         one = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_ref)
     # the whole timed workload, with the 1 % drift fallback
     coarse = args.steps // factor
     with fresh() as c:
         c.synchronize()
         t0 = time.perf_counter()
-        rep = c.step_learned(model, coarse, factor=factor, tol=args.tol)
+        rep = c.step_learned(model, coarse, factor=factor, tol=args.tol, skin=args.skin)
         c.synchronize()
         secs = time.perf_counter() - t0
         run = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_plain_end)
+    # the plain step at factor x dt over the same workload, for comparison
+    big = q.copy()
+    big.time_step = factor * float(p.time_step)
+    with fresh(big) as c:
+        c.update(coarse)
+        run_plain_big = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_plain_end)
     # kernel times (HIP events) over a few more surrogate steps
     with fresh() as c:
         c.set_timing(True)
-        krep = c.step_learned(model, 40, factor=factor, tol=1e9)
+        krep = c.step_learned(model, 40, factor=factor, tol=1e9, skin=args.skin)
     sim_steps = factor * rep["coarse_steps"] + rep["plain_steps"]
     speedup = (sim_steps / secs) / (args.steps / plain_secs)
     flops = pair_flops(krep["pairs"])
@@ -105,17 +114,24 @@ def learned_section(args, p, snap_pos, snap_vel, rho_plain_end, plain_secs) -> d
         "fallbacks": rep["fallbacks"],
         "coarse_steps": rep["coarse_steps"],
         "tol": args.tol,
+        "skin": args.skin,
+        "steps_per_list_build": args.check if args.skin > 0 else 1,
+        "list_builds": rep["rebuilds"],
+        "early_list_builds": rep["early_rebuilds"],
         "density_error_one_step": one,
         "density_error_workload": run,
         "density_error_note": "|rho - rho_plain| / rho0 over particles: one surrogate step vs 4 plain "
                               "steps from the same state, and the whole timed workload vs the plain run",
+        "plain_at_factor_dt": {"density_error_workload": run_plain_big,
+                               "note": "the plain step at 4 x dt over the same workload (same cost per "
+                                       "step as the plain step, so 4x the simulated-time rate)"},
         "roofline": {"bound": "mfma", "kernel": "surrogate pair network",
                      "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": None,
                      "pairs_per_launch": krep["pairs"], "flops_per_launch": flops,
                      "mean_launch_ms": round(krep["mlp_ms"], 5)},
         "kernels_ms": {"density_lists": round(krep["density_ms"], 5), "surrogate": round(krep["mlp_ms"], 5),
-                       "coarse_step": round(krep["coarse_ms"], 5)},
+                       "list_build": round(krep["rebuild_ms"], 5), "coarse_step": round(krep["coarse_ms"], 5)},
         "training": {"seconds": round(hist["seconds"], 2), "trajectories": hist["scenes"],
                      "particles": hist["particles"], "plain_steps": hist["steps"],
                      "updates": len(L) * hist["iters"], "final_loss": float(np.mean(tail)),
@@ -189,6 +205,10 @@ def main() -> int:
     ap.add_argument("--train-iters", type=int, default=16)
     ap.add_argument("--train-lr", type=float, default=3e-3)
     ap.add_argument("--tol", type=float, default=0.01)
+    ap.add_argument("--skin", type=float, default=0.2,
+                    help="surrogate neighbour-list skin as a fraction of h (0 = sort every step)")
+    ap.add_argument("--check", type=int, default=4,
+                    help="surrogate steps per host check (and per list build with a skin)")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -63,7 +63,10 @@ enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1 };
  * float4 (x,y,z,w) per particle; DENSITY/PRESSURE are float per particle;
  * DVEL is the last step's force velocity increment as float4 (w = 0);
  * TARGET is the window-mean pair-force acceleration of a kept training
- * window (float4, w = 0; sph_learn_get_window only).
+ * window (float4, w = 0; sph_learn_get_window only); NEIGHBOURS is the
+ * number of neighbours within h the last surrogate step listed for each
+ * particle (uint32 per particle, in the float buffer's words; sph_get_array
+ * after sph_step_learned only).
  * All arrays are indexed by the ORIGINAL particle index.                   */
 enum {
     SPH_ARRAY_POSITION = 0,
@@ -71,7 +74,8 @@ enum {
     SPH_ARRAY_DENSITY = 2,
     SPH_ARRAY_PRESSURE = 3,
     SPH_ARRAY_DVEL = 4,
-    SPH_ARRAY_TARGET = 5
+    SPH_ARRAY_TARGET = 5,
+    SPH_ARRAY_NEIGHBOURS = 6
 };
 
 /* Context flags (sph_create). */
@@ -377,30 +381,44 @@ typedef struct sph_learned_report {
     uint32_t coarse_steps;  /* surrogate steps kept                         */
     uint32_t fallbacks;     /* surrogate steps undone and replaced          */
     uint32_t plain_steps;   /* plain steps run in their place               */
-    uint32_t reserved0;
+    uint32_t rebuilds;      /* sorts of skin > 0 runs, replays included     */
     double drift_max;       /* largest |D - D_anchor| of a kept step        */
     double drift_last;      /* |D - D_anchor| of the last surrogate step    */
     double anchor;          /* D of the last plain step                     */
     double sim_time;        /* clock after the call                         */
     /* With sph_set_timing on: mean device time (ms) per kept surrogate step
      * of the density-and-neighbour-list kernel, of the surrogate kernel and
-     * of the whole step (HIP events on the context stream); else 0.       */
+     * of the whole step including its share of the sorts and snapshots
+     * (HIP events on the context stream), and of one sort with its reorder
+     * (skin > 0); else 0.                                                  */
     double density_ms, mlp_ms, coarse_ms;
     double pairs;           /* mean neighbour pairs per kept surrogate step  */
+    double rebuild_ms;
+    uint32_t early_rebuilds; /* runs cut short: a particle moved skin h      */
+    uint32_t reserved0;
 } sph_learned_report;
 
 /* Advance ncoarse surrogate steps of factor x time_step each: the plain
- * integrate (coarse step), sort, reorder and density kernels, then
- * v += a dt_coarse from the surrogate.  D = mean_i |rho_i / rho0 - 1| of
- * each surrogate step is compared with D of the last plain step (the
- * anchor); a step whose drift |D - anchor| exceeds tol (or whose
- * neighbour lists overflow) is undone and replaced by `factor` plain steps,
- * which set a new anchor.  D is read back once per tune[2] steps (the
- * steps are deterministic, so a late check gives the same trajectory as a
- * check after every step).  The current state must come from a step.
- * Single-scene contexts with a fixed time step (cfl = 0, t_end = 0).       */
+ * integrate (coarse step), the density and neighbour lists of the new
+ * positions, then v += a dt_coarse from the surrogate.
+ *
+ * skin = 0: every step sorts, reorders and walks the 27 neighbour cells.
+ * skin > 0: the particles are sorted once per tune[2] steps; the steps in
+ * between keep that sort's slot order and cell ranges and walk every cell
+ * within (1 + skin) h.  A step in which some particle has moved more than
+ * skin h since the sort ends the run early: it is taken again after a new
+ * sort.  Either way each step lists exactly the pairs within h.
+ *
+ * D = mean_i |rho_i / rho0 - 1| of each surrogate step is compared with D
+ * of the last plain step (the anchor); a step whose drift |D - anchor|
+ * exceeds tol (or whose neighbour lists overflow) is undone and replaced
+ * by `factor` plain steps, which set a new anchor.  D is read back once per
+ * tune[2] steps (the steps are deterministic, so a late check gives the
+ * same trajectory as a check after every step).  The current state must
+ * come from a step.  Single-scene contexts with a fixed time step (cfl = 0,
+ * t_end = 0).                                                              */
 int sph_step_learned(sph_ctx* ctx, sph_model* model, uint32_t ncoarse, uint32_t factor, float tol,
-                     sph_learned_report* out);
+                     float skin, sph_learned_report* out);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,49 @@ struct CellWalk {
     }
 };
 
+// ------------------------------------------------------- neighbour lists
+// Group-major layout: entries 4g .. 4g + 3 of slot i's list are the uint4 at
+// index g * n + i (n = slots of the launch), so the lanes of a wave, which
+// hold consecutive slots, read and write consecutive 16-byte words.
+__device__ __forceinline__ size_t nbr_group_index(uint32_t g, uint32_t n, uint32_t i) {
+    return (size_t)g * n + i;
+}
+
+// Appends a slot's neighbours four at a time; entries past kNbrCap are
+// counted but not stored.
+struct ListWriter {
+    uint4 stage;
+    uint32_t count;
+
+    __device__ __forceinline__ void init() {
+        stage = make_uint4(0u, 0u, 0u, 0u);
+        count = 0u;
+    }
+    __device__ __forceinline__ void push(uint4* __restrict__ nbr, uint32_t n, uint32_t i, uint32_t j) {
+        const uint32_t q = count & 3u;
+        stage.x = q == 0u ? j : stage.x;
+        stage.y = q == 1u ? j : stage.y;
+        stage.z = q == 2u ? j : stage.z;
+        stage.w = q == 3u ? j : stage.w;
+        if (q == 3u && count < kNbrCap) nbr[nbr_group_index(count >> 2, n, i)] = stage;
+        ++count;
+    }
+    // the last, partly filled group (its unused entries are never read)
+    __device__ __forceinline__ void flush(uint4* __restrict__ nbr, uint32_t n, uint32_t i) const {
+        if ((count & 3u) != 0u && count < kNbrCap) nbr[nbr_group_index(count >> 2, n, i)] = stage;
+    }
+};
+
+// XCD-aware tile order: the hardware hands consecutive workgroups to the
+// eight XCDs in turn; renumbering gives each XCD a contiguous run of tiles,
+// so tiles that share neighbour cells also share an L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t g) {
+    const uint32_t x = b % kXcds, local = b / kXcds;
+    const uint32_t per = g / kXcds, rem = g % kXcds;
+    return x * per + min(x, rem) + local;
+}
+
 // The cell tables of a block's neighbour rows, staged in LDS.
 struct CellView {
     const uint2* t;   // 9 rows of `cells` (start, end) pairs

@@ -92,6 +92,8 @@ using namespace detail;
 
 struct LearnState;
 void learn_release(struct Engine& e);
+// Neighbour counts of the last surrogate step into staging (original order).
+int learn_neighbour_counts(struct Engine& e);
 
 struct Engine {
     int device = 0;
@@ -129,7 +131,7 @@ struct Engine {
     bool device_clock = false;  // clock and time step advanced on the device each step
 
     bool timing = false;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};
     float last_ms[6] = {0, 0, 0, 0, 0, 0};
 
     hipGraph_t graph = nullptr;
@@ -265,7 +267,7 @@ struct Engine {
         if (e == hipSuccess && as_batch) e = dalloc(&d_tab, (size_t)nscenes);
         if (e == hipSuccess) e = hipMalloc(&sort_scratch, radix_scratch_bytes((uint32_t)N));
         if (e == hipSuccess) e = radix_plan_init(sort, (uint32_t)N, sort_scratch, stream);
-        for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
+        for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_scene_off, scene_off.data(), (nscenes + 1) * sizeof(uint32_t),
                                hipMemcpyHostToDevice, stream);
@@ -442,6 +444,11 @@ struct Engine {
             case SPH_ARRAY_DVEL:
                 if (!dvel) return SPH_ERR_UNSUPPORTED;
                 SPH_TRY(launch_scatter_to_original(n, p, dvel, staging, 4, 4, stream));
+                break;
+            case SPH_ARRAY_NEIGHBOURS:
+                if (last_kind != 2) return SPH_ERR_UNSUPPORTED;
+                SPH_TRY_ST(learn_neighbour_counts(*this));
+                rec = 1;
                 break;
             default:
                 return SPH_ERR_INVALID_ARGUMENT;

@@ -20,7 +20,8 @@ enum Counter : int {
     kCntOutOfGrid = 1,  // particles whose own cell index was clamped
     kCntNbrOverflow = 2,  // particles with more than kNbrCap listed neighbours
     kCntPairs = 3,        // listed neighbour pairs (min(count, kNbrCap) summed)
-    kCntNum = 4
+    kCntSkinMoved = 4,    // particles farther than the skin from their position at the last sort
+    kCntNum = 5
 };
 
 // Per-scene FP64 diagnostics: stats[scene * kNumStats + k].
@@ -119,10 +120,19 @@ hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipSt
 hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
                           const uint32_t* old_slot, hipStream_t s);
 hipError_t launch_density(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
-// Density plus each slot's neighbour list: nbr[k * kNbrCap + c] (c < kNbrCap),
+// Density plus each slot's neighbour list, group-major: entries 4g .. 4g + 3
+// of slot k are nbr[4 (g n + k) .. 4 (g n + k) + 3] (g < kNbrCap / 4), and
 // nnb[k] = neighbour count (may exceed kNbrCap: counted in kCntNbrOverflow).
 hipError_t launch_density_lists(const LaunchCfg& c, const StepBuffers& b, uint32_t* nbr, uint32_t* nnb,
                                 hipStream_t s);
+// Density, pressure and neighbour lists (as launch_density_lists) of one
+// scene whose slots moved since the last sort (sph_skin.hip): walks every
+// cell within (1 + skin) h of each slot's current position using the cell
+// ranges of that sort, counts the pairs in kCntPairs and the slots farther
+// than skin h from their position at the sort (bpos) in kCntSkinMoved; with
+// none of those, the lists hold exactly the pairs now within h.
+hipError_t launch_density_reach(const LaunchCfg& c, const StepBuffers& b, float skin, const float4* bpos,
+                                uint32_t* nbr, uint32_t* nnb, hipStream_t s);
 hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, bool keep_dvel,
                         hipStream_t s);
 // FP64 per-scene reductions of (mass, kinetic, potential, max speed) over

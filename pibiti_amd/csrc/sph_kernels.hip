@@ -235,10 +235,9 @@ __device__ bool stage_block(Stage<kCap>& S, uint32_t b0, uint32_t count, const s
 
 // ------------------------------------------------------------------ density
 // With kLists the density walk also records each particle's neighbours
-// (the slots that pass r2 < h2, j != i, in walk order) in its own row,
-// nbr[i * kNbrCap + c], c < kNbrCap, written four entries at a time, and
-// their count in nnb[i]; a count above kNbrCap raises the overflow counter
-// (entries past the cap are dropped).
+// (the slots that pass r2 < h2, j != i, in walk order) in the group-major
+// lists (ListWriter), and their count in nnb[i]; a count above kNbrCap
+// raises the overflow counter (entries past the cap are dropped).
 template <bool kBatch, int kPrec, bool kLists = false, typename Src = PosGlobal, bool kCells = false>
 __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& one,
                                             const sph_params* __restrict__ tab,
@@ -265,9 +264,9 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     const float h2 = p.h2;
     float sum = 0.0f;
     KahanF ks{0.0f, 0.0f};
-    uint32_t count = 0;
-    uint4 stage = make_uint4(0u, 0u, 0u, 0u);  // entries (count & ~3) .. (count & ~3) + 3
-    uint32_t* row = kLists ? nbr + (size_t)i * kNbrCap : nullptr;
+    ListWriter lw;
+    lw.init();
+    uint4* const lists = reinterpret_cast<uint4*>(nbr);
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
     for (int ord = 0; ord < 27; ++ord) {
@@ -303,15 +302,7 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                     cell = cell + (hit ? wgt : 0.0f);
                 }
                 if constexpr (kLists) {
-                    if (hit) {
-                        const uint32_t q = count & 3u;
-                        stage.x = q == 0u ? j : stage.x;
-                        stage.y = q == 1u ? j : stage.y;
-                        stage.z = q == 2u ? j : stage.z;
-                        stage.w = q == 3u ? j : stage.w;
-                        if (q == 3u && count < kNbrCap) *reinterpret_cast<uint4*>(row + (count - 3u)) = stage;
-                        ++count;
-                    }
+                    if (hit) lw.push(lists, n, i, j);
                 }
             }
         }
@@ -321,12 +312,11 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
     rhop[i] = make_float2(rho, eos(p, rho));
     if constexpr (kLists) {
-        // the last, partly filled group of four (its unused entries are never read)
-        if ((count & 3u) != 0u && count < kNbrCap) *reinterpret_cast<uint4*>(row + (count & ~3u)) = stage;
-        nnb[i] = count;
-        if (count > kNbrCap) atomicAdd(&counters[kCntNbrOverflow], 1u);
+        lw.flush(lists, n, i);
+        nnb[i] = lw.count;
+        if (lw.count > kNbrCap) atomicAdd(&counters[kCntNbrOverflow], 1u);
     }
-    return count;
+    return lw.count;
 }
 
 
@@ -576,16 +566,6 @@ __global__ __launch_bounds__(kBlock) void k_force_persistent(
 }
 
 // ------------------------------------------------------ staged gathers
-// XCD-aware tile order: the hardware hands consecutive workgroups to the
-// eight XCDs in turn; renumbering gives each XCD a contiguous run of tiles,
-// so tiles that share neighbour cells also share an L2.
-constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t g) {
-    const uint32_t x = b % kXcds, local = b / kXcds;
-    const uint32_t per = g / kXcds, rem = g % kXcds;
-    return x * per + min(x, rem) + local;
-}
-
 constexpr int kDensityStage = 2816;  // positions staged per 256-slot tile (44 KB)
 constexpr int kForceTile = 128;      // slots per force tile
 constexpr int kForceStage = 1536;    // positions staged per force tile (24 KB)

@@ -49,8 +49,9 @@ struct LearnState {
     uint32_t* spid = nullptr;
     double* dev_part = nullptr;
     double* dev = nullptr;       // D of each surrogate step of a chunk
-    uint32_t* ovf = nullptr;     // per step: neighbour-list overflow, listed pairs
+    uint32_t* ovf = nullptr;     // per step: list overflow, listed pairs, skin moves
     float* dt_coarse = nullptr;
+    float4* bpos = nullptr;      // positions at the last sort (skin > 0)
     double* host = nullptr;      // pinned copies of dev / ovf
     uint32_t* host_cnt = nullptr;
 };
@@ -58,6 +59,11 @@ struct LearnState {
 // Surrogate steps between host checks (sph_params::tune[2], 0 = default).
 constexpr uint32_t kDefaultCheck = 4;
 constexpr uint32_t kMaxCheck = 64;
+// Counters kept per surrogate step: list overflow, listed pairs, skin moves
+// (kCntNbrOverflow .. kCntSkinMoved, consecutive).
+constexpr uint32_t kStepCounters = 3;
+static_assert(kCntPairs == kCntNbrOverflow + 1 && kCntSkinMoved == kCntNbrOverflow + 2,
+              "step counters are copied as one range");
 
 void learn_release(Engine& e) {
     LearnState* L = e.learn;
@@ -66,12 +72,20 @@ void learn_release(Engine& e) {
                     L->nbr,      L->nnb,         L->acc,          L->tb.h,         L->tb.out,
                     L->tb.dh,    L->tb.part_pair, L->tb.part_node, L->tb.loss_part, L->tb.grad,
                     L->tb.loss,  L->tb.chunks,   L->lnbr,         L->lnnb,         L->spos,         L->svel,
-                    L->spid,     L->dev_part,    L->dev,          L->dt_coarse,   L->ovf};
+                    L->spid,     L->dev_part,    L->dev,          L->dt_coarse,   L->ovf,
+                    L->bpos};
     for (void* q : ptrs) (void)hipFree(q);
     if (L->host) (void)hipHostFree(L->host);
     if (L->host_cnt) (void)hipHostFree(L->host_cnt);
     delete L;
     e.learn = nullptr;
+}
+
+int learn_neighbour_counts(Engine& e) {
+    const LearnState* L = e.learn;
+    if (!L || !L->lnnb) return SPH_ERR_UNSUPPORTED;
+    SPH_TRY(launch_scatter_to_original(e.cfg.n, e.pid[e.cur_pid], L->lnnb, e.staging, 1, 1, e.stream));
+    return SPH_OK;
 }
 
 namespace {
@@ -125,13 +139,18 @@ hipError_t ensure_step(LearnState& L, uint32_t nscenes) {
     if (e == hipSuccess) e = dalloc(&L.spid, N);
     if (e == hipSuccess) e = dalloc(&L.dev_part, (size_t)256);
     if (e == hipSuccess) e = dalloc(&L.dev, (size_t)kMaxCheck);
-    if (e == hipSuccess) e = dalloc(&L.ovf, (size_t)2 * kMaxCheck);
+    if (e == hipSuccess) e = dalloc(&L.ovf, (size_t)kStepCounters * kMaxCheck);
     if (e == hipSuccess) e = dalloc(&L.dt_coarse, (size_t)nscenes);
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.host), kMaxCheck * sizeof(double));
     if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&L.host_cnt), 2 * kMaxCheck * sizeof(uint32_t));
+        e = hipHostMalloc(reinterpret_cast<void**>(&L.host_cnt), kStepCounters * kMaxCheck * sizeof(uint32_t));
     if (e == hipSuccess) L.step_ready = true;
     return e;
+}
+
+hipError_t ensure_skin(LearnState& L) {
+    if (L.bpos) return hipSuccess;
+    return dalloc(&L.bpos, (size_t)L.n);
 }
 
 MlpState window_state(const LearnState& L) {
@@ -303,31 +322,50 @@ int learn_get_window(Engine& e, uint32_t scene, int which, float* out) {
     return SPH_OK;
 }
 
-// One surrogate step: plain integrate (coarse dt), sort, reorder, density
-// with neighbour lists, density deviation into dev_out and the neighbour
-// overflow count into ovf_out (both device), surrogate velocity update.
-int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m, double* dev_out, uint32_t* ovf_out,
-                   bool timed) {
+// One surrogate step: plain integrate (coarse dt); with skin = 0 or on a
+// rebuild step, sort and reorder (with skin > 0 keeping the sorted
+// positions); density with neighbour lists (the 27-cell walk after a sort,
+// else the walk of launch_density_reach); the density deviation into
+// dev_out and the step's counters into cnt_out (both device); the surrogate
+// velocity update.  Events: ev[1]/ev[2] bracket the density kernel,
+// ev[3]/ev[4] the surrogate, ev[5]/ev[6] a rebuild's sort and reorder.
+int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m, float skin, bool rebuild, double* dev_out,
+                   uint32_t* cnt_out, bool timed) {
     const LaunchCfg c = e.launch_cfg();
-    if (timed) SPH_TRY(hipEventRecord(e.ev[0], e.stream));
     SPH_TRY(launch_fill_u32(e.counters, 0u, kCntNum, e.stream));
     StepBuffers b = e.buffers();
     b.dt_dev = L.dt_coarse;
-    SPH_TRY(launch_integrate_hash(c, b, e.stream));
-    uint32_t *skey = nullptr, *oslot = nullptr;
-    SPH_TRY(radix_sort_pairs(e.sort, c.key_bits, e.key_pid, e.val_pid, e.key_tmp, e.val_tmp, &skey, &oslot,
-                             e.stream));
-    SPH_TRY(launch_reorder(c, e.buffers(), skey, oslot, e.stream));
-    e.cur_pos ^= 1;
-    e.cur_vel ^= 1;
-    e.cur_pid ^= 1;
-    e.last_skey = skey;
+    const bool sorted = skin == 0.0f || rebuild;
+    // the integrate kernel clears the cell ranges for the next sort; a step
+    // that does not sort keeps them
+    LaunchCfg ci = c;
+    if (!sorted) ci.num_cells = 0;
+    SPH_TRY(launch_integrate_hash(ci, b, e.stream));
+    if (sorted) {
+        if (timed && skin > 0.0f) SPH_TRY(hipEventRecord(e.ev[5], e.stream));
+        uint32_t *skey = nullptr, *oslot = nullptr;
+        SPH_TRY(radix_sort_pairs(e.sort, c.key_bits, e.key_pid, e.val_pid, e.key_tmp, e.val_tmp, &skey, &oslot,
+                                 e.stream));
+        SPH_TRY(launch_reorder(c, e.buffers(), skey, oslot, e.stream));
+        e.cur_pos ^= 1;
+        e.cur_vel ^= 1;
+        e.cur_pid ^= 1;
+        e.last_skey = skey;
+        if (skin > 0.0f) {
+            SPH_TRY(hipMemcpyAsync(L.bpos, e.pos[e.cur_pos], (size_t)c.n * sizeof(float4),
+                                   hipMemcpyDeviceToDevice, e.stream));
+            if (timed) SPH_TRY(hipEventRecord(e.ev[6], e.stream));
+        }
+    }
     b = e.buffers();
     if (timed) SPH_TRY(hipEventRecord(e.ev[1], e.stream));
-    SPH_TRY(launch_density_lists(c, b, L.lnbr, L.lnnb, e.stream));
+    if (sorted)
+        SPH_TRY(launch_density_lists(c, b, L.lnbr, L.lnnb, e.stream));
+    else
+        SPH_TRY(launch_density_reach(c, b, skin, L.bpos, L.lnbr, L.lnnb, e.stream));
     if (timed) SPH_TRY(hipEventRecord(e.ev[2], e.stream));
     SPH_TRY(mlp_density_deviation(c.n, b.rhop, 1.0f / e.scenes[0].rest_density, L.dev_part, dev_out, e.stream));
-    SPH_TRY(hipMemcpyAsync(ovf_out, e.counters + kCntNbrOverflow, 2 * sizeof(uint32_t),
+    SPH_TRY(hipMemcpyAsync(cnt_out, e.counters + kCntNbrOverflow, kStepCounters * sizeof(uint32_t),
                            hipMemcpyDeviceToDevice, e.stream));
     MlpState st;
     st.n = c.n;
@@ -355,21 +393,26 @@ int plain_deviation(Engine& e, LearnState& L, double* out) {
 }
 
 // Surrogate steps run in chunks of `check` steps between host checks.  A
-// chunk starts from a snapshot of the state; its steps' deviations D are
-// read back once.  The first step whose drift |D - anchor| exceeds tol (or
-// whose neighbour lists overflowed) is replaced: the chunk is undone, the
-// steps before it are taken again (the surrogate step is deterministic, so
-// they reproduce exactly) and `factor` plain steps stand in for it.
-int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, float tol,
+// chunk starts from a snapshot of the state (and, with a skin, its first
+// step sorts while the others keep that sort's order); its steps'
+// deviations D and counters are read back once.  The first step whose drift |D - anchor| exceeds tol (or whose
+// neighbour lists overflowed) is replaced: the chunk is undone, the steps
+// before it are taken again (the surrogate step is deterministic, so they
+// reproduce exactly) and `factor` plain steps stand in for it.  A step in
+// which a particle moved farther than the skin since the sort ends the
+// chunk the same way, but is then taken again as the first step of the
+// next chunk, which sorts.
+int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, float tol, float skin,
                  sph_learned_report* rep) {
     SPH_TRY_ST(check_model(e, m));
-    if (factor == 0 || !(tol > 0.0f)) return SPH_ERR_INVALID_ARGUMENT;
+    if (factor == 0 || !(tol > 0.0f) || !(skin >= 0.0f) || !(skin <= 1.0f)) return SPH_ERR_INVALID_ARGUMENT;
     if (e.batch || e.device_clock) return SPH_ERR_UNSUPPORTED;
     if (!e.ready() || e.last_kind == 0) return SPH_ERR_NOT_INITIALISED;
     LearnState* L = learn_state(e);
     if (!L) return SPH_ERR_OUT_OF_MEMORY;
     DeviceGuard guard(e.device);
     SPH_TRY(ensure_step(*L, e.cfg.nscenes));
+    if (skin > 0.0f) SPH_TRY(ensure_skin(*L));
     e.drop_graph();
     const uint32_t check = std::min<uint32_t>(e.cfg.tune[2] ? e.cfg.tune[2] : kDefaultCheck, kMaxCheck);
     const size_t N = e.cfg.n;
@@ -380,42 +423,64 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
     SPH_TRY_ST(plain_deviation(e, *L, &anchor));
     sph_learned_report r{};
     uint32_t done = 0;
-    double t_dens = 0.0, t_mlp = 0.0, t_step = 0.0, pairs = 0.0;
-    uint32_t timed_steps = 0;
+    double t_dens = 0.0, t_mlp = 0.0, t_chunks = 0.0, t_build = 0.0, pairs = 0.0;
+    uint32_t timed_steps = 0, chunk_steps = 0, timed_builds = 0;
     while (done < ncoarse) {
         const uint32_t chunk = std::min(check, ncoarse - done);
         const int p0 = e.cur_pos, v0 = e.cur_vel, q0 = e.cur_pid;
+        // with timing on, ev[0]/ev[7] bracket the whole chunk and the other
+        // events its last step (and its first step's list build)
+        if (e.timing) SPH_TRY(hipEventRecord(e.ev[0], e.stream));
         SPH_TRY(hipMemcpyAsync(L->spos, e.pos[p0], N * 16, k, e.stream));
         SPH_TRY(hipMemcpyAsync(L->svel, e.vel[v0], N * 16, k, e.stream));
         SPH_TRY(hipMemcpyAsync(L->spid, e.pid[q0], N * 4, k, e.stream));
-        // with timing on, the events bracket the chunk's last step
-        for (uint32_t i = 0; i < chunk; ++i)
-            SPH_TRY_ST(enqueue_coarse(e, *L, *m, L->dev + i, L->ovf + 2 * i, e.timing && i + 1 == chunk));
-        SPH_TRY(hipMemcpyAsync(L->host, L->dev, chunk * sizeof(double), hipMemcpyDeviceToHost, e.stream));
-        SPH_TRY(hipMemcpyAsync(L->host_cnt, L->ovf, 2 * chunk * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               e.stream));
-        SPH_TRY(hipStreamSynchronize(e.stream));
-        if (e.timing) {
-            float a = 0.0f, b = 0.0f, c = 0.0f;
-            SPH_TRY(hipEventElapsedTime(&a, e.ev[1], e.ev[2]));
-            SPH_TRY(hipEventElapsedTime(&b, e.ev[3], e.ev[4]));
-            SPH_TRY(hipEventElapsedTime(&c, e.ev[0], e.ev[4]));
-            t_dens += a;
-            t_mlp += b;
-            t_step += c;
-            timed_steps += 1;
-        }
-        uint32_t bad = chunk;
         for (uint32_t i = 0; i < chunk; ++i) {
+            const bool timed = e.timing && (i + 1 == chunk || (i == 0 && skin > 0.0f));
+            SPH_TRY_ST(enqueue_coarse(e, *L, *m, skin, i == 0, L->dev + i, L->ovf + kStepCounters * i, timed));
+            if (timed && i == 0 && skin > 0.0f) {
+                // the events of the first step are read before the last step reuses them
+                SPH_TRY(hipEventSynchronize(e.ev[6]));
+                float bt = 0.0f;
+                SPH_TRY(hipEventElapsedTime(&bt, e.ev[5], e.ev[6]));
+                t_build += bt;
+                timed_builds += 1;
+            }
+        }
+        if (e.timing) SPH_TRY(hipEventRecord(e.ev[7], e.stream));
+        SPH_TRY(hipMemcpyAsync(L->host, L->dev, chunk * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+        SPH_TRY(hipMemcpyAsync(L->host_cnt, L->ovf, kStepCounters * chunk * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, e.stream));
+        SPH_TRY(hipStreamSynchronize(e.stream));
+        if (skin > 0.0f) r.rebuilds += 1;
+        uint32_t bad = chunk;
+        bool moved = false;
+        for (uint32_t i = 0; i < chunk; ++i) {
+            const uint32_t* cnt = L->host_cnt + kStepCounters * i;
+            if (cnt[2] != 0) {
+                bad = i;
+                moved = true;
+                break;
+            }
             const double drift = std::fabs(L->host[i] - anchor);
-            if (drift > (double)tol || L->host_cnt[2 * i] != 0) {
+            if (drift > (double)tol || cnt[0] != 0) {
                 bad = i;
                 r.drift_last = drift;
                 break;
             }
             r.drift_max = std::max(r.drift_max, drift);
             r.drift_last = drift;
-            pairs += L->host_cnt[2 * i + 1];
+            pairs += cnt[1];
+        }
+        if (e.timing && bad == chunk) {
+            float a = 0.0f, b = 0.0f, c = 0.0f;
+            SPH_TRY(hipEventElapsedTime(&a, e.ev[1], e.ev[2]));
+            SPH_TRY(hipEventElapsedTime(&b, e.ev[3], e.ev[4]));
+            SPH_TRY(hipEventElapsedTime(&c, e.ev[0], e.ev[7]));
+            t_dens += a;
+            t_mlp += b;
+            t_chunks += c;
+            timed_steps += 1;
+            chunk_steps += chunk;
         }
         if (bad < chunk) {
             e.cur_pos = p0;
@@ -425,14 +490,19 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
             SPH_TRY(hipMemcpyAsync(e.vel[v0], L->svel, N * 16, k, e.stream));
             SPH_TRY(hipMemcpyAsync(e.pid[q0], L->spid, N * 4, k, e.stream));
             for (uint32_t i = 0; i < bad; ++i)
-                SPH_TRY_ST(enqueue_coarse(e, *L, *m, L->dev + i, L->ovf + 2 * i, false));
-            for (uint32_t f = 0; f < factor; ++f) {
-                SPH_TRY_ST(e.enqueue_step(false));
-                e.after_steps(1);
+                SPH_TRY_ST(enqueue_coarse(e, *L, *m, skin, i == 0, L->dev + i, L->ovf + kStepCounters * i, false));
+            if (skin > 0.0f && bad > 0) r.rebuilds += 1;
+            if (moved) {
+                r.early_rebuilds += 1;
+            } else {
+                for (uint32_t f = 0; f < factor; ++f) {
+                    SPH_TRY_ST(e.enqueue_step(false));
+                    e.after_steps(1);
+                }
+                SPH_TRY_ST(plain_deviation(e, *L, &anchor));
+                r.fallbacks += 1;
+                r.plain_steps += factor;
             }
-            SPH_TRY_ST(plain_deviation(e, *L, &anchor));
-            r.fallbacks += 1;
-            r.plain_steps += factor;
         }
         const uint32_t kept = bad;
         r.coarse_steps += kept;
@@ -440,7 +510,7 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
         e.host_clock += dt * factor * kept;
         e.stepped = true;
         if (bad == chunk) e.last_kind = 2;
-        done += kept + (bad < chunk ? 1 : 0);
+        done += kept + (bad < chunk && !moved ? 1 : 0);
     }
     r.anchor = anchor;
     r.sim_time = e.host_clock;
@@ -448,8 +518,9 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
     if (timed_steps) {
         r.density_ms = t_dens / timed_steps;
         r.mlp_ms = t_mlp / timed_steps;
-        r.coarse_ms = t_step / timed_steps;
+        r.coarse_ms = t_chunks / chunk_steps;
     }
+    if (timed_builds) r.rebuild_ms = t_build / timed_builds;
     if (rep) *rep = r;
     return SPH_OK;
 }
@@ -616,10 +687,10 @@ int sph_batch_learn_get_window(sph_batch* b, uint32_t scene, int which, float* o
     return learn_get_window(b->eng, scene, which, out);
 }
 
-int sph_step_learned(sph_ctx* c, sph_model* m, uint32_t ncoarse, uint32_t factor, float tol,
+int sph_step_learned(sph_ctx* c, sph_model* m, uint32_t ncoarse, uint32_t factor, float tol, float skin,
                      sph_learned_report* out) {
     if (!c) return SPH_ERR_INVALID_ARGUMENT;
-    return step_learned(c->eng, m, ncoarse, factor, tol, out);
+    return step_learned(c->eng, m, ncoarse, factor, tol, skin, out);
 }
 
 }  // extern "C"

@@ -1,14 +1,15 @@
 // sph_mlp.hip - learned pair-force surrogate on BF16 matrix cores (gfx950).
 //
-// Work decomposition.  One wavefront serves 64 target particles (consecutive
-// sorted slots).  Each lane walks its target's 27 neighbour cells with the
-// density kernel's walk and predicate (r2 < h2, j != i) and appends the
-// neighbour slots to a list in LDS.  When a list could overflow, and at the
-// end of the walk, the wave drains the lists: for each half of 32 targets
-// it forms tiles of 32 pairs, row p of tile t being target p's t-th
-// neighbour, and runs both pair layers on v_mfma_f32_32x32x16_bf16.  The
-// rows of a tile are distinct targets, so the neighbour sum accumulates in
-// registers: no atomics, and a fixed summation order (the walk order).
+// Work decomposition.  The density kernel lists every particle's
+// neighbours (r2 < h2, j != i, walk order) in group-major lists.  One
+// wavefront serves 32 target particles (consecutive sorted slots): tile t
+// holds each target's t-th neighbour as one of 32 pair columns, lane halves
+// g = 0/1 holding the two halves of the pair's 16 input features, and both
+// pair layers run on v_mfma_f32_32x32x16_bf16.  The columns of a tile are
+// distinct targets, so the neighbour sum accumulates in registers: no
+// atomics, and a fixed summation order (the list order).  A wave runs as
+// many tiles as its largest neighbour count; shorter lists are padded with
+// masked pairs.
 //
 // Layouts (operand conventions in sph_mfma.hpp).  The forward pass runs
 // transposed, weights as the A operand and pairs as columns (lane = pair):
@@ -90,64 +91,88 @@ struct MlpArgs {
     float* part_pair;     // backward
 };
 
-// What one lane gathers for one pair: g = 0 lanes the neighbour's position,
-// g = 1 lanes its velocity and density.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short short2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Two floats as a packed BF16 pair (round to nearest even).
+__device__ __forceinline__ uint32_t cvt2(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// Square root rounded to nearest for x >= 0 in the normal range: the
+// hardware square root (within 1 ulp) moved to a neighbour when the
+// residual x - neighbour * s says that neighbour is nearer.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __int_as_float(__float_as_int(s) - 1);
+    const float hi = __int_as_float(__float_as_int(s) + 1);
+    float r = __builtin_fmaf(-lo, s, x) <= 0.0f ? lo : s;
+    r = __builtin_fmaf(-hi, s, x) > 0.0f ? hi : r;
+    return r;
+}
+
+// What one lane gathers for one pair: the neighbour's position (g = 0
+// lanes) or velocity (g = 1 lanes), and its density.
 struct PairRaw {
     float4 a;
     float b;
 };
 
-__device__ __forceinline__ PairRaw load_pair(int g, bool valid, uint32_t j, const MlpState& st) {
-    PairRaw r;
-    r.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    r.b = 0.0f;
-    if (valid) {
-        if (g == 0) {
-            r.a = st.pos[j];
-        } else {
-            r.a = st.vel[j];
-            r.b = st.rhop[j].x;
-        }
-    }
-    return r;
-}
+// A lane's stream over its target's neighbours.  Entry t of target i is
+// word 4 (floor(t / 4) n + i) + t % 4 of the group-major lists; entries at
+// and past the target's count stand for the target itself, so every load
+// is in bounds and no lane branches (those pairs are masked out of the
+// features).
+struct PairStream {
+    const uint32_t* words;  // this lane's target's words of group 0
+    const float4* src;      // positions (g = 0) or velocities (g = 1)
+    const float2* rhop;
+    size_t gstride;         // words from one group to the next: 4 n
+    uint32_t self;
+    int cnt;
 
-// Edge features of one pair as this lane's half of the k dimension.
+    __device__ __forceinline__ uint32_t index(int t) const {
+        const size_t off = t < cnt ? (size_t)(t >> 2) * gstride + (size_t)(t & 3) : 0;
+        const uint32_t w = words[off];
+        return t < cnt ? w : self;
+    }
+    __device__ __forceinline__ PairRaw load(uint32_t j) const {
+        PairRaw r;
+        r.a = src[j];
+        r.b = rhop[j].x;
+        return r;
+    }
+};
+
+// Per-lane constants of a target for the edge features: T is the target's
+// position (g = 0) or velocity (g = 1), s the scale with the sign that
+// turns (neighbour - target) into the feature, w2 the fixed third word.
+struct PairConst {
+    float4 T;
+    float s;
+    uint32_t w2;
+};
+
+// Edge features of one pair as this lane's half of the k dimension, the
+// same code on both halves:
 //   g = 0 (k 0..7):  (x_i - x_j)/h (3), |x_i - x_j|/h, s(rho_i/rho0 - 1), 1, 0, 0
 //   g = 1 (k 8..15): (v_j - v_i)/v (3), s(rho_j/rho0 - 1), 0 (4)
-__device__ __forceinline__ bf16x8 edge_features(int g, bool valid, const PairRaw& r, const float4& xi,
-                                                const float4& vi, float rhoi, const sph_mlp_norm& nm) {
-    bf16x8 f;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) f[e] = bf(0.0f);
-    if (!valid) return f;
-    if (g == 0) {
-        const float rx = xi.x - r.a.x, ry = xi.y - r.a.y, rz = xi.z - r.a.z;
-        const float d = sqrtf(rx * rx + ry * ry + rz * rz);
-        f[0] = bf(rx * nm.inv_h);
-        f[1] = bf(ry * nm.inv_h);
-        f[2] = bf(rz * nm.inv_h);
-        f[3] = bf(d * nm.inv_h);
-        f[4] = bf((rhoi * nm.inv_rho0 - 1.0f) * nm.rho_scale);
-        f[5] = bf(1.0f);  // feature kBiasFeat
-    } else {
-        f[0] = bf((r.a.x - vi.x) * nm.inv_v);
-        f[1] = bf((r.a.y - vi.y) * nm.inv_v);
-        f[2] = bf((r.a.z - vi.z) * nm.inv_v);
-        f[3] = bf((r.b * nm.inv_rho0 - 1.0f) * nm.rho_scale);
-    }
-    return f;
-}
-
-// A target's neighbour row is read four entries at a time: `cur` holds the
-// group of four containing the next tile's entry, `nxt` the group after it.
-__device__ __forceinline__ uint4 nbr_group(const uint32_t* row, int first, int cnt) {
-    return first < cnt ? *reinterpret_cast<const uint4*>(row + first) : make_uint4(0u, 0u, 0u, 0u);
-}
-
-__device__ __forceinline__ uint32_t nbr_pick(const uint4& q, int t) {
-    const int e = t & 3;
-    return e == 0 ? q.x : (e == 1 ? q.y : (e == 2 ? q.z : q.w));
+// (x_j - x_i) (-1/h) equals (x_i - x_j) (1/h) bit for bit.
+__device__ __forceinline__ bf16x8 edge_features(int g, bool valid, const PairRaw& r, const PairConst& k,
+                                                const sph_mlp_norm& nm) {
+    const float dx = r.a.x - k.T.x, dy = r.a.y - k.T.y, dz = r.a.z - k.T.z;
+    const float r2 = (dx * dx + dy * dy) + dz * dz;
+    const float dist = sqrt_rn(r2) * nm.inv_h;
+    const float rj = (r.b * nm.inv_rho0 - 1.0f) * nm.rho_scale;
+    u32x4 w;
+    w[0] = valid ? cvt2(dx * k.s, dy * k.s) : 0u;
+    w[1] = valid ? cvt2(dz * k.s, g ? rj : dist) : 0u;
+    w[2] = valid ? k.w2 : 0u;
+    w[3] = 0u;
+    return __builtin_bit_cast(bf16x8, w);
 }
 
 // Layer 1, transposed: the hidden units of pair (lane % 32) as the four
@@ -167,16 +192,10 @@ __device__ __forceinline__ bf16x8 keep_if(const bf16x8& v, bool keep) {
     return __builtin_bit_cast(bf16x8, s);
 }
 
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef short short2v __attribute__((ext_vector_type(2)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
 // Two floats to a relu'd BF16 pair: one packed conversion, one packed
 // 16-bit max.
 __device__ __forceinline__ uint32_t cvt_relu2(float a, float b) {
-    const f32x2 v = {a, b};
-    const short2v s = __builtin_bit_cast(short2v, __builtin_convertvector(v, bf16x2));
+    const short2v s = __builtin_bit_cast(short2v, cvt2(a, b));
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(s, (short2v)0));
 }
 
@@ -212,6 +231,7 @@ __device__ __forceinline__ bf16x8 unit_identity(int par, int g, int p) {
 // pair (lane % 32), its neighbours processed as one tile row per neighbour.
 struct Group {
     uint32_t iT;   // target slot of this lane
+    uint32_t ii;   // iT, or the group's first slot for a lane past the end
     int cT;        // its listed neighbours
     int m;         // tiles (largest count of the group)
     float4 xT, vT;
@@ -222,13 +242,32 @@ __device__ __forceinline__ Group open_group(uint32_t grp, int p, const MlpState&
     Group G;
     G.iT = grp * kGroup + p;
     const bool act = G.iT < st.n;
-    const uint32_t ii = act ? G.iT : grp * kGroup;
-    G.cT = act ? (int)min(st.nnb[ii], kNbrCap) : 0;
+    G.ii = act ? G.iT : grp * kGroup;
+    G.cT = act ? (int)min(st.nnb[G.ii], kNbrCap) : 0;
     G.m = wave_max(G.cT);
-    G.xT = st.pos[ii];
-    G.vT = st.vel[ii];
-    G.rT = st.rhop[ii].x;
+    G.xT = st.pos[G.ii];
+    G.vT = st.vel[G.ii];
+    G.rT = st.rhop[G.ii].x;
     return G;
+}
+
+__device__ __forceinline__ PairStream pair_stream(const Group& G, int g, const MlpState& st) {
+    PairStream ps;
+    ps.words = st.nbr + 4 * (size_t)G.ii;
+    ps.src = g ? st.vel : st.pos;
+    ps.rhop = st.rhop;
+    ps.gstride = 4 * (size_t)st.n;
+    ps.self = G.ii;
+    ps.cnt = G.cT;
+    return ps;
+}
+
+__device__ __forceinline__ PairConst pair_const(const Group& G, int g, const sph_mlp_norm& nm) {
+    PairConst k;
+    k.T = g ? G.vT : G.xT;
+    k.s = g ? nm.inv_v : -nm.inv_h;
+    k.w2 = g ? 0u : cvt2((G.rT * nm.inv_rho0 - 1.0f) * nm.rho_scale, 1.0f);
+    return k;
 }
 
 // Inference (kTrain = false: velocity update and/or acceleration) and the
@@ -284,20 +323,17 @@ __global__ __launch_bounds__(kBlk) void k_mlp_fwd(const MlpArgs a) {
         asm volatile("" ::: "memory");
         const Group G = open_group(grp, p, st);
         f32x16 hacc[2] = {zero16(), zero16()};
-        // neighbour indices four at a time; the neighbour's data one tile ahead
-        const uint32_t* nrow = st.nbr + (size_t)G.iT * kNbrCap;
-        uint4 ncur = nbr_group(nrow, 0, G.cT), nnxt = nbr_group(nrow, 4, G.cT);
-        PairRaw rn = load_pair(g, 0 < G.cT, ncur.x, st);
+        // neighbour indices two tiles ahead, the neighbour's data one tile ahead
+        const PairStream ps = pair_stream(G, g, st);
+        const PairConst kc = pair_const(G, g, nm);
+        PairRaw rn = ps.load(ps.index(0));
+        uint32_t jn = ps.index(1);
         for (int t = 0; t < G.m; ++t) {
             const bool valid = t < G.cT;
             const PairRaw rc = rn;
-            const int t1 = t + 1;
-            if ((t1 & 3) == 0) {
-                ncur = nnxt;
-                nnxt = nbr_group(nrow, t1 + 4, G.cT);
-            }
-            rn = load_pair(g, t1 < G.cT, nbr_pick(ncur, t1), st);
-            const bf16x8 E = edge_features(g, valid, rc, G.xT, G.vT, G.rT, nm);
+            rn = ps.load(jn);
+            jn = ps.index(t + 2);
+            const bf16x8 E = edge_features(g, valid, rc, kc, nm);
             bf16x8 h1t[4];
             layer1_t(w1f, E, h1t);
             const bf16x8 bias_b = valid ? one_valid : zero8;
@@ -435,20 +471,17 @@ __global__ __launch_bounds__(kBlk) void k_mlp_bwd(const MlpArgs a) {
                 dht[mt][4 * q + 3] = v.w;
             }
 
-        // neighbour indices four at a time; the neighbour's data one tile ahead
-        const uint32_t* nrow = st.nbr + (size_t)G.iT * kNbrCap;
-        uint4 ncur = nbr_group(nrow, 0, G.cT), nnxt = nbr_group(nrow, 4, G.cT);
-        PairRaw rn = load_pair(g, 0 < G.cT, ncur.x, st);
+        // neighbour indices two tiles ahead, the neighbour's data one tile ahead
+        const PairStream ps = pair_stream(G, g, st);
+        const PairConst kc = pair_const(G, g, nm);
+        PairRaw rn = ps.load(ps.index(0));
+        uint32_t jn = ps.index(1);
         for (int t = 0; t < G.m; ++t) {
             const bool valid = t < G.cT;
             const PairRaw rc = rn;
-            const int t1 = t + 1;
-            if ((t1 & 3) == 0) {
-                ncur = nnxt;
-                nnxt = nbr_group(nrow, t1 + 4, G.cT);
-            }
-            rn = load_pair(g, t1 < G.cT, nbr_pick(ncur, t1), st);
-            const bf16x8 E = edge_features(g, valid, rc, G.xT, G.vT, G.rT, nm);
+            rn = ps.load(jn);
+            jn = ps.index(t + 2);
+            const bf16x8 E = edge_features(g, valid, rc, kc, nm);
             const uint32_t vm = (uint32_t)__ballot(valid);  // bit r: pair row r valid
 
             bf16x8 h1t[4];

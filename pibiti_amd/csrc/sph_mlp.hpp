@@ -46,7 +46,7 @@ struct MlpState {
     const float4* vel = nullptr;
     const float2* rhop = nullptr;
     const uint32_t* sid = nullptr;  // batch only
-    const uint32_t* nbr = nullptr;  // neighbour lists, nbr[k * kNbrCap + c], c < kNbrCap
+    const uint32_t* nbr = nullptr;  // neighbour lists, group-major (nbr_group_index), kNbrCap per slot
     const uint32_t* nnb = nullptr;  // neighbour counts
 };
 

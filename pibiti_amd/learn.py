@@ -95,7 +95,7 @@ def train(model: sph.Model, *, scenes_count: int = 200, seed: int = 100, steps: 
 
 
 def rollout_error(params: sph.Params, pos: np.ndarray, vel: np.ndarray, model: sph.Model | None, *,
-                  warm: int, coarse: int, factor: int = 4, tol: float = 1e9) -> dict:
+                  warm: int, coarse: int, factor: int = 4, tol: float = 1e9, skin: float = 0.0) -> dict:
     """Density error of ``coarse`` surrogate steps against factor x coarse
     plain steps from the same state (after ``warm`` plain steps):
     RMS and max over particles of |rho_surrogate - rho_plain| / rho0, and
@@ -126,7 +126,7 @@ def rollout_error(params: sph.Params, pos: np.ndarray, vel: np.ndarray, model: s
         with sph.SphSystem(params) as c:
             c.reset(pos, vel)
             c.update(warm)
-            rep = c.step_learned(model, coarse, factor=factor, tol=tol)
+            rep = c.step_learned(model, coarse, factor=factor, tol=tol, skin=skin)
             out["surrogate"] = err(c.get_array(sph.SPH_ARRAY_DENSITY))
             out["report"] = rep
     return out

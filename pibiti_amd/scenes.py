@@ -123,8 +123,8 @@ def scaled_dam_break(lattice, *, seed: int = 3, **kw):
     s = 0.01
     b = fluid_origin(s)[0]
     wm = (2 * nx * s + 2 * b, 2 * ny * s + 2 * b, 2 * nz * s + 2 * b)
-    return dam_break(nx, ny, nz, spacing=s, h=0.02, world_max=wm, dt=5e-5, viscosity=0.1,
-                     vel_sigma=0.05, seed=seed, **kw)
+    kw.setdefault("vel_sigma", 0.05)
+    return dam_break(nx, ny, nz, spacing=s, h=0.02, world_max=wm, dt=5e-5, viscosity=0.1, seed=seed, **kw)
 
 
 def colliding_blocks(edge: int = 128, *, spacing: float = 0.01, h: float = 0.02,

@@ -20,7 +20,8 @@ SPH_OK = 0
 SPH_EOS_LINEAR, SPH_EOS_TAIT = 0, 1
 SPH_PREC_REFERENCE, SPH_PREC_KAHAN = 0, 1
 (SPH_ARRAY_POSITION, SPH_ARRAY_VELOCITY, SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_DVEL,
- SPH_ARRAY_TARGET) = range(6)
+ SPH_ARRAY_TARGET, SPH_ARRAY_NEIGHBOURS) = range(7)
+_SCALAR_ARRAYS = (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_NEIGHBOURS)
 MLP_NUM_PARAMS = 5388
 SPH_FLAG_KEEP_DVEL = 1
 EMPTY_CELL = 0xFFFFFFFF
@@ -154,7 +155,7 @@ class LearnedReport(C.Structure):
         ("coarse_steps", C.c_uint32),
         ("fallbacks", C.c_uint32),
         ("plain_steps", C.c_uint32),
-        ("reserved0", C.c_uint32),
+        ("rebuilds", C.c_uint32),
         ("drift_max", C.c_double),
         ("drift_last", C.c_double),
         ("anchor", C.c_double),
@@ -163,6 +164,9 @@ class LearnedReport(C.Structure):
         ("mlp_ms", C.c_double),
         ("coarse_ms", C.c_double),
         ("pairs", C.c_double),
+        ("rebuild_ms", C.c_double),
+        ("early_rebuilds", C.c_uint32),
+        ("reserved0", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -234,7 +238,7 @@ SIGNATURES = {
     "sph_batch_learn_train": (C.c_int, [_P, _P, C.c_uint32, C.c_float, C.POINTER(C.c_double)]),
     "sph_batch_learn_gradient": (C.c_int, [_P, _P, C.POINTER(C.c_double), _fp]),
     "sph_batch_learn_get_window": (C.c_int, [_P, C.c_uint32, C.c_int, _fp]),
-    "sph_step_learned": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_float,
+    "sph_step_learned": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                    C.POINTER(LearnedReport)]),
 }
 
@@ -343,10 +347,11 @@ class SphSystem:
         self.set_array(SPH_ARRAY_VELOCITY, vel4)
 
     def get_array(self, which: int) -> np.ndarray:
-        shape = (self.n,) if which in (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE) else (self.n, 4)
+        """One array in original particle order (NEIGHBOURS as uint32)."""
+        shape = (self.n,) if which in _SCALAR_ARRAYS else (self.n, 4)
         out = np.empty(shape, np.float32)
         _check(lib().sph_get_array(self._ctx, which, _f(out)), "sph_get_array")
-        return out
+        return out.view(np.uint32) if which == SPH_ARRAY_NEIGHBOURS else out
 
     def update(self, nsteps: int = 1) -> None:
         _check(lib().sph_step(self._ctx, nsteps), "sph_step")
@@ -480,9 +485,13 @@ def _sys_learn_window(self, which: int) -> np.ndarray:
     return out
 
 
-def _sys_step_learned(self, model: Model, ncoarse: int, factor: int = 4, tol: float = 0.01) -> dict:
+def _sys_step_learned(self, model: Model, ncoarse: int, factor: int = 4, tol: float = 0.01,
+                      skin: float = 0.0) -> dict:
+    """``ncoarse`` surrogate steps of ``factor`` x dt.  skin = 0 sorts every
+    step; skin > 0 sorts once per tune[2] steps and in between walks every
+    cell within (1 + skin) h of the last sort's cell ranges."""
     r = LearnedReport()
-    _check(lib().sph_step_learned(self._ctx, model.handle, ncoarse, factor, tol, C.byref(r)),
+    _check(lib().sph_step_learned(self._ctx, model.handle, ncoarse, factor, tol, skin, C.byref(r)),
            "sph_step_learned")
     return r.as_dict()
 
@@ -540,7 +549,7 @@ class SphBatch:
 
     def get_array(self, scene: int, which: int) -> np.ndarray:
         n = self.counts[scene]
-        shape = (n,) if which in (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE) else (n, 4)
+        shape = (n,) if which in _SCALAR_ARRAYS else (n, 4)
         out = np.empty(shape, np.float32)
         _check(lib().sph_batch_get_array(self._b, scene, which, _f(out)), "sph_batch_get_array")
         return out

@@ -193,6 +193,84 @@ def test_fallback_replays_plain_steps_exactly():
         assert abs(a.stats()["sim_time"] - b.stats()["sim_time"]) < 1e-12
 
 
+def skin_case(seed=12, sigma=1.0, check=4):
+    """A small dam break with fast random velocities, so that pairs cross h
+    within a few surrogate steps, and a random network."""
+    p, pos, vel, _ = scenes.scaled_dam_break((10, 8, 10), seed=seed, vel_sigma=sigma)
+    p.tune[2] = check
+    return p, pos, vel
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.15])
+def test_learned_neighbours_and_density_match_brute_force(skin):
+    """After surrogate steps (with a skin: a sort and two steps that walk its
+    cell ranges) each particle's listed neighbours are exactly the pairs
+    within h of its current position, and its density is the oracle's."""
+    p, pos, vel = skin_case()
+    print("case: skin", skin, flush=True)
+    with sph.SphSystem(p) as ctx, sph.Model(norm_for(p)) as model:
+        model.set_weights(random_weights(6))
+        ctx.reset(pos, vel)
+        ctx.update(2)
+        rep = ctx.step_learned(model, 3, factor=4, tol=1e9, skin=skin)
+        x = ctx.get_array(sph.SPH_ARRAY_POSITION)
+        rho = ctx.get_array(sph.SPH_ARRAY_DENSITY)
+        nn = ctx.get_array(sph.SPH_ARRAY_NEIGHBOURS)
+        assert ctx.stats()["cap_cells"] == 0
+    print("report", rep)
+    assert rep["coarse_steps"] == 3 and rep["fallbacks"] == 0
+    assert rep["rebuilds"] == (1 if skin else 0) and rep["early_rebuilds"] == 0
+    pi, _ = mo.pairs_bruteforce(x, p.h)
+    assert np.array_equal(nn, np.bincount(pi, minlength=x.shape[0]))
+    ref, _ = oracle.density_bruteforce_f64(p, x)
+    assert np.max(np.abs(rho - ref) / ref) < 1e-5
+
+
+def test_skin_steps_follow_the_resorting_path():
+    """Steps between sorts change only the order of the density and
+    neighbour sums: the trajectory stays within rounding of the path that
+    sorts every step, and is reproduced exactly by a second run."""
+    p, pos, vel = skin_case(seed=13)
+    out = {}
+    for skin in (0.0, 0.15, 0.15):
+        with sph.SphSystem(p) as ctx, sph.Model(norm_for(p)) as model:
+            model.set_weights(random_weights(7))
+            ctx.reset(pos, vel)
+            ctx.update(2)
+            rep = ctx.step_learned(model, 8, factor=4, tol=1e9, skin=skin)
+            assert rep["coarse_steps"] == 8
+            got = (ctx.get_array(sph.SPH_ARRAY_POSITION), ctx.get_array(sph.SPH_ARRAY_VELOCITY))
+        if skin in out:
+            assert all(np.array_equal(a, b) for a, b in zip(got, out[skin]))
+        out[skin] = got
+    (x0, v0), (x1, v1) = out[0.0], out[0.15]
+    dv = np.abs(v1 - v0)[:, :3].max() / np.abs(v0[:, :3]).max()
+    dx = np.abs(x1 - x0)[:, :3].max() / p.h
+    print("skin vs sorted: max |dv| / max |v|", dv, "max |dx| / h", dx)
+    assert dv < 1e-4 and dx < 1e-5
+
+
+def test_skin_exceeded_rebuilds_early():
+    """With a skin far thinner than one step's motion, every step past a
+    sort is cut short and taken again after a new sort: no step is lost to
+    the fallback, and the lists stay exact."""
+    p, pos, vel = skin_case(seed=14)
+    with sph.SphSystem(p) as ctx, sph.Model(norm_for(p)) as model:
+        model.set_weights(random_weights(8))
+        ctx.reset(pos, vel)
+        ctx.update(2)
+        rep = ctx.step_learned(model, 4, factor=4, tol=1e9, skin=1e-4)
+        x = ctx.get_array(sph.SPH_ARRAY_POSITION)
+        nn = ctx.get_array(sph.SPH_ARRAY_NEIGHBOURS)
+    print("report", rep)
+    assert rep["coarse_steps"] == 4 and rep["fallbacks"] == 0 and rep["plain_steps"] == 0
+    # four runs, each a sort and one step; the first three also a second
+    # step that is undone, and a sort replaying the kept step
+    assert rep["early_rebuilds"] == 3 and rep["rebuilds"] == 7
+    pi, _ = mo.pairs_bruteforce(x, p.h)
+    assert np.array_equal(nn, np.bincount(pi, minlength=x.shape[0]))
+
+
 def test_batch_window_gradient_matches_single_scenes():
     """A batch window's loss is the particle-weighted mean of its scenes'."""
     sc = [scenes.scaled_dam_break((8, 6, 8), seed=s) for s in (10, 11)]
