@@ -127,7 +127,7 @@ def learned_section(args, p, snap_pos, snap_vel, rho_plain_end, plain_secs) -> d
                                        "step as the plain step, so 4x the simulated-time rate)"},
         "roofline": {"bound": "mfma", "kernel": "surrogate pair network",
                      "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": None,
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": measured_traffic().get("surrogate"),
                      "pairs_per_launch": krep["pairs"], "flops_per_launch": flops,
                      "mean_launch_ms": round(krep["mlp_ms"], 5)},
         "kernels_ms": {"density_lists": round(krep["density_ms"], 5), "surrogate": round(krep["mlp_ms"], 5),
@@ -137,6 +137,45 @@ def learned_section(args, p, snap_pos, snap_vel, rho_plain_end, plain_secs) -> d
                      "updates": len(L) * hist["iters"], "final_loss": float(np.mean(tail)),
                      "first_loss": float(L[0])},
     }
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_dam_break_1m.txt"
+
+
+def measured_traffic() -> dict:
+    """HBM bytes per launch of the dominant kernels, from the committed
+    counter summary (lines 'traffic <kernel> <bytes>'), or {} if absent."""
+    out = {}
+    if TRAFFIC_FILE.exists():
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic":
+                out[f[1]] = int(f[2])
+    return out
+
+
+def copy_calibration() -> None:
+    """Five device-to-device copies of 256 MiB (the HIP runtime's copy
+    kernel): under a memory-counter profile their FETCH_SIZE / WRITE_SIZE
+    read against the known 256 MiB."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    size = C.c_size_t(256 << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), size) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, size) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(5):
+            if hip.hipMemcpy(b, a, size, 3) != 0:  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 def algorithmic_bytes(p) -> dict:
@@ -205,10 +244,13 @@ def main() -> int:
     ap.add_argument("--train-iters", type=int, default=16)
     ap.add_argument("--train-lr", type=float, default=3e-3)
     ap.add_argument("--tol", type=float, default=0.01)
-    ap.add_argument("--skin", type=float, default=0.2,
-                    help="surrogate neighbour-list skin as a fraction of h (0 = sort every step)")
+    ap.add_argument("--skin", type=float, default=0.05,
+                    help="surrogate steps between sorts walk cells within (1 + skin) h (0 = sort every step)")
     ap.add_argument("--check", type=int, default=4,
-                    help="surrogate steps per host check (and per list build with a skin)")
+                    help="surrogate steps per host check (and per sort with a skin)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with five device copies of 256 MiB (a plain copy kernel of known size, "
+                         "for scaling memory counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -249,6 +291,7 @@ def main() -> int:
     ab = algorithmic_bytes(p)
     kern_gbs = {k: ab[k] * n / (ms[k] * 1e-3) / 1e9 for k in ab if ms.get(k, 0) > 0}
     dom = max(("density", "force"), key=lambda k: ms[k])
+    traffic = measured_traffic()
     roof = {
         "bound": "hbm",
         "kernel": dom,
@@ -256,7 +299,7 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(kern_gbs[dom] / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": traffic.get(dom),
         "algorithmic_bytes_per_particle": round(ab[dom], 3),
         "mean_launch_ms": round(ms[dom], 5),
     }
@@ -298,6 +341,8 @@ def main() -> int:
         out["learned"] = learned_section(args, p, snap_pos, snap_vel, rho_plain_end, secs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget, p, pos, vel)
+    if args.copy_calibration:
+        copy_calibration()
     print(json.dumps(out), flush=True)
     return 0
 
