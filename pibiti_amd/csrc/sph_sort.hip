@@ -10,7 +10,8 @@
 //
 // Structure (one read of the keys for all digit histograms, then one kernel
 // per pass):
-//   k_hist        per-block digit histograms of every pass in one read,
+//   k_hist        per-block digit histograms of every pass in one read
+//                 (runs of equal digits in a wave counted by one atomic),
 //   k_hist_scan   sum over blocks + exclusive scan -> global digit offsets,
 //   k_pass        per tile: stable local ranking with 64-lane ballots (peer
 //                 mask per digit, rank = popcount of lower peers), tile digit
@@ -22,6 +23,8 @@
 // an error instead of hanging.  Ticket counter and look-back status words
 // carry the launch number, so nothing needs clearing between launches and
 // replays of a captured graph stay correct.
+#include <algorithm>
+
 #include "sph_internal.hpp"
 
 namespace sph {
@@ -35,6 +38,12 @@ constexpr int kTile = kSortBlock * kItems;  // keys per tile
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 4;
 constexpr int kHistBlocks = 64;
+constexpr int kHistThreads = 1024;
+constexpr int kHistKeys = 16;                       // keys per thread per round
+constexpr int kHistTile = kHistThreads * kHistKeys;  // keys per block per round
+constexpr int kScanThreads = 1024;
+constexpr int kScanQ = kScanThreads / kRadix;        // threads per digit in k_hist_scan
+static_assert(kHistBlocks % kScanQ == 0 && kHistBlocks / kScanQ <= 16, "k_hist_scan: one round of loads");
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 constexpr uint64_t kFlagAgg = 1ull << 30;
@@ -43,6 +52,24 @@ constexpr uint64_t kValMask = (1ull << 30) - 1;
 
 static_assert(kSortBlock == kRadix, "one thread per digit in the tile bookkeeping");
 
+// Inclusive scan of one value per thread over the block: a shuffle scan in
+// each wave, then the totals of the waves before it.
+template <int kThreads>
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wave) {
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off);
+        v += lane >= off ? u : 0u;
+    }
+    if (lane == kWave - 1) s_wave[w] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int k = 0; k < w; ++k) add += s_wave[k];
+    __syncthreads();
+    return v + add;
+}
+
 __device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -50,47 +77,77 @@ __device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                    uint32_t bits, uint32_t* __restrict__ block_hist) {
+// Digit histograms of every pass in one read of the keys.  Each thread
+// loads kHistKeys keys before counting (the loads overlap), and within a
+// wave a run of consecutive lanes with the same digit is counted by one
+// LDS atomic of its first lane: the keys arrive nearly sorted (the previous
+// step's order), so runs are long and the atomics rarely collide.
+__global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                      uint32_t bits, uint32_t* __restrict__ block_hist) {
     __shared__ uint32_t hist[kMaxPasses][kRadix];
     const int t = threadIdx.x;
-    for (int p = 0; p < kMaxPasses; ++p) hist[p][t] = 0;
+    const int lane = t % kWave;
+    for (int c = t; c < kMaxPasses * kRadix; c += kHistThreads) hist[c / kRadix][c % kRadix] = 0;
     __syncthreads();
     const uint32_t passes = (bits + 7) / 8;
-    for (uint32_t i = blockIdx.x * kSortBlock + t; i < n; i += gridDim.x * kSortBlock) {
-        const uint32_t k = keys[i];
-        for (uint32_t p = 0; p < passes; ++p) atomicAdd(&hist[p][(k >> (8 * p)) & 0xFFu], 1u);
+    const uint64_t after_me = lane == kWave - 1 ? 0ull : (~0ull << (lane + 1));
+    for (uint64_t base = (uint64_t)blockIdx.x * kHistTile; base < n; base += (uint64_t)gridDim.x * kHistTile) {
+        uint32_t k[kHistKeys];
+#pragma unroll
+        for (int it = 0; it < kHistKeys; ++it) {
+            const uint64_t e = base + (uint64_t)it * kHistThreads + t;
+            k[it] = keys[e < n ? e : n - 1];
+        }
+#pragma unroll
+        for (int it = 0; it < kHistKeys; ++it) {
+            // the valid lanes of a wave are a prefix of it (indices grow with the lane)
+            const bool valid = base + (uint64_t)it * kHistThreads + t < n;
+            const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+            for (uint32_t p = 0; p < passes; ++p) {
+                const uint32_t d = (k[it] >> (8 * p)) & 0xFFu;
+                const uint32_t dprev = __shfl_up(d, 1);
+                const bool head = valid && (lane == 0 || dprev != d);
+                const uint64_t later = __ballot(head) & after_me;
+                const uint32_t end = later ? (uint32_t)(__ffsll((unsigned long long)later) - 1) : nvalid;
+                if (head) atomicAdd(&hist[p][d], end - (uint32_t)lane);
+            }
+        }
     }
     __syncthreads();
-    for (uint32_t p = 0; p < passes; ++p)
-        block_hist[((size_t)blockIdx.x * kMaxPasses + p) * kRadix + t] = hist[p][t];
+    for (int c = t; c < (int)passes * kRadix; c += kHistThreads)
+        block_hist[((size_t)blockIdx.x * kMaxPasses + c / kRadix) * kRadix + c % kRadix] = hist[c / kRadix][c % kRadix];
 }
 
-// One block per pass: digit totals over the histogram blocks, exclusive scan.
-__global__ __launch_bounds__(kRadix) void k_hist_scan(const uint32_t* __restrict__ block_hist,
-                                                     uint32_t nblocks, uint32_t* __restrict__ offsets) {
-    __shared__ uint32_t s[kRadix];
-    const int t = threadIdx.x;
+// One block per pass: digit totals over the histogram blocks (kScanQ
+// threads per digit, every load of a thread in flight at once), then an
+// exclusive scan.
+__global__ __launch_bounds__(kScanThreads) void k_hist_scan(const uint32_t* __restrict__ block_hist,
+                                                           uint32_t nblocks, uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t part[kScanQ][kRadix];
+    __shared__ uint32_t s_wave[kScanThreads / kWave];
+    const int t = threadIdx.x % kRadix;
+    const int q = threadIdx.x / kRadix;
     const int p = blockIdx.x;
-    uint32_t part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // independent loads in flight: eight running sums
-    for (uint32_t b = 0; b < nblocks; b += 8) {
+    constexpr int kRows = kHistBlocks / kScanQ;
+    uint32_t v[kRows];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (b + u < nblocks) part[u] += block_hist[((size_t)(b + u) * kMaxPasses + p) * kRadix + t];
+    for (int u = 0; u < kRows; ++u) {
+        const uint32_t b = (uint32_t)(q + u * kScanQ);
+        v[u] = block_hist[((size_t)min(b, nblocks - 1) * kMaxPasses + p) * kRadix + t];
     }
-    uint32_t tot = 0;
+    uint32_t sum = 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) tot += part[u];
-    s[t] = tot;
+    for (int u = 0; u < kRows; ++u) sum += (uint32_t)(q + u * kScanQ) < nblocks ? v[u] : 0u;
+    part[q][t] = sum;
     __syncthreads();
-    for (int off = 1; off < kRadix; off <<= 1) {
-        const uint32_t add = t >= off ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += add;
-        __syncthreads();
+    uint32_t tot = 0;
+    if (q == 0) {
+#pragma unroll
+        for (int r = 0; r < kScanQ; ++r) tot += part[r][t];
     }
-    offsets[p * kRadix + t] = s[t] - tot;
+    // threads past the first kRadix add nothing and come after every digit
+    const uint32_t incl = block_incl_scan<kScanThreads>(tot, s_wave);
+    if (q == 0) offsets[p * kRadix + t] = incl - tot;
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_pass(
@@ -103,6 +160,7 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(
     __shared__ uint32_t gbase[kRadix];
     __shared__ uint32_t skeys[kTile];
     __shared__ uint32_t svals[kTile];
+    __shared__ uint32_t s_wave[kWaves];
     __shared__ uint64_t s_ticket;
 
     const int t = threadIdx.x;
@@ -180,16 +238,7 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(
         st_status(my, tag | kFlagInc | (uint64_t)(prefix + cnt));
         gbase[t] = digit_off[t] + prefix;
     }
-    dig_off[t] = cnt;
-    __syncthreads();
-    for (int off = 1; off < kRadix; off <<= 1) {
-        const uint32_t add = t >= off ? dig_off[t - off] : 0u;
-        __syncthreads();
-        dig_off[t] += add;
-        __syncthreads();
-    }
-    const uint32_t incl = dig_off[t];
-    __syncthreads();
+    const uint32_t incl = block_incl_scan<kSortBlock>(cnt, s_wave);
     dig_off[t] = incl - cnt;
     __syncthreads();
 
@@ -254,10 +303,10 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
     const uint32_t passes = (bits + 7) / 8;
     if (passes > (uint32_t)kMaxPasses) return hipErrorInvalidValue;
     if (passes > 0) {
-        hipLaunchKernelGGL(k_hist, dim3(kHistBlocks), dim3(kSortBlock), 0, stream, kin, n, bits,
-                           plan.block_hist);
-        hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(kRadix), 0, stream, plan.block_hist,
-                           (uint32_t)kHistBlocks, plan.digit_off);
+        const uint32_t hb = std::min<uint32_t>(kHistBlocks, std::max<uint32_t>(1u, (n + kHistTile - 1) / kHistTile));
+        hipLaunchKernelGGL(k_hist, dim3(hb), dim3(kHistThreads), 0, stream, kin, n, bits, plan.block_hist);
+        hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(kScanThreads), 0, stream, plan.block_hist, hb,
+                           plan.digit_off);
     }
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t shift = 8 * p;
