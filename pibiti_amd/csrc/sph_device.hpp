@@ -51,6 +51,24 @@ __device__ __forceinline__ float eos(const sph_params& p, float rho) {
     return (rho - p.rest_density) * p.stiffness;
 }
 
+// IEEE square root (rounded to nearest).  For x >= 2^-96 the hardware
+// square root (within 1 ulp) is moved to a neighbour when the residual
+// x - neighbour * s (one fused multiply-add, exact in sign) says that
+// neighbour is nearer; smaller arguments take the library routine.
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __int_as_float(__float_as_int(s) - 1);
+    const float hi = __int_as_float(__float_as_int(s) + 1);
+    float r = __builtin_fmaf(-lo, s, x) <= 0.0f ? lo : s;
+    r = __builtin_fmaf(-hi, s, x) > 0.0f ? hi : r;
+    return r;
+}
+
+__device__ __forceinline__ float sqrt_ieee(float x) {
+    if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(x);
+    return sqrt_rn_normal(x);
+}
+
 struct KahanF {
     float s, c;
     __device__ __forceinline__ void add(float v) {

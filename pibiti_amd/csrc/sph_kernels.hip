@@ -419,7 +419,7 @@ __device__ __forceinline__ void force_drain(const sph_params& p, const uint32_t*
         const float2 rj = rhop[jj];
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const float r = fmaxf(min_dist, sqrtf(r2));
+        const float r = fmaxf(min_dist, sqrt_ieee(r2));
         const float d12 = fminf(min_dens, 1.0f / (ri.x * rj.x));
         const float cc = h - r;
         const float pterm = ((cc * spiky) * (ri.y + rj.y)) / r;
@@ -502,13 +502,12 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
                 const float4 xj = xq[k];
                 const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
-                // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max.  Within the
-                // cell the store is unconditional (cnt < list size holds for the
-                // whole cell); only a hit advances cnt.
-                if (j < e) {
-                    list[cnt * kB + t] = j | tag;
-                    cnt += (r2 <= r2max && j != i) ? 1 : 0;
-                }
+                // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max.  The
+                // store is unconditional, also for the repeated slots past the
+                // cell's end (cnt <= list size holds for the whole cell, and the
+                // list has one spare row for a full list); only a hit advances cnt.
+                list[cnt * kB + t] = j | tag;
+                cnt += (r2 <= r2max && j != i && j < e) ? 1 : 0;
             }
         }
     }
@@ -539,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
     float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
-    __shared__ uint32_t list[kList * kBlock];
+    __shared__ uint32_t list[(kList + 1) * kBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
                                     cell_end, vel_out, dvel, dt_dev);
@@ -552,7 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_force_persistent(
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
     float4* __restrict__ dvel, const float* __restrict__ dt_dev, uint32_t* __restrict__ work) {
-    __shared__ uint32_t list[kList * kBlock];
+    __shared__ uint32_t list[(kList + 1) * kBlock];
     __shared__ uint32_t s_tile;
     const uint32_t ntiles = (n + kBlock - 1) / kBlock;
     for (;;) {
@@ -686,7 +685,7 @@ __global__ __launch_bounds__(kBlock) void k_force_cells(
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
     float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
-    __shared__ uint32_t list[kList * kBlock];
+    __shared__ uint32_t list[(kList + 1) * kBlock];
     __shared__ CellStage S;
     const uint32_t b0 = xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     const uint32_t count = min((uint32_t)kBlock, n - b0);
@@ -709,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_force_ordered(
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
     float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
-    __shared__ uint32_t list[kList * kBlock];
+    __shared__ uint32_t list[(kList + 1) * kBlock];
     const uint32_t i = (kOrder ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x;
     force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end,
                                     vel_out, dvel, dt_dev);
@@ -722,7 +721,7 @@ __global__ __launch_bounds__(kForceTile) void k_force_tiled(
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
     float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
-    __shared__ uint32_t list[kList * kForceTile];
+    __shared__ uint32_t list[(kList + 1) * kForceTile];
     __shared__ Stage<kForceStage> S;
     const uint32_t b0 = xcd_tile(blockIdx.x, gridDim.x) * kForceTile;
     const uint32_t count = min((uint32_t)kForceTile, n - b0);
@@ -775,7 +774,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
             if (r2 <= r2max) {
                 const float4 vj = vel[j];
                 const float2 rj = rhop[j];
-                const float r = fmaxf(min_dist, sqrtf(r2));
+                const float r = fmaxf(min_dist, sqrt_ieee(r2));
                 const float d12 = fminf(min_dens, 1.0f / (ri.x * rj.x));
                 const float cc = h - r;
                 const float pterm = ((cc * spiky) * (ri.y + rj.y)) / r;

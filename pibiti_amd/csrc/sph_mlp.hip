@@ -102,18 +102,6 @@ __device__ __forceinline__ uint32_t cvt2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
-// Square root rounded to nearest for x >= 0 in the normal range: the
-// hardware square root (within 1 ulp) moved to a neighbour when the
-// residual x - neighbour * s says that neighbour is nearer.
-__device__ __forceinline__ float sqrt_rn(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float lo = __int_as_float(__float_as_int(s) - 1);
-    const float hi = __int_as_float(__float_as_int(s) + 1);
-    float r = __builtin_fmaf(-lo, s, x) <= 0.0f ? lo : s;
-    r = __builtin_fmaf(-hi, s, x) > 0.0f ? hi : r;
-    return r;
-}
-
 // What one lane gathers for one pair: the neighbour's position (g = 0
 // lanes) or velocity (g = 1 lanes), and its density.
 struct PairRaw {
@@ -165,7 +153,7 @@ __device__ __forceinline__ bf16x8 edge_features(int g, bool valid, const PairRaw
                                                 const sph_mlp_norm& nm) {
     const float dx = r.a.x - k.T.x, dy = r.a.y - k.T.y, dz = r.a.z - k.T.z;
     const float r2 = (dx * dx + dy * dy) + dz * dz;
-    const float dist = sqrt_rn(r2) * nm.inv_h;
+    const float dist = sqrt_rn_normal(r2) * nm.inv_h;  // r2 is 0 or normal for a real pair
     const float rj = (r.b * nm.inv_rho0 - 1.0f) * nm.rho_scale;
     u32x4 w;
     w[0] = valid ? cvt2(dx * k.s, dy * k.s) : 0u;
