@@ -244,7 +244,7 @@ def main() -> int:
     ap.add_argument("--train-iters", type=int, default=16)
     ap.add_argument("--train-lr", type=float, default=3e-3)
     ap.add_argument("--tol", type=float, default=0.01)
-    ap.add_argument("--skin", type=float, default=0.05,
+    ap.add_argument("--skin", type=float, default=0.0,
                     help="surrogate steps between sorts walk cells within (1 + skin) h (0 = sort every step)")
     ap.add_argument("--check", type=int, default=4,
                     help="surrogate steps per host check (and per sort with a skin)")

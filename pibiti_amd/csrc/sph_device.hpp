@@ -168,28 +168,44 @@ __device__ __forceinline__ size_t nbr_group_index(uint32_t g, uint32_t n, uint32
     return (size_t)g * n + i;
 }
 
-// Appends a slot's neighbours four at a time; entries past kNbrCap are
-// counted but not stored.
-struct ListWriter {
-    uint4 stage;
-    uint32_t count;
+// Appends a slot's neighbours through a ring of kRingSlots entries per lane
+// in LDS (slot s of lane t at ring[s * kRingLanes + t]): every candidate is
+// written at the position of the next entry and only a hit advances the
+// count, so lanes never branch per candidate.  After each group of at most
+// four candidates a completed group of four entries (there is at most one)
+// leaves as one 16-byte store; finish() stores the partly filled last group
+// (its unused entries are never read).  Entries past kNbrCap are counted
+// but not stored.  Every block that builds lists has kRingLanes threads.
+constexpr uint32_t kRingSlots = 8;
+constexpr uint32_t kRingLanes = 256;
 
-    __device__ __forceinline__ void init() {
-        stage = make_uint4(0u, 0u, 0u, 0u);
+struct ListRing {
+    uint32_t* slot0;  // this lane's slot 0
+    uint32_t count;
+    uint32_t flushed;
+
+    __device__ __forceinline__ void init(uint32_t* ring) {
+        slot0 = ring + threadIdx.x;
         count = 0u;
+        flushed = 0u;
     }
-    __device__ __forceinline__ void push(uint4* __restrict__ nbr, uint32_t n, uint32_t i, uint32_t j) {
-        const uint32_t q = count & 3u;
-        stage.x = q == 0u ? j : stage.x;
-        stage.y = q == 1u ? j : stage.y;
-        stage.z = q == 2u ? j : stage.z;
-        stage.w = q == 3u ? j : stage.w;
-        if (q == 3u && count < kNbrCap) nbr[nbr_group_index(count >> 2, n, i)] = stage;
-        ++count;
+    __device__ __forceinline__ void put(uint32_t j, bool hit) {
+        slot0[(count & (kRingSlots - 1u)) * kRingLanes] = j;
+        count += hit ? 1u : 0u;
     }
-    // the last, partly filled group (its unused entries are never read)
-    __device__ __forceinline__ void flush(uint4* __restrict__ nbr, uint32_t n, uint32_t i) const {
-        if ((count & 3u) != 0u && count < kNbrCap) nbr[nbr_group_index(count >> 2, n, i)] = stage;
+    __device__ __forceinline__ uint4 group_at(uint32_t first) const {
+        const uint32_t b = first & (kRingSlots - 1u);
+        return make_uint4(slot0[b * kRingLanes], slot0[(b + 1u) * kRingLanes], slot0[(b + 2u) * kRingLanes],
+                          slot0[(b + 3u) * kRingLanes]);
+    }
+    __device__ __forceinline__ void drain(uint4* __restrict__ nbr, uint32_t n, uint32_t i) {
+        if (count - flushed >= 4u) {
+            if (flushed < kNbrCap) nbr[nbr_group_index(flushed >> 2, n, i)] = group_at(flushed);
+            flushed += 4u;
+        }
+    }
+    __device__ __forceinline__ void finish(uint4* __restrict__ nbr, uint32_t n, uint32_t i) {
+        if (count > flushed && flushed < kNbrCap) nbr[nbr_group_index(flushed >> 2, n, i)] = group_at(flushed);
     }
 };
 

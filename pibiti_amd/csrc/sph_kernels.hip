@@ -236,7 +236,7 @@ __device__ bool stage_block(Stage<kCap>& S, uint32_t b0, uint32_t count, const s
 // ------------------------------------------------------------------ density
 // With kLists the density walk also records each particle's neighbours
 // (the slots that pass r2 < h2, j != i, in walk order) in the group-major
-// lists (ListWriter), and their count in nnb[i]; a count above kNbrCap
+// lists (ListRing), and their count in nnb[i]; a count above kNbrCap
 // raises the overflow counter (entries past the cap are dropped).
 template <bool kBatch, int kPrec, bool kLists = false, typename Src = PosGlobal, bool kCells = false>
 __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& one,
@@ -264,8 +264,9 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     const float h2 = p.h2;
     float sum = 0.0f;
     KahanF ks{0.0f, 0.0f};
-    ListWriter lw;
-    lw.init();
+    __shared__ uint32_t ring[kLists ? kRingSlots * kRingLanes : 1];
+    ListRing lw;
+    if constexpr (kLists) lw.init(ring);
     uint4* const lists = reinterpret_cast<uint4*>(nbr);
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
@@ -301,10 +302,9 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                     // the partial sum is >= +0, so adding +0 for a miss is exact
                     cell = cell + (hit ? wgt : 0.0f);
                 }
-                if constexpr (kLists) {
-                    if (hit) lw.push(lists, n, i, j);
-                }
+                if constexpr (kLists) lw.put(j, hit);
             }
+            if constexpr (kLists) lw.drain(lists, n, i);
         }
         if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
     }
@@ -312,11 +312,12 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
     rhop[i] = make_float2(rho, eos(p, rho));
     if constexpr (kLists) {
-        lw.flush(lists, n, i);
+        lw.finish(lists, n, i);
         nnb[i] = lw.count;
         if (lw.count > kNbrCap) atomicAdd(&counters[kCntNbrOverflow], 1u);
+        return lw.count;
     }
-    return lw.count;
+    return 0u;
 }
 
 

@@ -23,6 +23,7 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;
+static_assert(kBlock == (int)kRingLanes, "list rings are sized for the block");
 
 // Cells of one axis that [x - r, x + r] reaches, clamped into the grid, and
 // the squared distance from x to a cell (zero inside), shrunk by a fuzz that
@@ -56,6 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_density_reach(sph_params p, uint32_t
                                                          float2* __restrict__ rhop, uint4* __restrict__ nbr,
                                                          uint32_t* __restrict__ nnb,
                                                          uint32_t* __restrict__ counters) {
+    __shared__ uint32_t ring[kRingSlots * kRingLanes];
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     uint32_t listed = 0;
     if (i < n) {
@@ -70,8 +72,8 @@ __global__ __launch_bounds__(kBlock) void k_density_reach(sph_params p, uint32_t
         const float cull2 = (reach * reach) * 1.00001f;
         const float h2 = p.h2;
         const uint32_t cap = p.max_par_in_cell;
-        ListWriter lw;
-        lw.init();
+        ListRing lw;
+        lw.init(ring);
         float sum = 0.0f;
         for (int gz = az.lo; gz <= az.hi; ++gz) {
             const float dz2 = az.d2(gz);
@@ -102,14 +104,15 @@ __global__ __launch_bounds__(kBlock) void k_density_reach(sph_params p, uint32_t
                             const float c3 = h2 - r2;
                             const float wgt = (c3 * c3) * c3;
                             cell = cell + (hit ? wgt : 0.0f);
-                            if (hit) lw.push(nbr, n, i, j);
+                            lw.put(j, hit);
                         }
+                        lw.drain(nbr, n, i);
                     }
                     sum = sum + cell;
                 }
             }
         }
-        lw.flush(nbr, n, i);
+        lw.finish(nbr, n, i);
         nnb[i] = lw.count;
         if (lw.count > kNbrCap) atomicAdd(&counters[kCntNbrOverflow], 1u);
         const float rho = (sum * p.poly6_kern) * p.particle_mass;
