@@ -241,7 +241,7 @@ def main() -> int:
     ap.add_argument("--roofline-steps", type=int, default=50)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-steps", type=int, default=1600)
-    ap.add_argument("--train-iters", type=int, default=16)
+    ap.add_argument("--train-iters", type=int, default=32)
     ap.add_argument("--train-lr", type=float, default=3e-3)
     ap.add_argument("--tol", type=float, default=0.01)
     ap.add_argument("--skin", type=float, default=0.0,

@@ -33,7 +33,7 @@ namespace {
 constexpr int kSortBlock = 256;
 constexpr int kWave = 64;
 constexpr int kWaves = kSortBlock / kWave;
-constexpr int kItems = 8;
+constexpr int kItems = 16;
 constexpr int kTile = kSortBlock * kItems;  // keys per tile
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 4;
