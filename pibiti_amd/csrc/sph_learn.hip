@@ -52,8 +52,9 @@ struct LearnState {
     uint32_t* ovf = nullptr;     // per step: list overflow, listed pairs, skin moves
     float* dt_coarse = nullptr;
     float4* bpos = nullptr;      // positions at the last sort (skin > 0)
-    double* host = nullptr;      // pinned copies of dev / ovf
+    double* host = nullptr;      // pinned copies of dev / ovf, one region per chunk slot
     uint32_t* host_cnt = nullptr;
+    hipEvent_t done[2] = {};     // end of the chunk in each slot
 };
 
 // Surrogate steps between host checks (sph_params::tune[2], 0 = default).
@@ -77,6 +78,8 @@ void learn_release(Engine& e) {
     for (void* q : ptrs) (void)hipFree(q);
     if (L->host) (void)hipHostFree(L->host);
     if (L->host_cnt) (void)hipHostFree(L->host_cnt);
+    for (hipEvent_t ev : L->done)
+        if (ev) (void)hipEventDestroy(ev);
     delete L;
     e.learn = nullptr;
 }
@@ -134,16 +137,18 @@ hipError_t ensure_step(LearnState& L, uint32_t nscenes) {
     const size_t N = L.n;
     hipError_t e = dalloc(&L.lnbr, N * kNbrCap);
     if (e == hipSuccess) e = dalloc(&L.lnnb, N);
-    if (e == hipSuccess) e = dalloc(&L.spos, N);
-    if (e == hipSuccess) e = dalloc(&L.svel, N);
-    if (e == hipSuccess) e = dalloc(&L.spid, N);
+    if (e == hipSuccess) e = dalloc(&L.spos, 2 * N);
+    if (e == hipSuccess) e = dalloc(&L.svel, 2 * N);
+    if (e == hipSuccess) e = dalloc(&L.spid, 2 * N);
     if (e == hipSuccess) e = dalloc(&L.dev_part, (size_t)256);
     if (e == hipSuccess) e = dalloc(&L.dev, (size_t)kMaxCheck);
     if (e == hipSuccess) e = dalloc(&L.ovf, (size_t)kStepCounters * kMaxCheck);
     if (e == hipSuccess) e = dalloc(&L.dt_coarse, (size_t)nscenes);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.host), kMaxCheck * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.host), 2 * kMaxCheck * sizeof(double));
     if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&L.host_cnt), kStepCounters * kMaxCheck * sizeof(uint32_t));
+        e = hipHostMalloc(reinterpret_cast<void**>(&L.host_cnt),
+                          2 * kStepCounters * kMaxCheck * sizeof(uint32_t));
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&L.done[k], hipEventDisableTiming);
     if (e == hipSuccess) L.step_ready = true;
     return e;
 }
@@ -364,9 +369,8 @@ int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m, float skin, boo
     else
         SPH_TRY(launch_density_reach(c, b, skin, L.bpos, L.lnbr, L.lnnb, e.stream));
     if (timed) SPH_TRY(hipEventRecord(e.ev[2], e.stream));
-    SPH_TRY(mlp_density_deviation(c.n, b.rhop, 1.0f / e.scenes[0].rest_density, L.dev_part, dev_out, e.stream));
-    SPH_TRY(hipMemcpyAsync(cnt_out, e.counters + kCntNbrOverflow, kStepCounters * sizeof(uint32_t),
-                           hipMemcpyDeviceToDevice, e.stream));
+    SPH_TRY(mlp_density_deviation(c.n, b.rhop, 1.0f / e.scenes[0].rest_density, L.dev_part, dev_out, e.stream,
+                                  e.counters, kCntNbrOverflow, kStepCounters, cnt_out));
     MlpState st;
     st.n = c.n;
     st.pos = b.pos_cur;
@@ -395,13 +399,29 @@ int plain_deviation(Engine& e, LearnState& L, double* out) {
 // Surrogate steps run in chunks of `check` steps between host checks.  A
 // chunk starts from a snapshot of the state (and, with a skin, its first
 // step sorts while the others keep that sort's order); its steps'
-// deviations D and counters are read back once.  The first step whose drift |D - anchor| exceeds tol (or whose
-// neighbour lists overflowed) is replaced: the chunk is undone, the steps
-// before it are taken again (the surrogate step is deterministic, so they
-// reproduce exactly) and `factor` plain steps stand in for it.  A step in
-// which a particle moved farther than the skin since the sort ends the
-// chunk the same way, but is then taken again as the first step of the
-// next chunk, which sorts.
+// deviations D and counters are copied to the host at its end.  Two chunks
+// are in flight: the next one is queued before the host reads this one's
+// results, so the device never waits for the host.  The first step whose
+// drift |D - anchor| exceeds tol (or whose neighbour lists overflowed) is
+// replaced: the queue is drained, the chunk is undone from its snapshot,
+// the steps before the bad one are taken again (the surrogate step is
+// deterministic, so they reproduce exactly) and `factor` plain steps stand
+// in for it; the chunk queued behind it is discarded.  A step in which a
+// particle moved farther than the skin since the sort ends the chunk the
+// same way, but is then taken again as the first step of the next chunk,
+// which sorts.  With timing on, one chunk is in flight at a time and HIP
+// events bracket the chunk, its last step's kernels and its first step's
+// sort.
+namespace {
+
+struct ChunkRun {
+    uint32_t len = 0;
+    int slot = 0;
+    int p0 = 0, v0 = 0, q0 = 0;  // state buffers at its start
+};
+
+}  // namespace
+
 int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, float tol, float skin,
                  sph_learned_report* rep) {
     SPH_TRY_ST(check_model(e, m));
@@ -418,24 +438,27 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
     const size_t N = e.cfg.n;
     const double dt = (double)e.scenes[0].time_step;
     const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    const size_t depth = e.timing ? 1 : 2;
     SPH_TRY(mlp_scale_dt(e.cfg.nscenes, e.dt_dev, (float)factor, L->dt_coarse, e.stream));
     double anchor = 0.0;
     SPH_TRY_ST(plain_deviation(e, *L, &anchor));
     sph_learned_report r{};
-    uint32_t done = 0;
     double t_dens = 0.0, t_mlp = 0.0, t_chunks = 0.0, t_build = 0.0, pairs = 0.0;
     uint32_t timed_steps = 0, chunk_steps = 0, timed_builds = 0;
-    while (done < ncoarse) {
-        const uint32_t chunk = std::min(check, ncoarse - done);
-        const int p0 = e.cur_pos, v0 = e.cur_vel, q0 = e.cur_pid;
-        // with timing on, ev[0]/ev[7] bracket the whole chunk and the other
-        // events its last step (and its first step's list build)
+
+    // Snapshot, steps and result copies of one chunk, then its event.
+    auto enqueue_chunk = [&](uint32_t len, int slot, ChunkRun& c) -> int {
+        c.len = len;
+        c.slot = slot;
+        c.p0 = e.cur_pos;
+        c.v0 = e.cur_vel;
+        c.q0 = e.cur_pid;
         if (e.timing) SPH_TRY(hipEventRecord(e.ev[0], e.stream));
-        SPH_TRY(hipMemcpyAsync(L->spos, e.pos[p0], N * 16, k, e.stream));
-        SPH_TRY(hipMemcpyAsync(L->svel, e.vel[v0], N * 16, k, e.stream));
-        SPH_TRY(hipMemcpyAsync(L->spid, e.pid[q0], N * 4, k, e.stream));
-        for (uint32_t i = 0; i < chunk; ++i) {
-            const bool timed = e.timing && (i + 1 == chunk || (i == 0 && skin > 0.0f));
+        SPH_TRY(hipMemcpyAsync(L->spos + slot * N, e.pos[c.p0], N * 16, k, e.stream));
+        SPH_TRY(hipMemcpyAsync(L->svel + slot * N, e.vel[c.v0], N * 16, k, e.stream));
+        SPH_TRY(hipMemcpyAsync(L->spid + slot * N, e.pid[c.q0], N * 4, k, e.stream));
+        for (uint32_t i = 0; i < len; ++i) {
+            const bool timed = e.timing && (i + 1 == len || (i == 0 && skin > 0.0f));
             SPH_TRY_ST(enqueue_coarse(e, *L, *m, skin, i == 0, L->dev + i, L->ovf + kStepCounters * i, timed));
             if (timed && i == 0 && skin > 0.0f) {
                 // the events of the first step are read before the last step reuses them
@@ -447,21 +470,44 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
             }
         }
         if (e.timing) SPH_TRY(hipEventRecord(e.ev[7], e.stream));
-        SPH_TRY(hipMemcpyAsync(L->host, L->dev, chunk * sizeof(double), hipMemcpyDeviceToHost, e.stream));
-        SPH_TRY(hipMemcpyAsync(L->host_cnt, L->ovf, kStepCounters * chunk * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, e.stream));
-        SPH_TRY(hipStreamSynchronize(e.stream));
+        SPH_TRY(hipMemcpyAsync(L->host + slot * kMaxCheck, L->dev, len * sizeof(double), hipMemcpyDeviceToHost,
+                               e.stream));
+        SPH_TRY(hipMemcpyAsync(L->host_cnt + slot * kStepCounters * kMaxCheck, L->ovf,
+                               kStepCounters * len * sizeof(uint32_t), hipMemcpyDeviceToHost, e.stream));
+        SPH_TRY(hipEventRecord(L->done[slot], e.stream));
+        return SPH_OK;
+    };
+
+    ChunkRun pend[2];
+    size_t npend = 0;
+    int next_slot = 0;
+    uint32_t done = 0, queued = 0;
+    while (done < ncoarse) {
+        while (npend < depth && done + queued < ncoarse) {
+            const uint32_t len = std::min(check, ncoarse - done - queued);
+            SPH_TRY_ST(enqueue_chunk(len, next_slot, pend[npend]));
+            next_slot ^= 1;
+            queued += len;
+            npend += 1;
+        }
+        const ChunkRun c = pend[0];
+        pend[0] = pend[1];
+        npend -= 1;
+        queued -= c.len;
+        SPH_TRY(hipEventSynchronize(L->done[c.slot]));
+        const double* hd = L->host + c.slot * kMaxCheck;
+        const uint32_t* hc = L->host_cnt + c.slot * kStepCounters * kMaxCheck;
         if (skin > 0.0f) r.rebuilds += 1;
-        uint32_t bad = chunk;
+        uint32_t bad = c.len;
         bool moved = false;
-        for (uint32_t i = 0; i < chunk; ++i) {
-            const uint32_t* cnt = L->host_cnt + kStepCounters * i;
+        for (uint32_t i = 0; i < c.len; ++i) {
+            const uint32_t* cnt = hc + kStepCounters * i;
             if (cnt[2] != 0) {
                 bad = i;
                 moved = true;
                 break;
             }
-            const double drift = std::fabs(L->host[i] - anchor);
+            const double drift = std::fabs(hd[i] - anchor);
             if (drift > (double)tol || cnt[0] != 0) {
                 bad = i;
                 r.drift_last = drift;
@@ -471,24 +517,28 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
             r.drift_last = drift;
             pairs += cnt[1];
         }
-        if (e.timing && bad == chunk) {
-            float a = 0.0f, b = 0.0f, c = 0.0f;
+        if (e.timing && bad == c.len) {
+            float a = 0.0f, b = 0.0f, t = 0.0f;
             SPH_TRY(hipEventElapsedTime(&a, e.ev[1], e.ev[2]));
             SPH_TRY(hipEventElapsedTime(&b, e.ev[3], e.ev[4]));
-            SPH_TRY(hipEventElapsedTime(&c, e.ev[0], e.ev[7]));
+            SPH_TRY(hipEventElapsedTime(&t, e.ev[0], e.ev[7]));
             t_dens += a;
             t_mlp += b;
-            t_chunks += c;
+            t_chunks += t;
             timed_steps += 1;
-            chunk_steps += chunk;
+            chunk_steps += c.len;
         }
-        if (bad < chunk) {
-            e.cur_pos = p0;
-            e.cur_vel = v0;
-            e.cur_pid = q0;
-            SPH_TRY(hipMemcpyAsync(e.pos[p0], L->spos, N * 16, k, e.stream));
-            SPH_TRY(hipMemcpyAsync(e.vel[v0], L->svel, N * 16, k, e.stream));
-            SPH_TRY(hipMemcpyAsync(e.pid[q0], L->spid, N * 4, k, e.stream));
+        if (bad < c.len) {
+            // drop the chunk queued behind this one, undo this one
+            SPH_TRY(hipStreamSynchronize(e.stream));
+            npend = 0;
+            queued = 0;
+            e.cur_pos = c.p0;
+            e.cur_vel = c.v0;
+            e.cur_pid = c.q0;
+            SPH_TRY(hipMemcpyAsync(e.pos[c.p0], L->spos + c.slot * N, N * 16, k, e.stream));
+            SPH_TRY(hipMemcpyAsync(e.vel[c.v0], L->svel + c.slot * N, N * 16, k, e.stream));
+            SPH_TRY(hipMemcpyAsync(e.pid[c.q0], L->spid + c.slot * N, N * 4, k, e.stream));
             for (uint32_t i = 0; i < bad; ++i)
                 SPH_TRY_ST(enqueue_coarse(e, *L, *m, skin, i == 0, L->dev + i, L->ovf + kStepCounters * i, false));
             if (skin > 0.0f && bad > 0) r.rebuilds += 1;
@@ -509,8 +559,8 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
         e.steps += kept;
         e.host_clock += dt * factor * kept;
         e.stepped = true;
-        if (bad == chunk) e.last_kind = 2;
-        done += kept + (bad < chunk && !moved ? 1 : 0);
+        if (bad == c.len) e.last_kind = 2;
+        done += kept + (bad < c.len && !moved ? 1 : 0);
     }
     r.anchor = anchor;
     r.sim_time = e.host_clock;

@@ -758,8 +758,12 @@ __global__ __launch_bounds__(256) void k_density_dev_partials(uint32_t n, const 
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// With counters: counters[first .. first + ncopy) are copied to cnt_out.
 __global__ __launch_bounds__(256) void k_density_dev_final(uint32_t n, const double* __restrict__ part,
-                                                           double* __restrict__ out) {
+                                                           double* __restrict__ out,
+                                                           const uint32_t* __restrict__ counters,
+                                                           uint32_t first, uint32_t ncopy,
+                                                           uint32_t* __restrict__ cnt_out) {
     __shared__ double sh[256];
     sh[threadIdx.x] = part[threadIdx.x];
     __syncthreads();
@@ -768,6 +772,7 @@ __global__ __launch_bounds__(256) void k_density_dev_final(uint32_t n, const dou
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = sh[0] / (double)n;
+    if (counters && threadIdx.x < ncopy) cnt_out[threadIdx.x] = counters[first + threadIdx.x];
 }
 
 __global__ void k_scale_dt(uint32_t n, const float* __restrict__ src, float f, float* __restrict__ dst) {
@@ -922,12 +927,14 @@ hipError_t mlp_accumulate_original(uint32_t n, const uint32_t* pid, const float4
 }
 
 hipError_t mlp_density_deviation(uint32_t n, const float2* rhop, float inv_rho0, double* partials,
-                                 double* out, hipStream_t s) {
+                                 double* out, hipStream_t s, const uint32_t* counters, uint32_t first,
+                                 uint32_t ncopy, uint32_t* cnt_out) {
     hipLaunchKernelGGL(k_density_dev_partials, dim3(kDevBlocks), dim3(256), 0, s, n, rhop, inv_rho0,
                        partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_density_dev_final, dim3(1), dim3(256), 0, s, n, partials, out);
+    hipLaunchKernelGGL(k_density_dev_final, dim3(1), dim3(256), 0, s, n, partials, out, counters, first, ncopy,
+                       cnt_out);
     return hipGetLastError();
 }
 

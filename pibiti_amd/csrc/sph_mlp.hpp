@@ -90,8 +90,11 @@ hipError_t mlp_accumulate_original(uint32_t n, const uint32_t* pid, const float4
                                    bool first, hipStream_t s);
 
 // out[0] = mean_k |rhop[k].x * inv_rho0 - 1| in FP64 (partials: 256 doubles).
+// With counters: counters[first .. first + ncopy) (ncopy <= 256) are copied
+// to cnt_out in the same launch.
 hipError_t mlp_density_deviation(uint32_t n, const float2* rhop, float inv_rho0, double* partials,
-                                 double* out, hipStream_t s);
+                                 double* out, hipStream_t s, const uint32_t* counters = nullptr,
+                                 uint32_t first = 0, uint32_t ncopy = 0, uint32_t* cnt_out = nullptr);
 
 // dst[k] = src[k] * factor for nscenes floats (coarse time steps).
 hipError_t mlp_scale_dt(uint32_t nscenes, const float* src, float factor, float* dst, hipStream_t s);
