@@ -105,7 +105,7 @@ struct LaunchCfg {
 
 // Launch tuning words (LaunchCfg::tune, from sph_params::tune):
 //   tune[0]  force kernel: 1 = fused candidate/pair loop, 16/24/32/48 =
-//            neighbour-list entries per lane (default 16)
+//            neighbour-list entries per lane (default 24)
 //   tune[1]  gather grid: 0 = one thread per slot over all slots, else a
 //            persistent grid of tune[1] blocks pulling 256-slot tiles
 //   tune[3]  gather variant (tune[1] == 0): 0 = cell tables and candidate

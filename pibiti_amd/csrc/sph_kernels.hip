@@ -386,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_density_persistent(
 // The list is drained by the whole wave before a cell whose candidates
 // could overflow some lane's list.  The list size (entries per lane) trades
 // LDS per wave (occupancy) against drains at partial lane occupancy.
-constexpr int kForceListDefault = 16;
+constexpr int kForceListDefault = 24;
 constexpr uint32_t kSlotBits = 27;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
 
