@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define SPH_ABI_VERSION 1u
+#define SPH_ABI_VERSION 2u
 
 /* ------------------------------------------------------------------------ */
 /* Status codes.  The reference exits on any CUDA error
@@ -40,10 +40,27 @@ typedef enum sph_status {
     SPH_ERR_CAPACITY = 8          /* size above a fixed capacity               */
 } sph_status;
 
-/* Boundary types (source/CUDA/Params.cuh:24).  Only the box is on the path. */
-enum { SPH_BND_BOX = 0 };
-/* Boundary effects along z (source/CUDA/Params.cuh:28).  Only NONE.        */
-enum { SPH_BND_EFF_NONE = 0 };
+/* Boundary types: the reference's BndType values (source/CUDA/Params.cuh:23).
+ * The soft walls of each are those of boundary() (source/CUDA/System.cu:41-159). */
+enum {
+    SPH_BND_BOX = 0,
+    SPH_BND_CYL_Y = 1,   /* cylinder around y of radius world_max[0] (no x, z walls) */
+    SPH_BND_CYL_Z = 2,   /* cylinder around z of radius -world_min[1] (no x, y walls) */
+    SPH_BND_CYL_YZ = 3,  /* box plus both cylinders                              */
+    SPH_BND_SPHERE = 4,  /* sphere of radius -world_min[1] about the origin        */
+    SPH_BND_PUMP_Y = 5   /* box plus the pump frame, outlet and inlet             */
+};
+/* Boundary effects along z: the reference's BndEff values
+ * (source/CUDA/Params.cuh:27).                                               */
+enum {
+    SPH_BND_EFF_NONE = 0,
+    SPH_BND_EFF_WRAP = 1,   /* no z- wall; particles leaving at z- re-enter at z+   */
+    SPH_BND_EFF_CYCLE = 2,  /* no z walls; particles wrap both ways                */
+    SPH_BND_EFF_WAVE = 3    /* sloped floor and a z- wall moving with the phase     */
+};
+/* Accelerator shapes: the reference's AccType values (source/CUDA/Params.cuh:38). */
+enum { SPH_ACC_OFF = 0, SPH_ACC_BOX = 1, SPH_ACC_CYL_Y = 2, SPH_ACC_CYL_Y_SMOOTH = 3 };
+#define SPH_NUM_ACCEL 4
 
 /* Equation of state.  LINEAR is the reference's p = k (rho - rho0)
  * (source/CUDA/Kernel_Cell.cui:195).  TAIT is an opt-in extension:
@@ -83,6 +100,16 @@ enum {
     SPH_FLAG_KEEP_DVEL = 1u /* force kernel also stores dv for readback */
 };
 
+/* One accelerator (struct Accel, source/CUDA/Params.cuh:40-45): particles
+ * inside the shape about `pos` gain acc * dt in the step's force update
+ * (computeForceD, source/CUDA/System.cu:379-399).                          */
+typedef struct sph_accel {
+    float pos[3];
+    float size[3];
+    float acc[3];
+    uint32_t type;              /* SPH_ACC_*                                 */
+} sph_accel;
+
 /* ------------------------------------------------------------------------ */
 /* User-level scene description: the raw values that Scene holds before
  * Scene::_UpdatePar/_UpdateGrid derive the kernel constants
@@ -112,6 +139,31 @@ typedef struct sph_scene {
     uint32_t eos;               /* SPH_EOS_*                                 */
     uint32_t precision;         /* SPH_PREC_*                                */
     float cfl;                  /* 0: fixed dt; > 0: dt = min(time_step, cfl*h/(c+vmax)) */
+    /* Boundary shape and z effect (boundary(), source/CUDA/System.cu:41-159)
+     * and the values it reads (source/CUDA/Params.cuh:99-107).             */
+    uint32_t bnd_type;          /* SPH_BND_*                                 */
+    uint32_t bnd_eff_z;         /* SPH_BND_EFF_*                             */
+    float wave_slope;           /* HSlope (r2Angle): floor slope along z     */
+    float wave_ampl;            /* WaveAmpl (rTwist)                         */
+    float wave_phase;           /* rAngle at the start                       */
+    float wave_speed;           /* WaveSpeed (rVel): every step first advances
+                                   wave_phase by wave_speed * time_step
+                                   (source/App/Update.cpp:15-16)             */
+    float exit_vel;             /* ExitVel (rVexit)                          */
+    float exit_dist;            /* ExitDist (rDexit)                         */
+    float pump_ang_out;         /* PumpAngOut                                */
+    float pump_h_close;         /* PumpHClose                                */
+    float pump_rad_in;          /* PumpRadIn (factor of world_max[0])        */
+    float pump_s[6];            /* s1..s6                                    */
+    /* DEM sphere collider (collideSpheres, source/CUDA/Kernel_Cell.cui:78-96).
+     * computeForceD applies it in every scene without a rotor: it is the
+     * else branch of the rotor test (source/CUDA/System.cu:310,373-375).  */
+    uint32_t collider;          /* 1: on                                     */
+    float coll_pos[3];          /* centre; coll_pos[0] == -77 places it at the
+                                   world's x-/y- edge (source/SPH/Scene.cpp:111-112) */
+    float coll_r;               /* ColliderR                                 */
+    float coll_spring, coll_damping, coll_shear;
+    sph_accel acc[SPH_NUM_ACCEL];
 } sph_scene;
 
 /* Derived kernel constants: the plain-scalar equivalent of SimParams
@@ -136,8 +188,8 @@ typedef struct sph_params {
     float min_dist;             /* min_dist * particle_r                     */
     float dist_bnd_hard, dist_bnd_soft;
     float bnd_damp, bnd_stiff, bnd_damp_c;
-    uint32_t bnd_type;          /* SPH_BND_BOX                               */
-    uint32_t bnd_eff_z;         /* SPH_BND_EFF_NONE                          */
+    uint32_t bnd_type;          /* SPH_BND_*                                 */
+    uint32_t bnd_eff_z;         /* SPH_BND_EFF_*                             */
     uint32_t eos;
     uint32_t precision;
     float cfl;
@@ -164,8 +216,31 @@ typedef struct sph_params {
     /* Simulated end time (s): the time step is clipped so the scene's clock
      * stops at t_end, after which the scene is frozen.  0 = no end time.    */
     float t_end;
+    /* Boundary shape: the sph_scene values, unscaled. */
+    float wave_slope, wave_ampl, wave_phase, wave_speed;
+    float exit_vel, exit_dist;
+    float pump_ang_out, pump_h_close, pump_rad_in;
+    float pump_s[6];
+    /* Derived from the fields above whenever the library takes parameters
+     * (values set by the caller are replaced):
+     *   wave_zmin_shift = wave_ampl (1 + sin(wave_phase)), the moving z- wall
+     *     of the wave effect (source/CUDA/System.cu:60);
+     *   pump_xs, pump_zs: the pump outlet's half width and the height above
+     *     which its side walls act (source/CUDA/System.cu:104-117).
+     * sin and cos here and in the pump's per-particle terms are correctly
+     * rounded float results.                                               */
+    float wave_zmin_shift, pump_xs, pump_zs;
+    /* Collider (centre already placed) and accelerators, as in sph_scene. */
+    uint32_t collider;
+    float coll_pos[3], coll_r, coll_spring, coll_damping, coll_shear;
+    sph_accel acc[SPH_NUM_ACCEL];
+    /* Derived: SPH_PHYS_COLLIDER | SPH_PHYS_ACCEL for the terms in use.    */
+    uint32_t physics;
     uint32_t reserved[2];
 } sph_params;
+
+/* sph_params::physics bits. */
+enum { SPH_PHYS_COLLIDER = 1u, SPH_PHYS_ACCEL = 2u };
 
 /* Global FP64 diagnostics reduced on the device (extension). */
 typedef struct sph_stats {
@@ -211,6 +286,74 @@ int sph_params_from_scene(const sph_scene* scene, sph_params* params);
 int sph_scene_lattice(uint32_t nx, uint32_t ny, uint32_t nz, const float region_min[3],
                       float spacing, float jitter, float vel_sigma, const float vel_offset[3],
                       uint64_t seed, uint64_t stream, float* pos4, float* vel4);
+
+/* ------------------------------------------------------------------------ */
+/* Scene files (host only).  A scene file is the reference's XML scene list
+ * (Scenes.xml): a root element whose <Scene> children carry the values of
+ * Scene::_FromXML as attributes (source/SPH/Scene_Load.cpp:11-111), with
+ * <Emitter> and <Accel> children.                                          */
+
+/* One emitter (class Emitter, source/SPH/Scene.h:7-17). */
+typedef struct sph_emitter {
+    int32_t size;               /* width in particles (0 = off)              */
+    int32_t size2;              /* height (0 = square)                       */
+    float pos[3];               /* centre                                    */
+    float rot[3];               /* rotation angles about x and y (degrees)   */
+    float vel;                  /* emission speed                            */
+} sph_emitter;
+
+/* Scene values that are not simulation parameters: the init volume and fill
+ * of cSPH::Reset, the drops and emitters of the app's frame update, the
+ * camera, and the rotor / height-map terms, which this library does not
+ * build (their scenes run without them; `unsupported` says which).       */
+typedef struct sph_scene_info {
+    char title[64];             /* name attribute ("No name" without one)   */
+    uint32_t chapter;           /* starts a chapter (always for scene 0)    */
+    uint32_t is_default;        /* reserved (0)                              */
+    float init_min[3];          /* init volume after Scene::_UpdateGrid moved */
+    float init_max[3];          /* it in by (soft band - particle_r)          */
+    int32_t init_type;          /* InitType: 1 random, otherwise lattice (9: no shape test) */
+    int32_t init_last;          /* InitLast: axis advanced last, 0 x, 1 y, 2 z */
+    float spacing;              /* lattice step (m): spacing * particle_r    */
+    float drop_r;               /* dropR                                     */
+    int32_t rain;               /* rain interval (frames)                    */
+    sph_emitter emit[4];
+    float cam_pos[3];
+    float cam_rot[3];
+    int32_t rotor_type;         /* RotorType (0 = none)                      */
+    int32_t hmap_type;          /* HmapType (0 = none)                       */
+    uint32_t unsupported;       /* SPH_FEAT_* terms the scene uses that are not built */
+    uint32_t reserved[3];
+} sph_scene_info;
+
+enum { SPH_FEAT_ROTOR = 1u, SPH_FEAT_HMAP = 2u };
+
+/* cSPH::LoadScenes (source/SPH/SPH_Scenes.cpp:53-83): the number of scenes
+ * in the file text (len bytes) and the index of the startup scene (the last
+ * one with a default or def attribute, else 0).  A file whose markup breaks
+ * off keeps the scenes read before the break, as the reference's reader
+ * does.                                                                    */
+int sph_xml_scene_count(const char* text, size_t len, uint32_t* count, uint32_t* default_index);
+
+/* Scene `index` of the file: Scene::InitDefault then Scene::_FromXML
+ * (source/SPH/Scene.cpp:7-66, source/SPH/Scene_Load.cpp:11-111), as an
+ * sph_scene for sph_params_from_scene (raw factors, as the reference keeps
+ * them before _UpdatePar) and the scene's other values (info may be NULL).
+ * The collider is on unless the scene has a rotor (the reference applies
+ * it in the rotor test's else branch, source/CUDA/System.cu:310,373-375);
+ * wave_speed is the scene's rVel for the wave effect, else 0.              */
+int sph_xml_scene(const char* text, size_t len, uint32_t index, sph_scene* scene, sph_scene_info* info);
+
+/* cSPH::Reset(initType) (source/SPH/SPH_Init.cpp:23-76): scene->num_particles
+ * positions (float4, w = 1) and zero velocities.  The lattice walk starts at
+ * info->init_min (y raised by particle_r), steps by info->spacing with the
+ * init_last axis order, keeps the sites inside the boundary shape and
+ * outside the collider, and wraps around the volume until every particle
+ * has a site.  init_type 1 fills the world at random from `seed` (the
+ * reference draws rand()).  SPH_ERR_INVALID_PARAMS if no site of the
+ * volume qualifies.                                                        */
+int sph_scene_fill(const sph_scene* scene, const sph_scene_info* info, uint64_t seed, float* pos4,
+                   float* vel4);
 
 /* ------------------------------------------------------------------------ */
 /* Device context (replaces cSPH::_InitMem/_FreeMem + setParameters,

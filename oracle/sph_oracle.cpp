@@ -6,7 +6,8 @@
 //
 // It restates, in plain C++, the arithmetic of the reference's CUDA step
 // (the reference has no CPU path of its own):
-//   integrate + box boundary ... source/CUDA/System.cu:41-77 (box, no z effect), :165-205
+//   integrate + boundary ....... source/CUDA/System.cu:41-159, :165-205
+//   collider + accelerators .... source/CUDA/Kernel_Cell.cui:78-96, source/CUDA/System.cu:374-399
 //   cell hash ................... source/CUDA/Kernel_Cell.cui:5-34
 //   stable key/value sort ....... source/CUDA/radixsort_kernel.cu:445-472 (LSD, stable)
 //   cellStart + reorder ......... source/CUDA/Kernel_Cell.cui:40-67, source/CUDA/System.cu:689-713
@@ -68,21 +69,139 @@ inline void bnd_wall(T diff, T nx, T ny, T nz, T stiff, T damp, T dt, T v[3]) {
     }
 }
 
-// integrateD + boundary() for one particle (source/CUDA/System.cu:63-77,165-205).
+// sin, cos and atan as the nearest T of the double result (the CUDA float
+// functions the reference calls are not reproduced; both sides of the parity
+// tests use correctly rounded values instead).
 template <typename T>
-inline void integrate_one(const sph_params& p, T x[3], T v[3]) {
+inline T sin_t(T a) { return (T)std::sin((double)a); }
+template <typename T>
+inline T cos_t(T a) { return (T)std::cos((double)a); }
+template <typename T>
+inline T atan_t(T a) { return (T)std::atan((double)a); }
+
+// boundary() (source/CUDA/System.cu:41-159) for one particle: the soft walls
+// of the boundary type and z effect in the reference's order, the wrap/cycle
+// position shifts and the pump.  addB walls damp with bndDamp, addC walls
+// (z walls of the box, cylinders, sphere) with bndDampC.
+template <typename T>
+inline void boundary(const sph_params& p, T x[3], T v[3]) {
     const T b = (T)p.dist_bnd_soft, stiff = (T)p.bnd_stiff, damp = (T)p.bnd_damp,
             damp2 = (T)p.bnd_damp_c, dt = (T)p.time_step;
+    T wmin[3] = {(T)p.world_min[0], (T)p.world_min[1], (T)p.world_min[2]};
+    const T wmax[3] = {(T)p.world_max[0], (T)p.world_max[1], (T)p.world_max[2]};
+    const T o = 0, one = 1;
+    const uint32_t type = p.bnd_type, eff = p.bnd_eff_z;
+    const bool cyl_y = type == SPH_BND_CYL_Y, cyl_z = type == SPH_BND_CYL_Z;
+    const bool cyl_yz = type == SPH_BND_CYL_YZ, sphere = type == SPH_BND_SPHERE;
+    const bool wave = eff == SPH_BND_EFF_WAVE, none = eff == SPH_BND_EFF_NONE,
+               cycle = eff == SPH_BND_EFF_CYCLE;
+    if (wave) {  // :55-61
+        const T sl = -(T)p.wave_slope;
+        bnd_wall<T>((b - (x[1] - wmin[1])) - (x[2] - wmin[2]) * sl, o, one - sl, sl, stiff, damp, dt, v);
+        wmin[2] = wmin[2] + (T)p.wave_ampl * (one + sin_t<T>((T)p.wave_phase));
+    }
+    if (!sphere) {  // :64-77
+        if (!cyl_y) {
+            if (none || wave) bnd_wall<T>((b - x[2]) + wmin[2], o, o, one, stiff, damp2, dt, v);
+            if (!cycle) bnd_wall<T>((b + x[2]) - wmax[2], o, o, -one, stiff, damp2, dt, v);
+        }
+        if (!cyl_y && !cyl_z) {
+            bnd_wall<T>((b - x[0]) + wmin[0], one, o, o, stiff, damp, dt, v);
+            bnd_wall<T>((b + x[0]) - wmax[0], -one, o, o, stiff, damp, dt, v);
+        }
+        if (!cyl_z) {
+            bnd_wall<T>((b - x[1]) + wmin[1], o, one, o, stiff, damp, dt, v);
+            bnd_wall<T>((b + x[1]) - wmax[1], o, -one, o, stiff, damp, dt, v);
+        }
+    } else {  // :78-81
+        const T len = std::sqrt((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]);
+        const T diff = (b + len) + wmin[1];
+        if (diff > (T)kBndEps) bnd_wall<T>(diff, -x[0] / len, -x[1] / len, -x[2] / len, stiff, damp2, dt, v);
+    }
+    if (cyl_y || cyl_yz) {  // :84-86
+        const T len = std::sqrt(x[0] * x[0] + x[2] * x[2]);
+        const T diff = (b + len) - wmax[0];
+        if (diff > (T)kBndEps) bnd_wall<T>(diff, -x[0] / len, o, -x[2] / len, stiff, damp2, dt, v);
+    }
+    if (cyl_z || cyl_yz) {  // :89-91
+        const T len = std::sqrt(x[0] * x[0] + x[1] * x[1]);
+        const T diff = (b + len) + wmin[1];
+        if (diff > (T)kBndEps) bnd_wall<T>(diff, -x[0] / len, -x[1] / len, o, stiff, damp2, dt, v);
+    }
+    if (!wave && !none) {  // :94-97
+        const T dr = (T)1 * (T)p.particle_r;
+        const T ev = (T)p.exit_vel;
+        if (cycle && v[2] > ev && x[2] > (wmax[2] - b) - dr)
+            x[2] = x[2] - (((wmax[2] - wmin[2]) - (T)2 * b) - dr);
+        else if (v[2] < -ev && x[2] < (wmin[2] + b) + dr)
+            x[2] = x[2] + (((wmax[2] - wmin[2]) - (T)2 * b) - dr);
+    }
+    if (type == SPH_BND_PUMP_Y) {  // :101-158
+        const T rad = wmax[0], ang = (T)p.pump_ang_out, hc = (T)p.pump_h_close;
+        const T rin = rad * (T)p.pump_rad_in;
+        T sv[6];
+        for (int k = 0; k < 6; ++k) sv[k] = (T)p.pump_s[k];
+        const T len = std::sqrt(x[0] * x[0] + x[1] * x[1]);
+        T diff = (b + len) - rad;
+        if (diff > (T)kBndEps) {
+            const T a = atan_t<T>(x[0] / x[1]);
+            bool wall;
+            if (ang < (T)0.5f)
+                wall = a < -ang || a > ang || x[1] < o;
+            else
+                wall = x[1] < o || (len < rad * sv[4] && a < ang);
+            if (wall) bnd_wall<T>(diff, -x[0] / len, -x[1] / len, o, stiff, damp, dt, v);
+        }
+        T xs;
+        if (ang < (T)0.5f) {
+            xs = sin_t<T>(ang * sv[2]) * rad;
+            const T zs = (cos_t<T>(ang * sv[2]) * rad) * sv[3];
+            if (x[1] > zs) {
+                bnd_wall<T>((b - x[0]) - xs, one, o, o, stiff, damp, dt, v);
+                bnd_wall<T>((b + x[0]) - xs, -one, o, o, stiff, damp, dt, v);
+            }
+        } else {
+            xs = (T)0.09f * sv[3];
+            if (len >= rad * sv[5]) bnd_wall<T>((b - x[0]) + xs, one, o, o, stiff, damp, dt, v);
+        }
+        if (x[2] > hc - b * sv[0]) {
+            diff = (b + len) - rin;
+            if (diff > (T)kBndEps) bnd_wall<T>(diff, -x[0] / len, -x[1] / len, o, stiff, damp, dt, v);
+        }
+        if (x[2] < hc - b * sv[1]) bnd_wall<T>((b + x[2]) - hc, o, o, -one, stiff, damp, dt, v);
+        diff = (x[1] - wmax[1]) + (T)p.exit_dist;
+        if (diff > (T)kBndEps && v[1] > (T)p.exit_vel) {
+            T rr, aa;
+            const T zz = std::fabs(hc - wmin[2]);
+            if (ang < (T)0.5f) {
+                const T xx = xs * (T)2;
+                rr = ((x[0] + xx / (T)2) / xx) * (T)0.7f;
+                aa = ((x[2] - zz / (T)2) / zz) * (T)1.6f;
+            } else {
+                rr = ((wmax[0] - x[0]) / xs) * (T)0.45f;
+                aa = ((x[2] - zz / (T)2) / zz) * (T)1.8f;
+            }
+            rr = rr * rin;
+            aa = aa * ((T)2.0f * (T)3.141592654f);
+            const T nx = cos_t<T>(aa) * rr, ny = sin_t<T>(aa) * rr;
+            const T nz = (T)((double)(wmax[2] - b) - (double)std::fabs(v[1] - (T)p.exit_vel) * 0.01);
+            x[0] = nx;
+            x[1] = ny;
+            x[2] = nz;
+            const T vy = v[1];
+            v[1] = v[2];
+            v[2] = -vy;
+        }
+    }
+}
+
+// integrateD + boundary() for one particle (source/CUDA/System.cu:41-159,165-205).
+template <typename T>
+inline void integrate_one(const sph_params& p, T x[3], T v[3]) {
+    const T dt = (T)p.time_step;
     const T wmin[3] = {(T)p.world_min[0], (T)p.world_min[1], (T)p.world_min[2]};
     const T wmax[3] = {(T)p.world_max[0], (T)p.world_max[1], (T)p.world_max[2]};
-    const T z = 0, one = 1;
-    // z walls use bndDampC, x and y walls bndDamp, in this order.
-    bnd_wall<T>((b - x[2]) + wmin[2], z, z, one, stiff, damp2, dt, v);
-    bnd_wall<T>((b + x[2]) - wmax[2], z, z, -one, stiff, damp2, dt, v);
-    bnd_wall<T>((b - x[0]) + wmin[0], one, z, z, stiff, damp, dt, v);
-    bnd_wall<T>((b + x[0]) - wmax[0], -one, z, z, stiff, damp, dt, v);
-    bnd_wall<T>((b - x[1]) + wmin[1], z, one, z, stiff, damp, dt, v);
-    bnd_wall<T>((b + x[1]) - wmax[1], z, -one, z, stiff, damp, dt, v);
+    boundary<T>(p, x, v);
     for (int c = 0; c < 3; ++c) v[c] = v[c] + (T)p.gravity[c] * dt;
     for (int c = 0; c < 3; ++c) v[c] = v[c] * (T)p.global_damping;
     for (int c = 0; c < 3; ++c) x[c] = x[c] + v[c] * dt;
@@ -224,6 +343,61 @@ inline T eos(const sph_params& p, T rho) {
 }
 
 // ------------------------------------------------------------------ force
+// What computeForceD adds to addVel after the pair sum
+// (source/CUDA/System.cu:374-399): the sphere collider's DEM force
+// (collideSpheres, source/CUDA/Kernel_Cell.cui:78-96; float3/float there is
+// a multiplication by the reciprocal, source/external/cutil_math.h:358-362)
+// and then each accelerator whose shape holds the particle.
+template <typename T>
+inline void extra_terms(const sph_params& p, const T xi[3], const T vi[3], T out[3]) {
+    const T dt = (T)p.time_step;
+    if (p.collider) {
+        T rel[3], u[3], nrm[3];
+        for (int k = 0; k < 3; ++k) rel[k] = (T)p.coll_pos[k] - xi[k];
+        const T dist = std::sqrt((rel[0] * rel[0] + rel[1] * rel[1]) + rel[2] * rel[2]);
+        const T rad = (T)p.particle_r + (T)p.coll_r;
+        if (dist < rad) {
+            for (int k = 0; k < 3; ++k) u[k] = -vi[k];
+            const T inv = (T)1 / dist;
+            for (int k = 0; k < 3; ++k) nrm[k] = rel[k] * inv;
+            const T d = (u[0] * nrm[0] + u[1] * nrm[1]) + u[2] * nrm[2];
+            const T spring = (T)p.coll_spring * (dist - rad);
+            for (int k = 0; k < 3; ++k) {
+                const T tan = u[k] - d * nrm[k];
+                T f = spring * nrm[k];
+                f = f + (T)p.coll_damping * u[k];
+                f = f + (T)p.coll_shear * tan;
+                out[k] = out[k] + f;
+            }
+        }
+    }
+    for (int a = 0; a < SPH_NUM_ACCEL; ++a) {
+        const sph_accel& ac = p.acc[a];
+        if (ac.type == SPH_ACC_OFF) continue;
+        T rel[3], size[3], acc[3];
+        for (int k = 0; k < 3; ++k) {
+            rel[k] = xi[k] - (T)ac.pos[k];
+            size[k] = (T)ac.size[k];
+            acc[k] = (T)ac.acc[k];
+        }
+        if (ac.type == SPH_ACC_BOX) {
+            if (std::fabs(rel[0]) < size[0] && std::fabs(rel[1]) < size[1] && std::fabs(rel[2]) < size[2])
+                for (int k = 0; k < 3; ++k) out[k] = out[k] + acc[k] * dt;
+        } else if (ac.type == SPH_ACC_CYL_Y) {
+            const T qx = rel[0] / size[0], qz = rel[2] / size[2];
+            const T r = std::sqrt(qx * qx + qz * qz);
+            if (std::fabs(rel[1]) < size[1] && r < (T)1)
+                for (int k = 0; k < 3; ++k) out[k] = out[k] + acc[k] * dt;
+        } else {
+            const T r = std::sqrt(rel[0] * rel[0] + rel[2] * rel[2]);
+            if (std::fabs(rel[1]) < size[1] && r < size[0]) {
+                const T g = (T)1 - r / size[2];
+                for (int k = 0; k < 3; ++k) out[k] = out[k] + (acc[k] * g) * dt;
+            }
+        }
+    }
+}
+
 // computeForceD/compForceCell/compForcePair (source/CUDA/Kernel_Cell.cui:210-261,
 // source/CUDA/System.cu:228-250) for sorted slot i.  Writes the velocity
 // increment addVel = F*m*dt.  For double also returns the absolute-value
@@ -297,6 +471,7 @@ void force_sum(const sph_params& p, uint32_t i, const std::vector<T>& spos,
             }
     const T mdt = (T)p.particle_mass * (T)p.time_step;
     for (int k = 0; k < 3; ++k) out[k] = (kKahan ? ktot[k].s : tot[k].s) * mdt;
+    extra_terms<T>(p, xi, vi, out);
     if (scale) *scale = sc * std::fabs(mdt);
 }
 
@@ -425,6 +600,14 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
     if (o.cnt) *o.cnt = cnt;
 }
 
+// App::UpdateEmitter runs before every cSPH::Update (source/App/Render.cpp:10-11)
+// and advances the wave phase: rAngle += rVel * timeStep (source/App/Update.cpp:15-16).
+inline sph_params advanced(const sph_params& p) {
+    sph_params q = p;
+    if (q.wave_speed != 0.0f) q.wave_phase = q.wave_phase + q.wave_speed * q.time_step;
+    return q;
+}
+
 }  // namespace
 
 extern "C" {
@@ -446,6 +629,11 @@ void oracle_set_num_threads(int n) {
 #endif
 }
 
+// Every step entry point below first advances wave_phase by
+// wave_speed * time_step (no-op for a still scene), as each library step
+// does; oracle_run_f32 carries the phase over its steps.  The stage
+// functions further down use the parameters as given.
+//
 // One FP32 step.  pos4/vel4 are updated in place (original order); every
 // other output pointer may be NULL.  prec is SPH_PREC_REFERENCE or
 // SPH_PREC_KAHAN.  counters receives {cap_cells, out_of_grid, max_cell_count, 0}.
@@ -455,7 +643,7 @@ int oracle_step_f32(const sph_params* p, int prec, uint32_t n, float* pos4, floa
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
     StepOut<float> o{pos4, vel4, nullptr, dens, pres, dvel4, nullptr, nullptr, nullptr, sorted_hash,
                      perm, cell_start, cell_end, reinterpret_cast<Counters*>(counters)};
-    step_impl<float>(*p, prec, n, pos4, vel4, false, o);
+    step_impl<float>(advanced(*p), prec, n, pos4, vel4, false, o);
     return 0;
 }
 
@@ -468,7 +656,7 @@ int oracle_step_f32_ex(const sph_params* p, int prec, uint32_t n, float* pos4, f
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
     StepOut<float> o{pos4, vel4, vel_int4, dens, pres, dvel4, nullptr, nullptr, nullptr,
                      sorted_hash, perm, cell_start, cell_end, reinterpret_cast<Counters*>(counters)};
-    step_impl<float>(*p, prec, n, pos4, vel4, false, o);
+    step_impl<float>(advanced(*p), prec, n, pos4, vel4, false, o);
     return 0;
 }
 
@@ -476,10 +664,12 @@ int oracle_step_f32_ex(const sph_params* p, int prec, uint32_t n, float* pos4, f
 int oracle_run_f32(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, float* pos4,
                    float* vel4, float* dens, float* pres) {
     if (!p || !pos4 || !vel4 || n != p->num_particles) return 1;
+    sph_params q = *p;
     for (uint32_t s = 0; s < nsteps; ++s) {
         StepOut<float> o{pos4, vel4, nullptr, dens, pres, nullptr, nullptr, nullptr, nullptr,
                          nullptr, nullptr, nullptr, nullptr, nullptr};
-        step_impl<float>(*p, prec, n, pos4, vel4, false, o);
+        q = advanced(q);
+        step_impl<float>(q, prec, n, pos4, vel4, false, o);
     }
     return 0;
 }
@@ -500,7 +690,7 @@ int oracle_step_f64(const sph_params* p, uint32_t n, const float* pos4_in, const
     std::vector<double> pin(pos4_in, pos4_in + 4 * (size_t)n), vin(vel4_in, vel4_in + 4 * (size_t)n);
     StepOut<double> o{pos4, vel4, nullptr, dens, pres, dvel4, dvel_scale, dens_scale, pres_scale,
                       nullptr, nullptr, nullptr, nullptr, nullptr};
-    step_impl<double>(*p, SPH_PREC_REFERENCE, n, pin.data(), vin.data(), skip_integrate != 0, o);
+    step_impl<double>(advanced(*p), SPH_PREC_REFERENCE, n, pin.data(), vin.data(), skip_integrate != 0, o);
     return 0;
 }
 

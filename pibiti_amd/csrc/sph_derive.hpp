@@ -1,0 +1,19 @@
+// sph_derive.hpp - host-side derivations of sph_params shared by the scene
+// helpers and the device contexts.
+#pragma once
+
+#include "../../include/pibiti_sph.h"
+
+namespace sph {
+
+// Fill the derived boundary fields (wave_zmin_shift, pump_xs, pump_zs) and
+// the physics bits from the raw fields.
+void derive_shape(sph_params& p);
+
+// SPH_ERR_INVALID_PARAMS for an unknown boundary type, effect or accelerator.
+int check_shape(const sph_params& p);
+
+// The per-step phase advance of a moving wave (no-op when wave_speed == 0).
+void advance_wave(sph_params& p);
+
+}  // namespace sph

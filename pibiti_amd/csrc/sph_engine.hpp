@@ -71,9 +71,9 @@ inline int validate(const sph_params* p) {
     if (!p) return SPH_ERR_INVALID_ARGUMENT;
     if (p->num_particles == 0 || p->num_particles > kMaxParticles || p->num_cells == 0 ||
         p->max_par_in_cell == 0 || !(p->h > 0.0f) || !(p->time_step > 0.0f) ||
-        p->bnd_type != SPH_BND_BOX || p->bnd_eff_z != SPH_BND_EFF_NONE || p->eos > SPH_EOS_TAIT ||
-        p->precision > SPH_PREC_KAHAN || !(p->t_end >= 0.0f))
+        p->eos > SPH_EOS_TAIT || p->precision > SPH_PREC_KAHAN || !(p->t_end >= 0.0f))
         return SPH_ERR_INVALID_PARAMS;
+    if (check_shape(*p) != SPH_OK) return SPH_ERR_INVALID_PARAMS;
     if ((uint64_t)p->grid_size[0] * p->grid_size[1] * p->grid_size[2] != p->num_cells ||
         p->grid_size_yx != p->grid_size[0] * p->grid_size[1])
         return SPH_ERR_INVALID_PARAMS;
@@ -202,6 +202,11 @@ struct Engine {
         flags = fl;
         batch = as_batch;
         scenes.assign(in, in + nscenes);
+        for (auto& sc : scenes) {
+            // a moving wave needs new parameters every step: single scenes only
+            if (as_batch && sc.wave_speed != 0.0f) return SPH_ERR_UNSUPPORTED;
+            derive_shape(sc);
+        }
         scene_off.assign(nscenes + 1, 0);
         uint64_t n = 0, cells = 0;
         uint32_t max_cap = 0, max_n = 0;
@@ -350,6 +355,12 @@ struct Engine {
         DeviceGuard guard(device);
         for (uint32_t s = 0; s < nsteps; ++s) {
             const bool tm = timing && s + 1 == nsteps;
+            if (!batch && scenes[0].wave_speed != 0.0f) {
+                // the moving wave's phase advances before every step; the
+                // kernels take the single scene's parameters by value
+                advance_wave(scenes[0]);
+                cfg.one = scenes[0];
+            }
             SPH_TRY_ST(enqueue_step(tm));
             after_steps(1);
             if (tm) {
@@ -366,6 +377,8 @@ struct Engine {
     int capture(uint32_t nsteps) {
         if (nsteps == 0 || (nsteps & 1u)) return SPH_ERR_INVALID_ARGUMENT;
         if (!ready()) return SPH_ERR_NOT_INITIALISED;
+        // a recording replays fixed parameters: no moving wave
+        if (scenes[0].wave_speed != 0.0f) return SPH_ERR_UNSUPPORTED;
         DeviceGuard guard(device);
         drop_graph();
         SPH_TRY(hipStreamSynchronize(stream));
@@ -547,6 +560,7 @@ struct Engine {
             return SPH_ERR_INVALID_PARAMS;
         if (!batch && ((1ull << p->key_bits) < cfg.num_cells || p->key_bits > 31))
             return SPH_ERR_INVALID_PARAMS;
+        if (batch && p->wave_speed != 0.0f) return SPH_ERR_UNSUPPORTED;
         DeviceGuard guard(device);
         if ((p->cfl > 0.0f || p->t_end > 0.0f) && !device_clock) {
             // from here on the clock and time step advance on the device
@@ -556,6 +570,7 @@ struct Engine {
         }
         const uint32_t base = cur.cell_base, kb = cur.key_bits;
         cur = *p;
+        derive_shape(cur);
         cur.cell_base = base;
         if (batch) cur.key_bits = kb;
         if (!batch) cfg.key_bits = cur.key_bits;

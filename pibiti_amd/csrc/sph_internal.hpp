@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "../../include/pibiti_sph.h"
+#include "sph_derive.hpp"
 
 namespace sph {
 

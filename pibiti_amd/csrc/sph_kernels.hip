@@ -19,7 +19,7 @@
 // contraction: the library is built with -ffp-contract=off and correctly
 // rounded fp32 division/sqrt), so one thread per particle reproduces the
 // reference's FP32 result bit for bit:
-//   integrateD + boundary (box) .... source/CUDA/System.cu:41-77,165-205
+//   integrateD + boundary .......... source/CUDA/System.cu:41-159,165-205
 //   calcGridPos/calcGridHash ....... source/CUDA/Kernel_Cell.cui:5-34
 //   reorderD ........................ source/CUDA/Kernel_Cell.cui:40-67
 //   computeDensityD ................. source/CUDA/Kernel_Cell.cui:142-199
@@ -48,6 +48,183 @@ __device__ __forceinline__ void bnd_wall(float diff, float nx, float ny, float n
     }
 }
 
+// Correctly rounded float atan, sin and cos for the pump's per-particle
+// terms: one rounding of the double result, so the CPU oracle (libm) agrees.
+__device__ __forceinline__ float atan_rn(float x) { return (float)atan((double)x); }
+__device__ __forceinline__ float sin_rn(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float cos_rn(float x) { return (float)cos((double)x); }
+
+// boundary() (source/CUDA/System.cu:41-159): the soft walls of the scene's
+// boundary type and z effect, in the reference's order, plus the position
+// wraps of the wrap/cycle effects and the pump outlet.  The shape branches
+// are uniform over a scene.
+__device__ __forceinline__ void boundary(const sph_params& p, float dt, float& x, float& y, float& z,
+                                         float& vx, float& vy, float& vz) {
+    const float b = p.dist_bnd_soft, stiff = p.bnd_stiff, damp = p.bnd_damp, damp2 = p.bnd_damp_c;
+    const float wmin_x = p.world_min[0], wmin_y = p.world_min[1];
+    float wmin_z = p.world_min[2];
+    const float wmax_x = p.world_max[0], wmax_y = p.world_max[1], wmax_z = p.world_max[2];
+    const uint32_t t = p.bnd_type, eff = p.bnd_eff_z;
+    const bool cyl_y = t == SPH_BND_CYL_Y, cyl_z = t == SPH_BND_CYL_Z;
+    const bool wave = eff == SPH_BND_EFF_WAVE, no_eff = eff == SPH_BND_EFF_NONE,
+               cycle = eff == SPH_BND_EFF_CYCLE;
+    if (wave) {
+        // sloped floor, then the moving z- wall
+        const float sl = -p.wave_slope;
+        bnd_wall((b - (y - wmin_y)) - (z - wmin_z) * sl, 0.0f, 1.0f - sl, sl, stiff, damp, dt, vx, vy, vz);
+        wmin_z = wmin_z + p.wave_zmin_shift;
+    }
+    if (t != SPH_BND_SPHERE) {
+        if (!cyl_y) {
+            if (no_eff || wave) bnd_wall((b - z) + wmin_z, 0.0f, 0.0f, 1.0f, stiff, damp2, dt, vx, vy, vz);
+            if (!cycle) bnd_wall((b + z) - wmax_z, 0.0f, 0.0f, -1.0f, stiff, damp2, dt, vx, vy, vz);
+        }
+        if (!cyl_y && !cyl_z) {
+            bnd_wall((b - x) + wmin_x, 1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+            bnd_wall((b + x) - wmax_x, -1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+        }
+        if (!cyl_z) {
+            bnd_wall((b - y) + wmin_y, 0.0f, 1.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+            bnd_wall((b + y) - wmax_y, 0.0f, -1.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+        }
+    } else {
+        const float len = sqrt_ieee((x * x + y * y) + z * z);
+        const float diff = (b + len) + wmin_y;
+        if (diff > kBndEps) bnd_wall(diff, -x / len, -y / len, -z / len, stiff, damp2, dt, vx, vy, vz);
+    }
+    if (cyl_y || t == SPH_BND_CYL_YZ) {
+        const float len = sqrt_ieee(x * x + z * z);
+        const float diff = (b + len) - wmax_x;
+        if (diff > kBndEps) bnd_wall(diff, -x / len, 0.0f, -z / len, stiff, damp2, dt, vx, vy, vz);
+    }
+    if (cyl_z || t == SPH_BND_CYL_YZ) {
+        const float len = sqrt_ieee(x * x + y * y);
+        const float diff = (b + len) + wmin_y;
+        if (diff > kBndEps) bnd_wall(diff, -x / len, -y / len, 0.0f, stiff, damp2, dt, vx, vy, vz);
+    }
+    if (!wave && !no_eff) {
+        // wrap / cycle along z
+        const float dr = 1.0f * p.particle_r;
+        const float span = ((wmax_z - wmin_z) - 2.0f * b) - dr;
+        if (cycle && vz > p.exit_vel && z > (wmax_z - b) - dr)
+            z = z - span;
+        else if (vz < -p.exit_vel && z < (wmin_z + b) + dr)
+            z = z + span;
+    }
+    if (t == SPH_BND_PUMP_Y) {
+        const float rad = wmax_x, ang = p.pump_ang_out, hc = p.pump_h_close;
+        const float rin = rad * p.pump_rad_in;
+        const float len = sqrt_ieee(x * x + y * y);
+        float diff = (b + len) - rad;
+        if (diff > kBndEps) {
+            // frame: the cylinder wall except over the outlet
+            const float a = atan_rn(x / y);
+            const bool wall = ang < 0.5f ? (a < -ang || a > ang || y < 0.0f)
+                                         : (y < 0.0f || (len < rad * p.pump_s[4] && a < ang));
+            if (wall) bnd_wall(diff, -x / len, -y / len, 0.0f, stiff, damp, dt, vx, vy, vz);
+        }
+        const float xs = p.pump_xs;
+        if (ang < 0.5f) {
+            if (y > p.pump_zs) {
+                bnd_wall((b - x) - xs, 1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+                bnd_wall((b + x) - xs, -1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+            }
+        } else if (len >= rad * p.pump_s[5]) {
+            bnd_wall((b - x) + xs, 1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+        }
+        // inlet hole (closed above) and the lid
+        if (z > hc - b * p.pump_s[0]) {
+            diff = (b + len) - rin;
+            if (diff > kBndEps) bnd_wall(diff, -x / len, -y / len, 0.0f, stiff, damp, dt, vx, vy, vz);
+        }
+        if (z < hc - b * p.pump_s[1]) bnd_wall((b + z) - hc, 0.0f, 0.0f, -1.0f, stiff, damp, dt, vx, vy, vz);
+        // outlet -> inlet: map the outlet rectangle onto the inlet circle
+        diff = (y - wmax_y) + p.exit_dist;
+        if (diff > kBndEps && vy > p.exit_vel) {
+            const float zz = fabsf(hc - wmin_z);
+            float rr, aa;
+            if (ang < 0.5f) {
+                const float xx = xs * 2.0f;
+                rr = ((x + xx / 2.0f) / xx) * 0.7f;
+                aa = ((z - zz / 2.0f) / zz) * 1.6f;
+            } else {
+                rr = ((wmax_x - x) / xs) * 0.45f;
+                aa = ((z - zz / 2.0f) / zz) * 1.8f;
+            }
+            rr = rr * rin;
+            aa = aa * (2.0f * 3.141592654f);
+            const float nx = cos_rn(aa) * rr, ny = sin_rn(aa) * rr;
+            // the 0.01 of the reference is a double literal: this term is double
+            const float nz = (float)((double)(wmax_z - b) - (double)fabsf(vy - p.exit_vel) * 0.01);
+            x = nx;
+            y = ny;
+            z = nz;
+            const float nvy = vz, nvz = -vy;
+            vy = nvy;
+            vz = nvz;
+        }
+    }
+}
+
+// Terms computeForceD adds after the pair sum (source/CUDA/System.cu:374-399):
+// the DEM sphere collider (collideSpheres, source/CUDA/Kernel_Cell.cui:78-96)
+// and the accelerators, in that order, to the velocity increment a.
+__device__ __forceinline__ void extra_dv(const sph_params& p, float dt, const float4& xi,
+                                         const float4& vi, float& ax, float& ay, float& az) {
+    if (p.physics & SPH_PHYS_COLLIDER) {
+        const float rx = p.coll_pos[0] - xi.x, ry = p.coll_pos[1] - xi.y, rz = p.coll_pos[2] - xi.z;
+        const float dist = sqrt_ieee((rx * rx + ry * ry) + rz * rz);
+        const float rad = p.particle_r + p.coll_r;
+        if (dist < rad) {
+            const float ux = -vi.x, uy = -vi.y, uz = -vi.z;  // relative velocity
+            const float inv = 1.0f / dist;                    // float3 / float
+            const float nx = rx * inv, ny = ry * inv, nz = rz * inv;
+            const float d = (ux * nx + uy * ny) + uz * nz;
+            const float tx = ux - d * nx, ty = uy - d * ny, tz = uz - d * nz;
+            const float sp = p.coll_spring * (dist - rad);
+            float fx = sp * nx, fy = sp * ny, fz = sp * nz;
+            fx = fx + p.coll_damping * ux;
+            fy = fy + p.coll_damping * uy;
+            fz = fz + p.coll_damping * uz;
+            fx = fx + p.coll_shear * tx;
+            fy = fy + p.coll_shear * ty;
+            fz = fz + p.coll_shear * tz;
+            ax = ax + fx;
+            ay = ay + fy;
+            az = az + fz;
+        }
+    }
+    if (p.physics & SPH_PHYS_ACCEL) {
+        for (int k = 0; k < SPH_NUM_ACCEL; ++k) {
+            const sph_accel& c = p.acc[k];
+            if (c.type == SPH_ACC_OFF) continue;
+            const float rx = xi.x - c.pos[0], ry = xi.y - c.pos[1], rz = xi.z - c.pos[2];
+            float g = 0.0f;
+            bool in = false;
+            if (c.type == SPH_ACC_BOX) {
+                in = fabsf(rx) < c.size[0] && fabsf(ry) < c.size[1] && fabsf(rz) < c.size[2];
+            } else if (c.type == SPH_ACC_CYL_Y) {
+                const float qx = rx / c.size[0], qz = rz / c.size[2];
+                in = fabsf(ry) < c.size[1] && sqrt_ieee(qx * qx + qz * qz) < 1.0f;
+            } else {
+                const float r = sqrt_ieee(rx * rx + rz * rz);
+                in = fabsf(ry) < c.size[1] && r < c.size[0];
+                g = 1.0f - r / c.size[2];
+            }
+            if (!in) continue;
+            if (c.type == SPH_ACC_CYL_Y_SMOOTH) {
+                ax = ax + (c.acc[0] * g) * dt;
+                ay = ay + (c.acc[1] * g) * dt;
+                az = az + (c.acc[2] * g) * dt;
+            } else {
+                ax = ax + c.acc[0] * dt;
+                ay = ay + c.acc[1] * dt;
+                az = az + c.acc[2] * dt;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------ integrate+hash
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_integrate_hash(
@@ -69,13 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     float4 v4 = vel[tid];
     float x = x4.x, y = x4.y, z = x4.z;
     float vx = v4.x, vy = v4.y, vz = v4.z;
-    const float b = p.dist_bnd_soft, stiff = p.bnd_stiff, damp = p.bnd_damp, damp2 = p.bnd_damp_c;
-    bnd_wall((b - z) + p.world_min[2], 0.0f, 0.0f, 1.0f, stiff, damp2, dt, vx, vy, vz);
-    bnd_wall((b + z) - p.world_max[2], 0.0f, 0.0f, -1.0f, stiff, damp2, dt, vx, vy, vz);
-    bnd_wall((b - x) + p.world_min[0], 1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
-    bnd_wall((b + x) - p.world_max[0], -1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
-    bnd_wall((b - y) + p.world_min[1], 0.0f, 1.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
-    bnd_wall((b + y) - p.world_max[1], 0.0f, -1.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
+    boundary(p, dt, x, y, z, vx, vy, vz);
     vx = vx + p.gravity[0] * dt;
     vy = vy + p.gravity[1] * dt;
     vz = vz + p.gravity[2] * dt;
@@ -524,10 +695,12 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
         ay = a.ay + a.fy;
         az = a.az + a.fz;
     }
-    const float mdt = p.particle_mass * dt_dev[scene];
+    const float dt = dt_dev[scene];
+    const float mdt = p.particle_mass * dt;
     ax = ax * mdt;
     ay = ay * mdt;
     az = az * mdt;
+    if (p.physics) extra_dv(p, dt, xi, vi, ax, ay, az);
     vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
     if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
@@ -805,10 +978,12 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
         ay = ky.s;
         az = kz.s;
     }
-    const float mdt = p.particle_mass * dt_dev[scene];
+    const float dt = dt_dev[scene];
+    const float mdt = p.particle_mass * dt;
     ax = ax * mdt;
     ay = ay * mdt;
     az = az * mdt;
+    if (p.physics) extra_dv(p, dt, xi, vi, ax, ay, az);
     vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
     if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }

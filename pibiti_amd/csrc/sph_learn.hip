@@ -180,6 +180,9 @@ int check_model(const Engine& e, const sph_model* m) {
 int learn_begin(Engine& e, const sph_model* m) {
     SPH_TRY_ST(check_model(e, m));
     if (!(e.flags & SPH_FLAG_KEEP_DVEL)) return SPH_ERR_UNSUPPORTED;
+    // the targets are pair-force increments: no collider or accelerator terms
+    for (const auto& sc : e.scenes)
+        if (sc.physics != 0) return SPH_ERR_UNSUPPORTED;
     if (e.last_kind != 1) return SPH_ERR_NOT_INITIALISED;
     LearnState* L = learn_state(e);
     if (!L) return SPH_ERR_OUT_OF_MEMORY;
@@ -426,7 +429,10 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
                  sph_learned_report* rep) {
     SPH_TRY_ST(check_model(e, m));
     if (factor == 0 || !(tol > 0.0f) || !(skin >= 0.0f) || !(skin <= 1.0f)) return SPH_ERR_INVALID_ARGUMENT;
-    if (e.batch || e.device_clock) return SPH_ERR_UNSUPPORTED;
+    // the surrogate replaces the pair force only: no collider, accelerators
+    // or moving wave
+    if (e.batch || e.device_clock || e.scenes[0].physics != 0 || e.scenes[0].wave_speed != 0.0f)
+        return SPH_ERR_UNSUPPORTED;
     if (!e.ready() || e.last_kind == 0) return SPH_ERR_NOT_INITIALISED;
     LearnState* L = learn_state(e);
     if (!L) return SPH_ERR_OUT_OF_MEMORY;

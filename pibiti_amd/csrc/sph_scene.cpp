@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "../../include/pibiti_sph.h"
+#include "sph_derive.hpp"
 
 namespace {
 
@@ -45,7 +46,50 @@ float force_threshold(float h, float min_dist) {
     return t;
 }
 
+// Correctly rounded float sine and cosine (double evaluation, one rounding).
+inline float sin_rn(float x) { return (float)std::sin((double)x); }
+inline float cos_rn(float x) { return (float)std::cos((double)x); }
+
 }  // namespace
+
+namespace sph {
+
+void derive_shape(sph_params& p) {
+    // wmin.z += rTwist*(1 + sinf(rAngle))  (source/CUDA/System.cu:60)
+    p.wave_zmin_shift = p.wave_ampl * (1.0f + sin_rn(p.wave_phase));
+    // pump outlet (source/CUDA/System.cu:104-117): rad = worldMax.x
+    const float rad = p.world_max[0], ang = p.pump_ang_out;
+    if (ang < 0.5f) {
+        p.pump_xs = sin_rn(ang * p.pump_s[2]) * rad;
+        p.pump_zs = (cos_rn(ang * p.pump_s[2]) * rad) * p.pump_s[3];
+    } else {
+        p.pump_xs = 0.09f * p.pump_s[3];
+        p.pump_zs = 0.0f;
+    }
+    uint32_t phys = p.collider ? (uint32_t)SPH_PHYS_COLLIDER : 0u;
+    for (int k = 0; k < SPH_NUM_ACCEL; ++k)
+        if (p.acc[k].type != SPH_ACC_OFF) phys |= SPH_PHYS_ACCEL;
+    p.physics = phys;
+}
+
+int check_shape(const sph_params& p) {
+    if (p.bnd_type > SPH_BND_PUMP_Y || p.bnd_eff_z > SPH_BND_EFF_WAVE || p.collider > 1u)
+        return SPH_ERR_INVALID_PARAMS;
+    for (int k = 0; k < SPH_NUM_ACCEL; ++k)
+        if (p.acc[k].type > SPH_ACC_CYL_Y_SMOOTH) return SPH_ERR_INVALID_PARAMS;
+    return SPH_OK;
+}
+
+void advance_wave(sph_params& p) {
+    // App::UpdateEmitter before each cSPH::Update (source/App/Update.cpp:15-16,
+    // source/App/Render.cpp:10-11): rAngle += rVel * timeStep.
+    if (p.wave_speed != 0.0f) {
+        p.wave_phase = p.wave_phase + p.wave_speed * p.time_step;
+        derive_shape(p);
+    }
+}
+
+}  // namespace sph
 
 extern "C" {
 
@@ -99,6 +143,38 @@ int sph_scene_default(sph_scene* s) {
     s->eos = SPH_EOS_LINEAR;
     s->precision = SPH_PREC_REFERENCE;
     s->cfl = 0.0f;
+    // source/SPH/Scene.cpp:31-48 (boundary, collider, pump, rotor twist)
+    s->bnd_type = SPH_BND_BOX;
+    s->bnd_eff_z = SPH_BND_EFF_NONE;
+    s->wave_slope = 0.0f;
+    s->wave_ampl = 0.04f;
+    s->wave_phase = 0.0f;
+    s->wave_speed = 0.0f;
+    s->exit_vel = 0.03f;
+    s->exit_dist = 0.04f;
+    s->pump_ang_out = 0.7f;
+    s->pump_h_close = -0.03f;
+    s->pump_rad_in = 0.4f;
+    const float ps[6] = {0.90f, 0.90f, 1.198f, 0.7782f, 0.8696f, 0.843f};
+    for (int k = 0; k < 6; ++k) s->pump_s[k] = ps[k];
+    // The collider's values are the reference's, but it starts switched off:
+    // the hot path's workloads run without it.  The scene-file loader turns
+    // it on, as the reference applies it in every scene without a rotor.
+    s->collider = 0;
+    s->coll_pos[0] = -77.0f;
+    s->coll_r = 0.05f;
+    s->coll_spring = 80.0f;
+    s->coll_damping = 0.02f;
+    s->coll_shear = 0.1f;
+    // source/SPH/Scene.cpp:54-57
+    for (int k = 0; k < SPH_NUM_ACCEL; ++k) {
+        sph_accel& a = s->acc[k];
+        a.size[0] = 0.005f;
+        a.size[1] = 0.010f;
+        a.size[2] = 0.005f;
+        a.acc[1] = 10.0f;
+        a.type = SPH_ACC_OFF;
+    }
     return SPH_OK;
 }
 
@@ -143,8 +219,24 @@ int sph_params_from_scene(const sph_scene* s, sph_params* p) {
     p->bnd_stiff = s->bnd_stiff;
     p->bnd_damp = s->bnd_damp;
     p->bnd_damp_c = s->bnd_damp_c;
-    p->bnd_type = SPH_BND_BOX;
-    p->bnd_eff_z = SPH_BND_EFF_NONE;
+    p->bnd_type = s->bnd_type;
+    p->bnd_eff_z = s->bnd_eff_z;
+    p->wave_slope = s->wave_slope;
+    p->wave_ampl = s->wave_ampl;
+    p->wave_phase = s->wave_phase;
+    p->wave_speed = s->wave_speed;
+    p->exit_vel = s->exit_vel;
+    p->exit_dist = s->exit_dist;
+    p->pump_ang_out = s->pump_ang_out;
+    p->pump_h_close = s->pump_h_close;
+    p->pump_rad_in = s->pump_rad_in;
+    for (int k = 0; k < 6; ++k) p->pump_s[k] = s->pump_s[k];
+    p->collider = s->collider;
+    p->coll_r = s->coll_r;
+    p->coll_spring = s->coll_spring;
+    p->coll_damping = s->coll_damping;
+    p->coll_shear = s->coll_shear;
+    for (int k = 0; k < SPH_NUM_ACCEL; ++k) p->acc[k] = s->acc[k];
 
     // Scene::_UpdateGrid (source/SPH/Scene.cpp:98-109).
     uint64_t cells = 1;
@@ -170,6 +262,19 @@ int sph_params_from_scene(const sph_scene* s, sph_params* p) {
     p->cfl = s->cfl;
     p->sound_speed = std::sqrt(s->stiffness);
     p->force_r2_max = force_threshold(h, p->min_dist);
+
+    // Collider placement of Scene::_UpdateGrid (source/SPH/Scene.cpp:100,111-112).
+    if (s->coll_pos[0] == -77.0f) {
+        const float b = p->dist_bnd_soft - r;
+        p->coll_pos[0] = (p->world_min[0] + b) - p->coll_r * 1.2f;
+        p->coll_pos[1] = (p->world_min[1] + b) + p->coll_r * 1.0f;
+        p->coll_pos[2] = 0.0f;
+    } else {
+        for (int c = 0; c < 3; ++c) p->coll_pos[c] = s->coll_pos[c];
+    }
+    const int st = sph::check_shape(*p);
+    if (st != SPH_OK) return st;
+    sph::derive_shape(*p);
     return SPH_OK;
 }
 

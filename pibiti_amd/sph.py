@@ -29,6 +29,36 @@ EMPTY_CELL = 0xFFFFFFFF
 f3 = C.c_float * 3
 u3 = C.c_uint32 * 3
 
+# boundary types, z effects and accelerator shapes (source/CUDA/Params.cuh:23-38)
+(SPH_BND_BOX, SPH_BND_CYL_Y, SPH_BND_CYL_Z, SPH_BND_CYL_YZ, SPH_BND_SPHERE,
+ SPH_BND_PUMP_Y) = range(6)
+SPH_BND_EFF_NONE, SPH_BND_EFF_WRAP, SPH_BND_EFF_CYCLE, SPH_BND_EFF_WAVE = range(4)
+SPH_ACC_OFF, SPH_ACC_BOX, SPH_ACC_CYL_Y, SPH_ACC_CYL_Y_SMOOTH = range(4)
+SPH_NUM_ACCEL = 4
+SPH_PHYS_COLLIDER, SPH_PHYS_ACCEL = 1, 2
+
+
+class Accel(C.Structure):
+    """sph_accel: one accelerator (struct Accel, source/CUDA/Params.cuh:40-45)."""
+
+    _fields_ = [("pos", f3), ("size", f3), ("acc", f3), ("type", C.c_uint32)]
+
+
+_SHAPE_FIELDS = [
+    ("bnd_type", C.c_uint32),
+    ("bnd_eff_z", C.c_uint32),
+    ("wave_slope", C.c_float),
+    ("wave_ampl", C.c_float),
+    ("wave_phase", C.c_float),
+    ("wave_speed", C.c_float),
+    ("exit_vel", C.c_float),
+    ("exit_dist", C.c_float),
+    ("pump_ang_out", C.c_float),
+    ("pump_h_close", C.c_float),
+    ("pump_rad_in", C.c_float),
+    ("pump_s", C.c_float * 6),
+]
+
 
 class Scene(C.Structure):
     """sph_scene: raw scene values before Scene::_UpdatePar/_UpdateGrid."""
@@ -58,6 +88,48 @@ class Scene(C.Structure):
         ("eos", C.c_uint32),
         ("precision", C.c_uint32),
         ("cfl", C.c_float),
+        *_SHAPE_FIELDS,
+        ("collider", C.c_uint32),
+        ("coll_pos", f3),
+        ("coll_r", C.c_float),
+        ("coll_spring", C.c_float),
+        ("coll_damping", C.c_float),
+        ("coll_shear", C.c_float),
+        ("acc", Accel * SPH_NUM_ACCEL),
+    ]
+
+
+class Emitter(C.Structure):
+    """sph_emitter (class Emitter, source/SPH/Scene.h:7-17)."""
+
+    _fields_ = [("size", C.c_int32), ("size2", C.c_int32), ("pos", f3), ("rot", f3), ("vel", C.c_float)]
+
+
+SPH_FEAT_ROTOR, SPH_FEAT_HMAP = 1, 2
+
+
+class SceneInfo(C.Structure):
+    """sph_scene_info: a scene file's non-parameter values (init volume,
+    drops, emitters, camera, unsupported terms)."""
+
+    _fields_ = [
+        ("title", C.c_char * 64),
+        ("chapter", C.c_uint32),
+        ("is_default", C.c_uint32),
+        ("init_min", f3),
+        ("init_max", f3),
+        ("init_type", C.c_int32),
+        ("init_last", C.c_int32),
+        ("spacing", C.c_float),
+        ("drop_r", C.c_float),
+        ("rain", C.c_int32),
+        ("emit", Emitter * 4),
+        ("cam_pos", f3),
+        ("cam_rot", f3),
+        ("rotor_type", C.c_int32),
+        ("hmap_type", C.c_int32),
+        ("unsupported", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -104,6 +176,18 @@ class Params(C.Structure):
         ("tune", C.c_uint32 * 4),
         ("cell_base", C.c_uint32),
         ("t_end", C.c_float),
+        *[f for f in _SHAPE_FIELDS if f[0] not in ("bnd_type", "bnd_eff_z")],
+        ("wave_zmin_shift", C.c_float),
+        ("pump_xs", C.c_float),
+        ("pump_zs", C.c_float),
+        ("collider", C.c_uint32),
+        ("coll_pos", f3),
+        ("coll_r", C.c_float),
+        ("coll_spring", C.c_float),
+        ("coll_damping", C.c_float),
+        ("coll_shear", C.c_float),
+        ("acc", Accel * SPH_NUM_ACCEL),
+        ("physics", C.c_uint32),
         ("reserved", C.c_uint32 * 2),
     ]
 
@@ -193,6 +277,10 @@ SIGNATURES = {
     "sph_params_from_scene": (C.c_int, [C.POINTER(Scene), C.POINTER(Params)]),
     "sph_scene_lattice": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, _fp, C.c_float, C.c_float,
                                     C.c_float, _fp, C.c_uint64, C.c_uint64, _fp, _fp]),
+    "sph_xml_scene_count": (C.c_int, [C.c_char_p, C.c_size_t, _up, _up]),
+    "sph_xml_scene": (C.c_int, [C.c_char_p, C.c_size_t, C.c_uint32, C.POINTER(Scene),
+                                C.POINTER(SceneInfo)]),
+    "sph_scene_fill": (C.c_int, [C.POINTER(Scene), C.POINTER(SceneInfo), C.c_uint64, _fp, _fp]),
     "sph_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_uint32, C.POINTER(_P)]),
     "sph_destroy": (C.c_int, [_P]),
     "sph_set_params": (C.c_int, [_P, C.POINTER(Params)]),
@@ -298,6 +386,38 @@ def scene_lattice(nx: int, ny: int, nz: int, region_min, spacing: float, jitter:
     voff = np.asarray(vel_offset, np.float32)
     _check(lib().sph_scene_lattice(nx, ny, nz, _f(rmin), spacing, jitter, vel_sigma, _f(voff),
                                    seed, stream, _f(pos), _f(vel)), "sph_scene_lattice")
+    return pos, vel
+
+
+def _xml_bytes(text) -> bytes:
+    return text.encode("latin-1") if isinstance(text, str) else bytes(text)
+
+
+def xml_scene_count(text) -> tuple[int, int]:
+    """(number of scenes, startup scene index) of a scene file's text
+    (cSPH::LoadScenes, source/SPH/SPH_Scenes.cpp:53-83)."""
+    b = _xml_bytes(text)
+    n, d = C.c_uint32(), C.c_uint32()
+    _check(lib().sph_xml_scene_count(b, len(b), C.byref(n), C.byref(d)), "sph_xml_scene_count")
+    return int(n.value), int(d.value)
+
+
+def xml_scene(text, index: int) -> tuple[Scene, SceneInfo]:
+    """Scene `index` of a scene file: Scene::InitDefault + Scene::_FromXML
+    (source/SPH/Scene.cpp:7-66, source/SPH/Scene_Load.cpp:11-111)."""
+    b = _xml_bytes(text)
+    s, info = Scene(), SceneInfo()
+    _check(lib().sph_xml_scene(b, len(b), index, C.byref(s), C.byref(info)), "sph_xml_scene")
+    return s, info
+
+
+def scene_fill(s: Scene, info: SceneInfo, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """cSPH::Reset(initType) (source/SPH/SPH_Init.cpp:23-76): positions and
+    zero velocities of the scene's particles."""
+    n = int(s.num_particles)
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    _check(lib().sph_scene_fill(C.byref(s), C.byref(info), seed, _f(pos), _f(vel)), "sph_scene_fill")
     return pos, vel
 
 

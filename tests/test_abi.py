@@ -40,7 +40,7 @@ def test_binding_covers_every_declared_symbol():
 
 
 def test_abi_version_and_strerror():
-    assert sph.lib().sph_abi_version() == 1
+    assert sph.lib().sph_abi_version() == 2
     assert sph.strerror(0) == "ok"
     for code in range(1, 9):
         assert sph.strerror(code) not in ("ok", "unknown status")
@@ -99,9 +99,59 @@ def test_create_rejects_invalid_params_before_touching_a_device():
     p = sph.params_from_scene(sph.scene_default())
     p.key_bits = 2  # fewer bits than the cell count needs
     assert sph.lib().sph_create(C.byref(p), 0, 0, C.byref(ctx)) == 2
+    for field, value in (("bnd_type", 6), ("bnd_eff_z", 4), ("collider", 2)):
+        p = sph.params_from_scene(sph.scene_default())
+        setattr(p, field, value)  # no such boundary type, effect or collider mode
+        assert sph.lib().sph_create(C.byref(p), 0, 0, C.byref(ctx)) == 2
     p = sph.params_from_scene(sph.scene_default())
-    p.bnd_type = 3  # only the box boundary is built
+    p.acc[2].type = 4  # no such accelerator shape
     assert sph.lib().sph_create(C.byref(p), 0, 0, C.byref(ctx)) == 2
+
+
+def test_batch_rejects_a_moving_wave():
+    """A moving wave needs new parameters every step, which a batch's device
+    table does not take: refused before any device use."""
+    L = sph.lib()
+    good = sph.params_from_scene(sph.scene_default())
+    wave = good.copy()
+    wave.bnd_eff_z = sph.SPH_BND_EFF_WAVE
+    wave.wave_speed = 6.0
+    arr = (sph.Params * 2)(good, wave)
+    out = C.c_void_p()
+    assert L.sph_batch_create(arr, 2, 0, 0, C.byref(out)) in (3, 7)  # no device here, or unsupported
+
+
+C_TYPES = {C.c_float: "float", C.c_uint32: "uint32_t", C.c_double: "double", C.c_uint64: "uint64_t"}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every field of the ctypes mirrors sits at the offset the C compiler
+    gives the header's struct, and the sizes agree."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"sph_scene": sph.Scene, "sph_params": sph.Params, "sph_accel": sph.Accel,
+               "sph_emitter": sph.Emitter, "sph_scene_info": sph.SceneInfo,
+               "sph_stats": sph.Stats, "sph_mlp_norm": sph.MlpNorm,
+               "sph_learned_report": sph.LearnedReport}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pibiti_sph.h"', "int main(void) {"]
+    expect = []
+    for cname, py in structs.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(C.sizeof(py))
+        for fname, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            expect.append(getattr(py, fname).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got == expect
 
 
 def test_batch_create_rejects_bad_scenes_and_null():

@@ -1,0 +1,167 @@
+"""GPU parity of the boundary shapes, z effects, collider and accelerators
+against the FP32 oracle, through the C ABI: positions, velocities, densities
+and force increments bit for bit after several steps.
+
+Each scene fills its shape with a jittered lattice (the sites inside the
+shape, as cSPH::Reset keeps them, source/SPH/SPH_Init.cpp:58-69) with random
+velocities, so particles sit in every soft band, cross the wrap planes and
+the pump outlet, and move through the collider and the accelerators.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pibiti_amd import sph
+
+R = 0.00625        # particle radius; spacing 2 R, h = cell = 4 R
+STEPS = 12
+
+
+def shape_scene(bnd=sph.SPH_BND_BOX, eff=sph.SPH_BND_EFF_NONE, world=(0.2, 0.2, 0.2), keep=None,
+                seed=7, vel_sigma=0.6, prec=sph.SPH_PREC_REFERENCE, **kw):
+    spacing = 2 * R
+    axes = [np.arange(-w + R, w - R + 1e-9, spacing) for w in world]
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    g = g + rng.uniform(-0.1, 0.1, g.shape) * spacing
+    if keep is not None:
+        g = g[keep(g)]
+    n = len(g)
+    s = sph.scene_default()
+    s.num_particles = n
+    s.time_step = 2e-4
+    s.gravity[:] = (0.0, -9.81, 0.0)
+    for c in range(3):
+        s.world_min[c] = -world[c]
+        s.world_max[c] = world[c]
+    s.particle_r = R
+    s.h = 4 * R
+    s.cell_size = 4 * R
+    s.particle_mass = 1000.0 * spacing ** 3
+    s.viscosity = 0.1
+    s.bnd_type = bnd
+    s.bnd_eff_z = eff
+    s.precision = prec
+    for k, v in kw.items():
+        if isinstance(v, (list, tuple)):
+            arr = getattr(s, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(s, k, v)
+    p = sph.params_from_scene(s)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = g
+    pos[:, 3] = 1
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.normal(0.0, vel_sigma, (n, 3))
+    return p, pos, vel
+
+
+def in_sphere(g):
+    return np.linalg.norm(g, axis=1) < 0.2 - R
+
+
+def in_cyl_y(g):
+    return np.hypot(g[:, 0], g[:, 2]) < 0.2 - R
+
+
+def in_cyl_z(g):
+    return np.hypot(g[:, 0], g[:, 1]) < 0.2 - R
+
+
+def accel(kind, pos, size, acc):
+    a = sph.Accel()
+    a.type = kind
+    for c in range(3):
+        a.pos[c], a.size[c], a.acc[c] = pos[c], size[c], acc[c]
+    return a
+
+
+CASES = {
+    "sphere": dict(bnd=sph.SPH_BND_SPHERE, keep=in_sphere),
+    "cylinder_y": dict(bnd=sph.SPH_BND_CYL_Y, keep=in_cyl_y),
+    "cylinder_z": dict(bnd=sph.SPH_BND_CYL_Z, keep=in_cyl_z),
+    "cylinders_yz": dict(bnd=sph.SPH_BND_CYL_YZ),
+    "cycle_z": dict(eff=sph.SPH_BND_EFF_CYCLE),
+    "wrap_z": dict(eff=sph.SPH_BND_EFF_WRAP),
+    "wave_moving": dict(eff=sph.SPH_BND_EFF_WAVE, wave_slope=0.2, wave_ampl=0.05,
+                        wave_phase=-1.5707963, wave_speed=6.0),
+    "pump_narrow_outlet": dict(bnd=sph.SPH_BND_PUMP_Y, world=(0.18, 0.3, 0.15), pump_ang_out=0.15,
+                               pump_s=[0.9, 0.9, 2.01, 0.8585, 0.9, 0.9]),
+    "pump_s_outlet": dict(bnd=sph.SPH_BND_PUMP_Y, world=(0.2, 0.3, 0.15), pump_ang_out=0.69,
+                          pump_h_close=-0.02, pump_s=[0.72, 0.68, 1.34, 0.89, 0.878, 0.883]),
+    "collider": dict(collider=1, coll_pos=[0.02, -0.05, 0.01], coll_r=0.05),
+    "collider_default_place_kahan": dict(collider=1, prec=sph.SPH_PREC_KAHAN),
+    "accelerators": dict(acc=[accel(sph.SPH_ACC_BOX, (0.05, -0.1, 0.0), (0.03, 0.03, 0.03), (0, 90, 30)),
+                              accel(sph.SPH_ACC_CYL_Y, (-0.06, 0.0, 0.05), (0.04, 0.05, 0.03), (20, 0, 0)),
+                              accel(sph.SPH_ACC_CYL_Y_SMOOTH, (0.0, 0.1, -0.05), (0.05, 0.04, 0.06),
+                                    (0, -40, 10))]),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_shape_steps_bit_exact(case):
+    kw = dict(CASES[case])
+    p, pos, vel = shape_scene(**kw)
+    print(f"case: {case}, n = {p.num_particles}, {STEPS} steps", flush=True)
+    with sph.SphSystem(p, keep_dvel=True) as s:
+        s.reset(pos, vel)
+        s.update(STEPS)
+        g = dict(pos=s.get_array(sph.SPH_ARRAY_POSITION), vel=s.get_array(sph.SPH_ARRAY_VELOCITY),
+                 dens=s.get_array(sph.SPH_ARRAY_DENSITY))
+    o = oracle.run_f32(p, pos, vel, STEPS, prec=int(p.precision))
+    for k in ("pos", "vel", "dens"):
+        bad = np.flatnonzero(np.any((g[k] != o[k]).reshape(len(pos), -1), axis=1))
+        assert bad.size == 0, f"{case} {k}: {bad.size} particles differ, first {bad[:5]}"
+    moved = np.abs(o["vel"][:, :3] - vel[:, :3]).max()
+    assert moved > 0.0
+
+
+def test_shape_walls_act_in_the_run():
+    """(CPU) The parity cases exercise their branches: in the pump scene some
+    particles leave through the outlet (their z jumps to the inlet side),
+    and in the cycle scene some wrap across z."""
+    p, pos, vel = shape_scene(**CASES["pump_narrow_outlet"])
+    o = oracle.run_f32(p, pos, vel, STEPS)
+    assert np.any(o["pos"][:, 2] - pos[:, 2] > 0.1)
+    p, pos, vel = shape_scene(**CASES["cycle_z"])
+    o = oracle.run_f32(p, pos, vel, STEPS)
+    assert np.any(np.abs(o["pos"][:, 2] - pos[:, 2]) > 0.1)
+
+
+@pytest.mark.gpu
+def test_shape_scenes_in_one_batch():
+    """Still shapes in a batch (each scene with its own shape, collider and
+    accelerators) give each scene's single-scene result."""
+    names = ["sphere", "cylinder_z", "pump_narrow_outlet", "accelerators"]
+    scenes = [shape_scene(**CASES[k]) for k in names]
+    ps = [sc[0] for sc in scenes]
+    print("case: batch of", names, flush=True)
+    with sph.SphBatch(ps) as b:
+        for i, (p, pos, vel) in enumerate(scenes):
+            b.reset(i, pos, vel)
+        b.update(STEPS)
+        for i, (p, pos, vel) in enumerate(scenes):
+            o = oracle.run_f32(p, pos, vel, STEPS)
+            assert np.array_equal(b.get_array(i, sph.SPH_ARRAY_POSITION), o["pos"]), names[i]
+            assert np.array_equal(b.get_array(i, sph.SPH_ARRAY_VELOCITY), o["vel"]), names[i]
+
+
+@pytest.mark.gpu
+def test_graph_replay_of_a_still_shape_and_no_capture_of_a_moving_wave():
+    p, pos, vel = shape_scene(**CASES["cylinders_yz"])
+    print("case: graph replay, cylinders yz", flush=True)
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        s.capture(4)
+        s.replay(3)
+        got = s.get_array(sph.SPH_ARRAY_VELOCITY)
+    o = oracle.run_f32(p, pos, vel, 12)
+    assert np.array_equal(got, o["vel"])
+    p, pos, vel = shape_scene(**CASES["wave_moving"])
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        with pytest.raises(sph.SphError):
+            s.capture(2)
