@@ -11,6 +11,7 @@
 #define PIBITI_SPH_HPP
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -46,23 +47,56 @@ struct InitVolume {
 class cSPH {
 public:
     sph_scene scn{};    /* raw scene values (Scene)                          */
+    sph_scene_info info{}; /* the scene's init volume, emitters, ... (file scenes) */
     sph_params par{};   /* derived constants (Scene::_UpdatePar/_UpdateGrid) */
     InitVolume vol{};
     int device = 0;
     uint32_t flags = 0; /* SPH_FLAG_*                                        */
     bool bInitialized = false;
+    int curScene = -1;  /* index into the loaded scene file, -1: scn set by hand */
+    uint64_t seed = 0;  /* random fill (initType 1)                          */
 
     cSPH() { check(sph_scene_default(&scn), "sph_scene_default"); }
     ~cSPH() { _FreeMem(); }
     cSPH(const cSPH&) = delete;
     cSPH& operator=(const cSPH&) = delete;
 
-    /* Derive the constants, allocate, fill the init volume. */
+    /* cSPH::LoadScenes (source/SPH/SPH_Scenes.cpp:53-83): read a scene file
+     * and select its startup scene (not yet initialised).                  */
+    void LoadScenes(const std::string& path) {
+        FILE* f = std::fopen(path.c_str(), "rb");
+        if (!f) throw Error(SPH_ERR_INVALID_ARGUMENT, ("cSPH::LoadScenes " + path).c_str());
+        std::string text;
+        char buf[65536];
+        size_t got;
+        while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+        std::fclose(f);
+        uint32_t count = 0, def = 0;
+        check(sph_xml_scene_count(text.data(), text.size(), &count, &def), "sph_xml_scene_count");
+        xml_ = std::move(text);
+        count_ = count;
+        SetScene(def);
+    }
+
+    uint32_t SceneCount() const { return count_; }
+
+    /* scn = scenes[index] (the value part of cSPH::UpdScene). */
+    void SetScene(uint32_t index) {
+        check(sph_xml_scene(xml_.data(), xml_.size(), index, &scn, &info), "sph_xml_scene");
+        curScene = (int)index;
+    }
+
+    /* cSPH::NextScene / PrevScene (source/SPH/SPH_Scenes.cpp:36-47). */
+    void NextScene(bool chapter = false) { Step(+1, chapter); }
+    void PrevScene(bool chapter = false) { Step(-1, chapter); }
+
+    /* cSPH::InitScene: derive the constants, allocate, fill the init volume
+     * (a file scene with cSPH::Reset(initType), else the `vol` lattice). */
     void InitScene() {
         check(sph_params_from_scene(&scn, &par), "sph_params_from_scene");
         _FreeMem();
         _InitMem();
-        Reset(0);
+        Reset(curScene >= 0 ? info.init_type : 0);
     }
 
     void _InitMem() {
@@ -80,10 +114,19 @@ public:
         bInitialized = false;
     }
 
-    /* type 0: the lattice block of `vol` (its size must be the scene's
-     * particle count); other reference fill types are not provided.        */
+    /* A file scene: its init volume or random fill (sph_scene_fill, any
+     * type).  Otherwise type 0: the lattice block of `vol` (its size must be
+     * the scene's particle count).                                         */
     void Reset(int type) {
         require();
+        if (curScene >= 0) {
+            sph_scene_info in = info;
+            in.init_type = type;
+            check(sph_scene_fill(&scn, &in, seed, hPos_.data(), hVel_.data()), "sph_scene_fill");
+            setArray(SPH_ARRAY_POSITION, hPos_.data(), 0, (int)par.num_particles);
+            setArray(SPH_ARRAY_VELOCITY, hVel_.data(), 0, (int)par.num_particles);
+            return;
+        }
         if (type != 0) throw Error(SPH_ERR_UNSUPPORTED, "cSPH::Reset");
         if ((uint64_t)vol.nx * vol.ny * vol.nz != par.num_particles)
             throw Error(SPH_ERR_INVALID_ARGUMENT, "cSPH::Reset (volume size)");
@@ -137,8 +180,24 @@ private:
         if (!bInitialized) throw Error(SPH_ERR_NOT_INITIALISED, "cSPH");
     }
 
+    void Step(int dir, bool chapter) {
+        if (count_ == 0) throw Error(SPH_ERR_NOT_INITIALISED, "cSPH: no scene file");
+        int k = curScene < 0 ? 0 : curScene;
+        sph_scene s{};
+        sph_scene_info in{};
+        for (uint32_t tries = 0; tries < count_; ++tries) {
+            k = (k + dir + (int)count_) % (int)count_;
+            check(sph_xml_scene(xml_.data(), xml_.size(), (uint32_t)k, &s, &in), "sph_xml_scene");
+            if (!chapter || in.chapter) break;
+        }
+        SetScene((uint32_t)k);
+        InitScene();
+    }
+
     sph_ctx* ctx_ = nullptr;
     std::vector<float> hPos_, hVel_, hDens_;
+    std::string xml_;
+    uint32_t count_ = 0;
 };
 
 }  // namespace pibiti

@@ -2,18 +2,61 @@
 //
 //   sph_run SCENE NX NY NZ MINX MINY MINZ SPACING JITTER VEL_SIGMA SEED STEPS OUT
 //
+//   sph_run --xml SCENES.xml INDEX STEPS OUT
+//
 // SCENE is a raw sph_scene record (sizeof(sph_scene) bytes).  The lattice
-// block NX x NY x NZ from (MINX, MINY, MINZ) is advanced STEPS steps; OUT
-// receives, as little-endian binary: uint32 n, float4 positions[n], float4
-// velocities[n], float densities[n], then double mass, kinetic energy,
-// max speed and simulated time.
+// block NX x NY x NZ from (MINX, MINY, MINZ) is advanced STEPS steps.  With
+// --xml, scene INDEX of a scene file is initialised as cSPH::InitScene does
+// (its init volume fill) and advanced STEPS steps.  OUT receives, as
+// little-endian binary: uint32 n, float4 positions[n], float4 velocities[n],
+// float densities[n], then double mass, kinetic energy, max speed and
+// simulated time.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "pibiti_sph.hpp"
 
+static int write_out(pibiti::cSPH& sim, uint32_t steps, const char* path) {
+    const uint32_t n = sim.par.num_particles;
+    std::vector<float> pos(sim.getArray(SPH_ARRAY_POSITION), sim.getArray(SPH_ARRAY_POSITION) + 4 * n);
+    const float* vel = sim.getArray(SPH_ARRAY_VELOCITY);
+    const float* dens = sim.getDensity();
+    const sph_stats st = sim.Stats();
+    FILE* o = std::fopen(path, "wb");
+    if (!o) {
+        std::fprintf(stderr, "cannot write %s\n", path);
+        return 2;
+    }
+    const double tail[4] = {st.mass, st.kinetic_energy, st.max_speed, st.sim_time};
+    bool ok = std::fwrite(&n, 4, 1, o) == 1;
+    ok = ok && std::fwrite(pos.data(), 16, n, o) == n;
+    ok = ok && std::fwrite(vel, 16, n, o) == n;
+    ok = ok && std::fwrite(dens, 4, n, o) == n;
+    ok = ok && std::fwrite(tail, 8, 4, o) == 4;
+    std::fclose(o);
+    if (!ok) return 2;
+    std::printf("sph_run: %u particles, %u steps, sim_time %.6g s, max speed %.6g m/s\n", n, steps,
+                st.sim_time, st.max_speed);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 6 && std::strcmp(argv[1], "--xml") == 0) {
+        try {
+            pibiti::cSPH sim;
+            sim.LoadScenes(argv[2]);
+            sim.SetScene((uint32_t)std::strtoul(argv[3], nullptr, 10));
+            const uint32_t steps = (uint32_t)std::strtoul(argv[4], nullptr, 10);
+            sim.InitScene();
+            sim.Update(steps);
+            return write_out(sim, steps, argv[5]);
+        } catch (const pibiti::Error& e) {
+            std::fprintf(stderr, "sph_run: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc != 14) {
         std::fprintf(stderr,
                      "usage: %s SCENE NX NY NZ MINX MINY MINZ SPACING JITTER VEL_SIGMA SEED STEPS OUT\n",
@@ -40,26 +83,7 @@ int main(int argc, char** argv) {
         const uint32_t steps = (uint32_t)std::strtoul(argv[12], nullptr, 10);
         sim.InitScene();
         sim.Update(steps);
-        const uint32_t n = sim.par.num_particles;
-        std::vector<float> pos(sim.getArray(SPH_ARRAY_POSITION), sim.getArray(SPH_ARRAY_POSITION) + 4 * n);
-        const float* vel = sim.getArray(SPH_ARRAY_VELOCITY);
-        const float* dens = sim.getDensity();
-        const sph_stats st = sim.Stats();
-        FILE* o = std::fopen(argv[13], "wb");
-        if (!o) {
-            std::fprintf(stderr, "cannot write %s\n", argv[13]);
-            return 2;
-        }
-        const double tail[4] = {st.mass, st.kinetic_energy, st.max_speed, st.sim_time};
-        bool ok = std::fwrite(&n, 4, 1, o) == 1;
-        ok = ok && std::fwrite(pos.data(), 16, n, o) == n;
-        ok = ok && std::fwrite(vel, 16, n, o) == n;
-        ok = ok && std::fwrite(dens, 4, n, o) == n;
-        ok = ok && std::fwrite(tail, 8, 4, o) == 4;
-        std::fclose(o);
-        if (!ok) return 2;
-        std::printf("sph_run: %u particles, %u steps, sim_time %.6g s, max speed %.6g m/s\n", n, steps,
-                    st.sim_time, st.max_speed);
+        return write_out(sim, steps, argv[13]);
     } catch (const pibiti::Error& e) {
         std::fprintf(stderr, "sph_run: %s\n", e.what());
         return 1;
