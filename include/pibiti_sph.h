@@ -139,31 +139,52 @@ typedef struct sph_scene {
     uint32_t eos;               /* SPH_EOS_*                                 */
     uint32_t precision;         /* SPH_PREC_*                                */
     float cfl;                  /* 0: fixed dt; > 0: dt = min(time_step, cfl*h/(c+vmax)) */
-    /* Boundary shape and z effect (boundary(), source/CUDA/System.cu:41-159)
-     * and the values it reads (source/CUDA/Params.cuh:99-107).             */
+    /* Boundary shape and z effect (boundary(), source/CUDA/System.cu:41-159),
+     * rotor and height map (computeForceD, source/CUDA/System.cu:253-372),
+     * and the SimParams values they read (source/CUDA/Params.cuh:98-113).
+     * As in the reference, several features share one value.              */
     uint32_t bnd_type;          /* SPH_BND_*                                 */
     uint32_t bnd_eff_z;         /* SPH_BND_EFF_*                             */
-    float wave_slope;           /* HSlope (r2Angle): floor slope along z     */
-    float wave_ampl;            /* WaveAmpl (rTwist)                         */
-    float wave_phase;           /* rAngle at the start                       */
-    float wave_speed;           /* WaveSpeed (rVel): every step first advances
-                                   wave_phase by wave_speed * time_step
-                                   (source/App/Update.cpp:15-16)             */
+    float rot2_angle;           /* r2Angle: wave floor slope (HSlope), height-map
+                                   level (Hheight), second propeller's angle
+                                   (Rotor2Angle, degrees in the file)        */
+    float rot_twist;            /* rTwist: wave amplitude (WaveAmpl), rotor twist */
+    float rot_angle;            /* rAngle: wave phase, rotor angle (rad)     */
+    float rot_vel;              /* rVel (WaveSpeed, RotorVel): every step first
+                                   advances rot_angle by rot_vel * time_step
+                                   (source/App/Update.cpp:15-20)             */
     float exit_vel;             /* ExitVel (rVexit)                          */
     float exit_dist;            /* ExitDist (rDexit)                         */
     float pump_ang_out;         /* PumpAngOut                                */
-    float pump_h_close;         /* PumpHClose                                */
+    float h_close;              /* hClose: pump lid (PumpHClose), height-map
+                                   amplitude (Hscale)                        */
     float pump_rad_in;          /* PumpRadIn (factor of world_max[0])        */
-    float pump_s[6];            /* s1..s6                                    */
+    float shape_s[6];           /* s1..s6: pump shape; height-map frequencies,
+                                   phases (degrees) and threshold (s1..s5)   */
     /* DEM sphere collider (collideSpheres, source/CUDA/Kernel_Cell.cui:78-96).
      * computeForceD applies it in every scene without a rotor: it is the
      * else branch of the rotor test (source/CUDA/System.cu:310,373-375).  */
-    uint32_t collider;          /* 1: on                                     */
-    float coll_pos[3];          /* centre; coll_pos[0] == -77 places it at the
-                                   world's x-/y- edge (source/SPH/Scene.cpp:111-112) */
+    uint32_t collider;          /* 1: on (where no rotor runs)               */
+    float coll_pos[3];          /* centre (also the rotor's); coll_pos[0] == -77
+                                   places it at the world's x-/y- edge
+                                   (source/SPH/Scene.cpp:111-112)            */
     float coll_r;               /* ColliderR                                 */
-    float coll_spring, coll_damping, coll_shear;
+    float coll_spring, coll_damping, coll_shear; /* also the rotor's and height map's */
     sph_accel acc[SPH_NUM_ACCEL];
+    /* Rotor (source/CUDA/System.cu:309-372): spheres of radius rot_r laid
+     * out in blades about coll_pos and turning with rot_angle.             */
+    int32_t rot_type;           /* SPH_ROTOR_*                               */
+    int32_t rot_size[3];        /* RotorSizes: blade extents (x, y, z)       */
+    int32_t rot_blades;         /* RotorBlades                               */
+    float rot_r;                /* colParR: sphere radius, factor of particle_r */
+    float rot_spc;              /* colParSpc: sphere spacing, factor of the radius */
+    float rot2_dist;            /* Rotor2Dist: propeller pair separation     */
+    float rot2_vel;             /* Rotor2Vel: second propeller's speed factor */
+    float rot2_twist;           /* r2twist: second propeller's twist sign    */
+    /* Height map (source/CUDA/System.cu:253-307): a sheet of spheres of
+     * radius rot_r, spacing rot_spc, at height
+     * h_close sin(s1 u + s2) sin(s3 w + s4) + rot2_angle above world_min.  */
+    int32_t hmap_type;          /* SPH_HMAP_*                                */
 } sph_scene;
 
 /* Derived kernel constants: the plain-scalar equivalent of SimParams
@@ -216,31 +237,44 @@ typedef struct sph_params {
     /* Simulated end time (s): the time step is clipped so the scene's clock
      * stops at t_end, after which the scene is frozen.  0 = no end time.    */
     float t_end;
-    /* Boundary shape: the sph_scene values, unscaled. */
-    float wave_slope, wave_ampl, wave_phase, wave_speed;
+    /* Boundary shape, rotor and height map: the sph_scene values, with
+     * rot_r and rot_spc scaled to metres as Scene::_UpdatePar does
+     * (rot_r *= particle_r, then rot_spc *= rot_r, source/SPH/Scene.cpp:93). */
+    float rot2_angle, rot_twist, rot_angle, rot_vel;
     float exit_vel, exit_dist;
-    float pump_ang_out, pump_h_close, pump_rad_in;
-    float pump_s[6];
+    float pump_ang_out, h_close, pump_rad_in;
+    float shape_s[6];
     /* Derived from the fields above whenever the library takes parameters
      * (values set by the caller are replaced):
-     *   wave_zmin_shift = wave_ampl (1 + sin(wave_phase)), the moving z- wall
+     *   wave_zmin_shift = rot_twist (1 + sin(rot_angle)), the moving z- wall
      *     of the wave effect (source/CUDA/System.cu:60);
      *   pump_xs, pump_zs: the pump outlet's half width and the height above
-     *     which its side walls act (source/CUDA/System.cu:104-117).
-     * sin and cos here and in the pump's per-particle terms are correctly
-     * rounded float results.                                               */
+     *     which its side walls act (source/CUDA/System.cu:104-117);
+     *   world_size_d: worldSizeD, the world less the soft band less r on
+     *     each side (source/SPH/Scene.cpp:99-103), the height map's period.
+     * sin and cos here and in the per-particle terms of the pump, rotor and
+     * height map are correctly rounded float results.                     */
     float wave_zmin_shift, pump_xs, pump_zs;
+    float world_size_d[3];
     /* Collider (centre already placed) and accelerators, as in sph_scene. */
     uint32_t collider;
     float coll_pos[3], coll_r, coll_spring, coll_damping, coll_shear;
     sph_accel acc[SPH_NUM_ACCEL];
-    /* Derived: SPH_PHYS_COLLIDER | SPH_PHYS_ACCEL for the terms in use.    */
+    int32_t rot_type, rot_size[3], rot_blades;
+    float rot_r, rot_spc, rot2_dist, rot2_vel, rot2_twist;
+    int32_t hmap_type;
+    /* Derived: SPH_PHYS_* bits of the terms in use.                         */
     uint32_t physics;
     uint32_t reserved[2];
 } sph_params;
 
-/* sph_params::physics bits. */
-enum { SPH_PHYS_COLLIDER = 1u, SPH_PHYS_ACCEL = 2u };
+/* Rotor kinds (RotorType) and height-map planes (HmapType). */
+enum { SPH_ROTOR_NONE = 0, SPH_ROTOR_PUMP_Z = 1, SPH_ROTOR_PUMP_Y = 2, SPH_ROTOR_PROPELLER_Z = 3 };
+enum { SPH_HMAP_NONE = 0, SPH_HMAP_XZ = 1, SPH_HMAP_YZ = 2 };
+
+/* sph_params::physics bits: the force-update terms in use.  The collider
+ * bit is set only when no rotor runs.                                      */
+enum { SPH_PHYS_COLLIDER = 1u, SPH_PHYS_ACCEL = 2u, SPH_PHYS_ROTOR = 4u, SPH_PHYS_HMAP = 8u };
 
 /* Global FP64 diagnostics reduced on the device (extension). */
 typedef struct sph_stats {
@@ -303,9 +337,8 @@ typedef struct sph_emitter {
 } sph_emitter;
 
 /* Scene values that are not simulation parameters: the init volume and fill
- * of cSPH::Reset, the drops and emitters of the app's frame update, the
- * camera, and the rotor / height-map terms, which this library does not
- * build (their scenes run without them; `unsupported` says which).       */
+ * of cSPH::Reset, the drops and emitters of the app's frame update, and the
+ * camera.                                                                  */
 typedef struct sph_scene_info {
     char title[64];             /* name attribute ("No name" without one)   */
     uint32_t chapter;           /* starts a chapter (always for scene 0)    */
@@ -320,13 +353,8 @@ typedef struct sph_scene_info {
     sph_emitter emit[4];
     float cam_pos[3];
     float cam_rot[3];
-    int32_t rotor_type;         /* RotorType (0 = none)                      */
-    int32_t hmap_type;          /* HmapType (0 = none)                       */
-    uint32_t unsupported;       /* SPH_FEAT_* terms the scene uses that are not built */
-    uint32_t reserved[3];
+    uint32_t reserved[4];
 } sph_scene_info;
-
-enum { SPH_FEAT_ROTOR = 1u, SPH_FEAT_HMAP = 2u };
 
 /* cSPH::LoadScenes (source/SPH/SPH_Scenes.cpp:53-83): the number of scenes
  * in the file text (len bytes) and the index of the startup scene (the last
@@ -339,9 +367,7 @@ int sph_xml_scene_count(const char* text, size_t len, uint32_t* count, uint32_t*
  * (source/SPH/Scene.cpp:7-66, source/SPH/Scene_Load.cpp:11-111), as an
  * sph_scene for sph_params_from_scene (raw factors, as the reference keeps
  * them before _UpdatePar) and the scene's other values (info may be NULL).
- * The collider is on unless the scene has a rotor (the reference applies
- * it in the rotor test's else branch, source/CUDA/System.cu:310,373-375);
- * wave_speed is the scene's rVel for the wave effect, else 0.              */
+ * The collider is on (the force update applies it where no rotor runs).  */
 int sph_xml_scene(const char* text, size_t len, uint32_t index, sph_scene* scene, sph_scene_info* info);
 
 /* cSPH::Reset(initType) (source/SPH/SPH_Init.cpp:23-76): scene->num_particles

@@ -236,14 +236,28 @@ def scene_values(sc: dict, index: int) -> dict:
     for k in range(6):
         if f"s{k + 1}" in A:
             s[k] = to_float(A[f"s{k + 1}"])
+    rot = dict(rot_size=[0, 4, 12], rot_blades=4, rot_r=f32(3), rot_spc=f32(0.7), rot2_dist=f32(0),
+               rot2_vel=f32(1), rot2_twist=f32(1))
+    if "RotorSizes" in A:
+        rot["rot_size"] = [int(x) for x in to_vec3(A["RotorSizes"])]  # make_int3 of floats
     if "RotorType" in A:
         rot_type = to_int(A["RotorType"])
+    if "Rotor2Dist" in A:
+        rot["rot2_dist"] = to_float(A["Rotor2Dist"])
     if "RotorVel" in A:
         r_vel = to_float(A["RotorVel"])
+    if "Rotor2Vel" in A:
+        rot["rot2_vel"] = to_float(A["Rotor2Vel"])
     if "RotorAngle" in A:
         r_angle = f32(f32(to_float(A["RotorAngle"]) * PI) / f32(180.0))
     if "Rotor2Angle" in A:
         r2_angle = f32(f32(to_float(A["Rotor2Angle"]) * PI) / f32(180.0))
+    if "colParR" in A:
+        rot["rot_r"] = to_float(A["colParR"])
+    if "colParSpc" in A:
+        rot["rot_spc"] = to_float(A["colParSpc"])
+    if "RotorBlades" in A:
+        rot["rot_blades"] = to_int(A["RotorBlades"])
     if "RotorTwist" in A:
         r_twist = to_float(A["RotorTwist"])
     if "WaveSpeed" in A:
@@ -263,11 +277,13 @@ def scene_values(sc: dict, index: int) -> dict:
 
     # what the library reports besides the raw values
     v["cell_size"] = cell
-    v["pump_h_close"] = h_close
-    v["pump_s"] = s
-    v["wave_phase"], v["wave_ampl"], v["wave_slope"] = r_angle, r_twist, r2_angle
-    v["wave_speed"] = r_vel if v["bnd_eff_z"] == 3 else f32(0)
-    v["collider"] = 0 if rot_type > 0 else 1
+    v["h_close"] = h_close
+    v["shape_s"] = s
+    v["rot_angle"], v["rot_twist"], v["rot2_angle"] = r_angle, r_twist, r2_angle
+    v["rot_vel"] = r_vel
+    v["collider"] = 1
+    v["rot_type"], v["hmap_type"] = rot_type, hmap
+    v.update(rot)
     v["coll_pos"] = coll_pos
     v["acc"] = acc
     r = v["particle_r"]
@@ -277,8 +293,6 @@ def scene_values(sc: dict, index: int) -> dict:
         "init_min": [f32(x + b) for x in init_min], "init_max": [f32(x - b) for x in init_max],
         "init_type": init_type, "init_last": init_last, "spacing": f32(spacing * r),
         "drop_r": drop_r, "rain": rain, "emit": emit, "cam_pos": cam_pos, "cam_rot": cam_rot,
-        "rotor_type": rot_type, "hmap_type": hmap,
-        "unsupported": (1 if rot_type > 0 else 0) | (2 if hmap > 0 else 0),
     }
     v.update({"info." + k: x for k, x in info.items()})
     return v

@@ -7,7 +7,8 @@
 // It restates, in plain C++, the arithmetic of the reference's CUDA step
 // (the reference has no CPU path of its own):
 //   integrate + boundary ....... source/CUDA/System.cu:41-159, :165-205
-//   collider + accelerators .... source/CUDA/Kernel_Cell.cui:78-96, source/CUDA/System.cu:374-399
+//   height map, rotor, collider,
+//   accelerators ............... source/CUDA/System.cu:253-399, source/CUDA/Kernel_Cell.cui:78-115
 //   cell hash ................... source/CUDA/Kernel_Cell.cui:5-34
 //   stable key/value sort ....... source/CUDA/radixsort_kernel.cu:445-472 (LSD, stable)
 //   cellStart + reorder ......... source/CUDA/Kernel_Cell.cui:40-67, source/CUDA/System.cu:689-713
@@ -96,9 +97,9 @@ inline void boundary(const sph_params& p, T x[3], T v[3]) {
     const bool wave = eff == SPH_BND_EFF_WAVE, none = eff == SPH_BND_EFF_NONE,
                cycle = eff == SPH_BND_EFF_CYCLE;
     if (wave) {  // :55-61
-        const T sl = -(T)p.wave_slope;
+        const T sl = -(T)p.rot2_angle;
         bnd_wall<T>((b - (x[1] - wmin[1])) - (x[2] - wmin[2]) * sl, o, one - sl, sl, stiff, damp, dt, v);
-        wmin[2] = wmin[2] + (T)p.wave_ampl * (one + sin_t<T>((T)p.wave_phase));
+        wmin[2] = wmin[2] + (T)p.rot_twist * (one + sin_t<T>((T)p.rot_angle));
     }
     if (!sphere) {  // :64-77
         if (!cyl_y) {
@@ -137,10 +138,10 @@ inline void boundary(const sph_params& p, T x[3], T v[3]) {
             x[2] = x[2] + (((wmax[2] - wmin[2]) - (T)2 * b) - dr);
     }
     if (type == SPH_BND_PUMP_Y) {  // :101-158
-        const T rad = wmax[0], ang = (T)p.pump_ang_out, hc = (T)p.pump_h_close;
+        const T rad = wmax[0], ang = (T)p.pump_ang_out, hc = (T)p.h_close;
         const T rin = rad * (T)p.pump_rad_in;
         T sv[6];
-        for (int k = 0; k < 6; ++k) sv[k] = (T)p.pump_s[k];
+        for (int k = 0; k < 6; ++k) sv[k] = (T)p.shape_s[k];
         const T len = std::sqrt(x[0] * x[0] + x[1] * x[1]);
         T diff = (b + len) - rad;
         if (diff > (T)kBndEps) {
@@ -343,35 +344,150 @@ inline T eos(const sph_params& p, T rho) {
 }
 
 // ------------------------------------------------------------------ force
+// collideSpheresR (source/CUDA/Kernel_Cell.cui:101-115) added to out: the
+// spring and damping force of a rotor or height-map sphere at offset rel
+// (relative velocity u = -v); a zero vector outside its reach.
+template <typename T>
+inline void collide_r(const sph_params& p, const T rel[3], const T u[3], T rad, T out[3]) {
+    const T dist = std::sqrt((rel[0] * rel[0] + rel[1] * rel[1]) + rel[2] * rel[2]);
+    T f[3] = {0, 0, 0};
+    if (dist < rad) {
+        const T inv = (T)1 / dist;  // float3 / float (cutil_math.h:358-362)
+        const T spring = (T)p.coll_spring * (dist - rad);
+        for (int k = 0; k < 3; ++k) f[k] = spring * (rel[k] * inv) + (T)p.coll_damping * u[k];
+    }
+    for (int k = 0; k < 3; ++k) out[k] = out[k] + f[k];
+}
+
+constexpr float kPiRef = 3.141592654f;  // PI, source/CUDA/Params.cuh:18
+
 // What computeForceD adds to addVel after the pair sum
-// (source/CUDA/System.cu:374-399): the sphere collider's DEM force
-// (collideSpheres, source/CUDA/Kernel_Cell.cui:78-96; float3/float there is
-// a multiplication by the reciprocal, source/external/cutil_math.h:358-362)
-// and then each accelerator whose shape holds the particle.
+// (source/CUDA/System.cu:253-399), in its order: the height map; the rotor
+// or, with no rotor, the sphere collider's DEM force (collideSpheres,
+// source/CUDA/Kernel_Cell.cui:78-96; float3/float is a multiplication by the
+// reciprocal, source/external/cutil_math.h:358-362); each accelerator whose
+// shape holds the particle.
 template <typename T>
 inline void extra_terms(const sph_params& p, const T xi[3], const T vi[3], T out[3]) {
     const T dt = (T)p.time_step;
-    if (p.collider) {
-        T rel[3], u[3], nrm[3];
+    const T pi = (T)kPiRef;
+    T u[3];
+    for (int k = 0; k < 3; ++k) u[k] = vi[k] * (T)-1;
+    if (p.hmap_type > 0) {  // :254-306
+        const bool xz = p.hmap_type == SPH_HMAP_XZ;
+        const int ax = xz ? 0 : 1;  // the in-plane axis besides z
+        const int lv = xz ? 1 : 0;  // the axis the sheet's height is along
+        const T spc = (T)p.rot_spc, rr = (T)p.particle_r + (T)p.rot_r;
+        // worldSizeD (Scene.cpp:99-103)
+        const T bd = (T)p.dist_bnd_soft - (T)p.particle_r;
+        T wsd[3];
+        for (int c = 0; c < 3; ++c) wsd[c] = ((T)p.world_max[c] - bd) - ((T)p.world_min[c] + bd);
+        const int ia = (int)((xi[ax] - (T)p.world_min[ax]) / spc);
+        const int iz = (int)((xi[2] - (T)p.world_min[2]) / spc);
+        const T af = (T)ia * spc + (T)p.world_min[ax];
+        const T zf = (T)iz * spc + (T)p.world_min[2];
+        for (int j = -2; j <= 2; ++j)
+            for (int i = -2; i <= 2; ++i) {
+                const T ah = af + (T)i * spc, zh = zf + (T)j * spc;
+                for (int k = 0; k <= 1; ++k) {
+                    // xh / worldSizeD.x * PI2 with PI2 = 2.f*PI unparenthesised
+                    const T an = ((ah / wsd[ax]) * (T)2) * pi;
+                    const T zn = ((zh / wsd[2]) * (T)2) * pi;
+                    const T ss = sin_t<T>((T)p.shape_s[0] * an + ((T)p.shape_s[1] * pi) / (T)180) *
+                                 sin_t<T>((T)p.shape_s[2] * zn + ((T)p.shape_s[3] * pi) / (T)180);
+                    if ((T)p.shape_s[4] <= (T)-1 || ss > (T)p.shape_s[4]) {
+                        const T lf = ((ss * (T)p.h_close) + (T)p.rot2_angle) + (T)p.world_min[lv];
+                        const T lh = lf - (T)k * spc;
+                        T rel[3];
+                        rel[ax] = ah - xi[ax];
+                        rel[lv] = lh - xi[lv];
+                        rel[2] = zh - xi[2];
+                        collide_r<T>(p, rel, u, rr, out);
+                    }
+                }
+            }
+    }
+    if (p.rot_type > 0) {  // :309-372
+        const int sx = p.rot_size[0], sy = p.rot_size[1], sz = p.rot_size[2], cb = p.rot_blades;
+        const T sp = (T)p.rot_spc, ca = ((T)2 * pi) / (T)cb;
+        const T x2 = (T)sx * (T)0.5f, y2 = (T)sy * (T)0.5f;
+        const T rr = (T)p.particle_r + (T)p.rot_r;
+        const T cp[3] = {(T)p.coll_pos[0], (T)p.coll_pos[1], (T)p.coll_pos[2]};
+        if (p.rot_type == SPH_ROTOR_PUMP_Z || p.rot_type == SPH_ROTOR_PUMP_Y) {
+            const int ax = p.rot_type == SPH_ROTOR_PUMP_Z ? 2 : 1;  // blade axis
+            const int bx = p.rot_type == SPH_ROTOR_PUMP_Z ? 1 : 2;  // the other turning axis
+            if (xi[ax] > cp[ax] - rr && xi[ax] < (cp[ax] + rr) + sp * (T)sy)
+                for (int z = 1; z <= sz; ++z)
+                    for (int x = 0; x <= sx; ++x)
+                        for (int c = 0; c < cb; ++c) {
+                            const T a = (-(T)p.rot_angle + (T)c * ca) + (T)z * (T)p.rot_twist;
+                            const T cs = cos_t<T>(a) * sp, sn = -sin_t<T>(a) * sp;
+                            for (int y = 0; y <= sy; ++y) {
+                                T pc[3];
+                                pc[0] = ((T)x - x2) * cs - (T)z * sn;
+                                pc[bx] = ((T)x - x2) * sn + (T)z * cs;
+                                pc[ax] = sp * (T)y;
+                                T rel[3];
+                                for (int k = 0; k < 3; ++k) rel[k] = (cp[k] + pc[k]) - xi[k];
+                                collide_r<T>(p, rel, u, rr, out);
+                            }
+                        }
+        } else {
+            const T half = (sp * (T)sz) / (T)2;
+            if (xi[2] > (cp[2] - rr) - half && xi[2] < (cp[2] + rr) + half) {
+                T rot[3] = {cp[0], cp[1], cp[2]};
+                T aa = (T)p.rot_angle, tw = (T)p.rot_twist;
+                if (p.rot2_dist > 0.0f) {
+                    if (xi[0] > (T)0) {
+                        rot[0] = rot[0] + (T)p.rot2_dist * (T)0.5f;
+                    } else {
+                        rot[0] = rot[0] - (T)p.rot2_dist * (T)0.5f;
+                        aa = (T)p.rot2_angle;
+                        tw = tw * (T)p.rot2_twist;
+                    }
+                }
+                for (int c = 0; c < cb; ++c) {
+                    const T a = aa + (T)c * ca;
+                    for (int h = 0; h <= sy; ++h) {
+                        T dh = 0;
+                        if (h == sy - 1)
+                            dh = (T)0.5f;
+                        else if (h == sy)
+                            dh = (T)1.2f;
+                        for (int x = 0; x <= sz; ++x) {
+                            T d = dh;
+                            if (x == sz && d == (T)0) d = (T)0.4f;
+                            const T k = cos_t<T>(((T)x - x2) * (T)0.2f + pi * (T)0.6f);
+                            const T at = tw * (d * (T)-0.05f - ((T)h - y2) * k);
+                            const T ac = a + at, cs = cos_t<T>(ac) * sp, sn = -sin_t<T>(ac) * sp;
+                            const T pc[3] = {(T)x * cs, (T)x * sn, (sp * ((-d + (T)h) - y2)) * k};
+                            const T r = (T)p.rot_r * std::fabs((T)1 - d);
+                            T rel[3];
+                            for (int q = 0; q < 3; ++q) rel[q] = (rot[q] + pc[q]) - xi[q];
+                            collide_r<T>(p, rel, u, (T)p.particle_r + r, out);
+                        }
+                    }
+                }
+            }
+        }
+    } else if (p.collider) {
+        T rel[3], nrm[3], f[3] = {0, 0, 0};
         for (int k = 0; k < 3; ++k) rel[k] = (T)p.coll_pos[k] - xi[k];
         const T dist = std::sqrt((rel[0] * rel[0] + rel[1] * rel[1]) + rel[2] * rel[2]);
         const T rad = (T)p.particle_r + (T)p.coll_r;
         if (dist < rad) {
-            for (int k = 0; k < 3; ++k) u[k] = -vi[k];
             const T inv = (T)1 / dist;
             for (int k = 0; k < 3; ++k) nrm[k] = rel[k] * inv;
             const T d = (u[0] * nrm[0] + u[1] * nrm[1]) + u[2] * nrm[2];
             const T spring = (T)p.coll_spring * (dist - rad);
             for (int k = 0; k < 3; ++k) {
                 const T tan = u[k] - d * nrm[k];
-                T f = spring * nrm[k];
-                f = f + (T)p.coll_damping * u[k];
-                f = f + (T)p.coll_shear * tan;
-                out[k] = out[k] + f;
+                f[k] = (spring * nrm[k] + (T)p.coll_damping * u[k]) + (T)p.coll_shear * tan;
             }
         }
+        for (int k = 0; k < 3; ++k) out[k] = out[k] + f[k];
     }
-    for (int a = 0; a < SPH_NUM_ACCEL; ++a) {
+    for (int a = 0; a < SPH_NUM_ACCEL; ++a) {  // :379-399
         const sph_accel& ac = p.acc[a];
         if (ac.type == SPH_ACC_OFF) continue;
         T rel[3], size[3], acc[3];
@@ -601,10 +717,17 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
 }
 
 // App::UpdateEmitter runs before every cSPH::Update (source/App/Render.cpp:10-11)
-// and advances the wave phase: rAngle += rVel * timeStep (source/App/Update.cpp:15-16).
+// and advances the rotor angle, which is also the wave phase:
+// rAngle += rVel * timeStep (source/App/Update.cpp:15-16).
 inline sph_params advanced(const sph_params& p) {
     sph_params q = p;
-    if (q.wave_speed != 0.0f) q.wave_phase = q.wave_phase + q.wave_speed * q.time_step;
+    if (q.rot_vel != 0.0f) {
+        q.rot_angle = q.rot_angle + q.rot_vel * q.time_step;
+        if (q.rot2_dist > 0.0f) {  // the second propeller (source/App/Update.cpp:17-20)
+            q.rot2_angle = q.rot2_angle + (q.rot_vel * q.rot2_vel) * q.time_step;
+            q.rot2_twist = q.rot2_vel < 0.0f ? -1.0f : 1.0f;
+        }
+    }
     return q;
 }
 
@@ -629,9 +752,10 @@ void oracle_set_num_threads(int n) {
 #endif
 }
 
-// Every step entry point below first advances wave_phase by
-// wave_speed * time_step (no-op for a still scene), as each library step
-// does; oracle_run_f32 carries the phase over its steps.  The stage
+// Every step entry point below first advances rot_angle by
+// rot_vel * time_step (and the second propeller's angle; no-op for a still
+// scene), as each library step does; oracle_run_f32 carries the angles over
+// its steps.  The stage
 // functions further down use the parameters as given.
 //
 // One FP32 step.  pos4/vel4 are updated in place (original order); every

@@ -13,7 +13,8 @@ void derive_shape(sph_params& p);
 // SPH_ERR_INVALID_PARAMS for an unknown boundary type, effect or accelerator.
 int check_shape(const sph_params& p);
 
-// The per-step phase advance of a moving wave (no-op when wave_speed == 0).
-void advance_wave(sph_params& p);
+// The per-step advance of the rotor angle (the wave phase) and of the second
+// propeller's angle (no-op when rot_vel == 0).
+void advance_motion(sph_params& p);
 
 }  // namespace sph

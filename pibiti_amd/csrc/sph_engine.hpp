@@ -203,8 +203,9 @@ struct Engine {
         batch = as_batch;
         scenes.assign(in, in + nscenes);
         for (auto& sc : scenes) {
-            // a moving wave needs new parameters every step: single scenes only
-            if (as_batch && sc.wave_speed != 0.0f) return SPH_ERR_UNSUPPORTED;
+            // a turning rotor or moving wave needs new parameters every step:
+            // single scenes only
+            if (as_batch && sc.rot_vel != 0.0f) return SPH_ERR_UNSUPPORTED;
             derive_shape(sc);
         }
         scene_off.assign(nscenes + 1, 0);
@@ -239,6 +240,8 @@ struct Engine {
         cfg.precision = scenes[0].precision;
         for (int k = 0; k < 4; ++k) cfg.tune[k] = scenes[0].tune[k];
         cfg.one = scenes[0];
+        cfg.physics = 0;
+        for (const auto& sc : scenes) cfg.physics |= sc.physics;
         have_pos.assign(nscenes, 0);
         have_vel.assign(nscenes, 0);
         batch = as_batch;
@@ -355,10 +358,10 @@ struct Engine {
         DeviceGuard guard(device);
         for (uint32_t s = 0; s < nsteps; ++s) {
             const bool tm = timing && s + 1 == nsteps;
-            if (!batch && scenes[0].wave_speed != 0.0f) {
-                // the moving wave's phase advances before every step; the
+            if (!batch && scenes[0].rot_vel != 0.0f) {
+                // the rotor angle (wave phase) advances before every step; the
                 // kernels take the single scene's parameters by value
-                advance_wave(scenes[0]);
+                advance_motion(scenes[0]);
                 cfg.one = scenes[0];
             }
             SPH_TRY_ST(enqueue_step(tm));
@@ -377,8 +380,8 @@ struct Engine {
     int capture(uint32_t nsteps) {
         if (nsteps == 0 || (nsteps & 1u)) return SPH_ERR_INVALID_ARGUMENT;
         if (!ready()) return SPH_ERR_NOT_INITIALISED;
-        // a recording replays fixed parameters: no moving wave
-        if (scenes[0].wave_speed != 0.0f) return SPH_ERR_UNSUPPORTED;
+        // a recording replays fixed parameters: no turning rotor or moving wave
+        if (scenes[0].rot_vel != 0.0f) return SPH_ERR_UNSUPPORTED;
         DeviceGuard guard(device);
         drop_graph();
         SPH_TRY(hipStreamSynchronize(stream));
@@ -560,7 +563,7 @@ struct Engine {
             return SPH_ERR_INVALID_PARAMS;
         if (!batch && ((1ull << p->key_bits) < cfg.num_cells || p->key_bits > 31))
             return SPH_ERR_INVALID_PARAMS;
-        if (batch && p->wave_speed != 0.0f) return SPH_ERR_UNSUPPORTED;
+        if (batch && p->rot_vel != 0.0f) return SPH_ERR_UNSUPPORTED;
         DeviceGuard guard(device);
         if ((p->cfl > 0.0f || p->t_end > 0.0f) && !device_clock) {
             // from here on the clock and time step advance on the device
@@ -579,6 +582,8 @@ struct Engine {
         for (auto& s : scenes) cfg.max_cap = std::max(cfg.max_cap, s.max_par_in_cell);
         for (int k = 0; k < 4; ++k) cfg.tune[k] = scenes[0].tune[k];
         cfg.one = scenes[0];
+        cfg.physics = 0;
+        for (const auto& sc : scenes) cfg.physics |= sc.physics;
         if (batch)
             SPH_TRY(hipMemcpyAsync(d_tab + scene, &cur, sizeof(sph_params), hipMemcpyHostToDevice, stream));
         SPH_TRY(hipMemcpyAsync(dt_dev + scene, &cur.time_step, sizeof(float), hipMemcpyHostToDevice, stream));

@@ -99,6 +99,7 @@ struct LaunchCfg {
     uint32_t max_scene_n = 0;   // largest scene
     uint32_t precision = 0;     // shared by every scene
     uint32_t tune[4] = {0, 0, 0, 0};
+    uint32_t physics = 0;       // union of the scenes' sph_params::physics bits
     const sph_params* tab = nullptr;      // device table (batch) or null
     const uint32_t* scene_off = nullptr;  // device: first slot of each scene, nscenes + 1
     sph_params one{};                     // the scene (one-scene launches)

@@ -70,7 +70,7 @@ __device__ __forceinline__ void boundary(const sph_params& p, float dt, float& x
                cycle = eff == SPH_BND_EFF_CYCLE;
     if (wave) {
         // sloped floor, then the moving z- wall
-        const float sl = -p.wave_slope;
+        const float sl = -p.rot2_angle;
         bnd_wall((b - (y - wmin_y)) - (z - wmin_z) * sl, 0.0f, 1.0f - sl, sl, stiff, damp, dt, vx, vy, vz);
         wmin_z = wmin_z + p.wave_zmin_shift;
     }
@@ -112,7 +112,7 @@ __device__ __forceinline__ void boundary(const sph_params& p, float dt, float& x
             z = z + span;
     }
     if (t == SPH_BND_PUMP_Y) {
-        const float rad = wmax_x, ang = p.pump_ang_out, hc = p.pump_h_close;
+        const float rad = wmax_x, ang = p.pump_ang_out, hc = p.h_close;
         const float rin = rad * p.pump_rad_in;
         const float len = sqrt_ieee(x * x + y * y);
         float diff = (b + len) - rad;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void boundary(const sph_params& p, float dt, float& x
             // frame: the cylinder wall except over the outlet
             const float a = atan_rn(x / y);
             const bool wall = ang < 0.5f ? (a < -ang || a > ang || y < 0.0f)
-                                         : (y < 0.0f || (len < rad * p.pump_s[4] && a < ang));
+                                         : (y < 0.0f || (len < rad * p.shape_s[4] && a < ang));
             if (wall) bnd_wall(diff, -x / len, -y / len, 0.0f, stiff, damp, dt, vx, vy, vz);
         }
         const float xs = p.pump_xs;
@@ -129,15 +129,15 @@ __device__ __forceinline__ void boundary(const sph_params& p, float dt, float& x
                 bnd_wall((b - x) - xs, 1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
                 bnd_wall((b + x) - xs, -1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
             }
-        } else if (len >= rad * p.pump_s[5]) {
+        } else if (len >= rad * p.shape_s[5]) {
             bnd_wall((b - x) + xs, 1.0f, 0.0f, 0.0f, stiff, damp, dt, vx, vy, vz);
         }
         // inlet hole (closed above) and the lid
-        if (z > hc - b * p.pump_s[0]) {
+        if (z > hc - b * p.shape_s[0]) {
             diff = (b + len) - rin;
             if (diff > kBndEps) bnd_wall(diff, -x / len, -y / len, 0.0f, stiff, damp, dt, vx, vy, vz);
         }
-        if (z < hc - b * p.pump_s[1]) bnd_wall((b + z) - hc, 0.0f, 0.0f, -1.0f, stiff, damp, dt, vx, vy, vz);
+        if (z < hc - b * p.shape_s[1]) bnd_wall((b + z) - hc, 0.0f, 0.0f, -1.0f, stiff, damp, dt, vx, vy, vz);
         // outlet -> inlet: map the outlet rectangle onto the inlet circle
         diff = (y - wmax_y) + p.exit_dist;
         if (diff > kBndEps && vy > p.exit_vel) {
@@ -166,33 +166,139 @@ __device__ __forceinline__ void boundary(const sph_params& p, float dt, float& x
     }
 }
 
-// Terms computeForceD adds after the pair sum (source/CUDA/System.cu:374-399):
-// the DEM sphere collider (collideSpheres, source/CUDA/Kernel_Cell.cui:78-96)
-// and the accelerators, in that order, to the velocity increment a.
+// collideSpheresR (source/CUDA/Kernel_Cell.cui:101-115): the spring and
+// damping force of one rotor or height-map sphere at offset r from the
+// particle (u = -v), added to a (zero outside the sphere's reach).
+__device__ __forceinline__ void collide_r(const sph_params& p, float rx, float ry, float rz, float ux,
+                                          float uy, float uz, float rad, float& ax, float& ay, float& az) {
+    const float dist = sqrt_ieee((rx * rx + ry * ry) + rz * rz);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (dist < rad) {
+        const float inv = 1.0f / dist;  // float3 / float
+        const float sp = p.coll_spring * (dist - rad);
+        fx = sp * (rx * inv) + p.coll_damping * ux;
+        fy = sp * (ry * inv) + p.coll_damping * uy;
+        fz = sp * (rz * inv) + p.coll_damping * uz;
+    }
+    ax = ax + fx;
+    ay = ay + fy;
+    az = az + fz;
+}
+
+constexpr float kPiRef = 3.141592654f;  // PI of source/CUDA/Params.cuh:18
+
+// The terms computeForceD adds to the velocity increment after the pair sum
+// (source/CUDA/System.cu:253-399), in its order: the height map, then the
+// rotor or, without one, the sphere collider (collideSpheres,
+// source/CUDA/Kernel_Cell.cui:78-96), then the accelerators.  Every branch
+// is uniform over a scene.
 __device__ __forceinline__ void extra_dv(const sph_params& p, float dt, const float4& xi,
                                          const float4& vi, float& ax, float& ay, float& az) {
-    if (p.physics & SPH_PHYS_COLLIDER) {
+    const float ux = vi.x * -1.0f, uy = vi.y * -1.0f, uz = vi.z * -1.0f;
+    if (p.physics & SPH_PHYS_HMAP) {
+        // a sheet of spheres on the rot_spc lattice of the x (or y) and z
+        // axes, 5 x 5 around the particle's own site, two layers deep
+        const bool xz = p.hmap_type == SPH_HMAP_XZ;
+        const float rr = p.particle_r + p.rot_r, spc = p.rot_spc;
+        const float xa = xz ? xi.x : xi.y, wa = xz ? p.world_min[0] : p.world_min[1];
+        const float wl = xz ? p.world_min[1] : p.world_min[0];
+        const float sa = xz ? p.world_size_d[0] : p.world_size_d[1];
+        const int ia = (int)((xa - wa) / spc);
+        const int iz = (int)((xi.z - p.world_min[2]) / spc);
+        const float af = (float)ia * spc + wa, zf = (float)iz * spc + p.world_min[2];
+        const float ph1 = (p.shape_s[1] * kPiRef) / 180.0f, ph3 = (p.shape_s[3] * kPiRef) / 180.0f;
+        for (int j = -2; j <= 2; ++j)
+            for (int i = -2; i <= 2; ++i) {
+                const float ah = af + (float)i * spc, zh = zf + (float)j * spc;
+                const float an = ((ah / sa) * 2.0f) * kPiRef, zn = ((zh / p.world_size_d[2]) * 2.0f) * kPiRef;
+                const float ss = sin_rn(p.shape_s[0] * an + ph1) * sin_rn(p.shape_s[2] * zn + ph3);
+                if (!(p.shape_s[4] <= -1.0f || ss > p.shape_s[4])) continue;
+                const float lf = ((ss * p.h_close) + p.rot2_angle) + wl;
+                for (int k = 0; k <= 1; ++k) {
+                    const float lh = lf - (float)k * spc;
+                    if (xz)
+                        collide_r(p, ah - xi.x, lh - xi.y, zh - xi.z, ux, uy, uz, rr, ax, ay, az);
+                    else
+                        collide_r(p, lh - xi.x, ah - xi.y, zh - xi.z, ux, uy, uz, rr, ax, ay, az);
+                }
+            }
+    }
+    if (p.physics & SPH_PHYS_ROTOR) {
+        const int sx = p.rot_size[0], sy = p.rot_size[1], sz = p.rot_size[2], cb = p.rot_blades;
+        const float sp = p.rot_spc, ca = (2.0f * kPiRef) / (float)cb;
+        const float x2 = (float)sx * 0.5f, y2 = (float)sy * 0.5f;
+        const float rr = p.particle_r + p.rot_r;
+        const float cx = p.coll_pos[0], cy = p.coll_pos[1], cz = p.coll_pos[2];
+        if (p.rot_type != SPH_ROTOR_PROPELLER_Z) {
+            // pump rotor: blades of spheres along z (type 1) or y (type 2)
+            const bool zax = p.rot_type == SPH_ROTOR_PUMP_Z;
+            const float q = zax ? xi.z : xi.y, c0 = zax ? cz : cy;
+            if (q > c0 - rr && q < (c0 + rr) + sp * (float)sy) {
+                for (int z = 1; z <= sz; ++z)
+                    for (int x = 0; x <= sx; ++x)
+                        for (int c = 0; c < cb; ++c) {
+                            const float a = ((-p.rot_angle) + (float)c * ca) + (float)z * p.rot_twist;
+                            const float cs = cos_rn(a) * sp, sn = (-sin_rn(a)) * sp;
+                            const float u = (float)x - x2;
+                            const float pa = u * cs - (float)z * sn, pb = u * sn + (float)z * cs;
+                            for (int y = 0; y <= sy; ++y) {
+                                const float py = sp * (float)y;
+                                const float qx = cx + pa, qy = cy + (zax ? pb : py), qz = cz + (zax ? py : pb);
+                                collide_r(p, qx - xi.x, qy - xi.y, qz - xi.z, ux, uy, uz, rr, ax, ay, az);
+                            }
+                        }
+            }
+        } else {
+            // propeller about z (two of them rot2_dist apart when set)
+            const float half = (sp * (float)sz) / 2.0f;
+            if (xi.z > (cz - rr) - half && xi.z < (cz + rr) + half) {
+                float px0 = cx, aa = p.rot_angle, tw = p.rot_twist;
+                if (p.rot2_dist > 0.0f) {
+                    if (xi.x > 0.0f) {
+                        px0 = px0 + p.rot2_dist * 0.5f;
+                    } else {
+                        px0 = px0 - p.rot2_dist * 0.5f;
+                        aa = p.rot2_angle;
+                        tw = tw * p.rot2_twist;
+                    }
+                }
+                for (int c = 0; c < cb; ++c) {
+                    const float a = aa + (float)c * ca;
+                    for (int h = 0; h <= sy; ++h) {
+                        const float dh = h == sy - 1 ? 0.5f : (h == sy ? 1.2f : 0.0f);
+                        for (int x = 0; x <= sz; ++x) {
+                            const float d = (x == sz && dh == 0.0f) ? 0.4f : dh;
+                            const float k = cos_rn(((float)x - x2) * 0.2f + kPiRef * 0.6f);
+                            const float at = tw * (d * -0.05f - ((float)h - y2) * k);
+                            const float ac = a + at;
+                            const float cs = cos_rn(ac) * sp, sn = (-sin_rn(ac)) * sp;
+                            const float pz = (sp * (((-d) + (float)h) - y2)) * k;
+                            const float r = p.rot_r * fabsf(1.0f - d);
+                            collide_r(p, (px0 + (float)x * cs) - xi.x, (cy + (float)x * sn) - xi.y,
+                                      (cz + pz) - xi.z, ux, uy, uz, p.particle_r + r, ax, ay, az);
+                        }
+                    }
+                }
+            }
+        }
+    } else if (p.physics & SPH_PHYS_COLLIDER) {
         const float rx = p.coll_pos[0] - xi.x, ry = p.coll_pos[1] - xi.y, rz = p.coll_pos[2] - xi.z;
         const float dist = sqrt_ieee((rx * rx + ry * ry) + rz * rz);
         const float rad = p.particle_r + p.coll_r;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         if (dist < rad) {
-            const float ux = -vi.x, uy = -vi.y, uz = -vi.z;  // relative velocity
-            const float inv = 1.0f / dist;                    // float3 / float
+            const float inv = 1.0f / dist;  // float3 / float
             const float nx = rx * inv, ny = ry * inv, nz = rz * inv;
             const float d = (ux * nx + uy * ny) + uz * nz;
             const float tx = ux - d * nx, ty = uy - d * ny, tz = uz - d * nz;
             const float sp = p.coll_spring * (dist - rad);
-            float fx = sp * nx, fy = sp * ny, fz = sp * nz;
-            fx = fx + p.coll_damping * ux;
-            fy = fy + p.coll_damping * uy;
-            fz = fz + p.coll_damping * uz;
-            fx = fx + p.coll_shear * tx;
-            fy = fy + p.coll_shear * ty;
-            fz = fz + p.coll_shear * tz;
-            ax = ax + fx;
-            ay = ay + fy;
-            az = az + fz;
+            fx = (sp * nx + p.coll_damping * ux) + p.coll_shear * tx;
+            fy = (sp * ny + p.coll_damping * uy) + p.coll_shear * ty;
+            fz = (sp * nz + p.coll_damping * uz) + p.coll_shear * tz;
         }
+        ax = ax + fx;
+        ay = ay + fy;
+        az = az + fz;
     }
     if (p.physics & SPH_PHYS_ACCEL) {
         for (int k = 0; k < SPH_NUM_ACCEL; ++k) {
@@ -223,6 +329,31 @@ __device__ __forceinline__ void extra_dv(const sph_params& p, float dt, const fl
             }
         }
     }
+}
+
+// The rest of computeForceD's velocity update for scenes with a height map,
+// rotor, collider or accelerators: vel_out holds the pair increment the
+// force kernel left there; add the other terms in the reference's order and
+// store v + increment.  A kernel of its own so that the force kernels keep
+// their register budget.
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_extra_terms(sph_params one, const sph_params* __restrict__ tab,
+                                                        const uint32_t* __restrict__ sid, uint32_t n,
+                                                        const float4* __restrict__ pos,
+                                                        const float4* __restrict__ vel,
+                                                        float4* __restrict__ vel_out,
+                                                        float4* __restrict__ dvel,
+                                                        const float* __restrict__ dt_dev) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
+    if (!p.physics) return;
+    const float4 xi = pos[i], vi = vel[i], a = vel_out[i];
+    float ax = a.x, ay = a.y, az = a.z;
+    extra_dv(p, dt_dev[scene], xi, vi, ax, ay, az);
+    vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
+    if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
 
 // ------------------------------------------------------------ integrate+hash
@@ -695,12 +826,16 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
         ay = a.ay + a.fy;
         az = a.az + a.fz;
     }
-    const float dt = dt_dev[scene];
-    const float mdt = p.particle_mass * dt;
+    const float mdt = p.particle_mass * dt_dev[scene];
     ax = ax * mdt;
     ay = ay * mdt;
     az = az * mdt;
-    if (p.physics) extra_dv(p, dt, xi, vi, ax, ay, az);
+    if (p.physics) {
+        // the pair increment only: k_extra_terms adds the scene's other
+        // terms and finishes the update
+        vel_out[i] = make_float4(ax, ay, az, 0.0f);
+        return;
+    }
     vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
     if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
@@ -978,12 +1113,16 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
         ay = ky.s;
         az = kz.s;
     }
-    const float dt = dt_dev[scene];
-    const float mdt = p.particle_mass * dt;
+    const float mdt = p.particle_mass * dt_dev[scene];
     ax = ax * mdt;
     ay = ay * mdt;
     az = az * mdt;
-    if (p.physics) extra_dv(p, dt, xi, vi, ax, ay, az);
+    if (p.physics) {
+        // the pair increment only: k_extra_terms adds the scene's other
+        // terms and finishes the update
+        vel_out[i] = make_float4(ax, ay, az, 0.0f);
+        return;
+    }
     vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
     if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
@@ -1340,6 +1479,15 @@ hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_ou
             force_impl<false, SPH_PREC_KAHAN>(c, b, vel_out, dvel, s);
         else
             force_impl<false, SPH_PREC_REFERENCE>(c, b, vel_out, dvel, s);
+    }
+    if (c.physics) {
+        const dim3 grid(blocks_for(c.n, kBlock));
+        if (c.tab)
+            hipLaunchKernelGGL(k_extra_terms<true>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n,
+                               b.pos_cur, b.vel_cur, vel_out, dvel, b.dt_dev);
+        else
+            hipLaunchKernelGGL(k_extra_terms<false>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n,
+                               b.pos_cur, b.vel_cur, vel_out, dvel, b.dt_dev);
     }
     return hipGetLastError();
 }

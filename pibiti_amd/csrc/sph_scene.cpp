@@ -56,17 +56,25 @@ namespace sph {
 
 void derive_shape(sph_params& p) {
     // wmin.z += rTwist*(1 + sinf(rAngle))  (source/CUDA/System.cu:60)
-    p.wave_zmin_shift = p.wave_ampl * (1.0f + sin_rn(p.wave_phase));
+    p.wave_zmin_shift = p.rot_twist * (1.0f + sin_rn(p.rot_angle));
     // pump outlet (source/CUDA/System.cu:104-117): rad = worldMax.x
     const float rad = p.world_max[0], ang = p.pump_ang_out;
     if (ang < 0.5f) {
-        p.pump_xs = sin_rn(ang * p.pump_s[2]) * rad;
-        p.pump_zs = (cos_rn(ang * p.pump_s[2]) * rad) * p.pump_s[3];
+        p.pump_xs = sin_rn(ang * p.shape_s[2]) * rad;
+        p.pump_zs = (cos_rn(ang * p.shape_s[2]) * rad) * p.shape_s[3];
     } else {
-        p.pump_xs = 0.09f * p.pump_s[3];
+        p.pump_xs = 0.09f * p.shape_s[3];
         p.pump_zs = 0.0f;
     }
-    uint32_t phys = p.collider ? (uint32_t)SPH_PHYS_COLLIDER : 0u;
+    // worldSizeD (source/SPH/Scene.cpp:99-103): b = soft band - r per side
+    const float bd = p.dist_bnd_soft - p.particle_r;
+    for (int c = 0; c < 3; ++c) p.world_size_d[c] = (p.world_max[c] - bd) - (p.world_min[c] + bd);
+    uint32_t phys = 0;
+    if (p.rot_type > 0)
+        phys |= SPH_PHYS_ROTOR;
+    else if (p.collider)
+        phys |= SPH_PHYS_COLLIDER;
+    if (p.hmap_type > 0) phys |= SPH_PHYS_HMAP;
     for (int k = 0; k < SPH_NUM_ACCEL; ++k)
         if (p.acc[k].type != SPH_ACC_OFF) phys |= SPH_PHYS_ACCEL;
     p.physics = phys;
@@ -77,14 +85,28 @@ int check_shape(const sph_params& p) {
         return SPH_ERR_INVALID_PARAMS;
     for (int k = 0; k < SPH_NUM_ACCEL; ++k)
         if (p.acc[k].type > SPH_ACC_CYL_Y_SMOOTH) return SPH_ERR_INVALID_PARAMS;
+    if (p.rot_type < 0 || p.rot_type > SPH_ROTOR_PROPELLER_Z || p.hmap_type < 0 || p.hmap_type > SPH_HMAP_YZ)
+        return SPH_ERR_INVALID_PARAMS;
+    if (p.rot_type > 0) {
+        // the rotor's loops run over these counts: bound them
+        if (p.rot_blades < 1 || p.rot_blades > 64) return SPH_ERR_INVALID_PARAMS;
+        for (int c = 0; c < 3; ++c)
+            if (p.rot_size[c] < 0 || p.rot_size[c] > 256) return SPH_ERR_INVALID_PARAMS;
+    }
+    if (p.hmap_type > 0 && !(p.rot_spc > 0.0f)) return SPH_ERR_INVALID_PARAMS;
     return SPH_OK;
 }
 
-void advance_wave(sph_params& p) {
-    // App::UpdateEmitter before each cSPH::Update (source/App/Update.cpp:15-16,
-    // source/App/Render.cpp:10-11): rAngle += rVel * timeStep.
-    if (p.wave_speed != 0.0f) {
-        p.wave_phase = p.wave_phase + p.wave_speed * p.time_step;
+void advance_motion(sph_params& p) {
+    // App::UpdateEmitter before each cSPH::Update (source/App/Update.cpp:15-20,
+    // source/App/Render.cpp:10-11): the rotor angle (also the wave phase)
+    // advances by rVel dt, the second propeller's by rVel r2Vel dt.
+    if (p.rot_vel != 0.0f) {
+        p.rot_angle = p.rot_angle + p.rot_vel * p.time_step;
+        if (p.rot2_dist > 0.0f) {
+            p.rot2_angle = p.rot2_angle + (p.rot_vel * p.rot2_vel) * p.time_step;
+            p.rot2_twist = p.rot2_vel < 0.0f ? -1.0f : 1.0f;
+        }
         derive_shape(p);
     }
 }
@@ -146,17 +168,17 @@ int sph_scene_default(sph_scene* s) {
     // source/SPH/Scene.cpp:31-48 (boundary, collider, pump, rotor twist)
     s->bnd_type = SPH_BND_BOX;
     s->bnd_eff_z = SPH_BND_EFF_NONE;
-    s->wave_slope = 0.0f;
-    s->wave_ampl = 0.04f;
-    s->wave_phase = 0.0f;
-    s->wave_speed = 0.0f;
+    s->rot2_angle = 0.0f;
+    s->rot_twist = 0.04f;
+    s->rot_angle = 0.0f;
+    s->rot_vel = 0.0f;
     s->exit_vel = 0.03f;
     s->exit_dist = 0.04f;
     s->pump_ang_out = 0.7f;
-    s->pump_h_close = -0.03f;
+    s->h_close = -0.03f;
     s->pump_rad_in = 0.4f;
     const float ps[6] = {0.90f, 0.90f, 1.198f, 0.7782f, 0.8696f, 0.843f};
-    for (int k = 0; k < 6; ++k) s->pump_s[k] = ps[k];
+    for (int k = 0; k < 6; ++k) s->shape_s[k] = ps[k];
     // The collider's values are the reference's, but it starts switched off:
     // the hot path's workloads run without it.  The scene-file loader turns
     // it on, as the reference applies it in every scene without a rotor.
@@ -166,6 +188,18 @@ int sph_scene_default(sph_scene* s) {
     s->coll_spring = 80.0f;
     s->coll_damping = 0.02f;
     s->coll_shear = 0.1f;
+    // rotor (source/SPH/Scene.cpp:40-48)
+    s->rot_type = SPH_ROTOR_NONE;
+    s->rot_size[0] = 0;
+    s->rot_size[1] = 4;
+    s->rot_size[2] = 12;
+    s->rot_blades = 4;
+    s->rot_r = 3.0f;
+    s->rot_spc = 0.7f;
+    s->rot2_dist = 0.0f;
+    s->rot2_vel = 1.0f;
+    s->rot2_twist = 1.0f;
+    s->hmap_type = SPH_HMAP_NONE;
     // source/SPH/Scene.cpp:54-57
     for (int k = 0; k < SPH_NUM_ACCEL; ++k) {
         sph_accel& a = s->acc[k];
@@ -221,22 +255,31 @@ int sph_params_from_scene(const sph_scene* s, sph_params* p) {
     p->bnd_damp_c = s->bnd_damp_c;
     p->bnd_type = s->bnd_type;
     p->bnd_eff_z = s->bnd_eff_z;
-    p->wave_slope = s->wave_slope;
-    p->wave_ampl = s->wave_ampl;
-    p->wave_phase = s->wave_phase;
-    p->wave_speed = s->wave_speed;
+    p->rot2_angle = s->rot2_angle;
+    p->rot_twist = s->rot_twist;
+    p->rot_angle = s->rot_angle;
+    p->rot_vel = s->rot_vel;
     p->exit_vel = s->exit_vel;
     p->exit_dist = s->exit_dist;
     p->pump_ang_out = s->pump_ang_out;
-    p->pump_h_close = s->pump_h_close;
+    p->h_close = s->h_close;
     p->pump_rad_in = s->pump_rad_in;
-    for (int k = 0; k < 6; ++k) p->pump_s[k] = s->pump_s[k];
+    for (int k = 0; k < 6; ++k) p->shape_s[k] = s->shape_s[k];
     p->collider = s->collider;
     p->coll_r = s->coll_r;
     p->coll_spring = s->coll_spring;
     p->coll_damping = s->coll_damping;
     p->coll_shear = s->coll_shear;
     for (int k = 0; k < SPH_NUM_ACCEL; ++k) p->acc[k] = s->acc[k];
+    p->rot_type = s->rot_type;
+    for (int c = 0; c < 3; ++c) p->rot_size[c] = s->rot_size[c];
+    p->rot_blades = s->rot_blades;
+    p->rot_r = s->rot_r * r;               // rotR *= particleR
+    p->rot_spc = s->rot_spc * p->rot_r;    // rotSpc *= rotR (source/SPH/Scene.cpp:93)
+    p->rot2_dist = s->rot2_dist;
+    p->rot2_vel = s->rot2_vel;
+    p->rot2_twist = s->rot2_twist;
+    p->hmap_type = s->hmap_type;
 
     // Scene::_UpdateGrid (source/SPH/Scene.cpp:98-109).
     uint64_t cells = 1;

@@ -239,11 +239,7 @@ constexpr float kPi = 3.141592654f;  // source/CUDA/Params.cuh:18
 struct SceneVals {
     sph_scene s{};
     sph_scene_info info{};
-    float rot_vel = 0.0f, rot2_vel = 1.0f, r_angle = 0.0f, r_twist = 0.04f, r2_angle = 0.0f;
-    float h_close = -0.03f;
-    float pump_s[6] = {0.90f, 0.90f, 1.198f, 0.7782f, 0.8696f, 0.843f};
     float coll_pos[3] = {-77.0f, 0.0f, 0.0f};
-    int rot_type = 0, hmap = 0;
 };
 
 void defaults(SceneVals& v) {
@@ -367,41 +363,50 @@ void from_xml(const Node& n, SceneVals& v) {
     A("bndDampC") s.bnd_damp_c = to_float(a);
     A("bndType") s.bnd_type = (uint32_t)to_int(a);
     A("bndEffZ") s.bnd_eff_z = (uint32_t)to_int(a);
-    A("HmapType") v.hmap = to_int(a);
+    A("HmapType") s.hmap_type = to_int(a);
     // pump (:92-94)
     A("PumpAngOut") s.pump_ang_out = to_float(a);
-    A("PumpHClose") v.h_close = to_float(a);
+    A("PumpHClose") s.h_close = to_float(a);
     A("PumpRadIn") s.pump_rad_in = to_float(a);
     A("ExitVel") s.exit_vel = to_float(a);
     A("ExitDist") s.exit_dist = to_float(a);
-    A("s1") v.pump_s[0] = to_float(a);
-    A("s2") v.pump_s[1] = to_float(a);
-    A("s3") v.pump_s[2] = to_float(a);
-    A("s4") v.pump_s[3] = to_float(a);
-    A("s5") v.pump_s[4] = to_float(a);
-    A("s6") v.pump_s[5] = to_float(a);
+    A("s1") s.shape_s[0] = to_float(a);
+    A("s2") s.shape_s[1] = to_float(a);
+    A("s3") s.shape_s[2] = to_float(a);
+    A("s4") s.shape_s[3] = to_float(a);
+    A("s5") s.shape_s[4] = to_float(a);
+    A("s6") s.shape_s[5] = to_float(a);
     // rotor (:96-103)
-    A("RotorType") v.rot_type = to_int(a);
-    A("RotorVel") v.rot_vel = to_float(a);
-    A("Rotor2Vel") v.rot2_vel = to_float(a);
-    A("RotorAngle") v.r_angle = to_float(a) * kPi / 180.0f;
-    A("Rotor2Angle") v.r2_angle = to_float(a) * kPi / 180.0f;
-    A("RotorTwist") v.r_twist = to_float(a);
+    A("RotorSizes") {
+        float r3[3];
+        to_vec3(a, r3);
+        for (int c = 0; c < 3; ++c) s.rot_size[c] = (int32_t)r3[c];
+    }
+    A("RotorType") s.rot_type = to_int(a);
+    A("Rotor2Dist") s.rot2_dist = to_float(a);
+    A("RotorVel") s.rot_vel = to_float(a);
+    A("Rotor2Vel") s.rot2_vel = to_float(a);
+    A("RotorAngle") s.rot_angle = to_float(a) * kPi / 180.0f;
+    A("Rotor2Angle") s.rot2_angle = to_float(a) * kPi / 180.0f;
+    A("colParR") s.rot_r = to_float(a);
+    A("colParSpc") s.rot_spc = to_float(a);
+    A("RotorBlades") s.rot_blades = to_int(a);
+    A("RotorTwist") s.rot_twist = to_float(a);
     // waves, aliases of rotor values (:106-108)
     A("WaveSpeed") {
-        v.rot_vel = to_float(a);
-        v.r_angle = -kPi / 2.0f;
+        s.rot_vel = to_float(a);
+        s.rot_angle = -kPi / 2.0f;
     }
-    A("WaveAmpl") v.r_twist = to_float(a);
-    A("HSlope") v.r2_angle = to_float(a);
+    A("WaveAmpl") s.rot_twist = to_float(a);
+    A("HSlope") s.rot2_angle = to_float(a);
     // height map, aliases again (:111-113)
-    A("Hheight") v.r2_angle = to_float(a);
-    A("Hscale") v.h_close = to_float(a);
-    A("HxFq") v.pump_s[0] = to_float(a);
-    A("HxOfs") v.pump_s[1] = to_float(a);
-    A("HzFq") v.pump_s[2] = to_float(a);
-    A("HzOfs") v.pump_s[3] = to_float(a);
-    A("HholeR") v.pump_s[4] = to_float(a);
+    A("Hheight") s.rot2_angle = to_float(a);
+    A("Hscale") s.h_close = to_float(a);
+    A("HxFq") s.shape_s[0] = to_float(a);
+    A("HxOfs") s.shape_s[1] = to_float(a);
+    A("HzFq") s.shape_s[2] = to_float(a);
+    A("HzOfs") s.shape_s[3] = to_float(a);
+    A("HholeR") s.shape_s[4] = to_float(a);
 #undef A
 }
 
@@ -410,21 +415,9 @@ void from_xml(const Node& n, SceneVals& v) {
 void finish(SceneVals& v) {
     sph_scene& s = v.s;
     sph_scene_info& in = v.info;
-    s.pump_h_close = v.h_close;
-    for (int k = 0; k < 6; ++k) s.pump_s[k] = v.pump_s[k];
-    s.wave_phase = v.r_angle;
-    s.wave_ampl = v.r_twist;
-    s.wave_slope = v.r2_angle;
-    // the phase advances every frame by rVel dt (source/App/Update.cpp:15-16);
-    // only the wave effect reads it here
-    s.wave_speed = s.bnd_eff_z == SPH_BND_EFF_WAVE ? v.rot_vel : 0.0f;
-    // computeForceD applies the sphere collider unless a rotor runs
-    // (source/CUDA/System.cu:310,373-375)
-    s.collider = v.rot_type > 0 ? 0u : 1u;
+    // the collider acts wherever no rotor runs (the force update decides)
+    s.collider = 1;
     for (int c = 0; c < 3; ++c) s.coll_pos[c] = v.coll_pos[c];
-    in.rotor_type = v.rot_type;
-    in.hmap_type = v.hmap;
-    in.unsupported = (v.rot_type > 0 ? (uint32_t)SPH_FEAT_ROTOR : 0u) | (v.hmap > 0 ? (uint32_t)SPH_FEAT_HMAP : 0u);
     // _UpdatePar: spacing *= particleR; _UpdateGrid: the init volume moves in
     // by distBndSoft*particleR - particleR on every side (Scene.cpp:80,100-102)
     const float r = s.particle_r;
@@ -559,9 +552,9 @@ int sph_scene_fill(const sph_scene* scene, const sph_scene_info* info, uint64_t 
     };
     const bool pump = scene->bnd_type == SPH_BND_PUMP_Y;
     // pump shape (:50-62): rad is the (moved) initMax.x
-    const float prad = info->init_max[0], hc = p.pump_h_close - b * 0.90f;
+    const float prad = info->init_max[0], hc = p.h_close - b * 0.90f;
     const float rin = prad * p.pump_rad_in - b / 2.0f;
-    const float pxs = (float)std::sin((double)(p.pump_ang_out * p.pump_s[2])) * prad - b / 2.0f - r;
+    const float pxs = (float)std::sin((double)(p.pump_ang_out * p.shape_s[2])) * prad - b / 2.0f - r;
     uint32_t i = 0;
     uint64_t sites_since_hit = 0, period = 0;
     bool counted = false;
@@ -570,13 +563,13 @@ int sph_scene_fill(const sph_scene* scene, const sph_scene_info* info, uint64_t 
         if (pump) {
             const float ang = p.pump_ang_out;
             keep = (ang < 0.5f && x[1] > 0.0f && x[2] < hc && x[0] < pxs && x[0] > -pxs) ||
-                   (ang >= 0.5f && x[1] > 0.0f && x[2] < hc && x[0] > 0.09f * p.pump_s[3]) ||
+                   (ang >= 0.5f && x[1] > 0.0f && x[2] < hc && x[0] > 0.09f * p.shape_s[3]) ||
                    std::sqrt(x[0] * x[0] + x[1] * x[1]) * 1.01f < (x[2] < hc ? prad : rin);
         } else {
             // outside the collider (unless a rotor uses its place), inside the shape (:64-71)
             const float d[3] = {x[0] - p.coll_pos[0], x[1] - p.coll_pos[1], x[2] - p.coll_pos[2]};
             const bool outside = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) > p.coll_r ||
-                                 info->rotor_type > 0;
+                                 scene->rot_type > 0;
             const uint32_t t = scene->bnd_type;
             keep = outside &&
                    (t == SPH_BND_BOX || info->init_type >= 9 || t == SPH_BND_CYL_YZ ||

@@ -35,7 +35,7 @@ u3 = C.c_uint32 * 3
 SPH_BND_EFF_NONE, SPH_BND_EFF_WRAP, SPH_BND_EFF_CYCLE, SPH_BND_EFF_WAVE = range(4)
 SPH_ACC_OFF, SPH_ACC_BOX, SPH_ACC_CYL_Y, SPH_ACC_CYL_Y_SMOOTH = range(4)
 SPH_NUM_ACCEL = 4
-SPH_PHYS_COLLIDER, SPH_PHYS_ACCEL = 1, 2
+SPH_PHYS_COLLIDER, SPH_PHYS_ACCEL, SPH_PHYS_ROTOR, SPH_PHYS_HMAP = 1, 2, 4, 8
 
 
 class Accel(C.Structure):
@@ -44,19 +44,31 @@ class Accel(C.Structure):
     _fields_ = [("pos", f3), ("size", f3), ("acc", f3), ("type", C.c_uint32)]
 
 
+_ROTOR_FIELDS = [
+    ("rot_type", C.c_int32),
+    ("rot_size", C.c_int32 * 3),
+    ("rot_blades", C.c_int32),
+    ("rot_r", C.c_float),
+    ("rot_spc", C.c_float),
+    ("rot2_dist", C.c_float),
+    ("rot2_vel", C.c_float),
+    ("rot2_twist", C.c_float),
+    ("hmap_type", C.c_int32),
+]
+
 _SHAPE_FIELDS = [
     ("bnd_type", C.c_uint32),
     ("bnd_eff_z", C.c_uint32),
-    ("wave_slope", C.c_float),
-    ("wave_ampl", C.c_float),
-    ("wave_phase", C.c_float),
-    ("wave_speed", C.c_float),
+    ("rot2_angle", C.c_float),
+    ("rot_twist", C.c_float),
+    ("rot_angle", C.c_float),
+    ("rot_vel", C.c_float),
     ("exit_vel", C.c_float),
     ("exit_dist", C.c_float),
     ("pump_ang_out", C.c_float),
-    ("pump_h_close", C.c_float),
+    ("h_close", C.c_float),
     ("pump_rad_in", C.c_float),
-    ("pump_s", C.c_float * 6),
+    ("shape_s", C.c_float * 6),
 ]
 
 
@@ -96,6 +108,7 @@ class Scene(C.Structure):
         ("coll_damping", C.c_float),
         ("coll_shear", C.c_float),
         ("acc", Accel * SPH_NUM_ACCEL),
+        *_ROTOR_FIELDS,
     ]
 
 
@@ -105,12 +118,13 @@ class Emitter(C.Structure):
     _fields_ = [("size", C.c_int32), ("size2", C.c_int32), ("pos", f3), ("rot", f3), ("vel", C.c_float)]
 
 
-SPH_FEAT_ROTOR, SPH_FEAT_HMAP = 1, 2
+SPH_ROTOR_NONE, SPH_ROTOR_PUMP_Z, SPH_ROTOR_PUMP_Y, SPH_ROTOR_PROPELLER_Z = range(4)
+SPH_HMAP_NONE, SPH_HMAP_XZ, SPH_HMAP_YZ = range(3)
 
 
 class SceneInfo(C.Structure):
     """sph_scene_info: a scene file's non-parameter values (init volume,
-    drops, emitters, camera, unsupported terms)."""
+    drops, emitters, camera)."""
 
     _fields_ = [
         ("title", C.c_char * 64),
@@ -126,10 +140,7 @@ class SceneInfo(C.Structure):
         ("emit", Emitter * 4),
         ("cam_pos", f3),
         ("cam_rot", f3),
-        ("rotor_type", C.c_int32),
-        ("hmap_type", C.c_int32),
-        ("unsupported", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 
@@ -180,6 +191,7 @@ class Params(C.Structure):
         ("wave_zmin_shift", C.c_float),
         ("pump_xs", C.c_float),
         ("pump_zs", C.c_float),
+        ("world_size_d", f3),
         ("collider", C.c_uint32),
         ("coll_pos", f3),
         ("coll_r", C.c_float),
@@ -187,6 +199,7 @@ class Params(C.Structure):
         ("coll_damping", C.c_float),
         ("coll_shear", C.c_float),
         ("acc", Accel * SPH_NUM_ACCEL),
+        *_ROTOR_FIELDS,
         ("physics", C.c_uint32),
         ("reserved", C.c_uint32 * 2),
     ]

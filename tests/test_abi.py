@@ -115,7 +115,7 @@ def test_batch_rejects_a_moving_wave():
     good = sph.params_from_scene(sph.scene_default())
     wave = good.copy()
     wave.bnd_eff_z = sph.SPH_BND_EFF_WAVE
-    wave.wave_speed = 6.0
+    wave.rot_vel = 6.0
     arr = (sph.Params * 2)(good, wave)
     out = C.c_void_p()
     assert L.sph_batch_create(arr, 2, 0, 0, C.byref(out)) in (3, 7)  # no device here, or unsupported

@@ -85,14 +85,25 @@ CASES = {
     "cylinders_yz": dict(bnd=sph.SPH_BND_CYL_YZ),
     "cycle_z": dict(eff=sph.SPH_BND_EFF_CYCLE),
     "wrap_z": dict(eff=sph.SPH_BND_EFF_WRAP),
-    "wave_moving": dict(eff=sph.SPH_BND_EFF_WAVE, wave_slope=0.2, wave_ampl=0.05,
-                        wave_phase=-1.5707963, wave_speed=6.0),
+    "wave_moving": dict(eff=sph.SPH_BND_EFF_WAVE, rot2_angle=0.2, rot_twist=0.05,
+                        rot_angle=-1.5707963, rot_vel=6.0),
     "pump_narrow_outlet": dict(bnd=sph.SPH_BND_PUMP_Y, world=(0.18, 0.3, 0.15), pump_ang_out=0.15,
-                               pump_s=[0.9, 0.9, 2.01, 0.8585, 0.9, 0.9]),
+                               shape_s=[0.9, 0.9, 2.01, 0.8585, 0.9, 0.9]),
     "pump_s_outlet": dict(bnd=sph.SPH_BND_PUMP_Y, world=(0.2, 0.3, 0.15), pump_ang_out=0.69,
-                          pump_h_close=-0.02, pump_s=[0.72, 0.68, 1.34, 0.89, 0.878, 0.883]),
+                          h_close=-0.02, shape_s=[0.72, 0.68, 1.34, 0.89, 0.878, 0.883]),
     "collider": dict(collider=1, coll_pos=[0.02, -0.05, 0.01], coll_r=0.05),
     "collider_default_place_kahan": dict(collider=1, prec=sph.SPH_PREC_KAHAN),
+    "rotor_pump_z": dict(rot_type=sph.SPH_ROTOR_PUMP_Z, rot_size=[0, 4, 12], rot_blades=4, rot_vel=10.0,
+                         rot_twist=0.04, coll_pos=[0.0, 0.0, -0.05]),
+    "rotor_pump_y": dict(rot_type=sph.SPH_ROTOR_PUMP_Y, rot_size=[1, 3, 10], rot_blades=6, rot_vel=-8.0,
+                         rot_angle=0.5, coll_pos=[0.02, -0.1, 0.0]),
+    "propeller_pair": dict(rot_type=sph.SPH_ROTOR_PROPELLER_Z, rot_size=[0, 6, 10], rot_blades=3,
+                           rot_vel=15.0, rot_twist=0.3, rot2_dist=0.12, rot2_vel=-1.0, rot2_angle=0.7,
+                           coll_pos=[0.0, -0.02, 0.0]),
+    "height_map_xz": dict(hmap_type=sph.SPH_HMAP_XZ, collider=1, rot_r=2.0, rot_spc=0.8, h_close=0.03,
+                          rot2_angle=0.06, shape_s=[2.0, 20.0, 1.5, 40.0, -0.3, 0.9]),
+    "height_map_yz": dict(hmap_type=sph.SPH_HMAP_YZ, rot_r=2.0, rot_spc=0.8, h_close=0.02,
+                          rot2_angle=0.08, shape_s=[1.0, 0.0, 2.0, 0.0, -1.0, 0.9]),
     "accelerators": dict(acc=[accel(sph.SPH_ACC_BOX, (0.05, -0.1, 0.0), (0.03, 0.03, 0.03), (0, 90, 30)),
                               accel(sph.SPH_ACC_CYL_Y, (-0.06, 0.0, 0.05), (0.04, 0.05, 0.03), (20, 0, 0)),
                               accel(sph.SPH_ACC_CYL_Y_SMOOTH, (0.0, 0.1, -0.05), (0.05, 0.04, 0.06),

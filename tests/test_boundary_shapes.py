@@ -180,7 +180,7 @@ def test_wave_floor_and_moving_z_wall(phase):
     """Wave effect: the sloped floor (bndDamp, normal (0, 1-sl, sl) with
     sl = -slope) and the z- wall moved by ampl (1 + sin(phase)) (bndDampC)."""
     slope, ampl = 0.2, 0.05
-    p = shape_params(eff=sph.SPH_BND_EFF_WAVE, wave_slope=slope, wave_ampl=ampl, wave_phase=phase)
+    p = shape_params(eff=sph.SPH_BND_EFF_WAVE, rot2_angle=slope, rot_twist=ampl, rot_angle=phase)
     b, st, dp, dc = consts(p)
     shift = float(np.float32(ampl)) * (1 + math.sin(float(np.float32(phase))))
     # float rounding of sin (|sin| <= 1) and of 1 + sin: a few ulp of ampl
@@ -200,9 +200,9 @@ def test_wave_phase_advances_before_each_step():
     """A step of a moving wave equals a step of a still wave whose phase is
     one advance ahead (App::UpdateEmitter runs before cSPH::Update)."""
     speed = 6.0
-    kw = dict(eff=sph.SPH_BND_EFF_WAVE, wave_ampl=0.05, wave_slope=0.1, n=2)
-    moving = shape_params(wave_phase=-1.5707963, wave_speed=speed, **kw)
-    ahead = shape_params(wave_phase=float(np.float32(np.float32(-1.5707963) + np.float32(speed) * np.float32(DT))),
+    kw = dict(eff=sph.SPH_BND_EFF_WAVE, rot_twist=0.05, rot2_angle=0.1, n=2)
+    moving = shape_params(rot_angle=-1.5707963, rot_vel=speed, **kw)
+    ahead = shape_params(rot_angle=float(np.float32(np.float32(-1.5707963) + np.float32(speed) * np.float32(DT))),
                          **kw)
     b = float(moving.dist_bnd_soft)
     pos = np.zeros((2, 4), np.float32)
@@ -216,7 +216,7 @@ def test_wave_phase_advances_before_each_step():
     np.testing.assert_array_equal(a["vel"], c["vel"])
     np.testing.assert_array_equal(a["pos"], c["pos"])
     # and it did move: a still wave at the old phase differs
-    still = shape_params(wave_phase=-1.5707963, **kw)
+    still = shape_params(rot_angle=-1.5707963, **kw)
     d = oracle.run_f32(still, pos, vel, 1)
     assert not np.array_equal(a["vel"], d["vel"])
 
@@ -227,7 +227,7 @@ PUMP_WORLD = (0.18, 0.3, 0.15)   # half extents of the reference's pump scenes
 
 def pump(**kw):
     return shape_params(sph.SPH_BND_PUMP_Y, world=PUMP_WORLD, pump_ang_out=0.15,
-                        pump_s=[0.9, 0.9, 2.01, 0.8585, 0.9, 0.9], **kw)
+                        shape_s=[0.9, 0.9, 2.01, 0.8585, 0.9, 0.9], **kw)
 
 
 def test_pump_frame_outside_the_outlet():
@@ -254,7 +254,7 @@ def test_pump_frame_open_over_the_outlet():
 def test_pump_lid():
     p = pump()
     b, st, dp, dc = consts(p)
-    hc = float(p.pump_h_close)
+    hc = float(p.h_close)
     x0 = f32([0.0, 0.0, hc - 0.95 * b])  # below hc - b s2 (s2 = 0.9), within b of hc
     v0 = f32([0.0, 0.0, 0.1])
     v = wall(v0, (b + x0[2]) - hc, [0, 0, -1], st, dp)
@@ -267,7 +267,7 @@ def test_pump_outlet_maps_onto_the_inlet():
     (vx, vy, vz) -> (vx, vz, -vy).  The y+ box wall acts first."""
     p = pump()
     b, st, dp, dc = consts(p)
-    hc = float(p.pump_h_close)
+    hc = float(p.h_close)
     rad, wy, wz = PUMP_WORLD
     ev = float(p.exit_vel)
     x0 = f32([0.01, wy - 0.01, -0.1])
@@ -354,3 +354,114 @@ def test_accelerators(kind):
         # inside the box's corner but outside the ellipse
         dv, _, _ = force_side(p, [0.01 + 0.028, 0.0, 0.045], [0.0, 0.0, 0.0])
         np.testing.assert_array_equal(dv, 0)
+
+
+# ------------------------------------------------------ rotor, height map
+def dem_r(rel, u, rad, spring, damping):
+    """collideSpheresR: spring (dist - rad) rel/dist + damping u inside rad."""
+    dist = np.linalg.norm(rel)
+    if dist >= rad:
+        return np.zeros(3)
+    return spring * (dist - rad) * rel / dist + damping * u
+
+
+def test_pump_rotor_single_sphere():
+    """One blade of one sphere (RotorSizes 0 0 1, one blade) sits at
+    centre + sp (sin a, cos a, 0) with a = -rAngle + rTwist (the z = 1 row)."""
+    ang, twist = 0.3, 0.2
+    p = shape_params(rot_type=sph.SPH_ROTOR_PUMP_Z, rot_size=[0, 0, 1], rot_blades=1, rot_r=2.0, rot_spc=0.8,
+                     rot_angle=ang, rot_twist=twist, coll_pos=[0.01, 0.0, -0.02])
+    assert p.physics == sph.SPH_PHYS_ROTOR
+    sp, rr = float(p.rot_spc), float(np.float32(R) + p.rot_r)
+    a = -float(np.float32(ang)) + float(np.float32(twist))
+    c = f32(p.coll_pos)
+    sphere = c + np.array([math.sin(a) * sp, math.cos(a) * sp, 0.0])
+    x = sphere + np.array([0.3, -0.5, 0.2]) * rr
+    v = np.array([0.2, -0.1, 0.05])
+    dv, x1, v1 = force_side(p, x, v)
+    exp = dem_r(sphere - x1, -v1, rr, 80.0, float(np.float32(0.02)))
+    assert np.linalg.norm(exp) > 0
+    np.testing.assert_allclose(dv, exp, rtol=2e-5, atol=1e-9)
+    # outside the rotor's z band nothing acts
+    dv, _, _ = force_side(p, sphere + np.array([0, 0, 3 * rr]), v)
+    np.testing.assert_array_equal(dv, 0)
+
+
+def test_pump_rotor_y_lays_blades_in_xz():
+    """Type 2 turns the same blade into the x-z plane: centre + sp (sin a, 0, cos a)."""
+    ang = -0.4
+    p = shape_params(rot_type=sph.SPH_ROTOR_PUMP_Y, rot_size=[0, 0, 1], rot_blades=1, rot_r=2.0, rot_spc=0.8,
+                     rot_angle=ang, rot_twist=0.0, coll_pos=[0.0, 0.02, 0.0])
+    sp, rr = float(p.rot_spc), float(np.float32(R) + p.rot_r)
+    a = -float(np.float32(ang))
+    sphere = f32(p.coll_pos) + np.array([math.sin(a) * sp, 0.0, math.cos(a) * sp])
+    x = sphere + np.array([-0.4, 0.3, 0.3]) * rr
+    dv, x1, v1 = force_side(p, x, np.zeros(3))
+    np.testing.assert_allclose(dv, dem_r(sphere - x1, -v1, rr, 80.0, 0.02), rtol=2e-5, atol=1e-9)
+
+
+def test_propeller_tip_sphere():
+    """RotorSizes 0 0 0: the only sphere is the tip row (h = sy, dh = 1.2) at
+    centre + (0, 0, -1.2 sp k), k = cos(-0.2 x2 + 0.6 pi), radius r + 0.2 rot_r."""
+    p = shape_params(rot_type=sph.SPH_ROTOR_PROPELLER_Z, rot_size=[2, 0, 0], rot_blades=1, rot_r=2.0,
+                     rot_spc=0.8, coll_pos=[0.0, 0.0, 0.0])
+    sp = float(p.rot_spc)
+    k = math.cos(-1.0 * 0.2 + float(np.float32(3.141592654) * np.float32(0.6)))
+    sphere = np.array([0.0, 0.0, -1.2 * sp * k])
+    rad = float(np.float32(R)) + float(p.rot_r) * abs(1 - 1.2)
+    x = sphere + np.array([0.2, -0.3, 0.1]) * rad
+    dv, x1, v1 = force_side(p, x, [0.1, 0.0, 0.0])
+    exp = dem_r(sphere - x1, -v1, rad, 80.0, 0.02)
+    assert np.linalg.norm(exp) > 0
+    np.testing.assert_allclose(dv, exp, rtol=5e-5, atol=1e-9)
+
+
+def hmap_expect(p, x, v, xz=True):
+    """All 5 x 5 x 2 spheres of the sheet near x, summed in float64."""
+    spc, rr = float(p.rot_spc), float(np.float32(R) + p.rot_r)
+    wmin = f32(p.world_min)
+    b = float(np.float32(p.dist_bnd_soft) - np.float32(R))
+    wsd = f32(p.world_max) - b - (wmin + b)
+    s = f32(p.shape_s)
+    a_ax, l_ax = (0, 1) if xz else (1, 0)
+    ia = int((x[a_ax] - wmin[a_ax]) / spc)
+    iz = int((x[2] - wmin[2]) / spc)
+    out = np.zeros(3)
+    for j in range(-2, 3):
+        for i in range(-2, 3):
+            ah = ia * spc + wmin[a_ax] + i * spc
+            zh = iz * spc + wmin[2] + j * spc
+            ss = math.sin(s[0] * ah / wsd[a_ax] * 2 * math.pi + s[1] * math.pi / 180) * \
+                math.sin(s[2] * zh / wsd[2] * 2 * math.pi + s[3] * math.pi / 180)
+            if not (s[4] <= -1 or ss > s[4]):
+                continue
+            for k in range(2):
+                sph_pos = np.zeros(3)
+                sph_pos[a_ax] = ah
+                sph_pos[2] = zh
+                sph_pos[l_ax] = ss * float(p.h_close) + float(p.rot2_angle) + wmin[l_ax] - k * spc
+                out += dem_r(sph_pos - x, -np.asarray(v), rr, 80.0, 0.02)
+    return out
+
+
+@pytest.mark.parametrize("plane", [sph.SPH_HMAP_XZ, sph.SPH_HMAP_YZ])
+@pytest.mark.parametrize("thresh", [-1.0, 0.2])
+def test_height_map_sheet(plane, thresh):
+    """The sheet of spheres at h_close sin(s1 u + s2) sin(s3 w + s4) +
+    rot2_angle above world_min (two layers), only where the product exceeds
+    s5 (s5 <= -1: everywhere), summed over the 5 x 5 sites around the particle."""
+    p = shape_params(hmap_type=plane, rot_r=2.0, rot_spc=0.7, h_close=0.01, rot2_angle=0.05,
+                     shape_s=[2.0, 30.0, 3.0, 10.0, thresh, 0.9])
+    assert p.physics & sph.SPH_PHYS_HMAP
+    xz = plane == sph.SPH_HMAP_XZ
+    lv = 1 if xz else 0
+    hits = 0
+    for frac in (0.013, 0.037, 0.061):
+        x = np.array([0.0123, 0.0231, frac])
+        x[lv] = -0.2 + 0.05 + 0.004  # just above the sheet's mean level
+        dv, x1, v1 = force_side(p, x, [0.0, -0.2, 0.0])
+        exp = hmap_expect(p, x1, v1, xz)
+        # the collider (default place) is far away
+        np.testing.assert_allclose(dv, exp, rtol=2e-4, atol=1e-8)
+        hits += np.linalg.norm(exp) > 0
+    assert hits > 0

@@ -90,7 +90,7 @@ def test_defaults_without_attributes():
     for name in ("num_particles", "time_step", "h", "particle_r", "stiffness", "viscosity", "bnd_stiff",
                  "bnd_damp", "bnd_damp_c", "cell_size", "rest_density"):
         assert getattr(s, name) == getattr(d, name), name
-    assert s.collider == 1 and s.coll_pos[0] == -77.0 and s.wave_speed == 0.0
+    assert s.collider == 1 and s.coll_pos[0] == -77.0 and s.rot_vel == 0.0
     assert info.title == b"No name" and info.chapter == 1 and info.init_type == 0 and info.init_last == 1
     r = f32(0.004)
     b = f32(f32(f32(8) * r) - r)
@@ -108,23 +108,33 @@ def test_aliases_apply_in_statement_order():
                'WaveSpeed="4" Hscale="0.2" PumpHClose="-0.01" HxFq="2" s1="0.5" ParticlesK="3" '
                'Particles="100" RotorAngle="90"/>')
     s, info = sph.xml_scene(text, 0)
-    assert s.wave_ampl == f32(0.07)
-    assert s.wave_slope == f32(0.3)
-    assert s.pump_h_close == f32(0.2)
-    assert s.pump_s[0] == f32(2)
+    assert s.rot_twist == f32(0.07)
+    assert s.rot2_angle == f32(0.3)
+    assert s.h_close == f32(0.2)
+    assert s.shape_s[0] == f32(2)
     assert s.num_particles == 3 * 1024
-    assert s.wave_phase == f32(-f32(3.141592654) / f32(2))  # WaveSpeed after RotorAngle
-    assert s.wave_speed == f32(4)
-    assert info.unsupported == 0
+    assert s.rot_angle == f32(-f32(3.141592654) / f32(2))  # WaveSpeed after RotorAngle
+    assert s.rot_vel == f32(4)
 
 
-def test_rotor_scene_turns_the_collider_off_and_reports_it():
-    text = xml('<Scene RotorType="1" RotorVel="10" HmapType="0"/><Scene HmapType="1" name="h"/>')
+def test_rotor_and_height_map_values():
+    """Rotor values (RotorSizes truncated to integers, angles in degrees) and
+    the force terms they select: a rotor replaces the collider."""
+    text = xml('<Scene RotorType="3" RotorVel="10" RotorSizes="0 5.7 13" RotorBlades="6" colParR="4" '
+               'colParSpc="0.5" Rotor2Dist="0.1" Rotor2Vel="-1" Rotor2Angle="45"/>'
+               '<Scene HmapType="2" name="h" Hheight="0.05" Hscale="0.03" HholeR="-1"/>')
     s, info = sph.xml_scene(text, 0)
-    assert s.collider == 0 and info.rotor_type == 1 and info.unsupported == sph.SPH_FEAT_ROTOR
-    assert s.wave_speed == 0.0  # rotor speed does not move a wave
+    assert s.rot_type == 3 and list(s.rot_size) == [0, 5, 13] and s.rot_blades == 6
+    assert s.rot_vel == f32(10) and s.rot2_vel == f32(-1) and s.rot2_dist == f32(0.1)
+    assert s.rot2_angle == f32(f32(f32(45) * f32(3.141592654)) / f32(180))
+    p = sph.params_from_scene(s)
+    r = f32(0.004)
+    assert p.rot_r == f32(f32(4) * r) and p.rot_spc == f32(f32(0.5) * p.rot_r)
+    assert p.physics == sph.SPH_PHYS_ROTOR  # the collider yields to the rotor
     s, info = sph.xml_scene(text, 1)
-    assert s.collider == 1 and info.unsupported == sph.SPH_FEAT_HMAP and info.title == b"h"
+    p = sph.params_from_scene(s)
+    assert s.hmap_type == 2 and p.physics == sph.SPH_PHYS_COLLIDER | sph.SPH_PHYS_HMAP
+    assert s.rot2_angle == f32(0.05) and s.h_close == f32(0.03) and s.shape_s[4] == -1
 
 
 def test_children_emitters_and_accels():
