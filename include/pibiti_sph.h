@@ -381,6 +381,30 @@ int sph_xml_scene(const char* text, size_t len, uint32_t index, sph_scene* scene
 int sph_scene_fill(const sph_scene* scene, const sph_scene_info* info, uint64_t seed, float* pos4,
                    float* vel4);
 
+/* Particle injection between steps (host only): the records a drop, rain
+ * or an emitter writes through cSPH::setArray; upload them with
+ * sph_set_array (positions and velocities) at the caller's write index.   */
+
+/* Largest emitter block (size x size2) sph_scene_emit writes.             */
+#define SPH_MAX_EMIT 65536
+
+/* cSPH::Drop (source/SPH/SPH_Init.cpp:82-118): a ball of radius info->drop_r
+ * lattice steps (info->spacing) at rest, centred in the world inside the
+ * soft band, or with random = 1 at a random place (high over a box or
+ * cylinder-Y floor; the reference draws rand(), here the seeded generator,
+ * draw number `draw`).  Writes the records that fit below num_particles
+ * starting at particle `first` (as the reference stops there): *count
+ * records to pos4/vel4 (float4, w = 1 and 0).                             */
+int sph_scene_drop(const sph_scene* scene, const sph_scene_info* info, int random, uint64_t seed,
+                   uint64_t draw, uint32_t first, float* pos4, float* vel4, uint32_t* count);
+
+/* One frame of emitter `emitter` (App::UpdateEmitter, source/App/Update.cpp:62-92):
+ * a size x size2 block of particles one spacing ahead of the emitter, laid
+ * out and moving along the emitter's direction (rotations rot[0] about x
+ * and -rot[1] about y, degrees, as the reference's GL modelview).  *count
+ * records (0 for an emitter that is off, at most SPH_MAX_EMIT).           */
+int sph_scene_emit(const sph_scene_info* info, uint32_t emitter, float* pos4, float* vel4, uint32_t* count);
+
 /* ------------------------------------------------------------------------ */
 /* Device context (replaces cSPH::_InitMem/_FreeMem + setParameters,
  * source/SPH/SPH_Mem.cpp:11-82, source/CUDA/System.cu:634-637).            */

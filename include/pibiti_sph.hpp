@@ -10,6 +10,7 @@
 #ifndef PIBITI_SPH_HPP
 #define PIBITI_SPH_HPP
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -54,7 +55,10 @@ public:
     uint32_t flags = 0; /* SPH_FLAG_*                                        */
     bool bInitialized = false;
     int curScene = -1;  /* index into the loaded scene file, -1: scn set by hand */
-    uint64_t seed = 0;  /* random fill (initType 1)                          */
+    uint64_t seed = 0;  /* random fill (initType 1) and random drops         */
+    uint32_t emitId = 0; /* next particle the emitters and drops overwrite (App::emitId) */
+    int cntRain = 0;     /* frames since the last rain drop (App::cntRain)   */
+    uint64_t drops = 0;  /* random drops so far (the generator's draw number) */
 
     cSPH() { check(sph_scene_default(&scn), "sph_scene_default"); }
     ~cSPH() { _FreeMem(); }
@@ -138,6 +142,45 @@ public:
         setArray(SPH_ARRAY_VELOCITY, hVel_.data(), 0, (int)par.num_particles);
     }
 
+    /* The particle part of App::UpdateEmitter (source/App/Update.cpp:62-97),
+     * run before each frame's Update: every emitter writes its block from
+     * emitId on, and every info.rain frames a drop falls at a random place.
+     * emitId wraps at the particle count; a block that crosses the end is
+     * split there (the reference writes past the end of its arrays).      */
+    void UpdateEmitter() {
+        require();
+        for (uint32_t e = 0; e < 4; ++e) {
+            const sph_emitter& em = info.emit[e];
+            if (em.size <= 0) continue;
+            const int64_t need = (int64_t)em.size * (em.size2 == 0 ? em.size : em.size2);
+            if (need <= 0 || need > SPH_MAX_EMIT) throw Error(SPH_ERR_CAPACITY, "cSPH::UpdateEmitter");
+            emPos_.resize(4 * (size_t)need);
+            emVel_.resize(4 * (size_t)need);
+            uint32_t cnt = 0;
+            check(sph_scene_emit(&info, e, emPos_.data(), emVel_.data(), &cnt), "sph_scene_emit");
+            PutWrapped(emPos_.data(), emVel_.data(), cnt);
+        }
+        if (info.rain > 0 && ++cntRain >= info.rain) {
+            cntRain = 0;
+            Drop(true);
+        }
+    }
+
+    /* cSPH::Drop (source/SPH/SPH_Init.cpp:82-118): a ball of particles at
+     * rest from emitId on (those below the particle count).               */
+    void Drop(bool random) {
+        require();
+        uint32_t cnt = 0;
+        check(sph_scene_drop(&scn, &info, random ? 1 : 0, seed, random ? drops++ : 0, emitId, hPos_.data(),
+                             hVel_.data(), &cnt),
+              "sph_scene_drop");
+        if (cnt == 0) return;
+        setArray(SPH_ARRAY_POSITION, hPos_.data(), (int)emitId, (int)cnt);
+        setArray(SPH_ARRAY_VELOCITY, hVel_.data(), (int)emitId, (int)cnt);
+        emitId += cnt;
+        if (emitId >= par.num_particles) emitId -= par.num_particles;
+    }
+
     /* One step (cSPH::Update, source/SPH/SPH_Update.cpp:12-81), or several. */
     void Update(uint32_t nsteps = 1) {
         require();
@@ -180,6 +223,20 @@ private:
         if (!bInitialized) throw Error(SPH_ERR_NOT_INITIALISED, "cSPH");
     }
 
+    // cnt records at emitId, wrapping at the particle count; emitId advances.
+    void PutWrapped(const float* pos, const float* vel, uint32_t cnt) {
+        const uint32_t n = par.num_particles;
+        uint32_t done = 0;
+        while (done < cnt) {
+            const uint32_t part = std::min(cnt - done, n - emitId);
+            setArray(SPH_ARRAY_POSITION, pos + 4 * (size_t)done, (int)emitId, (int)part);
+            setArray(SPH_ARRAY_VELOCITY, vel + 4 * (size_t)done, (int)emitId, (int)part);
+            done += part;
+            emitId += part;
+            if (emitId >= n) emitId -= n;
+        }
+    }
+
     void Step(int dir, bool chapter) {
         if (count_ == 0) throw Error(SPH_ERR_NOT_INITIALISED, "cSPH: no scene file");
         int k = curScene < 0 ? 0 : curScene;
@@ -195,7 +252,7 @@ private:
     }
 
     sph_ctx* ctx_ = nullptr;
-    std::vector<float> hPos_, hVel_, hDens_;
+    std::vector<float> hPos_, hVel_, hDens_, emPos_, emVel_;
     std::string xml_;
     uint32_t count_ = 0;
 };

@@ -2,12 +2,14 @@
 //
 //   sph_run SCENE NX NY NZ MINX MINY MINZ SPACING JITTER VEL_SIGMA SEED STEPS OUT
 //
-//   sph_run --xml SCENES.xml INDEX STEPS OUT
+//   sph_run --xml SCENES.xml INDEX STEPS OUT [frames]
 //
 // SCENE is a raw sph_scene record (sizeof(sph_scene) bytes).  The lattice
 // block NX x NY x NZ from (MINX, MINY, MINZ) is advanced STEPS steps.  With
 // --xml, scene INDEX of a scene file is initialised as cSPH::InitScene does
-// (its init volume fill) and advanced STEPS steps.  OUT receives, as
+// (its init volume fill) and advanced STEPS steps; with `frames`, each step
+// is a frame of the reference's app: the emitters and rain
+// (cSPH::UpdateEmitter), then the step.  OUT receives, as
 // little-endian binary: uint32 n, float4 positions[n], float4 velocities[n],
 // float densities[n], then double mass, kinetic energy, max speed and
 // simulated time.
@@ -43,14 +45,21 @@ static int write_out(pibiti::cSPH& sim, uint32_t steps, const char* path) {
 }
 
 int main(int argc, char** argv) {
-    if (argc == 6 && std::strcmp(argv[1], "--xml") == 0) {
+    if ((argc == 6 || (argc == 7 && std::strcmp(argv[6], "frames") == 0)) && std::strcmp(argv[1], "--xml") == 0) {
         try {
             pibiti::cSPH sim;
             sim.LoadScenes(argv[2]);
             sim.SetScene((uint32_t)std::strtoul(argv[3], nullptr, 10));
             const uint32_t steps = (uint32_t)std::strtoul(argv[4], nullptr, 10);
             sim.InitScene();
-            sim.Update(steps);
+            if (argc == 7) {
+                for (uint32_t k = 0; k < steps; ++k) {
+                    sim.UpdateEmitter();
+                    sim.Update(1);
+                }
+            } else {
+                sim.Update(steps);
+            }
             return write_out(sim, steps, argv[5]);
         } catch (const pibiti::Error& e) {
             std::fprintf(stderr, "sph_run: %s\n", e.what());

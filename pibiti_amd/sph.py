@@ -294,6 +294,9 @@ SIGNATURES = {
     "sph_xml_scene": (C.c_int, [C.c_char_p, C.c_size_t, C.c_uint32, C.POINTER(Scene),
                                 C.POINTER(SceneInfo)]),
     "sph_scene_fill": (C.c_int, [C.POINTER(Scene), C.POINTER(SceneInfo), C.c_uint64, _fp, _fp]),
+    "sph_scene_drop": (C.c_int, [C.POINTER(Scene), C.POINTER(SceneInfo), C.c_int, C.c_uint64, C.c_uint64,
+                                 C.c_uint32, _fp, _fp, _up]),
+    "sph_scene_emit": (C.c_int, [C.POINTER(SceneInfo), C.c_uint32, _fp, _fp, _up]),
     "sph_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_uint32, C.POINTER(_P)]),
     "sph_destroy": (C.c_int, [_P]),
     "sph_set_params": (C.c_int, [_P, C.POINTER(Params)]),
@@ -432,6 +435,34 @@ def scene_fill(s: Scene, info: SceneInfo, seed: int = 0) -> tuple[np.ndarray, np
     vel = np.zeros((n, 4), np.float32)
     _check(lib().sph_scene_fill(C.byref(s), C.byref(info), seed, _f(pos), _f(vel)), "sph_scene_fill")
     return pos, vel
+
+
+SPH_MAX_EMIT = 65536
+
+
+def scene_drop(s: Scene, info: SceneInfo, random: bool = False, seed: int = 0, draw: int = 0,
+               first: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """cSPH::Drop (source/SPH/SPH_Init.cpp:82-118): the records of one drop
+    written from particle `first` (those that fit below the particle count)."""
+    r = int(info.drop_r) + 1
+    cap = min((2 * r + 1) ** 3, int(s.num_particles) - first)
+    pos = np.zeros((max(cap, 1), 4), np.float32)
+    vel = np.zeros((max(cap, 1), 4), np.float32)
+    n = C.c_uint32()
+    _check(lib().sph_scene_drop(C.byref(s), C.byref(info), int(random), seed, draw, first, _f(pos), _f(vel),
+                                C.byref(n)), "sph_scene_drop")
+    return pos[:n.value].copy(), vel[:n.value].copy()
+
+
+def scene_emit(info: SceneInfo, emitter: int) -> tuple[np.ndarray, np.ndarray]:
+    """One frame of an emitter's block (App::UpdateEmitter, source/App/Update.cpp:62-92)."""
+    e = info.emit[emitter]
+    cap = max(1, min(SPH_MAX_EMIT, max(e.size, 0) * max(e.size2 or e.size, 0)))
+    pos = np.zeros((cap, 4), np.float32)
+    vel = np.zeros((cap, 4), np.float32)
+    n = C.c_uint32()
+    _check(lib().sph_scene_emit(C.byref(info), emitter, _f(pos), _f(vel), C.byref(n)), "sph_scene_emit")
+    return pos[:n.value].copy(), vel[:n.value].copy()
 
 
 class SphSystem:

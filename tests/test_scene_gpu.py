@@ -36,6 +36,8 @@ SCENES = """<SPH>
   <Scene name="pump" bndType="5" bndStiff="60000" Gravity="0 -5 -1.5" Particles="14000"
          World="0.36 0.6 0.3" spacing="1.3" PumpAngOut="0.15" s3="2.01" s4="0.8585" s5="0.9" s6="0.9"
          ColliderPos="0 0 -0.11"/>
+  <Scene name="fountain and rain" Particles="9000" World="0.3 0.3 0.3" Init="0.24 0.1 0.24" dropR="2"
+         rain="3" EmitSize="5" EmitSize2="3" EmitPos="0.02 -0.08 0" EmitRot="70 30" EmitVel="1.5"/>
   <Scene name="accelerators" Particles="12000" World="0.3 0.3 0.3" ColliderR="0.03"
          ColliderPos="0.05 -0.1 0.02">
     <Accel type="1" size="0.02 0.02 0.02" pos="-0.05 -0.1 0" acc="0 80 0"/>
@@ -52,7 +54,7 @@ def load(index):
     return s, info, sph.params_from_scene(s), pos, vel
 
 
-@pytest.mark.parametrize("index", range(8))
+@pytest.mark.parametrize("index", range(9))
 def test_file_scene_steps_bit_exact(index):
     s, info, p, pos, vel = load(index)
     print(f"case: scene {index} '{info.title.decode()}', n = {p.num_particles}, {STEPS} steps", flush=True)
@@ -66,6 +68,75 @@ def test_file_scene_steps_bit_exact(index):
         bad = np.flatnonzero(np.any((g[k] != o[k]).reshape(len(pos), -1), axis=1))
         assert bad.size == 0, f"scene {index} {k}: {bad.size} particles differ, first {bad[:5]}"
     assert np.abs(o["vel"][:, :3]).max() > 0.0
+
+
+FOUNTAIN = 7
+FRAMES = 12
+
+
+def run_frames(index, frames, seed=0):
+    """The reference app's frames (App::Simulate, source/App/Render.cpp:8-11):
+    emitter blocks and rain drops written at emitId, wrapping, then a step.
+    Returns the GPU arrays and the oracle's after the same injections."""
+    s, info, p, pos, vel = load(index)
+    n = int(p.num_particles)
+    opos, ovel = pos.copy(), vel.copy()
+    emit_id, cnt_rain, drops = 0, 0, 0
+    with sph.SphSystem(p) as g:
+        g.reset(pos, vel)
+
+        def put(ps, vs, first):
+            g.set_array(sph.SPH_ARRAY_POSITION, ps, first)
+            g.set_array(sph.SPH_ARRAY_VELOCITY, vs, first)
+            opos[first:first + len(ps)] = ps
+            ovel[first:first + len(vs)] = vs
+
+        for _ in range(frames):
+            for e in range(4):
+                bp, bv = sph.scene_emit(info, e)
+                done = 0
+                while done < len(bp):
+                    part = min(len(bp) - done, n - emit_id)
+                    put(bp[done:done + part], bv[done:done + part], emit_id)
+                    done += part
+                    emit_id = (emit_id + part) % n
+            if info.rain > 0:
+                cnt_rain += 1
+                if cnt_rain >= info.rain:
+                    cnt_rain = 0
+                    dp, dv = sph.scene_drop(s, info, random=True, seed=seed, draw=drops, first=emit_id)
+                    drops += 1
+                    if len(dp):
+                        put(dp, dv, emit_id)
+                        emit_id = (emit_id + len(dp)) % n
+            g.update(1)
+            o = oracle.step_f32(p, opos, ovel)
+            opos, ovel = o["pos"], o["vel"]
+        got = dict(pos=g.get_array(sph.SPH_ARRAY_POSITION), vel=g.get_array(sph.SPH_ARRAY_VELOCITY))
+    return got, dict(pos=opos, vel=ovel)
+
+
+def test_emitters_and_rain_between_steps_bit_exact():
+    print("case: fountain and rain,", FRAMES, "frames", flush=True)
+    g, o = run_frames(FOUNTAIN, FRAMES)
+    assert np.array_equal(g["pos"], o["pos"]) and np.array_equal(g["vel"], o["vel"])
+
+
+def test_cpp_host_class_runs_app_frames(tmp_path):
+    """pibiti::cSPH::UpdateEmitter + Update per frame gives the same arrays."""
+    f = tmp_path / "Scenes.xml"
+    f.write_text(SCENES)
+    out = tmp_path / "out.bin"
+    print("case: C++ host class frames, scene", FOUNTAIN, flush=True)
+    r = subprocess.run([str(build.RUN_BIN), "--xml", str(f), str(FOUNTAIN), str(FRAMES), str(out), "frames"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    n = int(np.frombuffer(raw[:4], np.uint32)[0])
+    arr = np.frombuffer(raw[4:4 + 32 * n], np.float32)
+    g, o = run_frames(FOUNTAIN, FRAMES)
+    assert np.array_equal(arr[:4 * n].reshape(n, 4), o["pos"])
+    assert np.array_equal(arr[4 * n:8 * n].reshape(n, 4), o["vel"])
 
 
 def test_cpp_host_class_runs_a_file_scene(tmp_path):
