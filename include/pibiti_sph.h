@@ -92,8 +92,18 @@ enum {
     SPH_ARRAY_PRESSURE = 3,
     SPH_ARRAY_DVEL = 4,
     SPH_ARRAY_TARGET = 5,
-    SPH_ARRAY_NEIGHBOURS = 6
+    SPH_ARRAY_NEIGHBOURS = 6,
+    SPH_ARRAY_COLOR = 7,        /* float4 (r, g, b, 1) of the last step (clr_type != NONE) */
+    SPH_ARRAY_DYE = 8           /* float dye intensity per particle          */
 };
+
+/* Colour modes: the reference's ClrType values (source/CUDA/Params.cuh:31). */
+enum {
+    SPH_CLR_DENS = 0, SPH_CLR_ACCEL = 1, SPH_CLR_DENS_ACC = 2, SPH_CLR_VEL = 3,  /* VEL: foam potential */
+    SPH_CLR_VEL_ACC = 4, SPH_CLR_VEL_RGB = 5, SPH_CLR_NONE = 6
+};
+/* Dye shapes (dyeType). */
+enum { SPH_DYE_OFF = 0, SPH_DYE_BOX = 1, SPH_DYE_SPHERE = 2 };
 
 /* Context flags (sph_create). */
 enum {
@@ -185,6 +195,20 @@ typedef struct sph_scene {
      * radius rot_r, spacing rot_spc, at height
      * h_close sin(s1 u + s2) sin(s3 w + s4) + rot2_angle above world_min.  */
     int32_t hmap_type;          /* SPH_HMAP_*                                */
+    /* Particle colours and dye (computeForceD's colouring,
+     * source/CUDA/System.cu:405-546): a per-particle colour for a renderer
+     * and a dye intensity that particles pick up in the dye region and lose
+     * at dye_fade per second.                                               */
+    uint32_t clr_type;          /* SPH_CLR_* (SPH_CLR_NONE: no colour output) */
+    int32_t hue;                /* iHue: 1 = (intensity, dye, b) channels     */
+    float brightness, contrast;
+    int32_t dye_type;           /* SPH_DYE_*                                 */
+    int32_t dye_clear;          /* steps left that clear the dye: every step
+                                   first counts it down by one, and a step that
+                                   sees it above 0 clears the dye (source/App/Update.cpp:23) */
+    float dye_fade;             /* intensity lost per second                 */
+    float dye_pos[3];
+    float dye_size[3];          /* box half sizes; the sphere's radius is dye_size[1] */
 } sph_scene;
 
 /* Derived kernel constants: the plain-scalar equivalent of SimParams
@@ -263,6 +287,11 @@ typedef struct sph_params {
     int32_t rot_type, rot_size[3], rot_blades;
     float rot_r, rot_spc, rot2_dist, rot2_vel, rot2_twist;
     int32_t hmap_type;
+    uint32_t clr_type;
+    int32_t hue;
+    float brightness, contrast;
+    int32_t dye_type, dye_clear;
+    float dye_fade, dye_pos[3], dye_size[3];
     /* Derived: SPH_PHYS_* bits of the terms in use.                         */
     uint32_t physics;
     uint32_t reserved[2];
@@ -441,6 +470,25 @@ int sph_step(sph_ctx* ctx, uint32_t nsteps);
  * pointer may be NULL to skip that array.                                  */
 int sph_get_grid(sph_ctx* ctx, uint32_t* sorted_hash, uint32_t* perm, uint32_t* cell_start,
                  uint32_t* cell_end);
+
+/* Device arrays of the current state, for a renderer that reads them in
+ * place (the role of the reference's GL buffer interop,
+ * source/CUDA/System.cu:556-560): positions, velocities and colours per
+ * sorted slot (float4; colour4 is NULL without a colour mode), the original
+ * particle index of each slot, the context's HIP stream (hipStream_t) and
+ * device.  The pointers stay valid until the next call that steps, uploads
+ * or changes parameters; wait for `stream` before reading them.           */
+typedef struct sph_device_view {
+    const float* pos4;
+    const float* vel4;
+    const float* colour4;
+    const uint32_t* index;
+    void* stream;
+    uint32_t n;
+    int32_t device;
+} sph_device_view;
+
+int sph_get_device_view(sph_ctx* ctx, sph_device_view* out);
 
 /* FP64 reductions of the current state plus counters of the last step.   */
 int sph_get_stats(sph_ctx* ctx, sph_stats* out);

@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
                                       _up, _up, _up, _up, _up]
         h.oracle_run_f32.restype = C.c_int
         h.oracle_run_f32.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, _fp, _fp, _fp, _fp]
+        h.oracle_run_f32_colour.restype = C.c_int
+        h.oracle_run_f32_colour.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, _fp, _fp, _fp, _fp]
         h.oracle_step_f32_ex.restype = C.c_int
         h.oracle_step_f32_ex.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _fp, _fp, _fp, _fp, _fp, _fp,
                                          _up, _up, _up, _up, _up]
@@ -185,3 +187,16 @@ def density_bruteforce_f64(params, pos4):
     nnb = np.empty(n, np.uint32)
     lib().oracle_density_bruteforce_f64(_pp(params), n, _f(pos), _d(dens), _u(nnb))
     return dens, nnb
+
+
+def run_f32_colour(params, pos4, vel4, nsteps: int, dye=None, prec: int = 0) -> dict:
+    """nsteps FP32 steps with the colour output; dye (float per particle,
+    original order) starts at `dye` (zeros by default)."""
+    n = int(params.num_particles)
+    pos = np.array(pos4, np.float32, copy=True).reshape(n, 4)
+    vel = np.array(vel4, np.float32, copy=True).reshape(n, 4)
+    d = np.zeros(n, np.float32) if dye is None else np.array(dye, np.float32, copy=True)
+    col = np.zeros((n, 4), np.float32)
+    st = lib().oracle_run_f32_colour(_pp(params), prec, n, nsteps, _f(pos), _f(vel), _f(d), _f(col))
+    assert st == 0
+    return dict(pos=pos, vel=vel, dye=d, colour=col)

@@ -282,6 +282,9 @@ def scene_values(sc: dict, index: int) -> dict:
     v["rot_angle"], v["rot_twist"], v["rot2_angle"] = r_angle, r_twist, r2_angle
     v["rot_vel"] = r_vel
     v["collider"] = 1
+    # colours and dye (Scene.cpp:61-64): no attributes set them
+    v.update(clr_type=4, hue=0, brightness=f32(0.3), contrast=f32(0.3), dye_type=0, dye_clear=2,
+             dye_fade=f32(1.0), dye_pos=[f32(0), f32(-0.21), f32(0)], dye_size=[f32(0.008), f32(0.012), f32(0.008)])
     v["rot_type"], v["hmap_type"] = rot_type, hmap
     v.update(rot)
     v["coll_pos"] = coll_pos

@@ -609,7 +609,125 @@ struct StepOut {
     uint32_t* cell_start;
     uint32_t* cell_end;
     Counters* cnt;
+    T* colour4;     // particle colours (clr_type != NONE)
+    T* dye;         // dye intensity, read and written
 };
+
+// ----------------------------------------------------------------- colours
+// computeForceD's colouring and dye (source/CUDA/System.cu:405-546) for
+// sorted slot k; dv is the step's whole velocity increment.  Returns false
+// for CLR_None (no colour, no dye update).
+template <typename T>
+inline T dot4(const T* a, const T* b) {
+    return ((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3];  // cutil_math.h:540-543
+}
+template <typename T>
+inline T phi_clamp(T v, T lo, T hi) {  // phiClamp (System.cu:221-223)
+    return (std::fmin(v, hi) - std::fmin(v, lo)) / (hi - lo);
+}
+
+template <typename T>
+bool colour_one(const sph_params& p, uint32_t k, uint32_t si, const std::vector<T>& spos,
+                const std::vector<T>& svel, const std::vector<T>& dens, const T* dv,
+                const std::vector<uint32_t>& cs, const std::vector<uint32_t>& ce, T* dye, T col[3]) {
+    if (p.clr_type == SPH_CLR_NONE) return false;
+    const T dt = (T)p.time_step, br = (T)p.brightness, ct = (T)p.contrast, rho0 = (T)p.rest_density;
+    const T* x = &spos[4 * (size_t)k];
+    const T* v = &svel[4 * (size_t)k];
+    auto len3 = [](T a, T b, T c) { return std::sqrt((a * a + b * b) + c * c); };
+    col[0] = (T)0.2f;
+    col[1] = (T)0.5f;
+    col[2] = (T)1;
+    T intens = 0;
+    const uint32_t t = p.clr_type;
+    if (t == SPH_CLR_VEL_ACC || t == SPH_CLR_DENS_ACC) {
+        const T f = ((T)0.02f * len3(dv[0], dv[1], dv[2])) / dt;
+        const T kf = ct * f;
+        T kb;
+        if (t == SPH_CLR_VEL_ACC)
+            kb = br + ct * ((T)2.5f * len3(v[0], v[1], v[2]));
+        else
+            kb = br + ct * (((T)4 * (dens[k] - rho0)) / rho0);
+        const T add[3] = {(T)0.7f, t == SPH_CLR_VEL_ACC ? (T)0.35f : (T)0.7f, (T)0};
+        for (int c = 0; c < 3; ++c) col[c] = col[c] * kb + add[c] * kf;
+    } else if (t == SPH_CLR_VEL) {
+        intens = br + ct * ((T)2.5f * len3(v[0], v[1], v[2]));
+        // foam potential (:424-486): cells in x, y, z order
+        int g[3];
+        grid_pos<T>(p, x, g, nullptr);
+        T vdiff = 0;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int cx = g[0] + dx, cy = g[1] + dy, cz = g[2] + dz;
+                    if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)p.grid_size[0] || cy >= (int)p.grid_size[1] ||
+                        cz >= (int)p.grid_size[2])
+                        continue;
+                    const uint32_t c = grid_hash(p, cx, cy, cz);
+                    if (cs[c] == kEmpty) continue;
+                    const uint32_t e = std::min(ce[c], cs[c] + p.max_par_in_cell);
+                    for (uint32_t j = cs[c]; j < e; ++j) {
+                        if (j == k) continue;
+                        const T* x2 = &spos[4 * (size_t)j];
+                        const T* v2 = &svel[4 * (size_t)j];
+                        T sv[4], sx[4], ddv[4], ddx[4];
+                        for (int q = 0; q < 4; ++q) {
+                            sv[q] = v[q] + v2[q];
+                            sx[q] = x[q] + x2[q];
+                            ddv[q] = v[q] - v2[q];
+                            ddx[q] = x[q] - x2[q];
+                        }
+                        const T iv = (T)1 / std::sqrt(dot4(sv, sv)), ix = (T)1 / std::sqrt(dot4(sx, sx));
+                        for (int q = 0; q < 4; ++q) {
+                            sv[q] = sv[q] * iv;  // normalize (cutil_math.h:552-556)
+                            sx[q] = sx[q] * ix;
+                        }
+                        const T r = std::sqrt(dot4(ddx, ddx));
+                        const T w = r <= (T)3 ? ((T)1 - r) / (T)3 : (T)0;
+                        vdiff = vdiff + (std::sqrt(dot4(ddv, ddv)) * ((T)1 - dot4(sv, sx))) * w;
+                    }
+                }
+        const T trapped = phi_clamp<T>(vdiff, (T)5, (T)20);
+        const T ke = (((T)p.particle_mass * (T)1000) * dot4(v, v)) * (T)0.5;
+        const T nd = phi_clamp<T>(ke, (T)0.1f, (T)1) * ((T)3 * trapped + (T)3 * (T)0);
+        col[0] = col[1] = col[2] = nd;
+    } else if (t == SPH_CLR_VEL_RGB) {
+        for (int c = 0; c < 3; ++c) col[c] = (T)0.5f + v[c] * (ct * (T)2.5f);
+    } else {  // ACCEL, DENS
+        const T kb = t == SPH_CLR_ACCEL ? br + ct * (((T)0.02f * len3(dv[0], dv[1], dv[2])) / dt)
+                                        : br + ct * (((T)4 * (dens[k] - rho0)) / rho0);
+        intens = kb;
+        for (int c = 0; c < 3; ++c) col[c] = col[c] * kb;
+    }
+    if (p.hue == 1) {
+        col[0] = intens;
+        col[1] = 0;
+    }
+    if (p.dye_clear > 0) {
+        dye[si] = 0;
+    } else if (p.dye_type > 0) {
+        T dc = dye[si];
+        const T rel[3] = {x[0] - (T)p.dye_pos[0], x[1] - (T)p.dye_pos[1], x[2] - (T)p.dye_pos[2]};
+        if (p.dye_type == SPH_DYE_BOX) {
+            if (std::fabs(rel[0]) < (T)p.dye_size[0] && std::fabs(rel[1]) < (T)p.dye_size[1] &&
+                std::fabs(rel[2]) < (T)p.dye_size[2])
+                dc = 1;
+        } else if (len3(rel[0], rel[1], rel[2]) < (T)p.dye_size[1]) {
+            dc = 1;
+        }
+        dc = dc - dt * (T)p.dye_fade;
+        if (dc < 0) dc = 0;
+        if (dc >= 0) dye[si] = dc;
+        if (p.hue == 0) {
+            col[0] = col[0] + (T)0.9f * dc;
+            col[1] = col[1] + (T)0.9f * dc;
+            col[2] = col[2] + (T)1 * dc;
+        } else {
+            col[1] = dc;
+        }
+    }
+    return true;
+}
 
 template <typename T>
 void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const T* vel_in,
@@ -709,6 +827,15 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
         if (o.sorted_hash) o.sorted_hash[k] = hash[si];
         if (o.perm) o.perm[k] = si;
     }
+    if (o.dye)
+        for (uint32_t k = 0; k < n; ++k) {
+            T col[3];
+            const uint32_t si = perm[k];
+            if (colour_one<T>(p, k, si, spos, svel, dens, &dv[4 * (size_t)k], cs, ce, o.dye, col) && o.colour4) {
+                for (int c = 0; c < 3; ++c) o.colour4[4 * (size_t)si + c] = col[c];
+                o.colour4[4 * (size_t)si + 3] = 1;
+            }
+        }
     if (o.pos4) std::copy(pos.begin(), pos.end(), o.pos4);
     if (o.vel4) std::copy(vel.begin(), vel.end(), o.vel4);
     if (o.cell_start) std::copy(cs.begin(), cs.end(), o.cell_start);
@@ -721,6 +848,7 @@ void step_impl(const sph_params& p, int prec, uint32_t n, const T* pos_in, const
 // rAngle += rVel * timeStep (source/App/Update.cpp:15-16).
 inline sph_params advanced(const sph_params& p) {
     sph_params q = p;
+    if (q.dye_clear > 0) q.dye_clear = q.dye_clear - 1;  // source/App/Update.cpp:23
     if (q.rot_vel != 0.0f) {
         q.rot_angle = q.rot_angle + q.rot_vel * q.time_step;
         if (q.rot2_dist > 0.0f) {  // the second propeller (source/App/Update.cpp:17-20)
@@ -792,6 +920,22 @@ int oracle_run_f32(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, f
     for (uint32_t s = 0; s < nsteps; ++s) {
         StepOut<float> o{pos4, vel4, nullptr, dens, pres, nullptr, nullptr, nullptr, nullptr,
                          nullptr, nullptr, nullptr, nullptr, nullptr};
+        q = advanced(q);
+        step_impl<float>(q, prec, n, pos4, vel4, false, o);
+    }
+    return 0;
+}
+
+// nsteps FP32 steps with the colour output: dye (float per particle,
+// original order) is read and updated; colour4 receives the last step's
+// colours (float4, original order; untouched for SPH_CLR_NONE).
+int oracle_run_f32_colour(const sph_params* p, int prec, uint32_t n, uint32_t nsteps, float* pos4,
+                          float* vel4, float* dye, float* colour4) {
+    if (!p || !pos4 || !vel4 || !dye || n != p->num_particles) return 1;
+    sph_params q = *p;
+    for (uint32_t s = 0; s < nsteps; ++s) {
+        StepOut<float> o{pos4, vel4, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, nullptr, nullptr, nullptr, colour4, dye};
         q = advanced(q);
         step_impl<float>(q, prec, n, pos4, vel4, false, o);
     }

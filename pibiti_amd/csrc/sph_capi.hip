@@ -1,5 +1,7 @@
 // sph_capi.hip - extern "C" entry points of include/pibiti_sph.h for single
 // scenes and batches; the device context is sph::Engine (sph_engine.hpp).
+#include <cstring>
+
 #include "sph_engine.hpp"
 
 using namespace sph;
@@ -74,6 +76,21 @@ int sph_get_grid(sph_ctx* c, uint32_t* sorted_hash, uint32_t* perm, uint32_t* ce
 int sph_get_stats(sph_ctx* c, sph_stats* out) {
     if (!c) return SPH_ERR_INVALID_ARGUMENT;
     return c->eng.get_stats(0, out);
+}
+
+int sph_get_device_view(sph_ctx* c, sph_device_view* out) {
+    if (!c || !out) return SPH_ERR_INVALID_ARGUMENT;
+    const Engine& e = c->eng;
+    if (!e.ready()) return SPH_ERR_NOT_INITIALISED;
+    std::memset(out, 0, sizeof(*out));
+    out->pos4 = reinterpret_cast<const float*>(e.pos[e.cur_pos]);
+    out->vel4 = reinterpret_cast<const float*>(e.vel[e.cur_vel]);
+    out->colour4 = reinterpret_cast<const float*>(e.colour);
+    out->index = e.pid[e.cur_pid];
+    out->stream = e.stream;
+    out->n = e.cfg.n;
+    out->device = e.device;
+    return SPH_OK;
 }
 
 int sph_synchronize(sph_ctx* c) {

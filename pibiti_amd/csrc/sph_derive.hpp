@@ -17,4 +17,12 @@ int check_shape(const sph_params& p);
 // propeller's angle (no-op when rot_vel == 0).
 void advance_motion(sph_params& p);
 
+// Everything the reference's app changes before each step: the dye-clear
+// count, then advance_motion.
+void advance_frame(sph_params& p);
+
+// True when advance_frame changes the parameters (a turning rotor or
+// moving wave, or dye clears still pending).
+bool changes_per_step(const sph_params& p);
+
 }  // namespace sph

@@ -112,6 +112,8 @@ struct Engine {
     uint32_t* sid[2] = {nullptr, nullptr};
     float2* rhop = nullptr;
     float4* dvel = nullptr;
+    float4* colour = nullptr;   // per sorted slot, scenes with clr_type != NONE
+    float* dye = nullptr;       // per ORIGINAL index (it follows the particle)
     uint32_t *key_pid = nullptr, *val_pid = nullptr, *key_tmp = nullptr, *val_tmp = nullptr;
     uint32_t *cell_start = nullptr, *cell_end = nullptr, *counters = nullptr, *cell_max = nullptr;
     uint32_t* work = nullptr;
@@ -184,7 +186,7 @@ struct Engine {
             (void)hipFree(pid[k]);
             (void)hipFree(sid[k]);
         }
-        void* ptrs[] = {rhop,     dvel,     key_pid,  val_pid,    key_tmp, val_tmp,
+        void* ptrs[] = {rhop,     dvel,     colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
                         cell_start, cell_end, counters, cell_max, work,    partials,
                         stats,    dt_dev,   staging,  sort_scratch, d_tab, d_scene_off};
         for (void* q : ptrs) (void)hipFree(q);
@@ -205,7 +207,7 @@ struct Engine {
         for (auto& sc : scenes) {
             // a turning rotor or moving wave needs new parameters every step:
             // single scenes only
-            if (as_batch && sc.rot_vel != 0.0f) return SPH_ERR_UNSUPPORTED;
+            if (as_batch && changes_per_step(sc)) return SPH_ERR_UNSUPPORTED;
             derive_shape(sc);
         }
         scene_off.assign(nscenes + 1, 0);
@@ -295,6 +297,7 @@ struct Engine {
                                 stream);
         if (e == hipSuccess && dvel)
             e = launch_fill_u32(reinterpret_cast<uint32_t*>(dvel), 0u, 4 * N, stream);
+        if (e == hipSuccess) e = ensure_colour();
         if (e == hipSuccess) {
             std::vector<float> dts(nscenes);
             for (uint32_t s = 0; s < nscenes; ++s) dts[s] = scenes[s].time_step;
@@ -303,6 +306,23 @@ struct Engine {
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
         }
         return to_status(e);
+    }
+
+    // Colour output: the colour and dye buffers, and the force increment the
+    // colours read, for any scene with a colour mode.
+    hipError_t ensure_colour() {
+        cfg.colour = 0;
+        for (const auto& sc : scenes)
+            if (sc.clr_type != SPH_CLR_NONE) cfg.colour = 1;
+        if (!cfg.colour || colour) return hipSuccess;
+        const size_t N = cfg.n;
+        hipError_t e = dalloc(&colour, N);
+        if (e == hipSuccess) e = dalloc(&dye, N);
+        if (e == hipSuccess && !dvel) e = dalloc(&dvel, N);
+        if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(colour), 0u, 4 * N, stream);
+        if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dye), 0u, N, stream);
+        if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dvel), 0u, 4 * N, stream);
+        return e;
     }
 
     LaunchCfg launch_cfg() const {
@@ -334,7 +354,8 @@ struct Engine {
         SPH_TRY(launch_density(c, buffers(), stream));
         if (timed) SPH_TRY(hipEventRecord(ev[4], stream));
         StepBuffers b = buffers();
-        SPH_TRY(launch_force(c, b, b.vel_alt, keep, stream));
+        SPH_TRY(launch_force(c, b, b.vel_alt, keep || cfg.colour, stream));
+        if (cfg.colour) SPH_TRY(launch_colour(c, b, colour, dye, stream));
         cur_vel ^= 1;
         last_kind = 1;
         if (timed) SPH_TRY(hipEventRecord(ev[5], stream));
@@ -358,10 +379,10 @@ struct Engine {
         DeviceGuard guard(device);
         for (uint32_t s = 0; s < nsteps; ++s) {
             const bool tm = timing && s + 1 == nsteps;
-            if (!batch && scenes[0].rot_vel != 0.0f) {
+            if (!batch && changes_per_step(scenes[0])) {
                 // the rotor angle (wave phase) advances before every step; the
                 // kernels take the single scene's parameters by value
-                advance_motion(scenes[0]);
+                advance_frame(scenes[0]);
                 cfg.one = scenes[0];
             }
             SPH_TRY_ST(enqueue_step(tm));
@@ -381,7 +402,7 @@ struct Engine {
         if (nsteps == 0 || (nsteps & 1u)) return SPH_ERR_INVALID_ARGUMENT;
         if (!ready()) return SPH_ERR_NOT_INITIALISED;
         // a recording replays fixed parameters: no turning rotor or moving wave
-        if (scenes[0].rot_vel != 0.0f) return SPH_ERR_UNSUPPORTED;
+        if (changes_per_step(scenes[0])) return SPH_ERR_UNSUPPORTED;
         DeviceGuard guard(device);
         drop_graph();
         SPH_TRY(hipStreamSynchronize(stream));
@@ -460,6 +481,15 @@ struct Engine {
             case SPH_ARRAY_DVEL:
                 if (!dvel) return SPH_ERR_UNSUPPORTED;
                 SPH_TRY(launch_scatter_to_original(n, p, dvel, staging, 4, 4, stream));
+                break;
+            case SPH_ARRAY_COLOR:
+                if (!colour) return SPH_ERR_UNSUPPORTED;
+                SPH_TRY(launch_scatter_to_original(n, p, colour, staging, 4, 4, stream));
+                break;
+            case SPH_ARRAY_DYE:
+                if (!dye) return SPH_ERR_UNSUPPORTED;
+                SPH_TRY(hipMemcpyAsync(staging, dye, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+                rec = 1;
                 break;
             case SPH_ARRAY_NEIGHBOURS:
                 if (last_kind != 2) return SPH_ERR_UNSUPPORTED;
@@ -563,7 +593,7 @@ struct Engine {
             return SPH_ERR_INVALID_PARAMS;
         if (!batch && ((1ull << p->key_bits) < cfg.num_cells || p->key_bits > 31))
             return SPH_ERR_INVALID_PARAMS;
-        if (batch && p->rot_vel != 0.0f) return SPH_ERR_UNSUPPORTED;
+        if (batch && changes_per_step(*p)) return SPH_ERR_UNSUPPORTED;
         DeviceGuard guard(device);
         if ((p->cfl > 0.0f || p->t_end > 0.0f) && !device_clock) {
             // from here on the clock and time step advance on the device
@@ -584,6 +614,7 @@ struct Engine {
         cfg.one = scenes[0];
         cfg.physics = 0;
         for (const auto& sc : scenes) cfg.physics |= sc.physics;
+        SPH_TRY(ensure_colour());
         if (batch)
             SPH_TRY(hipMemcpyAsync(d_tab + scene, &cur, sizeof(sph_params), hipMemcpyHostToDevice, stream));
         SPH_TRY(hipMemcpyAsync(dt_dev + scene, &cur.time_step, sizeof(float), hipMemcpyHostToDevice, stream));

@@ -100,6 +100,7 @@ struct LaunchCfg {
     uint32_t precision = 0;     // shared by every scene
     uint32_t tune[4] = {0, 0, 0, 0};
     uint32_t physics = 0;       // union of the scenes' sph_params::physics bits
+    uint32_t colour = 0;        // some scene has a colour mode
     const sph_params* tab = nullptr;      // device table (batch) or null
     const uint32_t* scene_off = nullptr;  // device: first slot of each scene, nscenes + 1
     sph_params one{};                     // the scene (one-scene launches)
@@ -119,6 +120,9 @@ uint32_t stats_chunks(const LaunchCfg& c);
 size_t stats_partials_count(const LaunchCfg& c);
 
 hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
+// Particle colours and dye after the force update: b holds the step's state
+// before the update (pos_cur, vel_cur, rhop, dvel, pid_cur).
+hipError_t launch_colour(const LaunchCfg& c, const StepBuffers& b, float4* colour, float* dye, hipStream_t s);
 hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
                           const uint32_t* old_slot, hipStream_t s);
 hipError_t launch_density(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);

@@ -356,6 +356,160 @@ __global__ __launch_bounds__(kBlock) void k_extra_terms(sph_params one, const sp
     if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
 
+// ---------------------------------------------------------------- colours
+__device__ __forceinline__ float dot4(const float4& a, const float4& b) {
+    return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;  // cutil_math.h:540-543
+}
+__device__ __forceinline__ float len3(float x, float y, float z) { return sqrt_ieee((x * x + y * y) + z * z); }
+__device__ __forceinline__ float phi_clamp(float v, float lo, float hi) {
+    return (fminf(v, hi) - fminf(v, lo)) / (hi - lo);  // phiClamp, System.cu:221-223
+}
+
+// Foam potential of the velocity colour mode (source/CUDA/System.cu:424-486):
+// the trapped-air potential from the velocity differences of every other
+// particle in the 27 cells (x, y, z order, at most max_par_in_cell per
+// cell) weighted by W(r) = (1 - r)/3 for r <= 3, clamped into [5, 20],
+// times the kinetic-energy potential clamped into [0.1, 1], times 3.
+// normalize() is v * (1/sqrtf(v.v)) over all four components
+// (cutil_math.h:552-556).
+__device__ float foam(const sph_params& p, uint32_t i, const float4& xi, const float4& vi,
+                      const float4* __restrict__ pos, const float4* __restrict__ vel,
+                      const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end) {
+    bool out = false;
+    const int gx = cell_coord(xi.x, p.world_min[0], p.cell_size[0], p.grid_size[0], out);
+    const int gy = cell_coord(xi.y, p.world_min[1], p.cell_size[1], p.grid_size[1], out);
+    const int gz = cell_coord(xi.z, p.world_min[2], p.cell_size[2], p.grid_size[2], out);
+    float vdiff = 0.0f;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int cx = gx + dx, cy = gy + dy, cz = gz + dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)p.grid_size[0] || cy >= (int)p.grid_size[1] ||
+                    cz >= (int)p.grid_size[2])
+                    continue;
+                const uint32_t c = grid_hash(p, cx, cy, cz);
+                const uint32_t s = cell_start[c];
+                if (s == kEmptyCell) continue;
+                const uint32_t e = min(cell_end[c], s + p.max_par_in_cell);
+                for (uint32_t j = s; j < e; ++j) {
+                    if (j == i) continue;
+                    const float4 v2 = vel[j], x2 = pos[j];
+                    const float4 sv = make_float4(vi.x + v2.x, vi.y + v2.y, vi.z + v2.z, vi.w + v2.w);
+                    const float4 sx = make_float4(xi.x + x2.x, xi.y + x2.y, xi.z + x2.z, xi.w + x2.w);
+                    const float iv = 1.0f / sqrt_ieee(dot4(sv, sv)), ix = 1.0f / sqrt_ieee(dot4(sx, sx));
+                    const float4 nv = make_float4(sv.x * iv, sv.y * iv, sv.z * iv, sv.w * iv);
+                    const float4 nx = make_float4(sx.x * ix, sx.y * ix, sx.z * ix, sx.w * ix);
+                    const float4 dv = make_float4(vi.x - v2.x, vi.y - v2.y, vi.z - v2.z, vi.w - v2.w);
+                    const float4 dp = make_float4(xi.x - x2.x, xi.y - x2.y, xi.z - x2.z, xi.w - x2.w);
+                    const float r = sqrt_ieee(dot4(dp, dp));
+                    const float w = r <= 3.0f ? (1.0f - r) / 3.0f : 0.0f;
+                    vdiff = vdiff + (sqrt_ieee(dot4(dv, dv)) * (1.0f - dot4(nv, nx))) * w;
+                }
+            }
+    const float trapped = phi_clamp(vdiff, 5.0f, 20.0f);
+    const float ke = ((p.particle_mass * 1000.0f) * dot4(vi, vi)) * 0.5f;
+    const float kpot = phi_clamp(ke, 0.1f, 1.0f);
+    return kpot * (3.0f * trapped + 3.0f * 0.0f);
+}
+
+// computeForceD's colouring and dye (source/CUDA/System.cu:405-546) for
+// one sorted slot: x, v the step's state before the force update, a the
+// step's whole velocity increment, rho the step's density.
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_colour(sph_params one, const sph_params* __restrict__ tab,
+                                                   const uint32_t* __restrict__ sid, uint32_t n,
+                                                   const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                   const float2* __restrict__ rhop,
+                                                   const float4* __restrict__ dvel, const uint32_t* __restrict__ pid,
+                                                   const uint32_t* __restrict__ cell_start,
+                                                   const uint32_t* __restrict__ cell_end,
+                                                   const float* __restrict__ dt_dev, float4* __restrict__ colour,
+                                                   float* __restrict__ dye) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
+    if (p.clr_type == SPH_CLR_NONE) return;  // no colour and no dye update
+    const float dt = dt_dev[scene];
+    const float4 xi = pos[i], vi = vel[i], a = dvel[i];
+    const float dens = rhop[i].x;
+    const float br = p.brightness, ct = p.contrast;
+    float cx = 0.2f, cy = 0.5f, cz = 1.0f, intens = 0.0f;
+    switch (p.clr_type) {
+        case SPH_CLR_VEL_ACC: {
+            const float v = 2.5f * len3(vi.x, vi.y, vi.z);
+            const float f = (0.02f * len3(a.x, a.y, a.z)) / dt;
+            const float kv = br + ct * v, kf = ct * f;
+            cx = cx * kv + 0.7f * kf;
+            cy = cy * kv + 0.35f * kf;
+            cz = cz * kv + 0.0f * kf;
+        } break;
+        case SPH_CLR_DENS_ACC: {
+            const float d = (4.0f * (dens - p.rest_density)) / p.rest_density;
+            const float f = (0.02f * len3(a.x, a.y, a.z)) / dt;
+            const float kd = br + ct * d, kf = ct * f;
+            cx = cx * kd + 0.7f * kf;
+            cy = cy * kd + 0.7f * kf;
+            cz = cz * kd + 0.0f * kf;
+        } break;
+        case SPH_CLR_VEL: {
+            const float v = 2.5f * len3(vi.x, vi.y, vi.z);
+            intens = br + ct * v;
+            const float nd = foam(p, i, xi, vi, pos, vel, cell_start, cell_end);
+            cx = cy = cz = nd;
+        } break;
+        case SPH_CLR_VEL_RGB: {
+            const float k = ct * 2.5f;
+            cx = 0.5f + vi.x * k;
+            cy = 0.5f + vi.y * k;
+            cz = 0.5f + vi.z * k;
+        } break;
+        case SPH_CLR_ACCEL: {
+            const float f = (0.02f * len3(a.x, a.y, a.z)) / dt;
+            const float kf = br + ct * f;
+            intens = kf;
+            cx = cx * kf;
+            cy = cy * kf;
+            cz = cz * kf;
+        } break;
+        default: {  // SPH_CLR_DENS
+            const float d = (4.0f * (dens - p.rest_density)) / p.rest_density;
+            const float kd = br + ct * d;
+            intens = kd;
+            cx = cx * kd;
+            cy = cy * kd;
+            cz = cz * kd;
+        } break;
+    }
+    if (p.hue == 1) {
+        cx = intens;
+        cy = 0.0f;
+    }
+    const uint32_t si = pid[i];
+    if (p.dye_clear > 0) {
+        dye[si] = 0.0f;
+    } else if (p.dye_type > 0) {
+        float dc = dye[si];
+        const float rx = xi.x - p.dye_pos[0], ry = xi.y - p.dye_pos[1], rz = xi.z - p.dye_pos[2];
+        if (p.dye_type == SPH_DYE_BOX) {
+            if (fabsf(rx) < p.dye_size[0] && fabsf(ry) < p.dye_size[1] && fabsf(rz) < p.dye_size[2]) dc = 1.0f;
+        } else if (len3(rx, ry, rz) < p.dye_size[1]) {
+            dc = 1.0f;
+        }
+        dc = dc - dt * p.dye_fade;
+        if (dc < 0.0f) dc = 0.0f;
+        if (dc >= 0.0f) dye[si] = dc;
+        if (p.hue == 0) {
+            cx = cx + 0.9f * dc;
+            cy = cy + 0.9f * dc;
+            cz = cz + 1.0f * dc;
+        } else {
+            cy = dc;
+        }
+    }
+    colour[i] = make_float4(cx, cy, cz, 1.0f);
+}
+
 // ------------------------------------------------------------ integrate+hash
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_integrate_hash(
@@ -1489,6 +1643,17 @@ hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_ou
             hipLaunchKernelGGL(k_extra_terms<false>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n,
                                b.pos_cur, b.vel_cur, vel_out, dvel, b.dt_dev);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_colour(const LaunchCfg& c, const StepBuffers& b, float4* colour, float* dye, hipStream_t s) {
+    const dim3 grid(blocks_for(c.n, kBlock));
+    if (c.tab)
+        hipLaunchKernelGGL(k_colour<true>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.vel_cur, b.rhop, b.dvel, b.pid_cur, b.cell_start, b.cell_end, b.dt_dev, colour, dye);
+    else
+        hipLaunchKernelGGL(k_colour<false>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.vel_cur, b.rhop, b.dvel, b.pid_cur, b.cell_start, b.cell_end, b.dt_dev, colour, dye);
     return hipGetLastError();
 }
 

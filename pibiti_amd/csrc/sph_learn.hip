@@ -429,9 +429,10 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
                  sph_learned_report* rep) {
     SPH_TRY_ST(check_model(e, m));
     if (factor == 0 || !(tol > 0.0f) || !(skin >= 0.0f) || !(skin <= 1.0f)) return SPH_ERR_INVALID_ARGUMENT;
-    // the surrogate replaces the pair force only: no collider, accelerators
-    // or moving wave
-    if (e.batch || e.device_clock || e.scenes[0].physics != 0 || e.scenes[0].rot_vel != 0.0f)
+    // the surrogate replaces the pair force only: no collider, accelerators,
+    // moving wave or colours
+    if (e.batch || e.device_clock || e.scenes[0].physics != 0 || changes_per_step(e.scenes[0]) ||
+        e.scenes[0].clr_type != SPH_CLR_NONE)
         return SPH_ERR_UNSUPPORTED;
     if (!e.ready() || e.last_kind == 0) return SPH_ERR_NOT_INITIALISED;
     LearnState* L = learn_state(e);

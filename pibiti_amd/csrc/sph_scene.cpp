@@ -94,7 +94,18 @@ int check_shape(const sph_params& p) {
             if (p.rot_size[c] < 0 || p.rot_size[c] > 256) return SPH_ERR_INVALID_PARAMS;
     }
     if (p.hmap_type > 0 && !(p.rot_spc > 0.0f)) return SPH_ERR_INVALID_PARAMS;
+    if (p.clr_type > SPH_CLR_NONE || p.dye_type < 0 || p.dye_type > SPH_DYE_SPHERE || p.hue < 0 || p.hue > 1)
+        return SPH_ERR_INVALID_PARAMS;
     return SPH_OK;
+}
+
+bool changes_per_step(const sph_params& p) { return p.rot_vel != 0.0f || p.dye_clear > 0; }
+
+void advance_frame(sph_params& p) {
+    // App::UpdateEmitter counts the dye clear down before each step
+    // (source/App/Update.cpp:23)
+    if (p.dye_clear > 0) p.dye_clear = p.dye_clear - 1;
+    advance_motion(p);
 }
 
 void advance_motion(sph_params& p) {
@@ -200,6 +211,22 @@ int sph_scene_default(sph_scene* s) {
     s->rot2_vel = 1.0f;
     s->rot2_twist = 1.0f;
     s->hmap_type = SPH_HMAP_NONE;
+    // colours and dye (source/SPH/Scene.cpp:61-64).  No colour output by
+    // default (the reference's app starts in CLR_VelAcc, which the scene-file
+    // loader keeps); the dye-clear count starts at 0 for the same reason.
+    s->clr_type = SPH_CLR_NONE;
+    s->hue = 0;
+    s->brightness = 0.3f;
+    s->contrast = 0.3f;
+    s->dye_type = SPH_DYE_OFF;
+    s->dye_clear = 0;
+    s->dye_fade = 1.0f;
+    s->dye_pos[0] = 0.0f;
+    s->dye_pos[1] = -0.21f;
+    s->dye_pos[2] = 0.0f;
+    s->dye_size[0] = 0.008f;
+    s->dye_size[1] = 0.012f;
+    s->dye_size[2] = 0.008f;
     // source/SPH/Scene.cpp:54-57
     for (int k = 0; k < SPH_NUM_ACCEL; ++k) {
         sph_accel& a = s->acc[k];
@@ -280,6 +307,17 @@ int sph_params_from_scene(const sph_scene* s, sph_params* p) {
     p->rot2_vel = s->rot2_vel;
     p->rot2_twist = s->rot2_twist;
     p->hmap_type = s->hmap_type;
+    p->clr_type = s->clr_type;
+    p->hue = s->hue;
+    p->brightness = s->brightness;
+    p->contrast = s->contrast;
+    p->dye_type = s->dye_type;
+    p->dye_clear = s->dye_clear;
+    p->dye_fade = s->dye_fade;
+    for (int c = 0; c < 3; ++c) {
+        p->dye_pos[c] = s->dye_pos[c];
+        p->dye_size[c] = s->dye_size[c];
+    }
 
     // Scene::_UpdateGrid (source/SPH/Scene.cpp:98-109).
     uint64_t cells = 1;

@@ -417,6 +417,9 @@ void finish(SceneVals& v) {
     sph_scene_info& in = v.info;
     // the collider acts wherever no rotor runs (the force update decides)
     s.collider = 1;
+    // the app's colour mode and dye clear count (source/SPH/Scene.cpp:61-63)
+    s.clr_type = SPH_CLR_VEL_ACC;
+    s.dye_clear = 2;
     for (int c = 0; c < 3; ++c) s.coll_pos[c] = v.coll_pos[c];
     // _UpdatePar: spacing *= particleR; _UpdateGrid: the init volume moves in
     // by distBndSoft*particleR - particleR on every side (Scene.cpp:80,100-102)

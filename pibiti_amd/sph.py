@@ -20,8 +20,11 @@ SPH_OK = 0
 SPH_EOS_LINEAR, SPH_EOS_TAIT = 0, 1
 SPH_PREC_REFERENCE, SPH_PREC_KAHAN = 0, 1
 (SPH_ARRAY_POSITION, SPH_ARRAY_VELOCITY, SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_DVEL,
- SPH_ARRAY_TARGET, SPH_ARRAY_NEIGHBOURS) = range(7)
-_SCALAR_ARRAYS = (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_NEIGHBOURS)
+ SPH_ARRAY_TARGET, SPH_ARRAY_NEIGHBOURS, SPH_ARRAY_COLOR, SPH_ARRAY_DYE) = range(9)
+_SCALAR_ARRAYS = (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_NEIGHBOURS, SPH_ARRAY_DYE)
+(SPH_CLR_DENS, SPH_CLR_ACCEL, SPH_CLR_DENS_ACC, SPH_CLR_VEL, SPH_CLR_VEL_ACC, SPH_CLR_VEL_RGB,
+ SPH_CLR_NONE) = range(7)
+SPH_DYE_OFF, SPH_DYE_BOX, SPH_DYE_SPHERE = range(3)
 MLP_NUM_PARAMS = 5388
 SPH_FLAG_KEEP_DVEL = 1
 EMPTY_CELL = 0xFFFFFFFF
@@ -54,6 +57,15 @@ _ROTOR_FIELDS = [
     ("rot2_vel", C.c_float),
     ("rot2_twist", C.c_float),
     ("hmap_type", C.c_int32),
+    ("clr_type", C.c_uint32),
+    ("hue", C.c_int32),
+    ("brightness", C.c_float),
+    ("contrast", C.c_float),
+    ("dye_type", C.c_int32),
+    ("dye_clear", C.c_int32),
+    ("dye_fade", C.c_float),
+    ("dye_pos", f3),
+    ("dye_size", f3),
 ]
 
 _SHAPE_FIELDS = [
@@ -210,6 +222,13 @@ class Params(C.Structure):
         return out
 
 
+class DeviceView(C.Structure):
+    """sph_device_view: device pointers of the current state."""
+
+    _fields_ = [("pos4", C.c_void_p), ("vel4", C.c_void_p), ("colour4", C.c_void_p), ("index", C.c_void_p),
+                ("stream", C.c_void_p), ("n", C.c_uint32), ("device", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("mass", C.c_double),
@@ -306,6 +325,7 @@ SIGNATURES = {
     "sph_get_grid": (C.c_int, [_P, _up, _up, _up, _up]),
     "sph_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "sph_synchronize": (C.c_int, [_P]),
+    "sph_get_device_view": (C.c_int, [_P, C.POINTER(DeviceView)]),
     "sph_set_timing": (C.c_int, [_P, C.c_int]),
     "sph_get_timing": (C.c_int, [_P, _fp]),
     "sph_capture": (C.c_int, [_P, C.c_uint32]),
@@ -519,6 +539,11 @@ class SphSystem:
 
     def update(self, nsteps: int = 1) -> None:
         _check(lib().sph_step(self._ctx, nsteps), "sph_step")
+
+    def device_view(self) -> "DeviceView":
+        v = DeviceView()
+        _check(lib().sph_get_device_view(self._ctx, C.byref(v)), "sph_get_device_view")
+        return v
 
     def synchronize(self) -> None:
         _check(lib().sph_synchronize(self._ctx), "sph_synchronize")

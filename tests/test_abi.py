@@ -133,7 +133,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     structs = {"sph_scene": sph.Scene, "sph_params": sph.Params, "sph_accel": sph.Accel,
-               "sph_emitter": sph.Emitter, "sph_scene_info": sph.SceneInfo,
+               "sph_emitter": sph.Emitter, "sph_scene_info": sph.SceneInfo, "sph_device_view": sph.DeviceView,
                "sph_stats": sph.Stats, "sph_mlp_norm": sph.MlpNorm,
                "sph_learned_report": sph.LearnedReport}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pibiti_sph.h"', "int main(void) {"]
