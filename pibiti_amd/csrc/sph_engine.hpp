@@ -112,6 +112,7 @@ struct Engine {
     uint32_t* sid[2] = {nullptr, nullptr};
     float2* rhop = nullptr;
     float4* dvel = nullptr;
+    uint32_t* masks = nullptr;  // force pair masks (kMaskWords per slot) while cells hold <= 16 candidates
     float4* colour = nullptr;   // per sorted slot, scenes with clr_type != NONE
     float* dye = nullptr;       // per ORIGINAL index (it follows the particle)
     uint32_t *key_pid = nullptr, *val_pid = nullptr, *key_tmp = nullptr, *val_tmp = nullptr;
@@ -166,6 +167,7 @@ struct Engine {
         b.partials = partials;
         b.stats = stats;
         b.dt_dev = dt_dev;
+        b.masks = cfg.max_cap <= kMaskCap ? masks : nullptr;
         return b;
     }
 
@@ -186,7 +188,7 @@ struct Engine {
             (void)hipFree(pid[k]);
             (void)hipFree(sid[k]);
         }
-        void* ptrs[] = {rhop,     dvel,     colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
+        void* ptrs[] = {rhop,     dvel,     masks,    colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
                         cell_start, cell_end, counters, cell_max, work,    partials,
                         stats,    dt_dev,   staging,  sort_scratch, d_tab, d_scene_off};
         for (void* q : ptrs) (void)hipFree(q);
@@ -259,6 +261,7 @@ struct Engine {
             if (e == hipSuccess && as_batch) e = dalloc(&sid[k], N);
         }
         if (e == hipSuccess) e = dalloc(&rhop, N);
+        if (e == hipSuccess && max_cap <= kMaskCap) e = dalloc(&masks, N * kMaskWords);
         if (e == hipSuccess && (flags & SPH_FLAG_KEEP_DVEL)) e = dalloc(&dvel, N);
         if (e == hipSuccess) e = dalloc(&key_pid, N);
         if (e == hipSuccess) e = dalloc(&val_pid, N);
@@ -628,7 +631,7 @@ struct Engine {
         hipDeviceProp_t prop{};
         (void)hipGetDeviceProperties(&prop, device);
         char buf[512];
-        std::snprintf(buf, sizeof(buf), "%s|cu%d|n%u|s%u|c%u|cap%u|p%u|nb%u", prop.gcnArchName,
+        std::snprintf(buf, sizeof(buf), "v2|%s|cu%d|n%u|s%u|c%u|cap%u|p%u|nb%u", prop.gcnArchName,
                       prop.multiProcessorCount, cfg.n, cfg.nscenes, cfg.num_cells, cfg.max_cap,
                       cfg.precision, cfg.max_scene_n);
         return buf;
@@ -734,7 +737,9 @@ struct Engine {
             return r;
         };
         e = snap(true);
-        const uint32_t lists[] = {16, 24, 32};
+        // 2 = the pair-mask path (force lists filled from the density
+        // kernel's masks), else the list size of the candidate-testing force
+        const uint32_t lists[] = {2, 16, 24, 32};
         // gather grid and variant: per-lane loads in XCD order, cell tables in
         // LDS, or a persistent grid
         const uint32_t shapes[][2] = {{0, 0}, {0, 2}, {2 * cus, 0}, {4 * cus, 0}, {8 * cus, 0}};
@@ -743,9 +748,10 @@ struct Engine {
         const uint32_t saved_tune[4] = {cfg.tune[0], cfg.tune[1], cfg.tune[2], cfg.tune[3]};
         int st = SPH_OK;
         for (uint32_t li : lists) {
-            if (cfg.max_cap > li) continue;
+            if (li == 2 ? (cfg.max_cap > kMaskCap || !masks) : cfg.max_cap > li) continue;
             for (const auto& sh : shapes) {
                 if (e != hipSuccess || st != SPH_OK) break;
+                if (li == 2 && (sh[0] != 0 || sh[1] != 0)) continue;  // one launch shape
                 const uint32_t g = sh[0];
                 const uint32_t cand[4] = {li, g, 0, sh[1]};
                 set_tune(cand);

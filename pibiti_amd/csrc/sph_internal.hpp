@@ -86,7 +86,13 @@ struct StepBuffers {
     double* partials;     // per (scene, chunk) FP64 partials
     double* stats;        // kNumStats doubles per scene
     float* dt_dev;        // time step of each scene
+    uint32_t* masks;      // force pair masks of the density kernel (kMaskWords per slot), or null
 };
+
+// Force pair masks: 27 cells x 16 bits in 14 words per slot (word w of slot
+// i at masks[w n + i]), usable while every cell holds at most 16 candidates.
+constexpr uint32_t kMaskWords = 14;
+constexpr uint32_t kMaskCap = 16;
 
 // What a launch covers: one scene (params by value) or a batch of scenes
 // (device parameter table + per-slot scene ids + static slot ranges).
@@ -107,8 +113,11 @@ struct LaunchCfg {
 };
 
 // Launch tuning words (LaunchCfg::tune, from sph_params::tune):
-//   tune[0]  force kernel: 1 = fused candidate/pair loop, 16/24/32/48 =
-//            neighbour-list entries per lane (default 24)
+//   tune[0]  force kernel: 1 = fused candidate/pair loop, 2 = pair masks
+//            written by the density kernel (max_par_in_cell <= 16; also the
+//            default when every tune word is 0), 16/24/32/48 = neighbour-list
+//            entries per lane filled by the force kernel's own candidate
+//            tests (the default otherwise: 24)
 //   tune[1]  gather grid: 0 = one thread per slot over all slots, else a
 //            persistent grid of tune[1] blocks pulling 256-slot tiles
 //   tune[3]  gather variant (tune[1] == 0): 0 = cell tables and candidate

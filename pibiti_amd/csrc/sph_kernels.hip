@@ -694,7 +694,15 @@ __device__ bool stage_block(Stage<kCap>& S, uint32_t b0, uint32_t count, const s
 // (the slots that pass r2 < h2, j != i, in walk order) in the group-major
 // lists (ListRing), and their count in nnb[i]; a count above kNbrCap
 // raises the overflow counter (entries past the cap are dropped).
-template <bool kBatch, int kPrec, bool kLists = false, typename Src = PosGlobal, bool kCells = false>
+//
+// With kMasks it records instead, per neighbour cell in walk order, a 16-bit
+// mask of the cell's slots (bit = slot - cell start) that pass the force
+// predicate (r2 <= force_r2_max, j != i): the force kernel's pairs, so that
+// it needs no distance tests of its own.  Cells hold at most 16 slots here.
+// Masks of cells 2w and 2w+1 share word w (low, high half): kMaskWords
+// words per slot, word w of slot i at nbr[w n + i].
+template <bool kBatch, int kPrec, bool kLists = false, typename Src = PosGlobal, bool kCells = false,
+          bool kMasks = false>
 __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& one,
                                             const sph_params* __restrict__ tab,
                                             const uint32_t* __restrict__ sid,
@@ -726,6 +734,8 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     uint4* const lists = reinterpret_cast<uint4*>(nbr);
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
+    const float r2f = p.force_r2_max;
+    uint32_t mword = 0;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
         const int dx = ord % 3 - 1, dy = (ord / 3) % 3 - 1, dz = ord / 9 - 1;
@@ -734,36 +744,46 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
             open = w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
         else
             open = w.open(dx, dy, dz, cell_start, cell_end, s, e);
-        if (!open) continue;
-        float cell = 0.0f;
-        const int32_t soff = src.offset(ord / 3);
-        // candidates four at a time: the loads go out together (indices past
-        // the cell repeat its last slot and are masked out of the sums)
-        for (uint32_t j0 = s; j0 < e; j0 += 4) {
-            float4 xq[4];
+        uint32_t cmask = 0;
+        if (open) {
+            float cell = 0.0f;
+            const int32_t soff = src.offset(ord / 3);
+            // candidates four at a time: the loads go out together (indices
+            // past the cell repeat its last slot and are masked out of the sums)
+            for (uint32_t j0 = s; j0 < e; j0 += 4) {
+                float4 xq[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) xq[k] = src.at(min(j0 + k, e - 1), soff);
+                for (int k = 0; k < 4; ++k) xq[k] = src.at(min(j0 + k, e - 1), soff);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t j = j0 + k;
-                const float4 xj = xq[k];
-                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                const bool hit = (r2 < h2) && (j != i) && (j < e);
-                const float c3 = h2 - r2;
-                const float wgt = (c3 * c3) * c3;
-                if (kPrec == SPH_PREC_KAHAN) {
-                    if (hit) ks.add(wgt);
-                } else {
-                    // the partial sum is >= +0, so adding +0 for a miss is exact
-                    cell = cell + (hit ? wgt : 0.0f);
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t j = j0 + k;
+                    const float4 xj = xq[k];
+                    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    const bool hit = (r2 < h2) && (j != i) && (j < e);
+                    const float c3 = h2 - r2;
+                    const float wgt = (c3 * c3) * c3;
+                    if (kPrec == SPH_PREC_KAHAN) {
+                        if (hit) ks.add(wgt);
+                    } else {
+                        // the partial sum is >= +0, so adding +0 for a miss is exact
+                        cell = cell + (hit ? wgt : 0.0f);
+                    }
+                    if constexpr (kLists) lw.put(j, hit);
+                    if constexpr (kMasks) cmask |= ((r2 <= r2f) && (j != i) && (j < e) ? 1u : 0u) << (j - s);
                 }
-                if constexpr (kLists) lw.put(j, hit);
+                if constexpr (kLists) lw.drain(lists, n, i);
             }
-            if constexpr (kLists) lw.drain(lists, n, i);
+            if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
         }
-        if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
+        if constexpr (kMasks) {
+            if (ord & 1)
+                nbr[(uint32_t)(ord >> 1) * n + i] = mword | (cmask << 16);
+            else
+                mword = cmask;
+        }
     }
+    if constexpr (kMasks) nbr[(kMaskWords - 1u) * n + i] = mword;
     if (kPrec == SPH_PREC_KAHAN) sum = ks.s;
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
     rhop[i] = make_float2(rho, eos(p, rho));
@@ -898,7 +918,12 @@ __device__ __forceinline__ void force_drain(const sph_params& p, const uint32_t*
 
 // One slot per lane; every lane of the wave must call this (inactive lanes
 // with active = false) because the list drains are wave-wide decisions.
-template <bool kBatch, int kPrec, int kList, int kB = kBlock, typename Src = PosGlobal, bool kCells = false>
+//
+// With kMasks the pair masks of the density kernel (density_one) replace the
+// candidate tests: each neighbour cell's passing slots go straight to the
+// lane's list.
+template <bool kBatch, int kPrec, int kList, int kB = kBlock, typename Src = PosGlobal, bool kCells = false,
+          bool kMasks = false>
 __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* list,
                                           const sph_params& one, const sph_params* __restrict__ tab,
                                           const uint32_t* __restrict__ sid,
@@ -909,7 +934,8 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
                                           const uint32_t* __restrict__ cell_end,
                                           float4* __restrict__ vel_out, float4* __restrict__ dvel,
                                           const float* __restrict__ dt_dev, const Src* src_in = nullptr,
-                                          const CellView* cv = nullptr) {
+                                          const CellView* cv = nullptr, const uint32_t* __restrict__ masks = nullptr,
+                                          uint32_t n = 0) {
     const int t = threadIdx.x;
     Src src{};
     if constexpr (std::is_same<Src, PosGlobal>::value) {
@@ -932,6 +958,7 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
     int cnt = 0;
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
+    uint32_t mword = 0;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s = 0, e = 0;
         const int dx = ord % 3 - 1, dy = (ord / 3) % 3 - 1, dz = ord / 9 - 1;
@@ -940,6 +967,21 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
             open = active && w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
         else
             open = active && w.open(dx, dy, dz, cell_start, cell_end, s, e);
+        if constexpr (kMasks) {
+            if ((ord & 1) == 0) mword = masks[(uint32_t)(ord >> 1) * n + ii];
+            uint32_t m = open ? ((ord & 1) ? (mword >> 16) : (mword & 0xFFFFu)) : 0u;
+            if (__any(cnt + __popc(m) > kList)) {
+                force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
+                cnt = 0;
+            }
+            const uint32_t tag = (uint32_t)ord << kSlotBits;
+            while (m != 0u) {
+                list[cnt * kB + t] = (s + (uint32_t)__builtin_ctz(m)) | tag;
+                ++cnt;
+                m &= m - 1u;
+            }
+            continue;
+        }
         // every candidate of this cell could pass: make room first
         if (__any(cnt + (int)(e - s) > kList)) {
             force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
@@ -1067,6 +1109,19 @@ __global__ __launch_bounds__(kBlock) void k_density_tiled(
     }
 }
 
+// Density plus the force kernel's pair masks (density_one with kMasks), one
+// slot per lane in XCD-ordered tiles.
+template <bool kBatch, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_density_masks(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ masks) {
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    density_one<kBatch, kPrec, false, PosGlobal, false, true>(i, one, tab, sid, pos, cell_start, cell_end, rhop,
+                                                             n, masks);
+}
+
 // Cell-table stage: the (start, end) pairs of the block's 9 neighbour rows
 // (the key ranges of stage_block), so that opening a neighbour cell reads
 // LDS instead of two dependent global loads.
@@ -1164,17 +1219,22 @@ __global__ __launch_bounds__(kBlock) void k_force_cells(
                                         vel_out, dvel, dt_dev);
 }
 
-template <bool kBatch, int kPrec, int kList, int kOrder>
+template <bool kBatch, int kPrec, int kList, int kOrder, bool kMasks = false>
 __global__ __launch_bounds__(kBlock) void k_force_ordered(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev, const uint32_t* __restrict__ masks = nullptr) {
     __shared__ uint32_t list[(kList + 1) * kBlock];
     const uint32_t i = (kOrder ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x;
-    force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end,
-                                    vel_out, dvel, dt_dev);
+    if constexpr (kMasks)
+        force_one<kBatch, kPrec, kList, kBlock, PosGlobal, false, true>(
+            i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end, vel_out, dvel, dt_dev, nullptr,
+            nullptr, masks, n);
+    else
+        force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end,
+                                        vel_out, dvel, dt_dev);
 }
 
 template <bool kBatch, int kPrec, int kList>
@@ -1454,10 +1514,21 @@ void reorder_impl(const LaunchCfg& c, const StepBuffers& b, const uint32_t* skey
                        b.counters);
 }
 
+// The pair-mask path (density writes the force's pair masks, the force
+// fills its lists from them): tune[0] == 2, and the default launch
+// (tune all 0) whenever cells hold at most 16 candidates.
+bool use_masks(const LaunchCfg& c, const StepBuffers& b) {
+    const bool asked = c.tune[0] == 2 || (c.tune[0] == 0 && c.tune[1] == 0 && c.tune[3] == 0);
+    return asked && b.masks && c.max_cap <= kMaskCap;
+}
+
 template <bool kBatch, int kPrec>
 void density_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
     const uint32_t grid = c.tune[1];
-    if (!grid && c.tune[3] == 2 && b.skey)
+    if (use_masks(c, b))
+        hipLaunchKernelGGL((k_density_masks<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
+                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop, b.masks);
+    else if (!grid && c.tune[3] == 2 && b.skey)
         hipLaunchKernelGGL((k_density_cells<kBatch, kPrec, false>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock),
                            0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start, b.cell_end,
                            b.rhop, nullptr, nullptr, b.counters);
@@ -1508,6 +1579,12 @@ void force_list_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, 
 template <bool kBatch, int kPrec>
 void force_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float4* dvel,
                 hipStream_t s) {
+    if (use_masks(c, b)) {
+        hipLaunchKernelGGL((k_force_ordered<kBatch, kPrec, kForceListDefault, 1, true>), dim3(blocks_for(c.n, kBlock)),
+                           dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.masks);
+        return;
+    }
     uint32_t list = c.tune[0] ? c.tune[0] : (uint32_t)kForceListDefault;
     if (list != 1 && list != 16 && list != 24 && list != 32 && list != 48) list = kForceListDefault;
     if (list != 1 && c.max_cap > list) list = 48;
