@@ -318,7 +318,9 @@ typedef struct sph_stats {
     uint32_t out_of_grid;       /* particles whose cell had to be clamped    */
     uint32_t max_cell_count;    /* largest particle count of one cell        */
     uint32_t sort_errors;       /* nonzero: a sort look-back wait hit its bound (results invalid) */
-    uint32_t reserved[6];
+    uint32_t full_sort;         /* 1: the last step ordered the slots with the full radix sort,
+                                   0: incrementally from the previous order (same order)      */
+    uint32_t reserved[5];
 } sph_stats;
 
 typedef struct sph_ctx sph_ctx;

@@ -116,15 +116,22 @@ struct Engine {
     float4* colour = nullptr;   // per sorted slot, scenes with clr_type != NONE
     float* dye = nullptr;       // per ORIGINAL index (it follows the particle)
     uint32_t *key_pid = nullptr, *val_pid = nullptr, *key_tmp = nullptr, *val_tmp = nullptr;
-    uint32_t *cell_start = nullptr, *cell_end = nullptr, *counters = nullptr, *cell_max = nullptr;
+    // cell ranges and slot keys, two of each: a step builds the next order
+    // from the current one (cur_cell / cur_key pick the current)
+    uint32_t* cell_start[2] = {nullptr, nullptr};
+    uint32_t* cell_end[2] = {nullptr, nullptr};
+    uint32_t* skey_buf[2] = {nullptr, nullptr};
+    uint32_t *counters = nullptr, *cell_max = nullptr;
     uint32_t* work = nullptr;
     double *partials = nullptr, *stats = nullptr;
     float* dt_dev = nullptr;
     float4* staging = nullptr;
     void* sort_scratch = nullptr;
     RadixSortPlan sort{};
+    void* resort_scratch = nullptr;
+    ResortPlan resort{};
 
-    int cur_pos = 0, cur_vel = 0, cur_pid = 0;
+    int cur_pos = 0, cur_vel = 0, cur_pid = 0, cur_cell = 0, cur_key = 0;
     const uint32_t* last_skey = nullptr;
     std::vector<uint8_t> have_pos, have_vel;  // per scene: full upload seen
     bool stepped = false;
@@ -156,12 +163,15 @@ struct Engine {
         b.rhop = rhop;
         b.dvel = dvel;
         b.skey = last_skey;
+        b.skey_alt = skey_buf[cur_key ^ 1];
         b.key_pid = key_pid;
         b.val_pid = val_pid;
         b.key_tmp = key_tmp;
         b.val_tmp = val_tmp;
-        b.cell_start = cell_start;
-        b.cell_end = cell_end;
+        b.cell_start = cell_start[cur_cell];
+        b.cell_end = cell_end[cur_cell];
+        b.cell_start_alt = cell_start[cur_cell ^ 1];
+        b.cell_end_alt = cell_end[cur_cell ^ 1];
         b.counters = counters;
         b.work = work;
         b.partials = partials;
@@ -187,10 +197,13 @@ struct Engine {
             (void)hipFree(vel[k]);
             (void)hipFree(pid[k]);
             (void)hipFree(sid[k]);
+            (void)hipFree(cell_start[k]);
+            (void)hipFree(cell_end[k]);
+            (void)hipFree(skey_buf[k]);
         }
         void* ptrs[] = {rhop,     dvel,     masks,    colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
-                        cell_start, cell_end, counters, cell_max, work,    partials,
-                        stats,    dt_dev,   staging,  sort_scratch, d_tab, d_scene_off};
+                        counters, cell_max, work,    partials,
+                        stats,    dt_dev,   staging,  sort_scratch, resort_scratch, d_tab, d_scene_off};
         for (void* q : ptrs) (void)hipFree(q);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -267,8 +280,11 @@ struct Engine {
         if (e == hipSuccess) e = dalloc(&val_pid, N);
         if (e == hipSuccess) e = dalloc(&key_tmp, N);
         if (e == hipSuccess) e = dalloc(&val_tmp, N);
-        if (e == hipSuccess) e = dalloc(&cell_start, NC);
-        if (e == hipSuccess) e = dalloc(&cell_end, NC);
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = dalloc(&cell_start[k], NC);
+            if (e == hipSuccess) e = dalloc(&cell_end[k], NC);
+            if (e == hipSuccess) e = dalloc(&skey_buf[k], N);
+        }
         if (e == hipSuccess) e = dalloc(&counters, (size_t)kCntNum);
         if (e == hipSuccess) e = dalloc(&cell_max, (size_t)1);
         if (e == hipSuccess) e = dalloc(&work, (size_t)4);
@@ -280,6 +296,8 @@ struct Engine {
         if (e == hipSuccess && as_batch) e = dalloc(&d_tab, (size_t)nscenes);
         if (e == hipSuccess) e = hipMalloc(&sort_scratch, radix_scratch_bytes((uint32_t)N));
         if (e == hipSuccess) e = radix_plan_init(sort, (uint32_t)N, sort_scratch, stream);
+        if (e == hipSuccess) e = hipMalloc(&resort_scratch, resort_scratch_bytes((uint32_t)N, (uint32_t)NC));
+        if (e == hipSuccess) e = resort_plan_init(resort, (uint32_t)N, (uint32_t)NC, resort_scratch, stream);
         for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_scene_off, scene_off.data(), (nscenes + 1) * sizeof(uint32_t),
@@ -290,8 +308,12 @@ struct Engine {
         if (e == hipSuccess) e = launch_iota_u32(pid[0], (uint32_t)N, stream);
         if (e == hipSuccess && as_batch)
             e = launch_scene_ids((uint32_t)N, nscenes, d_scene_off, pid[0], sid[0], stream);
-        if (e == hipSuccess) e = launch_fill_u32(cell_start, kEmptyCell, NC, stream);
-        if (e == hipSuccess) e = launch_fill_u32(cell_end, kEmptyCell, NC, stream);
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = launch_fill_u32(cell_start[k], kEmptyCell, NC, stream);
+            if (e == hipSuccess) e = launch_fill_u32(cell_end[k], kEmptyCell, NC, stream);
+            if (e == hipSuccess) e = launch_fill_u32(skey_buf[k], 0u, N, stream);
+        }
+        last_skey = skey_buf[0];
         if (e == hipSuccess) e = launch_fill_u32(counters, 0u, kCntNum, stream);
         if (e == hipSuccess) e = launch_fill_u32(work, 0u, 4, stream);
         if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(rhop), 0u, 2 * N, stream);
@@ -342,17 +364,17 @@ struct Engine {
         if (timed) SPH_TRY(hipEventRecord(ev[0], stream));
         if (device_clock) SPH_TRY(launch_stats(c, buffers(), true, stream));
         SPH_TRY(launch_fill_u32(counters, 0u, kCntNum, stream));
-        SPH_TRY(launch_integrate_hash(c, buffers(), stream));
+        SPH_TRY(launch_integrate_moves(c, buffers(), resort, key_tmp, stream));
         if (timed) SPH_TRY(hipEventRecord(ev[1], stream));
-        uint32_t *skey = nullptr, *oslot = nullptr;
-        SPH_TRY(radix_sort_pairs(sort, c.key_bits, key_pid, val_pid, key_tmp, val_tmp, &skey, &oslot,
-                                 stream));
+        SPH_TRY(resort_order(c, buffers(), resort, stream));
         if (timed) SPH_TRY(hipEventRecord(ev[2], stream));
-        SPH_TRY(launch_reorder(c, buffers(), skey, oslot, stream));
+        SPH_TRY(resort_apply(c, buffers(), resort, sort, stream));
         cur_pos ^= 1;
         cur_vel ^= 1;
         cur_pid ^= 1;
-        last_skey = skey;
+        cur_cell ^= 1;
+        cur_key ^= 1;
+        last_skey = skey_buf[cur_key];
         if (timed) SPH_TRY(hipEventRecord(ev[3], stream));
         SPH_TRY(launch_density(c, buffers(), stream));
         if (timed) SPH_TRY(hipEventRecord(ev[4], stream));
@@ -409,7 +431,8 @@ struct Engine {
         DeviceGuard guard(device);
         drop_graph();
         SPH_TRY(hipStreamSynchronize(stream));
-        const int p0 = cur_pos, v0 = cur_vel, q0 = cur_pid, k0 = last_kind;
+        const int p0 = cur_pos, v0 = cur_vel, q0 = cur_pid, k0 = last_kind, c0 = cur_cell, key0 = cur_key;
+        const uint32_t* s0 = last_skey;
         SPH_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int st = SPH_OK;
         for (uint32_t s = 0; s < nsteps && st == SPH_OK; ++s) st = enqueue_step(false);
@@ -418,6 +441,9 @@ struct Engine {
         cur_pos = p0;
         cur_vel = v0;
         cur_pid = q0;
+        cur_cell = c0;
+        cur_key = key0;
+        last_skey = s0;
         last_kind = k0;  // recording ran nothing
         if (st != SPH_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -526,11 +552,12 @@ struct Engine {
         SPH_TRY(launch_stats(c, buffers(), false, stream));
         SPH_TRY(launch_fill_u32(cell_max, 0u, 1, stream));
         if (stepped) SPH_TRY(launch_cell_stats(c, buffers(), cell_max, stream));
-        uint32_t cnt[kCntNum], cmax = 0, sort_err = 0;
+        uint32_t cnt[kCntNum], cmax = 0, sort_err = 0, resort_err = 0;
         SPH_TRY(hipMemcpyAsync(st.data(), stats, st.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipMemcpyAsync(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipMemcpyAsync(&cmax, cell_max, sizeof(cmax), hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipMemcpyAsync(&sort_err, sort.err, sizeof(sort_err), hipMemcpyDeviceToHost, stream));
+        SPH_TRY(hipMemcpyAsync(&resort_err, resort.err, sizeof(resort_err), hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipStreamSynchronize(stream));
         // the next step's time step (as the next step will compute it) for reporting
         const double dt_next = st[(size_t)scene * kNumStats + kSDt];
@@ -550,7 +577,8 @@ struct Engine {
         out->cap_cells = cnt[kCntCapCells];
         out->out_of_grid = cnt[kCntOutOfGrid];
         out->max_cell_count = cmax;
-        out->sort_errors = sort_err;
+        out->sort_errors = sort_err | resort_err;
+        out->full_sort = cnt[kCntFullSort] != 0;
         return SPH_OK;
     }
 
@@ -566,8 +594,8 @@ struct Engine {
             tmp.resize(ncell);
             csp = tmp.data();
         }
-        if (csp) SPH_TRY(hipMemcpyAsync(csp, cell_start, ncell * 4, hipMemcpyDeviceToHost, stream));
-        if (ce) SPH_TRY(hipMemcpyAsync(ce, cell_end, ncell * 4, hipMemcpyDeviceToHost, stream));
+        if (csp) SPH_TRY(hipMemcpyAsync(csp, cell_start[cur_cell], ncell * 4, hipMemcpyDeviceToHost, stream));
+        if (ce) SPH_TRY(hipMemcpyAsync(ce, cell_end[cur_cell], ncell * 4, hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipStreamSynchronize(stream));
         // cellEnd is only maintained for occupied cells; report empty ones as
         // the reference marks an empty cellStart.
@@ -759,6 +787,8 @@ struct Engine {
                 cur_vel = v0;
                 cur_pid = q0;
                 e = snap(false);
+                // the slots are back in the saved order, the cell ranges are not
+                if (e == hipSuccess) e = resort_invalidate(resort, stream);
                 if (e != hipSuccess) break;
                 st = enqueue_step(false);  // warm-up
                 if (st != SPH_OK) break;
@@ -782,6 +812,7 @@ struct Engine {
         cur_vel = v0;
         cur_pid = q0;
         if (e == hipSuccess) e = snap(false);
+        if (e == hipSuccess) e = resort_invalidate(resort, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         cleanup();
         steps = steps0;

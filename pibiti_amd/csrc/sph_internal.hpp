@@ -22,7 +22,8 @@ enum Counter : int {
     kCntNbrOverflow = 2,  // particles with more than kNbrCap listed neighbours
     kCntPairs = 3,        // listed neighbour pairs (min(count, kNbrCap) summed)
     kCntSkinMoved = 4,    // particles farther than the skin from their position at the last sort
-    kCntNum = 5
+    kCntFullSort = 5,     // nonzero: this step's order comes from the full radix sort
+    kCntNum = 6
 };
 
 // Per-scene FP64 diagnostics: stats[scene * kNumStats + k].
@@ -57,10 +58,44 @@ hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipSt
 // Stable LSD sort of (keys, vals) by the low `bits` bits of keys.  keys0/vals0
 // hold the input; keys1/vals1 are the ping-pong buffers.  On return
 // *keys_out/*vals_out point at whichever pair holds the sorted result.
+// With a gate, every kernel of the sort returns at once unless *gate != 0
+// (a device word written by an earlier launch on the stream).
 // Graph-capturable (no host-side state changes between launches).
 hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* keys0,
                             uint32_t* vals0, uint32_t* keys1, uint32_t* vals1,
-                            uint32_t** keys_out, uint32_t** vals_out, hipStream_t stream);
+                            uint32_t** keys_out, uint32_t** vals_out, hipStream_t stream,
+                            const uint32_t* gate = nullptr);
+
+// ---- incremental re-sort (sph_resort.hip) ----------------------------------
+// Between steps most particles keep their cell.  The step's new (cell, index)
+// order is built from the last one: the integrate kernel marks the slots
+// whose cell changed (movers) and counts them into their new cells; a scan
+// over the cells gives the new cell ranges; each slot's new position is its
+// new cell's start plus the number of particles of that cell with a smaller
+// original index (stayers counted from the old ranges and the mover bits,
+// movers from the cell's list of incoming indices).  The result equals the
+// stable radix sort of (cell, index) exactly.  The full radix sort runs
+// instead (decided on the device) when a cell receives more than
+// kMaxIncoming movers or holds more than kMaxCellCount particles.
+constexpr uint32_t kMaxIncoming = 64;
+constexpr uint32_t kMaxCellCount = 4096;
+
+struct ResortPlan {
+    uint32_t n = 0, num_cells = 0, num_tiles = 0;
+    uint32_t* valid = nullptr;   // device word: slot order, skey and the current cell ranges agree
+    uint32_t* in_cnt = nullptr;  // movers into each cell (zero between steps)
+    uint64_t* moved = nullptr;   // one bit per slot: the cell changed this step
+    uint64_t* status = nullptr;  // look-back words of the cell scan, one per tile
+    uint64_t* ticket = nullptr;  // tile ticket counter (monotone)
+    uint32_t* err = nullptr;     // set when a look-back wait hit its limit
+};
+size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells);
+// Carves the plan out of `scratch`, clears it (not graph-capturable) and
+// marks the order invalid.
+hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, void* scratch, hipStream_t s);
+// Marks the order invalid (graph-capturable): the next step places every
+// particle from scratch.
+hipError_t resort_invalidate(const ResortPlan& plan, hipStream_t s);
 
 // ---- step kernels (sph_kernels.hip) ---------------------------------------
 struct StepBuffers {
@@ -75,12 +110,15 @@ struct StepBuffers {
     float2* rhop;         // (density, pressure) per sorted slot
     float4* dvel;         // optional force increment per sorted slot
     const uint32_t* skey; // cell key of each sorted slot (from the last sort)
+    uint32_t* skey_alt;   // cell key of each slot of the next order (incremental re-sort)
     uint32_t* key_pid;    // cell key in original-index order (sort input)
     uint32_t* val_pid;    // current slot in original-index order (sort input)
     uint32_t* key_tmp;
     uint32_t* val_tmp;
     uint32_t* cell_start;
     uint32_t* cell_end;
+    uint32_t* cell_start_alt;  // cell ranges of the next order (incremental re-sort)
+    uint32_t* cell_end_alt;
     uint32_t* counters;
     uint32_t* work;       // persistent-kernel tickets: [ticket, done] per gather kernel
     double* partials;     // per (scene, chunk) FP64 partials
@@ -129,6 +167,19 @@ uint32_t stats_chunks(const LaunchCfg& c);
 size_t stats_partials_count(const LaunchCfg& c);
 
 hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
+// The step's integration for the incremental re-sort: the new cell key of
+// each slot into nkey (slot order), the mover bits and per-cell mover counts
+// into the plan (every slot is a mover while the plan's order is invalid).
+hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, uint32_t* nkey,
+                                  hipStream_t s);
+// The rest of the re-sort after launch_integrate_moves (nkey = key_tmp).
+// resort_order: new cell ranges into cell_start_alt/cell_end_alt and the
+// movers' incoming lists into val_tmp.  resort_apply: the state in the new
+// order into the *_alt buffers and skey_alt (through key_pid/val_pid and the
+// radix plan on a full-sort step); marks the plan's order valid.
+hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, hipStream_t s);
+hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp,
+                        const RadixSortPlan& sort, hipStream_t s);
 // Particle colours and dye after the force update: b holds the step's state
 // before the update (pos_cur, vel_cur, rhop, dvel, pid_cur).
 hipError_t launch_colour(const LaunchCfg& c, const StepBuffers& b, float4* colour, float* dye, hipStream_t s);

@@ -511,24 +511,13 @@ __global__ __launch_bounds__(kBlock) void k_colour(sph_params one, const sph_par
 }
 
 // ------------------------------------------------------------ integrate+hash
-template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_integrate_hash(
-    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid,
-    uint32_t n, uint32_t num_cells, float4* __restrict__ pos, float4* __restrict__ vel,
-    const uint32_t* __restrict__ pid, uint32_t* __restrict__ key_pid,
-    uint32_t* __restrict__ val_pid, uint32_t* __restrict__ cell_start,
-    uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    // cellStart of the previous step is dead from here on: clear it.
-    for (uint32_t c = tid; c < num_cells; c += nthreads) cell_start[c] = kEmptyCell;
-    if (tid >= n) return;
-
-    uint32_t scene;
-    const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
-    const float dt = dt_dev[scene];
-    float4 x4 = pos[tid];
-    float4 v4 = vel[tid];
+// One slot's boundary forces, gravity, damping, position update and clamp to
+// the hard walls (in place), and its new cell key.
+__device__ __forceinline__ uint32_t integrate_one(const sph_params& p, float dt, float4* __restrict__ pos,
+                                                  float4* __restrict__ vel, uint32_t k,
+                                                  uint32_t* __restrict__ counters) {
+    float4 x4 = pos[k];
+    float4 v4 = vel[k];
     float x = x4.x, y = x4.y, z = x4.z;
     float vx = v4.x, vy = v4.y, vz = v4.z;
     boundary(p, dt, x, y, z, vx, vy, vz);
@@ -557,17 +546,58 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
         if (z > hi) z = hi;
         if (z < lo) z = lo;
     }
-    pos[tid] = make_float4(x, y, z, x4.w);
-    vel[tid] = make_float4(vx, vy, vz, v4.w);
+    pos[k] = make_float4(x, y, z, x4.w);
+    vel[k] = make_float4(vx, vy, vz, v4.w);
 
     bool out = false;
     const int gx = cell_coord(x, p.world_min[0], p.cell_size[0], p.grid_size[0], out);
     const int gy = cell_coord(y, p.world_min[1], p.cell_size[1], p.grid_size[1], out);
     const int gz = cell_coord(z, p.world_min[2], p.cell_size[2], p.grid_size[2], out);
     if (out) atomicAdd(&counters[kCntOutOfGrid], 1u);
+    return grid_hash(p, gx, gy, gz);
+}
+
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_integrate_hash(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid,
+    uint32_t n, uint32_t num_cells, float4* __restrict__ pos, float4* __restrict__ vel,
+    const uint32_t* __restrict__ pid, uint32_t* __restrict__ key_pid,
+    uint32_t* __restrict__ val_pid, uint32_t* __restrict__ cell_start,
+    uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    // cellStart of the previous step is dead from here on: clear it.
+    for (uint32_t c = tid; c < num_cells; c += nthreads) cell_start[c] = kEmptyCell;
+    if (tid >= n) return;
+
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
+    const uint32_t h = integrate_one(p, dt_dev[scene], pos, vel, tid, counters);
     const uint32_t id = pid[tid];
-    key_pid[id] = grid_hash(p, gx, gy, gz);
+    key_pid[id] = h;
     val_pid[id] = tid;
+}
+
+// The same integration for the incremental re-sort: the new key stays in
+// slot order, a slot whose key changed (every slot while the order is not
+// valid) is a mover: its bit is set and its new cell's mover count raised.
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_integrate_moves(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ skey,
+    uint32_t* __restrict__ nkey, const uint32_t* __restrict__ valid, uint64_t* __restrict__ moved,
+    uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n) return;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
+    const uint32_t h = integrate_one(p, dt_dev[scene], pos, vel, tid, counters);
+    const bool mv = *valid == 0 || skey[tid] != h;
+    nkey[tid] = h;
+    // the active lanes of the last wave are a prefix of it: lane 0 writes
+    const uint64_t bits = __ballot(mv);
+    if ((threadIdx.x & 63) == 0) moved[tid >> 6] = bits;
+    if (mv) atomicAdd(&in_cnt[h], 1u);
 }
 
 // ------------------------------------------------------------------ reorder
@@ -1631,6 +1661,18 @@ hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipSt
         integrate_impl<true>(c, b, s);
     else
         integrate_impl<false>(c, b, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, uint32_t* nkey,
+                                  hipStream_t s) {
+    const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
+    if (c.tab)
+        hipLaunchKernelGGL((k_integrate_moves<true>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.vel_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt, b.counters, b.dt_dev);
+    else
+        hipLaunchKernelGGL((k_integrate_moves<false>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.vel_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt, b.counters, b.dt_dev);
     return hipGetLastError();
 }
 

@@ -341,6 +341,9 @@ int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m, float skin, boo
                    uint32_t* cnt_out, bool timed) {
     const LaunchCfg c = e.launch_cfg();
     SPH_TRY(launch_fill_u32(e.counters, 0u, kCntNum, e.stream));
+    // this step orders the slots its own way: the next plain step places
+    // every particle again
+    SPH_TRY(resort_invalidate(e.resort, e.stream));
     StepBuffers b = e.buffers();
     b.dt_dev = L.dt_coarse;
     const bool sorted = skin == 0.0f || rebuild;
@@ -546,6 +549,7 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
             SPH_TRY(hipMemcpyAsync(e.pos[c.p0], L->spos + c.slot * N, N * 16, k, e.stream));
             SPH_TRY(hipMemcpyAsync(e.vel[c.v0], L->svel + c.slot * N, N * 16, k, e.stream));
             SPH_TRY(hipMemcpyAsync(e.pid[c.q0], L->spid + c.slot * N, N * 4, k, e.stream));
+            SPH_TRY(resort_invalidate(e.resort, e.stream));  // slots back in the snapshot's order
             for (uint32_t i = 0; i < bad; ++i)
                 SPH_TRY_ST(enqueue_coarse(e, *L, *m, skin, i == 0, L->dev + i, L->ovf + kStepCounters * i, false));
             if (skin > 0.0f && bad > 0) r.rebuilds += 1;

@@ -83,7 +83,9 @@ __device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
 // LDS atomic of its first lane: the keys arrive nearly sorted (the previous
 // step's order), so runs are long and the atomics rarely collide.
 __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                      uint32_t bits, uint32_t* __restrict__ block_hist) {
+                                                      uint32_t bits, uint32_t* __restrict__ block_hist,
+                                                      const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ uint32_t hist[kMaxPasses][kRadix];
     const int t = threadIdx.x;
     const int lane = t % kWave;
@@ -122,7 +124,9 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
 // threads per digit, every load of a thread in flight at once), then an
 // exclusive scan.
 __global__ __launch_bounds__(kScanThreads) void k_hist_scan(const uint32_t* __restrict__ block_hist,
-                                                           uint32_t nblocks, uint32_t* __restrict__ offsets) {
+                                                           uint32_t nblocks, uint32_t* __restrict__ offsets,
+                                                           const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ uint32_t part[kScanQ][kRadix];
     __shared__ uint32_t s_wave[kScanThreads / kWave];
     const int t = threadIdx.x % kRadix;
@@ -154,7 +158,10 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
     uint32_t pass_bits, const uint32_t* __restrict__ digit_off, uint64_t* __restrict__ status,
-    uint64_t* __restrict__ ticket, uint32_t num_tiles, uint32_t* __restrict__ err) {
+    uint64_t* __restrict__ ticket, uint32_t num_tiles, uint32_t* __restrict__ err,
+    const uint32_t* __restrict__ gate) {
+    // a gated-off launch takes no ticket, so the epochs stay in step
+    if (gate && *gate == 0) return;
     __shared__ uint32_t wcount[kWaves][kRadix];
     __shared__ uint32_t dig_off[kRadix];
     __shared__ uint32_t gbase[kRadix];
@@ -294,7 +301,8 @@ hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipSt
 
 hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* keys0,
                             uint32_t* vals0, uint32_t* keys1, uint32_t* vals1,
-                            uint32_t** keys_out, uint32_t** vals_out, hipStream_t stream) {
+                            uint32_t** keys_out, uint32_t** vals_out, hipStream_t stream,
+                            const uint32_t* gate) {
     uint32_t* kin = keys0;
     uint32_t* vin = vals0;
     uint32_t* kout = keys1;
@@ -304,16 +312,16 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
     if (passes > (uint32_t)kMaxPasses) return hipErrorInvalidValue;
     if (passes > 0) {
         const uint32_t hb = std::min<uint32_t>(kHistBlocks, std::max<uint32_t>(1u, (n + kHistTile - 1) / kHistTile));
-        hipLaunchKernelGGL(k_hist, dim3(hb), dim3(kHistThreads), 0, stream, kin, n, bits, plan.block_hist);
+        hipLaunchKernelGGL(k_hist, dim3(hb), dim3(kHistThreads), 0, stream, kin, n, bits, plan.block_hist, gate);
         hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(kScanThreads), 0, stream, plan.block_hist, hb,
-                           plan.digit_off);
+                           plan.digit_off, gate);
     }
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t shift = 8 * p;
         const uint32_t pb = (bits - shift) < 8 ? (bits - shift) : 8;
         hipLaunchKernelGGL(k_pass, dim3(plan.num_tiles), dim3(kSortBlock), 0, stream, kin, vin, kout,
                            vout, n, shift, pb, plan.digit_off + p * kRadix, plan.status, plan.ticket,
-                           plan.num_tiles, plan.err);
+                           plan.num_tiles, plan.err, gate);
         uint32_t* tk = kin;
         kin = kout;
         kout = tk;

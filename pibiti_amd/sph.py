@@ -242,7 +242,8 @@ class Stats(C.Structure):
         ("out_of_grid", C.c_uint32),
         ("max_cell_count", C.c_uint32),
         ("sort_errors", C.c_uint32),
-        ("reserved", C.c_uint32 * 6),
+        ("full_sort", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
     ]
 
     def as_dict(self) -> dict:

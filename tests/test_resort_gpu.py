@@ -1,0 +1,151 @@
+"""The step's (cell, index) order built incrementally from the previous
+step's order, and the full radix sort it falls back to, against the oracle's
+sort: positions, velocities and densities bit for bit, the sorted keys and
+permutation exactly.
+
+The incremental order is the default on every step; the full sort runs when
+a cell receives more than 64 particles in one step (a dense first step) or
+holds more than 4096.  stats()["full_sort"] reports which one the last step
+took.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pibiti_amd import scenes, sph
+
+pytestmark = pytest.mark.gpu
+
+
+def arrays(s):
+    return dict(pos=s.get_array(sph.SPH_ARRAY_POSITION), vel=s.get_array(sph.SPH_ARRAY_VELOCITY),
+                dens=s.get_array(sph.SPH_ARRAY_DENSITY))
+
+
+def assert_equal_oracle(g, o, what):
+    for k in ("pos", "vel", "dens"):
+        bad = np.flatnonzero(np.any((g[k] != o[k]).reshape(len(g[k]), -1), axis=1))
+        assert bad.size == 0, f"{what} {k}: {bad.size} particles differ, first {bad[:5]}"
+
+
+def assert_sorted_like_oracle(s, p):
+    """The slots hold ascending (cell key, original index) of the current
+    positions, and the cell ranges cover them."""
+    gr = s.get_grid()
+    x = s.get_array(sph.SPH_ARRAY_POSITION)
+    h, _ = oracle.hash_f32(p, x)
+    order = np.lexsort((np.arange(len(h)), h)).astype(np.uint32)
+    assert np.array_equal(gr["perm"], order)
+    assert np.array_equal(gr["sorted_hash"], h[order])
+    occ = gr["cell_start"] != 0xFFFFFFFF
+    counts = np.bincount(h, minlength=len(gr["cell_start"]))
+    assert np.array_equal(np.flatnonzero(occ), np.flatnonzero(counts))
+    assert np.array_equal((gr["cell_end"][occ].astype(np.int64) - gr["cell_start"][occ]), counts[occ])
+
+
+def test_fast_particles_change_cells_every_step():
+    """Random velocities of 3 m/s: a few percent of the particles change cell
+    on every step, and every step takes the incremental order."""
+    p, pos, vel, _ = scenes.scaled_dam_break((16, 12, 16), seed=21, vel_sigma=3.0)
+    steps = 20
+    print("case: fast dam break 3072 particles,", steps, "steps", flush=True)
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        moved = []
+        h_prev = oracle.hash_f32(p, pos)[0]
+        for _ in range(steps):
+            s.update(1)
+            assert s.stats()["full_sort"] == 0
+            h = oracle.hash_f32(p, s.get_array(sph.SPH_ARRAY_POSITION))[0]
+            moved.append(int(np.sum(h != h_prev)))
+            h_prev = h
+        g = arrays(s)
+        assert_sorted_like_oracle(s, p)
+        assert s.stats()["sort_errors"] == 0
+    assert min(moved[1:]) > 0, moved
+    o = oracle.run_f32(p, pos, vel, steps)
+    assert_equal_oracle(g, o, "fast")
+
+
+def dense(lattice=(20, 20, 20), seed=22):
+    """Spacing h / 5: 125 particles per cell, more than one step may bring
+    into a cell, so the first step (every particle placed anew) sorts fully."""
+    return scenes.dam_break(*lattice, spacing=0.004, h=0.02, world_max=(0.3, 0.3, 0.3), dt=5e-5,
+                            viscosity=0.1, vel_sigma=0.05, seed=seed)
+
+
+def test_dense_first_step_takes_the_full_sort():
+    p, pos, vel, _ = dense()
+    print("case: dense 8000 particles, 4 steps", flush=True)
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        kinds = []
+        for _ in range(4):
+            s.update(1)
+            kinds.append(s.stats()["full_sort"])
+        g = arrays(s)
+        assert_sorted_like_oracle(s, p)
+    assert kinds == [1, 0, 0, 0], kinds
+    o = oracle.run_f32(p, pos, vel, 4)
+    assert_equal_oracle(g, o, "dense")
+
+
+def test_crowded_cell_sorts_fully_every_step():
+    """5000 particles packed into one cell: beyond the incremental order's
+    4096 per cell on every step."""
+    p, pos, vel, _ = scenes.scaled_dam_break((20, 16, 20), seed=23)
+    wmin = np.array(p.world_min, np.float64)
+    cs = np.array(p.cell_size, np.float64)
+    centre = wmin + (np.floor((pos[0, :3] - wmin) / cs) + 0.5) * cs
+    rng = np.random.default_rng(5)
+    pos[:5000, :3] = (centre + rng.uniform(-0.002, 0.002, (5000, 3))).astype(np.float32)
+    vel[:5000] = 0
+    print("case: 5000 particles in one cell, 3 steps", flush=True)
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        kinds = []
+        for _ in range(3):
+            s.update(1)
+            kinds.append(s.stats()["full_sort"])
+        g = arrays(s)
+        assert_sorted_like_oracle(s, p)
+    assert kinds == [1, 1, 1], kinds
+    o = oracle.run_f32(p, pos, vel, 3)
+    assert_equal_oracle(g, o, "crowded")
+
+
+def test_graph_replay_decides_the_sort_on_the_device():
+    """A recording made before the first step replays a full-sort step and
+    then incremental ones: the same as eager steps."""
+    p, pos, vel, _ = dense(seed=24)
+    print("case: dense graph replay 2 x 2 steps", flush=True)
+    with sph.SphSystem(p) as a, sph.SphSystem(p) as b:
+        a.reset(pos, vel)
+        b.reset(pos, vel)
+        a.capture(2)
+        a.replay(2)
+        b.update(4)
+        ga, gb = arrays(a), arrays(b)
+        assert a.stats()["full_sort"] == 0
+    for k in ga:
+        assert np.array_equal(ga[k], gb[k]), k
+
+
+def test_upload_that_moves_many_particles():
+    """Positions rewritten between steps (half the particles shifted by a
+    few cells): the next step still orders incrementally and matches."""
+    p, pos, vel, _ = scenes.scaled_dam_break((16, 12, 16), seed=25)
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        s.update(2)
+        x = s.get_array(sph.SPH_ARRAY_POSITION)
+        v = s.get_array(sph.SPH_ARRAY_VELOCITY)
+        x[::2, 0] += np.float32(0.05)
+        s.set_array(sph.SPH_ARRAY_POSITION, x)
+        s.update(1)
+        full = s.stats()["full_sort"]
+        g = arrays(s)
+        assert_sorted_like_oracle(s, p)
+    o = oracle.step_f32(p, x, v)
+    assert_equal_oracle(g, o, "upload")
+    assert full == 0
