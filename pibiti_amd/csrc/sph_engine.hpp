@@ -280,9 +280,10 @@ struct Engine {
         if (e == hipSuccess) e = dalloc(&val_pid, N);
         if (e == hipSuccess) e = dalloc(&key_tmp, N);
         if (e == hipSuccess) e = dalloc(&val_tmp, N);
+        // cell tables padded to a multiple of four cells (16-byte accesses)
         for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-            e = dalloc(&cell_start[k], NC);
-            if (e == hipSuccess) e = dalloc(&cell_end[k], NC);
+            e = dalloc(&cell_start[k], (NC + 3) & ~(size_t)3);
+            if (e == hipSuccess) e = dalloc(&cell_end[k], (NC + 3) & ~(size_t)3);
             if (e == hipSuccess) e = dalloc(&skey_buf[k], N);
         }
         if (e == hipSuccess) e = dalloc(&counters, (size_t)kCntNum);
@@ -364,11 +365,12 @@ struct Engine {
         if (timed) SPH_TRY(hipEventRecord(ev[0], stream));
         if (device_clock) SPH_TRY(launch_stats(c, buffers(), true, stream));
         SPH_TRY(launch_fill_u32(counters, 0u, kCntNum, stream));
-        SPH_TRY(launch_integrate_moves(c, buffers(), resort, key_tmp, stream));
+        // the mover counts alternate with the cell tables
+        SPH_TRY(launch_integrate_moves(c, buffers(), resort, cur_cell, key_tmp, stream));
         if (timed) SPH_TRY(hipEventRecord(ev[1], stream));
-        SPH_TRY(resort_order(c, buffers(), resort, stream));
+        SPH_TRY(resort_order(c, buffers(), resort, cur_cell, stream));
         if (timed) SPH_TRY(hipEventRecord(ev[2], stream));
-        SPH_TRY(resort_apply(c, buffers(), resort, sort, stream));
+        SPH_TRY(resort_apply(c, buffers(), resort, cur_cell, sort, stream));
         cur_pos ^= 1;
         cur_vel ^= 1;
         cur_pid ^= 1;

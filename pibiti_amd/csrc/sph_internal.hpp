@@ -73,21 +73,24 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
 // over the cells gives the new cell ranges; each slot's new position is its
 // new cell's start plus the number of particles of that cell with a smaller
 // original index (stayers counted from the old ranges and the mover bits,
-// movers from the cell's list of incoming indices).  The result equals the
+// movers from the cell's list of incoming indices).  The cell tables are
+// padded to a multiple of four cells.  The result equals the
 // stable radix sort of (cell, index) exactly.  The full radix sort runs
 // instead (decided on the device) when a cell receives more than
 // kMaxIncoming movers or holds more than kMaxCellCount particles.
-constexpr uint32_t kMaxIncoming = 64;
+constexpr uint32_t kMaxIncoming = 32;
+constexpr uint32_t kScanTile = 1024;  // cells per block of the cell scan
 constexpr uint32_t kMaxCellCount = 4096;
 
 struct ResortPlan {
     uint32_t n = 0, num_cells = 0, num_tiles = 0;
     uint32_t* valid = nullptr;   // device word: slot order, skey and the current cell ranges agree
-    uint32_t* in_cnt = nullptr;  // movers into each cell (zero between steps)
+    uint32_t* in_cnt[2] = {nullptr, nullptr};  // movers into each cell; steps alternate (parity)
+    uint32_t* inlist = nullptr;  // original indices of the movers, kMaxIncoming per cell
     uint64_t* moved = nullptr;   // one bit per slot: the cell changed this step
-    uint64_t* status = nullptr;  // look-back words of the cell scan, one per tile
-    uint64_t* ticket = nullptr;  // tile ticket counter (monotone)
-    uint32_t* err = nullptr;     // set when a look-back wait hit its limit
+    uint32_t* tile_total[2] = {nullptr, nullptr};  // particles per tile of kScanTile cells in the last order
+    uint32_t* tile_delta[2] = {nullptr, nullptr};  // movers in less movers out per tile this step
+    uint32_t* err = nullptr;     // set when the counts disagree (results invalid)
 };
 size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells);
 // Carves the plan out of `scratch`, clears it (not graph-capturable) and
@@ -170,15 +173,16 @@ hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipSt
 // The step's integration for the incremental re-sort: the new cell key of
 // each slot into nkey (slot order), the mover bits and per-cell mover counts
 // into the plan (every slot is a mover while the plan's order is invalid).
-hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, uint32_t* nkey,
-                                  hipStream_t s);
-// The rest of the re-sort after launch_integrate_moves (nkey = key_tmp).
-// resort_order: new cell ranges into cell_start_alt/cell_end_alt and the
-// movers' incoming lists into val_tmp.  resort_apply: the state in the new
-// order into the *_alt buffers and skey_alt (through key_pid/val_pid and the
+hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
+                                  uint32_t* nkey, hipStream_t s);
+// The rest of the re-sort after launch_integrate_moves (nkey = key_tmp),
+// with the same parity (the step's mover counts; parity ^ 1 is cleared for
+// the next step).  resort_order: new cell ranges into cell_start_alt /
+// cell_end_alt.  resort_apply: the state in the new order into the *_alt
+// buffers and skey_alt (through key_pid/val_pid, key_tmp/val_tmp and the
 // radix plan on a full-sort step); marks the plan's order valid.
-hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, hipStream_t s);
-hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp,
+hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity, hipStream_t s);
+hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
                         const RadixSortPlan& sort, hipStream_t s);
 // Particle colours and dye after the force update: b holds the step's state
 // before the update (pos_cur, vel_cur, rhop, dvel, pid_cur).

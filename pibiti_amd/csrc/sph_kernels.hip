@@ -578,26 +578,51 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     val_pid[id] = tid;
 }
 
+// counter[key] += d (mod 2^32) for every active lane with `on`, one atomic
+// per distinct key of the wave (neighbouring slots mostly share keys).
+__device__ __forceinline__ void wave_add(uint32_t* __restrict__ counter, uint32_t key, uint32_t d, bool on) {
+    uint64_t pending = __ballot(on);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const uint64_t same = __ballot(on && key == lk) & pending;
+        if ((int)(threadIdx.x & 63u) == leader) atomicAdd(&counter[lk], d * (uint32_t)__popcll(same));
+        pending &= ~same;
+    }
+}
+
 // The same integration for the incremental re-sort: the new key stays in
 // slot order, a slot whose key changed (every slot while the order is not
-// valid) is a mover: its bit is set and its new cell's mover count raised.
+// valid) is a mover: its bit is set, its new cell's mover count raised and
+// its original index listed there (the first kMaxIncoming of a cell).
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
-    float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ skey,
-    uint32_t* __restrict__ nkey, const uint32_t* __restrict__ valid, uint64_t* __restrict__ moved,
-    uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
+    float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ pid,
+    const uint32_t* __restrict__ skey, uint32_t* __restrict__ nkey, const uint32_t* __restrict__ valid,
+    uint64_t* __restrict__ moved, uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ inlist,
+    uint32_t* __restrict__ tile_delta, uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n) return;
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
     const uint32_t h = integrate_one(p, dt_dev[scene], pos, vel, tid, counters);
-    const bool mv = *valid == 0 || skey[tid] != h;
+    const bool ok = *valid != 0;
+    const uint32_t h_old = ok ? skey[tid] : kEmptyCell;
+    const bool mv = !ok || h_old != h;
     nkey[tid] = h;
     // the active lanes of the last wave are a prefix of it: lane 0 writes
     const uint64_t bits = __ballot(mv);
     if ((threadIdx.x & 63) == 0) moved[tid >> 6] = bits;
-    if (mv) atomicAdd(&in_cnt[h], 1u);
+    if (mv) {
+        const uint32_t t = atomicAdd(&in_cnt[h], 1u);
+        if (t < kMaxIncoming) inlist[(size_t)h * kMaxIncoming + t] = pid[tid];
+    }
+    // per scan tile: movers in less movers out (a move within a tile is none)
+    const uint32_t tile_new = h / kScanTile, tile_old = h_old / kScanTile;
+    const bool cross = mv && (!ok || tile_new != tile_old);
+    wave_add(tile_delta, tile_new, 1u, cross);
+    wave_add(tile_delta, tile_old, ~0u, cross && ok);
 }
 
 // ------------------------------------------------------------------ reorder
@@ -1664,15 +1689,17 @@ hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, uint32_t* nkey,
-                                  hipStream_t s) {
+hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
+                                  uint32_t* nkey, hipStream_t s) {
     const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
     if (c.tab)
         hipLaunchKernelGGL((k_integrate_moves<true>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.vel_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt, b.counters, b.dt_dev);
+                           b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
+                           rp.tile_delta[parity], b.counters, b.dt_dev);
     else
         hipLaunchKernelGGL((k_integrate_moves<false>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.vel_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt, b.counters, b.dt_dev);
+                           b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
+                           rp.tile_delta[parity], b.counters, b.dt_dev);
     return hipGetLastError();
 }
 

@@ -10,20 +10,22 @@
 // keeps its cell; only the movers need placing:
 //
 //   k_integrate_moves  (sph_kernels.hip) new key per slot, mover bits, movers
-//                      counted into their new cells;
+//                      counted into their new cells and their original
+//                      indices listed there (up to kMaxIncoming per cell),
+//                      movers in less movers out per tile of kScanTile cells;
 //   k_cell_scan        per cell: stayers = old count - movers out (from the
 //                      old range and the mover bits), new count = stayers +
-//                      movers in; exclusive scan over all cells (decoupled
-//                      look-back) -> new cell ranges; flags the step for the
+//                      movers in; exclusive scan over the tile's cells, the
+//                      tile's start summed from the earlier tiles' counts
+//                      of the last order and their deltas -> new cell
+//                      ranges; clears the other step's counts; flags the
+//                      step for the
 //                      full sort when a cell takes more than kMaxIncoming
 //                      movers or holds more than kMaxCellCount particles;
-//   k_place_movers     each mover's original index into its new cell's
-//                      incoming list (the first entries of the cell's new
-//                      range in a scratch array);
 //   k_resort_scatter   slot -> new slot: cell start + stayers of the cell
 //                      with a smaller index (a stayer: its rank among the old
 //                      range's stayers; a mover: a binary search of the old
-//                      range) + incoming entries with a smaller index; the
+//                      range) + listed movers in with a smaller index; the
 //                      state, index, scene and key scattered there.
 //                      On a full-sort step it writes the radix sort's input
 //                      instead;
@@ -39,19 +41,8 @@ namespace sph {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kCellItems = 16;
-constexpr uint32_t kCellTile = kBlock * kCellItems;  // cells per scan tile
-constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr uint64_t kFlagAgg = 1ull << 30;
-constexpr uint64_t kFlagInc = 2ull << 30;
-constexpr uint64_t kValMask = (1ull << 30) - 1;
-
-__device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+constexpr int kCellItems = 4;
+static_assert(kScanTile == kBlock * kCellItems, "k_cell_scan: four cells per thread, one uint4 each");
 
 __device__ __forceinline__ bool is_mover(const uint64_t* __restrict__ moved, uint32_t k) {
     return ((moved[k >> 6] >> (k & 63u)) & 1ull) != 0;
@@ -84,124 +75,91 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wave
     return v + add;
 }
 
-// LDS index of tile item i, one word of padding per 16 so that the threads'
-// runs of 16 consecutive items start in different banks.
-__host__ __device__ constexpr uint32_t pad(uint32_t i) { return i + (i >> 4); }
-
 __global__ __launch_bounds__(kBlock) void k_cell_scan(
     uint32_t num_cells, const uint32_t* __restrict__ valid, const uint32_t* __restrict__ old_start,
-    const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ in_cnt, const uint64_t* __restrict__ moved,
-    uint32_t* __restrict__ new_start, uint32_t* __restrict__ new_end, uint32_t* __restrict__ counters,
-    uint64_t* __restrict__ status, uint64_t* __restrict__ ticket, uint32_t num_tiles, uint32_t* __restrict__ err) {
-    __shared__ uint32_t s_cnt[pad(kCellTile - 1) + 1];
+    const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ in_next,
+    const uint64_t* __restrict__ moved, uint32_t* __restrict__ new_start, uint32_t* __restrict__ new_end,
+    uint32_t* __restrict__ counters, const uint32_t* __restrict__ tile_total, const uint32_t* __restrict__ tile_delta,
+    uint32_t* __restrict__ total_next, uint32_t* __restrict__ delta_next, uint32_t* __restrict__ err) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_total, s_excl;
-    __shared__ uint64_t s_ticket;
     const uint32_t t = threadIdx.x;
-    if (t == 0) s_ticket = atomicAdd((unsigned long long*)ticket, 1ull);
-    __syncthreads();
-    const uint32_t tile = (uint32_t)(s_ticket % num_tiles);
-    const uint64_t epoch = (s_ticket / num_tiles) & 0xFFFFFFFFull;
+    const uint32_t tile = blockIdx.x;
     const bool ok = *valid != 0;
-    const uint32_t base = tile * kCellTile;
 
-    // counts, one cell per thread per row (coalesced)
-    uint32_t cnt[kCellItems];
+    // four consecutive cells per thread (16-byte loads; the arrays are
+    // padded to a multiple of four cells)
+    const uint32_t c0 = tile * kScanTile + (uint32_t)kCellItems * t;
+    uint32_t cnt[kCellItems] = {0, 0, 0, 0};
     bool big = false;
-#pragma unroll
-    for (int i = 0; i < kCellItems; ++i) {
-        const uint32_t c = base + (uint32_t)i * kBlock + t;
-        uint32_t v = 0;
-        if (c < num_cells) {
-            const uint32_t in = in_cnt[c];
-            uint32_t stay = 0, old = 0;
-            if (ok) {
-                const uint32_t s = old_start[c];
-                if (s != kEmptyCell) {
-                    const uint32_t e = old_end[c];
-                    old = e - s;
-                    stay = old - movers_between(moved, s, e);
-                }
-            }
-            v = stay + in;
-            big = big || in > kMaxIncoming || v > kMaxCellCount || old > kMaxCellCount;
+    if (c0 < num_cells) {
+        const uint4 in4 = *reinterpret_cast<const uint4*>(in_cnt + c0);
+        uint4 s4 = make_uint4(kEmptyCell, kEmptyCell, kEmptyCell, kEmptyCell), e4 = make_uint4(0, 0, 0, 0);
+        if (ok) {
+            s4 = *reinterpret_cast<const uint4*>(old_start + c0);
+            e4 = *reinterpret_cast<const uint4*>(old_end + c0);
         }
-        cnt[i] = v;
-        s_cnt[pad((uint32_t)i * kBlock + t)] = v;
+        // the next step counts its movers into the other array
+        *reinterpret_cast<uint4*>(in_next + c0) = make_uint4(0, 0, 0, 0);
+        const uint32_t ins[kCellItems] = {in4.x, in4.y, in4.z, in4.w};
+        const uint32_t ss[kCellItems] = {s4.x, s4.y, s4.z, s4.w};
+        const uint32_t es[kCellItems] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+        for (int j = 0; j < kCellItems; ++j) {
+            if (c0 + (uint32_t)j >= num_cells) break;
+            uint32_t stay = 0, old = 0;
+            if (ss[j] != kEmptyCell) {
+                old = es[j] - ss[j];
+                stay = old - movers_between(moved, ss[j], es[j]);
+            }
+            cnt[j] = stay + ins[j];
+            big = big || ins[j] > kMaxIncoming || cnt[j] > kMaxCellCount || old > kMaxCellCount;
+        }
     }
     if (__any(big) && (t & 63u) == 0) atomicAdd(&counters[kCntFullSort], 1u);
-    __syncthreads();
-
-    // thread t scans items 16 t .. 16 t + 15 (cells in order)
-    uint32_t sum = 0;
-#pragma unroll
-    for (int j = 0; j < kCellItems; ++j) sum += s_cnt[pad(t * kCellItems + j)];
+    const uint32_t sum = cnt[0] + cnt[1] + cnt[2] + cnt[3];
     const uint32_t incl = block_incl_scan(sum, s_wave);
     if (t == kBlock - 1) s_total = incl;
-    uint32_t run = incl - sum;
-#pragma unroll
-    for (int j = 0; j < kCellItems; ++j) {
-        const uint32_t q = pad(t * kCellItems + j);
-        const uint32_t v = s_cnt[q];
-        s_cnt[q] = run;
-        run += v;
-    }
-    __syncthreads();
 
-    // tile prefix: publish the aggregate, look back to an inclusive prefix
+    // the tile's first slot: the particles of every earlier tile, each the
+    // tile's count of the previous order plus its movers in less its movers
+    // out (no wait on other blocks)
+    uint32_t pre = 0;
+    for (uint32_t q = t; q < tile; q += kBlock) pre += (ok ? tile_total[q] : 0u) + tile_delta[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off);
+    __syncthreads();  // s_wave of the scan is read; s_total written
+    if ((t & 63u) == 0) s_wave[t >> 6] = pre;
+    __syncthreads();
     if (t == 0) {
-        const uint64_t ep = epoch << 32;
-        const uint32_t total = s_total;
-        uint32_t excl = 0;
-        if (tile == 0) {
-            st_status(&status[0], ep | kFlagInc | total);
-        } else {
-            st_status(&status[tile], ep | kFlagAgg | total);
-            uint32_t spins = 0;
-            int32_t j = (int32_t)tile - 1;
-            while (j >= 0) {
-                const uint64_t w = ld_status(&status[j]);
-                if ((w >> 32) != epoch || (w & (kFlagAgg | kFlagInc)) == 0) {
-                    if (++spins > kSpinLimit) {
-                        atomicOr(err, 1u);
-                        break;
-                    }
-                    continue;
-                }
-                excl += (uint32_t)(w & kValMask);
-                if (w & kFlagInc) break;
-                --j;
-            }
-            st_status(&status[tile], ep | kFlagInc | ((excl + total) & kValMask));
-        }
-        s_excl = excl;
+        uint32_t e = 0;
+        for (int q = 0; q < kBlock / 64; ++q) e += s_wave[q];
+        s_excl = e;
+        // the tile's own count two ways
+        if ((ok ? tile_total[tile] : 0u) + tile_delta[tile] != s_total) atomicOr(err, 1u);
+        total_next[tile] = s_total;
+        delta_next[tile] = 0;
     }
     __syncthreads();
-    const uint32_t excl = s_excl;
+    if (c0 < num_cells) {
+        uint32_t s = s_excl + incl - sum;
+        uint32_t st[kCellItems], en[kCellItems];
 #pragma unroll
-    for (int i = 0; i < kCellItems; ++i) {
-        const uint32_t c = base + (uint32_t)i * kBlock + t;
-        if (c < num_cells) {
-            const uint32_t s = excl + s_cnt[pad((uint32_t)i * kBlock + t)];
-            new_start[c] = cnt[i] ? s : kEmptyCell;
-            new_end[c] = s + cnt[i];
+        for (int j = 0; j < kCellItems; ++j) {
+            st[j] = cnt[j] ? s : kEmptyCell;
+            s += cnt[j];
+            en[j] = s;
+        }
+        if (c0 + kCellItems <= num_cells) {
+            *reinterpret_cast<uint4*>(new_start + c0) = make_uint4(st[0], st[1], st[2], st[3]);
+            *reinterpret_cast<uint4*>(new_end + c0) = make_uint4(en[0], en[1], en[2], en[3]);
+        } else {
+            for (uint32_t j = 0; c0 + j < num_cells; ++j) {
+                new_start[c0 + j] = st[j];
+                new_end[c0 + j] = en[j];
+            }
         }
     }
-}
-
-__global__ __launch_bounds__(kBlock) void k_place_movers(uint32_t n, const uint32_t* __restrict__ nkey,
-                                                        const uint64_t* __restrict__ moved,
-                                                        uint32_t* __restrict__ in_cnt,
-                                                        const uint32_t* __restrict__ new_start,
-                                                        const uint32_t* __restrict__ pid,
-                                                        uint32_t* __restrict__ inbuf) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || !is_mover(moved, k)) return;
-    const uint32_t c = nkey[k];
-    // counting down leaves every count at zero for the next step
-    const uint32_t t = atomicSub(&in_cnt[c], 1u) - 1u;
-    const uint32_t q = new_start[c] + t;
-    if (q < n) inbuf[q] = pid[k];
 }
 
 template <bool kBatch>
@@ -209,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_resort_scatter(
     sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ nkey,
     const uint64_t* __restrict__ moved, const uint32_t* __restrict__ valid, const uint32_t* __restrict__ old_start,
     const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ new_start,
-    const uint32_t* __restrict__ new_end, const uint32_t* __restrict__ inbuf, const float4* __restrict__ pos_in,
+    const uint32_t* __restrict__ new_end, const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist,
+    const float4* __restrict__ pos_in,
     const float4* __restrict__ vel_in, const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in,
     float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out,
     uint32_t* __restrict__ sid_out, uint32_t* __restrict__ skey_out, uint32_t* __restrict__ key_pid,
@@ -239,11 +198,11 @@ __global__ __launch_bounds__(kBlock) void k_resort_scatter(
     }
     // a stayer lies in its cell's old range
     if (!mv && (os == kEmptyCell || k < os || k >= oe)) bad = true;
-    if (bad || end - base < stay) {
+    const uint32_t nin = in_cnt[c];
+    if (bad || nin > kMaxIncoming || end - base != stay + nin) {
         atomicOr(err, 1u);
         return;
     }
-    const uint32_t nin = (end - base) - stay;
     uint32_t local = 0;
     if (!mv) {
         local = (k - os) - movers_between(moved, os, k);
@@ -259,7 +218,8 @@ __global__ __launch_bounds__(kBlock) void k_resort_scatter(
         local = (lo - os) - movers_between(moved, os, lo);
     }
     uint32_t inr = 0;
-    for (uint32_t q = 0; q < nin; ++q) inr += inbuf[base + q] < id ? 1u : 0u;
+    const uint32_t* inc = inlist + (size_t)c * kMaxIncoming;
+    for (uint32_t q = 0; q < nin; ++q) inr += inc[q] < id ? 1u : 0u;
     const uint32_t r = base + local + inr;
     if (r >= end) {
         atomicOr(err, 1u);
@@ -309,17 +269,17 @@ __global__ __launch_bounds__(kBlock) void k_resort_gather(
 inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 inline uint32_t scan_tiles(uint32_t num_cells) {
-    const uint32_t t = (uint32_t)(((uint64_t)num_cells + kCellTile - 1) / kCellTile);
+    const uint32_t t = (uint32_t)(((uint64_t)num_cells + kScanTile - 1) / kScanTile);
     return t == 0 ? 1 : t;
 }
 
 template <bool kBatch>
-hipError_t apply_impl(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, const RadixSortPlan& sort,
-                      hipStream_t s) {
+hipError_t apply_impl(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
+                      const RadixSortPlan& sort, hipStream_t s) {
     const dim3 grid(blocks(c.n)), block(kBlock);
     hipLaunchKernelGGL((k_resort_scatter<kBatch>), grid, block, 0, s, c.one, c.tab, c.n, b.key_tmp, rp.moved,
-                       rp.valid, b.cell_start, b.cell_end, b.cell_start_alt, b.cell_end_alt, b.val_tmp,
-                       b.pos_cur, b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt,
+                       rp.valid, b.cell_start, b.cell_end, b.cell_start_alt, b.cell_end_alt, rp.in_cnt[parity],
+                       rp.inlist, b.pos_cur, b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt,
                        b.skey_alt, b.key_pid, b.val_pid, b.counters, rp.err);
     uint32_t *skey = nullptr, *oslot = nullptr;
     hipError_t e = radix_sort_pairs(sort, c.key_bits, b.key_pid, b.val_pid, b.key_tmp, b.val_tmp, &skey, &oslot, s,
@@ -333,9 +293,13 @@ hipError_t apply_impl(const LaunchCfg& c, const StepBuffers& b, const ResortPlan
 
 }  // namespace
 
+inline size_t pad4(uint32_t cells) { return ((size_t)cells + 3) & ~(size_t)3; }
+
 size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells) {
-    // valid | err | ticket | status words | mover bits | mover counts
-    return 16 + (size_t)scan_tiles(num_cells) * 8 + ((size_t)n / 64 + 2) * 8 + (size_t)num_cells * 4;
+    // mover counts x 2 (padded to four cells) | incoming lists | tile totals
+    // x 2 | tile deltas x 2 | valid, err | mover bits
+    return 2 * pad4(num_cells) * 4 + (size_t)num_cells * kMaxIncoming * 4 + 4 * (size_t)scan_tiles(num_cells) * 4 +
+           8 + ((size_t)n / 64 + 2) * 8;
 }
 
 hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, void* scratch, hipStream_t s) {
@@ -343,35 +307,43 @@ hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, vo
     plan.num_cells = num_cells;
     plan.num_tiles = scan_tiles(num_cells);
     char* p = static_cast<char*>(scratch);
+    plan.in_cnt[0] = reinterpret_cast<uint32_t*>(p);
+    plan.in_cnt[1] = plan.in_cnt[0] + pad4(num_cells);
+    hipError_t e = hipMemsetAsync(p, 0, 2 * pad4(num_cells) * 4, s);
+    p += 2 * pad4(num_cells) * 4;
+    plan.inlist = reinterpret_cast<uint32_t*>(p);
+    p += (size_t)num_cells * kMaxIncoming * 4;
+    char* q = p;
+    for (int k = 0; k < 2; ++k) {
+        plan.tile_total[k] = reinterpret_cast<uint32_t*>(p) + (size_t)k * plan.num_tiles;
+        plan.tile_delta[k] = reinterpret_cast<uint32_t*>(p) + (size_t)(2 + k) * plan.num_tiles;
+    }
+    p += 4 * (size_t)plan.num_tiles * 4;
     plan.valid = reinterpret_cast<uint32_t*>(p);
     plan.err = reinterpret_cast<uint32_t*>(p + 4);
-    plan.ticket = reinterpret_cast<uint64_t*>(p + 8);
-    p += 16;
-    plan.status = reinterpret_cast<uint64_t*>(p);
-    p += (size_t)plan.num_tiles * 8;
+    p += 8;
     plan.moved = reinterpret_cast<uint64_t*>(p);
     p += ((size_t)n / 64 + 2) * 8;
-    plan.in_cnt = reinterpret_cast<uint32_t*>(p);
-    return hipMemsetAsync(scratch, 0, resort_scratch_bytes(n, num_cells), s);
+    if (e == hipSuccess) e = hipMemsetAsync(q, 0, (size_t)(p - q), s);
+    return e;
 }
 
 hipError_t resort_invalidate(const ResortPlan& plan, hipStream_t s) {
     return launch_fill_u32(plan.valid, 0u, 1, s);
 }
 
-hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, hipStream_t s) {
+hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity, hipStream_t s) {
     if (rp.n != c.n || rp.num_cells != c.num_cells) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cell_scan, dim3(rp.num_tiles), dim3(kBlock), 0, s, c.num_cells, rp.valid, b.cell_start,
-                       b.cell_end, rp.in_cnt, rp.moved, b.cell_start_alt, b.cell_end_alt, b.counters, rp.status,
-                       rp.ticket, rp.num_tiles, rp.err);
-    hipLaunchKernelGGL(k_place_movers, dim3(blocks(c.n)), dim3(kBlock), 0, s, c.n, b.key_tmp, rp.moved, rp.in_cnt,
-                       b.cell_start_alt, b.pid_cur, b.val_tmp);
+                       b.cell_end, rp.in_cnt[parity], rp.in_cnt[parity ^ 1], rp.moved, b.cell_start_alt,
+                       b.cell_end_alt, b.counters, rp.tile_total[parity], rp.tile_delta[parity],
+                       rp.tile_total[parity ^ 1], rp.tile_delta[parity ^ 1], rp.err);
     return hipGetLastError();
 }
 
-hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, const RadixSortPlan& sort,
-                        hipStream_t s) {
-    return c.tab ? apply_impl<true>(c, b, rp, sort, s) : apply_impl<false>(c, b, rp, sort, s);
+hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
+                        const RadixSortPlan& sort, hipStream_t s) {
+    return c.tab ? apply_impl<true>(c, b, rp, parity, sort, s) : apply_impl<false>(c, b, rp, parity, sort, s);
 }
 
 }  // namespace sph
