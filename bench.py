@@ -3,8 +3,9 @@
 
 Workload: the 1,048,576-particle dam break (128x64x128 lattice, spacing
 0.01, h = 0.02, 2x1x2 box, velocities N(0, 0.05), dt = 5e-5).  A step is
-one full pass of the hot path: integrate + hash, stable radix sort,
-reorder + cellStart/cellEnd, density + EOS, force + velocity update.  The
+one full pass of the hot path: integrate + hash, the new (cell, index)
+order with cellStart/cellEnd (built from the last order; the stable radix
+sort when a cell crowds), reorder, density + EOS, force + velocity update.  The
 state is resident in HBM before the timed region; nothing is copied to or
 from the host inside it.
 
@@ -216,16 +217,20 @@ def copy_calibration() -> None:
 
 
 def algorithmic_bytes(p) -> dict:
-    """Minimal FP32 traffic per particle of each kernel (pos 12 B, vel 12 B,
+    """Minimal FP32 traffic per particle of each stage (pos 12 B, vel 12 B,
     rho 4 B, p 4 B, uint32 key and index; every array counted once per
-    kernel; cell tables 8 B per cell)."""
+    kernel; cell tables 8 B per cell).  integrate_hash: state read and
+    written, last key read, new key written; sort: the cell scan (old ranges
+    and mover counts read, new ranges and cleared counts written, 24 B per
+    cell, plus one mover bit per particle); reorder: the scatter (new key,
+    index and state read, state, index and key written, both cell tables
+    read)."""
     n = int(p.num_particles)
     c = int(p.num_cells) / n
-    passes = (int(p.key_bits) + 7) // 8
     return {
         "integrate_hash": 12 + 12 + 12 + 12 + 8,
-        "sort": 24 * passes,
-        "reorder": 8 + 24 + 24 + 8 * c,
+        "sort": 24 * c + 0.125,
+        "reorder": 8 + 24 + 24 + 8 + 16 * c,
         "density": 12 + 8 + 8 * c,
         "force": 32 + 12 + 8 * c,
     }
