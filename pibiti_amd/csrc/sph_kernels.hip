@@ -573,9 +573,11 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
     const uint32_t h = integrate_one(p, dt_dev[scene], pos, vel, tid, counters);
-    const uint32_t id = pid[tid];
-    key_pid[id] = h;
-    val_pid[id] = tid;
+    if (key_pid) {  // null: a step that keeps its order writes no sort input
+        const uint32_t id = pid[tid];
+        key_pid[id] = h;
+        val_pid[id] = tid;
+    }
 }
 
 // counter[key] += d (mod 2^32) for every active lane with `on`, one atomic

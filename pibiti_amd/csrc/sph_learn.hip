@@ -336,37 +336,40 @@ int learn_get_window(Engine& e, uint32_t scene, int which, float* out) {
 // else the walk of launch_density_reach); the density deviation into
 // dev_out and the step's counters into cnt_out (both device); the surrogate
 // velocity update.  Events: ev[1]/ev[2] bracket the density kernel,
-// ev[3]/ev[4] the surrogate, ev[5]/ev[6] a rebuild's sort and reorder.
+// ev[3]/ev[4] the surrogate, ev[5]/ev[6] a rebuild's integrate, order and
+// reorder.
 int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m, float skin, bool rebuild, double* dev_out,
                    uint32_t* cnt_out, bool timed) {
     const LaunchCfg c = e.launch_cfg();
     SPH_TRY(launch_fill_u32(e.counters, 0u, kCntNum, e.stream));
-    // this step orders the slots its own way: the next plain step places
-    // every particle again
-    SPH_TRY(resort_invalidate(e.resort, e.stream));
     StepBuffers b = e.buffers();
     b.dt_dev = L.dt_coarse;
     const bool sorted = skin == 0.0f || rebuild;
-    // the integrate kernel clears the cell ranges for the next sort; a step
-    // that does not sort keeps them
-    LaunchCfg ci = c;
-    if (!sorted) ci.num_cells = 0;
-    SPH_TRY(launch_integrate_hash(ci, b, e.stream));
     if (sorted) {
+        // the plain step's order: only the particles that changed cell are
+        // placed again (the full sort when a cell crowds)
         if (timed && skin > 0.0f) SPH_TRY(hipEventRecord(e.ev[5], e.stream));
-        uint32_t *skey = nullptr, *oslot = nullptr;
-        SPH_TRY(radix_sort_pairs(e.sort, c.key_bits, e.key_pid, e.val_pid, e.key_tmp, e.val_tmp, &skey, &oslot,
-                                 e.stream));
-        SPH_TRY(launch_reorder(c, e.buffers(), skey, oslot, e.stream));
+        SPH_TRY(launch_integrate_moves(c, b, e.resort, e.cur_cell, e.key_tmp, e.stream));
+        SPH_TRY(resort_order(c, b, e.resort, e.cur_cell, e.stream));
+        SPH_TRY(resort_apply(c, b, e.resort, e.cur_cell, e.sort, e.stream));
         e.cur_pos ^= 1;
         e.cur_vel ^= 1;
         e.cur_pid ^= 1;
-        e.last_skey = skey;
+        e.cur_cell ^= 1;
+        e.cur_key ^= 1;
+        e.last_skey = e.skey_buf[e.cur_key];
         if (skin > 0.0f) {
             SPH_TRY(hipMemcpyAsync(L.bpos, e.pos[e.cur_pos], (size_t)c.n * sizeof(float4),
                                    hipMemcpyDeviceToDevice, e.stream));
             if (timed) SPH_TRY(hipEventRecord(e.ev[6], e.stream));
         }
+    } else {
+        // a step between sorts keeps the slot order and the cell ranges
+        // (and so the order stays valid for the next sort)
+        LaunchCfg ci = c;
+        ci.num_cells = 0;
+        b.key_pid = nullptr;
+        SPH_TRY(launch_integrate_hash(ci, b, e.stream));
     }
     b = e.buffers();
     if (timed) SPH_TRY(hipEventRecord(e.ev[1], e.stream));
