@@ -219,8 +219,9 @@ def copy_calibration() -> None:
 def algorithmic_bytes(p) -> dict:
     """Minimal FP32 traffic per particle of each stage (pos 12 B, vel 12 B,
     rho 4 B, p 4 B, uint32 key and index; every array counted once per
-    kernel; cell tables 8 B per cell).  integrate_hash: state read and
-    written, last key read, new key written; sort: the cell scan (old ranges
+    kernel; cell tables 8 B per cell).  integrate_hash: state read, last key
+    read, new key written (the scatter stores the integrated state); sort:
+    the cell scan (old ranges
     and mover counts read, new ranges and cleared counts written, 24 B per
     cell, plus one mover bit per particle); reorder: the scatter (new key,
     index and state read, state, index and key written, both cell tables
@@ -228,7 +229,7 @@ def algorithmic_bytes(p) -> dict:
     n = int(p.num_particles)
     c = int(p.num_cells) / n
     return {
-        "integrate_hash": 12 + 12 + 12 + 12 + 8,
+        "integrate_hash": 12 + 12 + 4 + 4,
         "sort": 24 * c + 0.125,
         "reorder": 8 + 24 + 24 + 8 + 16 * c,
         "density": 12 + 8 + 8 * c,

@@ -35,6 +35,22 @@ __device__ __forceinline__ int cell_coord(float x, float wmin, float cs, uint32_
     return (int)f;
 }
 
+// Incremental re-sort: one bit per slot, set when the slot's cell changed.
+__device__ __forceinline__ bool is_mover(const uint64_t* __restrict__ moved, uint32_t k) {
+    return ((moved[k >> 6] >> (k & 63u)) & 1ull) != 0;
+}
+
+// Movers among slots [a, b).
+__device__ __forceinline__ uint32_t movers_between(const uint64_t* __restrict__ moved, uint32_t a, uint32_t b) {
+    if (a >= b) return 0;
+    const uint32_t wa = a >> 6, wb = b >> 6, la = a & 63u, lb = b & 63u;
+    if (wa == wb) return (uint32_t)__popcll((moved[wa] >> la) & ((1ull << (lb - la)) - 1ull));
+    uint32_t c = (uint32_t)__popcll(moved[wa] >> la);
+    for (uint32_t w = wa + 1; w < wb; ++w) c += (uint32_t)__popcll(moved[w]);
+    if (lb) c += (uint32_t)__popcll(moved[wb] & ((1ull << lb) - 1ull));
+    return c;
+}
+
 __device__ __forceinline__ uint32_t grid_hash(const sph_params& p, int gx, int gy, int gz) {
     return p.cell_base + (uint32_t)gz * p.grid_size_yx + (uint32_t)gy * p.grid_size[0] + (uint32_t)gx;
 }

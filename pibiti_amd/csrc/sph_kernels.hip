@@ -512,12 +512,10 @@ __global__ __launch_bounds__(kBlock) void k_colour(sph_params one, const sph_par
 
 // ------------------------------------------------------------ integrate+hash
 // One slot's boundary forces, gravity, damping, position update and clamp to
-// the hard walls (in place), and its new cell key.
-__device__ __forceinline__ uint32_t integrate_one(const sph_params& p, float dt, float4* __restrict__ pos,
-                                                  float4* __restrict__ vel, uint32_t k,
-                                                  uint32_t* __restrict__ counters) {
-    float4 x4 = pos[k];
-    float4 v4 = vel[k];
+// the hard walls, on the state in registers, and its new cell key.  An
+// out-of-grid key is counted when counters is given.
+__device__ __forceinline__ uint32_t integrate_state(const sph_params& p, float dt, float4& x4, float4& v4,
+                                                    uint32_t* __restrict__ counters) {
     float x = x4.x, y = x4.y, z = x4.z;
     float vx = v4.x, vy = v4.y, vz = v4.z;
     boundary(p, dt, x, y, z, vx, vy, vz);
@@ -546,15 +544,27 @@ __device__ __forceinline__ uint32_t integrate_one(const sph_params& p, float dt,
         if (z > hi) z = hi;
         if (z < lo) z = lo;
     }
-    pos[k] = make_float4(x, y, z, x4.w);
-    vel[k] = make_float4(vx, vy, vz, v4.w);
+    x4 = make_float4(x, y, z, x4.w);
+    v4 = make_float4(vx, vy, vz, v4.w);
 
     bool out = false;
     const int gx = cell_coord(x, p.world_min[0], p.cell_size[0], p.grid_size[0], out);
     const int gy = cell_coord(y, p.world_min[1], p.cell_size[1], p.grid_size[1], out);
     const int gz = cell_coord(z, p.world_min[2], p.cell_size[2], p.grid_size[2], out);
-    if (out) atomicAdd(&counters[kCntOutOfGrid], 1u);
+    if (out && counters) atomicAdd(&counters[kCntOutOfGrid], 1u);
     return grid_hash(p, gx, gy, gz);
+}
+
+// integrate_state on slot k of pos/vel, in place.
+__device__ __forceinline__ uint32_t integrate_one(const sph_params& p, float dt, float4* __restrict__ pos,
+                                                  float4* __restrict__ vel, uint32_t k,
+                                                  uint32_t* __restrict__ counters) {
+    float4 x4 = pos[k];
+    float4 v4 = vel[k];
+    const uint32_t h = integrate_state(p, dt, x4, v4, counters);
+    pos[k] = x4;
+    vel[k] = v4;
+    return h;
 }
 
 template <bool kBatch>
@@ -593,14 +603,15 @@ __device__ __forceinline__ void wave_add(uint32_t* __restrict__ counter, uint32_
     }
 }
 
-// The same integration for the incremental re-sort: the new key stays in
-// slot order, a slot whose key changed (every slot while the order is not
+// The same integration for the incremental re-sort, keeping only the new
+// key (in slot order; the state is integrated again where k_resort_scatter
+// moves it).  A slot whose key changed (every slot while the order is not
 // valid) is a mover: its bit is set, its new cell's mover count raised and
 // its original index listed there (the first kMaxIncoming of a cell).
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
-    float4* __restrict__ pos, float4* __restrict__ vel, const uint32_t* __restrict__ pid,
+    const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pid,
     const uint32_t* __restrict__ skey, uint32_t* __restrict__ nkey, const uint32_t* __restrict__ valid,
     uint64_t* __restrict__ moved, uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ inlist,
     uint32_t* __restrict__ tile_delta, uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
@@ -608,7 +619,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     if (tid >= n) return;
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
-    const uint32_t h = integrate_one(p, dt_dev[scene], pos, vel, tid, counters);
+    // the key only: k_resort_scatter integrates the state again as it moves it
+    float4 x4 = pos[tid];
+    float4 v4 = vel[tid];
+    const uint32_t h = integrate_state(p, dt_dev[scene], x4, v4, counters);
     const bool ok = *valid != 0;
     const uint32_t h_old = ok ? skey[tid] : kEmptyCell;
     const bool mv = !ok || h_old != h;
@@ -625,6 +639,115 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     const bool cross = mv && (!ok || tile_new != tile_old);
     wave_add(tile_delta, tile_new, 1u, cross);
     wave_add(tile_delta, tile_old, ~0u, cross && ok);
+}
+
+// ----------------------------------------------- incremental re-sort: apply
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_resort_scatter(
+    sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ nkey,
+    const uint64_t* __restrict__ moved, const uint32_t* __restrict__ valid, const uint32_t* __restrict__ old_start,
+    const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ new_start,
+    const uint32_t* __restrict__ new_end, const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist,
+    const float4* __restrict__ pos_in,
+    const float4* __restrict__ vel_in, const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in,
+    float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out,
+    uint32_t* __restrict__ sid_out, uint32_t* __restrict__ skey_out, uint32_t* __restrict__ key_pid,
+    uint32_t* __restrict__ val_pid, uint32_t* __restrict__ counters, uint32_t* __restrict__ err,
+    const float* __restrict__ dt_dev) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t c = nkey[k], id = pid_in[k];
+    if (counters[kCntFullSort]) {
+        // the radix sort's input: keys in original-index order
+        key_pid[id] = c;
+        val_pid[id] = k;
+        return;
+    }
+    const bool mv = is_mover(moved, k);
+    const uint32_t base = new_start[c], end = new_end[c];
+    bool bad = base >= end || end > n;
+    uint32_t os = kEmptyCell, oe = 0, stay = 0;
+    if (*valid) {
+        os = old_start[c];
+        if (os != kEmptyCell) {
+            oe = old_end[c];
+            if (oe > n || oe <= os)
+                bad = true;
+            else
+                stay = (oe - os) - movers_between(moved, os, oe);
+        }
+    }
+    // a stayer lies in its cell's old range
+    if (!mv && (os == kEmptyCell || k < os || k >= oe)) bad = true;
+    const uint32_t nin = in_cnt[c];
+    if (bad || nin > kMaxIncoming || end - base != stay + nin) {
+        atomicOr(err, 1u);
+        return;
+    }
+    uint32_t local = 0;
+    if (!mv) {
+        local = (k - os) - movers_between(moved, os, k);
+    } else if (os != kEmptyCell) {
+        uint32_t lo = os, hi = oe;
+        while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (pid_in[mid] < id)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        local = (lo - os) - movers_between(moved, os, lo);
+    }
+    uint32_t inr = 0;
+    const uint32_t* inc = inlist + (size_t)c * kMaxIncoming;
+    for (uint32_t q = 0; q < nin; ++q) inr += inc[q] < id ? 1u : 0u;
+    const uint32_t r = base + local + inr;
+    if (r >= end) {
+        atomicOr(err, 1u);
+        return;
+    }
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid_in, k, scene);
+    float4 x4 = pos_in[k];
+    float4 v4 = vel_in[k];
+    (void)integrate_state(p, dt_dev[scene], x4, v4, nullptr);
+    pos_out[r] = x4;
+    vel_out[r] = v4;
+    pid_out[r] = id;
+    skey_out[r] = c;
+    if constexpr (kBatch) sid_out[r] = scene;
+    if (r + 1 == end && end - base > p.max_par_in_cell) atomicAdd(&counters[kCntCapCells], 1u);
+}
+
+// A full-sort step's reorder (the cell ranges come from k_cell_scan): the
+// state integrated again and gathered into the sorted slots.  Marks the new
+// order valid on every step.
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_resort_gather(
+    sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ skey,
+    const uint32_t* __restrict__ old_slot, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
+    const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out, uint32_t* __restrict__ sid_out,
+    uint32_t* __restrict__ skey_out, uint32_t* __restrict__ counters, uint32_t* __restrict__ valid,
+    const float* __restrict__ dt_dev) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *valid = 1u;
+    if (k >= n || counters[kCntFullSort] == 0) return;
+    const uint32_t s = old_slot[k];
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid_in, s, scene);
+    float4 x4 = pos_in[s];
+    float4 v4 = vel_in[s];
+    (void)integrate_state(p, dt_dev[scene], x4, v4, nullptr);
+    pos_out[k] = x4;
+    vel_out[k] = v4;
+    pid_out[k] = pid_in[s];
+    const uint32_t h = skey[k];
+    skey_out[k] = h;
+    if constexpr (kBatch) sid_out[k] = scene;
+    const uint32_t cap = p.max_par_in_cell;
+    if ((k + 1 == n || skey[k + 1] != h) && k >= cap && skey[k - cap] == h)
+        atomicAdd(&counters[kCntCapCells], 1u);
 }
 
 // ------------------------------------------------------------------ reorder
@@ -1563,6 +1686,24 @@ void integrate_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
 }
 
 template <bool kBatch>
+hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
+                             const RadixSortPlan& sort, hipStream_t s) {
+    const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
+    hipLaunchKernelGGL((k_resort_scatter<kBatch>), grid, block, 0, s, c.one, c.tab, c.n, b.key_tmp, rp.moved,
+                       rp.valid, b.cell_start, b.cell_end, b.cell_start_alt, b.cell_end_alt, rp.in_cnt[parity],
+                       rp.inlist, b.pos_cur, b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt,
+                       b.sid_alt, b.skey_alt, b.key_pid, b.val_pid, b.counters, rp.err, b.dt_dev);
+    uint32_t *skey = nullptr, *oslot = nullptr;
+    hipError_t e = radix_sort_pairs(sort, c.key_bits, b.key_pid, b.val_pid, b.key_tmp, b.val_tmp, &skey, &oslot, s,
+                                    b.counters + kCntFullSort);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_resort_gather<kBatch>), grid, block, 0, s, c.one, c.tab, c.n, skey, oslot, b.pos_cur,
+                       b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt,
+                       b.counters, rp.valid, b.dt_dev);
+    return hipGetLastError();
+}
+
+template <bool kBatch>
 void reorder_impl(const LaunchCfg& c, const StepBuffers& b, const uint32_t* skey,
                   const uint32_t* oslot, hipStream_t s) {
     hipLaunchKernelGGL((k_reorder<kBatch>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, c.one,
@@ -1703,6 +1844,11 @@ hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, cons
                            b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
                            rp.tile_delta[parity], b.counters, b.dt_dev);
     return hipGetLastError();
+}
+
+hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
+                        const RadixSortPlan& sort, hipStream_t s) {
+    return c.tab ? resort_apply_impl<true>(c, b, rp, parity, sort, s) : resort_apply_impl<false>(c, b, rp, parity, sort, s);
 }
 
 hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,

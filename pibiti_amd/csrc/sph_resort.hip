@@ -9,29 +9,31 @@
 // steps a particle moves a small fraction of a cell, so almost every slot
 // keeps its cell; only the movers need placing:
 //
-//   k_integrate_moves  (sph_kernels.hip) new key per slot, mover bits, movers
-//                      counted into their new cells and their original
-//                      indices listed there (up to kMaxIncoming per cell),
-//                      movers in less movers out per tile of kScanTile cells;
-//   k_cell_scan        per cell: stayers = old count - movers out (from the
-//                      old range and the mover bits), new count = stayers +
-//                      movers in; exclusive scan over the tile's cells, the
-//                      tile's start summed from the earlier tiles' counts
-//                      of the last order and their deltas -> new cell
-//                      ranges; clears the other step's counts; flags the
-//                      step for the
-//                      full sort when a cell takes more than kMaxIncoming
-//                      movers or holds more than kMaxCellCount particles;
-//   k_resort_scatter   slot -> new slot: cell start + stayers of the cell
-//                      with a smaller index (a stayer: its rank among the old
-//                      range's stayers; a mover: a binary search of the old
-//                      range) + listed movers in with a smaller index; the
-//                      state, index, scene and key scattered there.
-//                      On a full-sort step it writes the radix sort's input
-//                      instead;
-//   radix sort + k_resort_gather  only on a full-sort step (gated on the
-//                      device, so the launch sequence never changes and
-//                      graphs replay it).
+//   k_integrate_moves  (sph_kernels.hip) the new key of each slot, mover
+//                      bits, movers counted into their new cells and their
+//                      original indices listed there (up to kMaxIncoming per
+//                      cell), movers in less movers out per tile of
+//                      kScanTile cells; the integrated state is not stored;
+//   k_cell_scan        (here) per cell: stayers = old count - movers out
+//                      (the old range and the mover bits), new count =
+//                      stayers + movers in; exclusive scan over the tile's
+//                      cells, the tile's start summed from the earlier
+//                      tiles' counts of the last order and their deltas ->
+//                      new cell ranges; clears the other step's counts;
+//                      flags the step for the full sort when a cell takes
+//                      more than kMaxIncoming movers or holds more than
+//                      kMaxCellCount particles;
+//   k_resort_scatter   (sph_kernels.hip) slot -> new slot: cell start +
+//                      stayers of the cell with a smaller index (a stayer:
+//                      its rank among the old range's stayers; a mover: a
+//                      binary search of the old range) + listed movers in
+//                      with a smaller index; integrates the slot's state
+//                      again (the same arithmetic) and writes it there with
+//                      its index, scene and key.  On a full-sort step it
+//                      writes the radix sort's input instead;
+//   radix sort + k_resort_gather (sph_kernels.hip)  only on a full-sort step
+//                      (gated on the device, so the launch sequence never
+//                      changes and graphs replay it).
 // While the order is invalid (first step, after the learned surrogate or
 // the tuner rearranged the slots) every slot counts as a mover with no old
 // ranges, which is a counting sort by cell with the same result.
@@ -40,24 +42,11 @@
 namespace sph {
 namespace {
 
+using namespace dev;
+
 constexpr int kBlock = 256;
 constexpr int kCellItems = 4;
 static_assert(kScanTile == kBlock * kCellItems, "k_cell_scan: four cells per thread, one uint4 each");
-
-__device__ __forceinline__ bool is_mover(const uint64_t* __restrict__ moved, uint32_t k) {
-    return ((moved[k >> 6] >> (k & 63u)) & 1ull) != 0;
-}
-
-// Movers among slots [a, b).
-__device__ __forceinline__ uint32_t movers_between(const uint64_t* __restrict__ moved, uint32_t a, uint32_t b) {
-    if (a >= b) return 0;
-    const uint32_t wa = a >> 6, wb = b >> 6, la = a & 63u, lb = b & 63u;
-    if (wa == wb) return (uint32_t)__popcll((moved[wa] >> la) & ((1ull << (lb - la)) - 1ull));
-    uint32_t c = (uint32_t)__popcll(moved[wa] >> la);
-    for (uint32_t w = wa + 1; w < wb; ++w) c += (uint32_t)__popcll(moved[w]);
-    if (lb) c += (uint32_t)__popcll(moved[wb] & ((1ull << lb) - 1ull));
-    return c;
-}
 
 // Inclusive scan of one value per thread over the block.
 __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wave) {
@@ -162,133 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
     }
 }
 
-template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_resort_scatter(
-    sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ nkey,
-    const uint64_t* __restrict__ moved, const uint32_t* __restrict__ valid, const uint32_t* __restrict__ old_start,
-    const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ new_start,
-    const uint32_t* __restrict__ new_end, const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist,
-    const float4* __restrict__ pos_in,
-    const float4* __restrict__ vel_in, const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in,
-    float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out,
-    uint32_t* __restrict__ sid_out, uint32_t* __restrict__ skey_out, uint32_t* __restrict__ key_pid,
-    uint32_t* __restrict__ val_pid, uint32_t* __restrict__ counters, uint32_t* __restrict__ err) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t c = nkey[k], id = pid_in[k];
-    if (counters[kCntFullSort]) {
-        // the radix sort's input: keys in original-index order
-        key_pid[id] = c;
-        val_pid[id] = k;
-        return;
-    }
-    const bool mv = is_mover(moved, k);
-    const uint32_t base = new_start[c], end = new_end[c];
-    bool bad = base >= end || end > n;
-    uint32_t os = kEmptyCell, oe = 0, stay = 0;
-    if (*valid) {
-        os = old_start[c];
-        if (os != kEmptyCell) {
-            oe = old_end[c];
-            if (oe > n || oe <= os)
-                bad = true;
-            else
-                stay = (oe - os) - movers_between(moved, os, oe);
-        }
-    }
-    // a stayer lies in its cell's old range
-    if (!mv && (os == kEmptyCell || k < os || k >= oe)) bad = true;
-    const uint32_t nin = in_cnt[c];
-    if (bad || nin > kMaxIncoming || end - base != stay + nin) {
-        atomicOr(err, 1u);
-        return;
-    }
-    uint32_t local = 0;
-    if (!mv) {
-        local = (k - os) - movers_between(moved, os, k);
-    } else if (os != kEmptyCell) {
-        uint32_t lo = os, hi = oe;
-        while (lo < hi) {
-            const uint32_t mid = lo + ((hi - lo) >> 1);
-            if (pid_in[mid] < id)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        local = (lo - os) - movers_between(moved, os, lo);
-    }
-    uint32_t inr = 0;
-    const uint32_t* inc = inlist + (size_t)c * kMaxIncoming;
-    for (uint32_t q = 0; q < nin; ++q) inr += inc[q] < id ? 1u : 0u;
-    const uint32_t r = base + local + inr;
-    if (r >= end) {
-        atomicOr(err, 1u);
-        return;
-    }
-    pos_out[r] = pos_in[k];
-    vel_out[r] = vel_in[k];
-    pid_out[r] = id;
-    skey_out[r] = c;
-    uint32_t cap = one.max_par_in_cell;
-    if constexpr (kBatch) {
-        const uint32_t sc = sid_in[k];
-        sid_out[r] = sc;
-        cap = tab[sc].max_par_in_cell;
-    }
-    if (r + 1 == end && end - base > cap) atomicAdd(&counters[kCntCapCells], 1u);
-}
-
-// A full-sort step's reorder (the cell ranges come from k_cell_scan); marks
-// the new order valid on every step.
-template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_resort_gather(
-    sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ skey,
-    const uint32_t* __restrict__ old_slot, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
-    const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in, float4* __restrict__ pos_out,
-    float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out, uint32_t* __restrict__ sid_out,
-    uint32_t* __restrict__ skey_out, uint32_t* __restrict__ counters, uint32_t* __restrict__ valid) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *valid = 1u;
-    if (k >= n || counters[kCntFullSort] == 0) return;
-    const uint32_t s = old_slot[k];
-    pos_out[k] = pos_in[s];
-    vel_out[k] = vel_in[s];
-    pid_out[k] = pid_in[s];
-    const uint32_t h = skey[k];
-    skey_out[k] = h;
-    uint32_t cap = one.max_par_in_cell;
-    if constexpr (kBatch) {
-        const uint32_t sc = sid_in[s];
-        sid_out[k] = sc;
-        cap = tab[sc].max_par_in_cell;
-    }
-    if ((k + 1 == n || skey[k + 1] != h) && k >= cap && skey[k - cap] == h)
-        atomicAdd(&counters[kCntCapCells], 1u);
-}
-
-inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
-
 inline uint32_t scan_tiles(uint32_t num_cells) {
     const uint32_t t = (uint32_t)(((uint64_t)num_cells + kScanTile - 1) / kScanTile);
     return t == 0 ? 1 : t;
-}
-
-template <bool kBatch>
-hipError_t apply_impl(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
-                      const RadixSortPlan& sort, hipStream_t s) {
-    const dim3 grid(blocks(c.n)), block(kBlock);
-    hipLaunchKernelGGL((k_resort_scatter<kBatch>), grid, block, 0, s, c.one, c.tab, c.n, b.key_tmp, rp.moved,
-                       rp.valid, b.cell_start, b.cell_end, b.cell_start_alt, b.cell_end_alt, rp.in_cnt[parity],
-                       rp.inlist, b.pos_cur, b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt,
-                       b.skey_alt, b.key_pid, b.val_pid, b.counters, rp.err);
-    uint32_t *skey = nullptr, *oslot = nullptr;
-    hipError_t e = radix_sort_pairs(sort, c.key_bits, b.key_pid, b.val_pid, b.key_tmp, b.val_tmp, &skey, &oslot, s,
-                                    b.counters + kCntFullSort);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_resort_gather<kBatch>), grid, block, 0, s, c.one, c.tab, c.n, skey, oslot, b.pos_cur,
-                       b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt,
-                       b.counters, rp.valid);
-    return hipGetLastError();
 }
 
 }  // namespace
@@ -339,11 +204,6 @@ hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPl
                        b.cell_end_alt, b.counters, rp.tile_total[parity], rp.tile_delta[parity],
                        rp.tile_total[parity ^ 1], rp.tile_delta[parity ^ 1], rp.err);
     return hipGetLastError();
-}
-
-hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
-                        const RadixSortPlan& sort, hipStream_t s) {
-    return c.tab ? apply_impl<true>(c, b, rp, parity, sort, s) : apply_impl<false>(c, b, rp, parity, sort, s);
 }
 
 }  // namespace sph
