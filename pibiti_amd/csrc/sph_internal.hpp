@@ -47,6 +47,7 @@ struct RadixSortPlan {
     uint64_t* status = nullptr;      // look-back words, one per (tile, digit)
     uint64_t* ticket = nullptr;      // tile ticket counter (monotone)
     uint32_t* err = nullptr;         // set when a look-back wait hit its limit
+    uint32_t* done = nullptr;        // histogram blocks finished (the last one scans; zero between sorts)
 };
 
 // Bytes of device scratch the sort of n keys needs, and the plan carved out
@@ -90,6 +91,7 @@ struct ResortPlan {
     uint64_t* moved = nullptr;   // one bit per slot: the cell changed this step
     uint32_t* tile_total[2] = {nullptr, nullptr};  // particles per tile of kScanTile cells in the last order
     uint32_t* tile_delta[2] = {nullptr, nullptr};  // movers in less movers out per tile this step
+    uint32_t* valid_step = nullptr;  // this step's copy of valid (the last kernel of a step sets valid)
     uint32_t* err = nullptr;     // set when the counts disagree (results invalid)
 };
 size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells);

@@ -604,7 +604,7 @@ __device__ __forceinline__ void wave_add(uint32_t* __restrict__ counter, uint32_
 }
 
 // The same integration for the incremental re-sort, keeping only the new
-// key (in slot order; the state is integrated again where k_resort_scatter
+// key (in slot order; the state is integrated again where k_resort_place
 // moves it).  A slot whose key changed (every slot while the order is not
 // valid) is a mover: its bit is set, its new cell's mover count raised and
 // its original index listed there (the first kMaxIncoming of a cell).
@@ -614,12 +614,13 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pid,
     const uint32_t* __restrict__ skey, uint32_t* __restrict__ nkey, const uint32_t* __restrict__ valid,
     uint64_t* __restrict__ moved, uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ inlist,
-    uint32_t* __restrict__ tile_delta, uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
+    uint32_t* __restrict__ tile_delta, uint32_t* __restrict__ key_pid, uint32_t* __restrict__ val_pid,
+    uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n) return;
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
-    // the key only: k_resort_scatter integrates the state again as it moves it
+    // the key only: k_resort_place integrates the state again as it moves it
     float4 x4 = pos[tid];
     float4 v4 = vel[tid];
     const uint32_t h = integrate_state(p, dt_dev[scene], x4, v4, counters);
@@ -627,6 +628,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     const uint32_t h_old = ok ? skey[tid] : kEmptyCell;
     const bool mv = !ok || h_old != h;
     nkey[tid] = h;
+    // the radix sort's input, for a step that turns out to need it
+    const uint32_t id = pid[tid];
+    key_pid[id] = h;
+    val_pid[id] = tid;
     // the active lanes of the last wave are a prefix of it: lane 0 writes
     const uint64_t bits = __ballot(mv);
     if ((threadIdx.x & 63) == 0) moved[tid >> 6] = bits;
@@ -642,112 +647,99 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
 }
 
 // ----------------------------------------------- incremental re-sort: apply
+// The state in the step's new order, one slot per thread, after the cell
+// scan (and the radix sort on a full-sort step).  Each slot's state is
+// integrated again from the last step's values (the integrate kernel keeps
+// only the keys; the same arithmetic gives the same bits).
+//   incremental: old slot k goes to its new cell's start + the cell's
+//     stayers with a smaller original index (a stayer: its rank among the
+//     old range's stayers; a mover: a binary search of the old range) + the
+//     cell's listed movers in with a smaller index;
+//   full sort: new slot k takes the sorted pair's old slot.
+// Marks the new order valid; the step's own validity was copied to
+// valid_step by the cell scan.
 template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_resort_scatter(
+__global__ __launch_bounds__(kBlock) void k_resort_place(
     sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ nkey,
-    const uint64_t* __restrict__ moved, const uint32_t* __restrict__ valid, const uint32_t* __restrict__ old_start,
-    const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ new_start,
-    const uint32_t* __restrict__ new_end, const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist,
-    const float4* __restrict__ pos_in,
-    const float4* __restrict__ vel_in, const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in,
-    float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out,
-    uint32_t* __restrict__ sid_out, uint32_t* __restrict__ skey_out, uint32_t* __restrict__ key_pid,
-    uint32_t* __restrict__ val_pid, uint32_t* __restrict__ counters, uint32_t* __restrict__ err,
-    const float* __restrict__ dt_dev) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t c = nkey[k], id = pid_in[k];
-    if (counters[kCntFullSort]) {
-        // the radix sort's input: keys in original-index order
-        key_pid[id] = c;
-        val_pid[id] = k;
-        return;
-    }
-    const bool mv = is_mover(moved, k);
-    const uint32_t base = new_start[c], end = new_end[c];
-    bool bad = base >= end || end > n;
-    uint32_t os = kEmptyCell, oe = 0, stay = 0;
-    if (*valid) {
-        os = old_start[c];
-        if (os != kEmptyCell) {
-            oe = old_end[c];
-            if (oe > n || oe <= os)
-                bad = true;
-            else
-                stay = (oe - os) - movers_between(moved, os, oe);
-        }
-    }
-    // a stayer lies in its cell's old range
-    if (!mv && (os == kEmptyCell || k < os || k >= oe)) bad = true;
-    const uint32_t nin = in_cnt[c];
-    if (bad || nin > kMaxIncoming || end - base != stay + nin) {
-        atomicOr(err, 1u);
-        return;
-    }
-    uint32_t local = 0;
-    if (!mv) {
-        local = (k - os) - movers_between(moved, os, k);
-    } else if (os != kEmptyCell) {
-        uint32_t lo = os, hi = oe;
-        while (lo < hi) {
-            const uint32_t mid = lo + ((hi - lo) >> 1);
-            if (pid_in[mid] < id)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        local = (lo - os) - movers_between(moved, os, lo);
-    }
-    uint32_t inr = 0;
-    const uint32_t* inc = inlist + (size_t)c * kMaxIncoming;
-    for (uint32_t q = 0; q < nin; ++q) inr += inc[q] < id ? 1u : 0u;
-    const uint32_t r = base + local + inr;
-    if (r >= end) {
-        atomicOr(err, 1u);
-        return;
-    }
-    uint32_t scene;
-    const sph_params& p = scene_params<kBatch>(one, tab, sid_in, k, scene);
-    float4 x4 = pos_in[k];
-    float4 v4 = vel_in[k];
-    (void)integrate_state(p, dt_dev[scene], x4, v4, nullptr);
-    pos_out[r] = x4;
-    vel_out[r] = v4;
-    pid_out[r] = id;
-    skey_out[r] = c;
-    if constexpr (kBatch) sid_out[r] = scene;
-    if (r + 1 == end && end - base > p.max_par_in_cell) atomicAdd(&counters[kCntCapCells], 1u);
-}
-
-// A full-sort step's reorder (the cell ranges come from k_cell_scan): the
-// state integrated again and gathered into the sorted slots.  Marks the new
-// order valid on every step.
-template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_resort_gather(
-    sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ skey,
-    const uint32_t* __restrict__ old_slot, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
+    const uint64_t* __restrict__ moved, const uint32_t* __restrict__ valid_step, uint32_t* __restrict__ valid,
+    const uint32_t* __restrict__ old_start, const uint32_t* __restrict__ old_end,
+    const uint32_t* __restrict__ new_start, const uint32_t* __restrict__ new_end,
+    const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist, const uint32_t* __restrict__ sorted_key,
+    const uint32_t* __restrict__ sorted_slot, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
     const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out, uint32_t* __restrict__ sid_out,
-    uint32_t* __restrict__ skey_out, uint32_t* __restrict__ counters, uint32_t* __restrict__ valid,
+    uint32_t* __restrict__ skey_out, uint32_t* __restrict__ counters, uint32_t* __restrict__ err,
     const float* __restrict__ dt_dev) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *valid = 1u;
-    if (k >= n || counters[kCntFullSort] == 0) return;
-    const uint32_t s = old_slot[k];
+    if (k >= n) return;
+    uint32_t src, dst, c, id;
+    if (counters[kCntFullSort]) {
+        src = sorted_slot[k];
+        dst = k;
+        c = sorted_key[k];
+        id = pid_in[src];
+    } else {
+        src = k;
+        c = nkey[k];
+        id = pid_in[k];
+        const bool mv = is_mover(moved, k);
+        const uint32_t base = new_start[c], end = new_end[c];
+        bool bad = base >= end || end > n;
+        uint32_t os = kEmptyCell, oe = 0, stay = 0;
+        if (*valid_step) {
+            os = old_start[c];
+            if (os != kEmptyCell) {
+                oe = old_end[c];
+                if (oe > n || oe <= os)
+                    bad = true;
+                else
+                    stay = (oe - os) - movers_between(moved, os, oe);
+            }
+        }
+        // a stayer lies in its cell's old range
+        if (!mv && (os == kEmptyCell || k < os || k >= oe)) bad = true;
+        const uint32_t nin = in_cnt[c];
+        if (bad || nin > kMaxIncoming || end - base != stay + nin) {
+            atomicOr(err, 1u);
+            return;
+        }
+        uint32_t local = 0;
+        if (!mv) {
+            local = (k - os) - movers_between(moved, os, k);
+        } else if (os != kEmptyCell) {
+            uint32_t lo = os, hi = oe;
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo) >> 1);
+                if (pid_in[mid] < id)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            local = (lo - os) - movers_between(moved, os, lo);
+        }
+        uint32_t inr = 0;
+        const uint32_t* inc = inlist + (size_t)c * kMaxIncoming;
+        for (uint32_t q = 0; q < nin; ++q) inr += inc[q] < id ? 1u : 0u;
+        dst = base + local + inr;
+        if (dst >= end) {
+            atomicOr(err, 1u);
+            return;
+        }
+    }
     uint32_t scene;
-    const sph_params& p = scene_params<kBatch>(one, tab, sid_in, s, scene);
-    float4 x4 = pos_in[s];
-    float4 v4 = vel_in[s];
+    const sph_params& p = scene_params<kBatch>(one, tab, sid_in, src, scene);
+    float4 x4 = pos_in[src];
+    float4 v4 = vel_in[src];
     (void)integrate_state(p, dt_dev[scene], x4, v4, nullptr);
-    pos_out[k] = x4;
-    vel_out[k] = v4;
-    pid_out[k] = pid_in[s];
-    const uint32_t h = skey[k];
-    skey_out[k] = h;
-    if constexpr (kBatch) sid_out[k] = scene;
-    const uint32_t cap = p.max_par_in_cell;
-    if ((k + 1 == n || skey[k + 1] != h) && k >= cap && skey[k - cap] == h)
-        atomicAdd(&counters[kCntCapCells], 1u);
+    pos_out[dst] = x4;
+    vel_out[dst] = v4;
+    pid_out[dst] = id;
+    skey_out[dst] = c;
+    if constexpr (kBatch) sid_out[dst] = scene;
+    // the cell's last slot counts a cell above the candidate cap
+    const uint32_t end = new_end[c];
+    if (dst + 1 == end && end - new_start[c] > p.max_par_in_cell) atomicAdd(&counters[kCntCapCells], 1u);
 }
 
 // ------------------------------------------------------------------ reorder
@@ -1688,18 +1680,16 @@ void integrate_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
 template <bool kBatch>
 hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
                              const RadixSortPlan& sort, hipStream_t s) {
-    const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
-    hipLaunchKernelGGL((k_resort_scatter<kBatch>), grid, block, 0, s, c.one, c.tab, c.n, b.key_tmp, rp.moved,
-                       rp.valid, b.cell_start, b.cell_end, b.cell_start_alt, b.cell_end_alt, rp.in_cnt[parity],
-                       rp.inlist, b.pos_cur, b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt,
-                       b.sid_alt, b.skey_alt, b.key_pid, b.val_pid, b.counters, rp.err, b.dt_dev);
+    // the full sort runs only when the cell scan flagged the step
     uint32_t *skey = nullptr, *oslot = nullptr;
     hipError_t e = radix_sort_pairs(sort, c.key_bits, b.key_pid, b.val_pid, b.key_tmp, b.val_tmp, &skey, &oslot, s,
                                     b.counters + kCntFullSort);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_resort_gather<kBatch>), grid, block, 0, s, c.one, c.tab, c.n, skey, oslot, b.pos_cur,
-                       b.vel_cur, b.pid_cur, b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt,
-                       b.counters, rp.valid, b.dt_dev);
+    hipLaunchKernelGGL((k_resort_place<kBatch>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, c.one, c.tab,
+                       c.n, b.key_tmp, rp.moved, rp.valid_step, rp.valid, b.cell_start, b.cell_end, b.cell_start_alt,
+                       b.cell_end_alt, rp.in_cnt[parity], rp.inlist, skey, oslot, b.pos_cur, b.vel_cur, b.pid_cur,
+                       b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt, b.counters, rp.err,
+                       b.dt_dev);
     return hipGetLastError();
 }
 
@@ -1838,11 +1828,11 @@ hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, cons
     if (c.tab)
         hipLaunchKernelGGL((k_integrate_moves<true>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
                            b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
-                           rp.tile_delta[parity], b.counters, b.dt_dev);
+                           rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
     else
         hipLaunchKernelGGL((k_integrate_moves<false>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
                            b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
-                           rp.tile_delta[parity], b.counters, b.dt_dev);
+                           rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
     return hipGetLastError();
 }
 

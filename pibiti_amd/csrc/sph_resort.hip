@@ -23,17 +23,19 @@
 //                      flags the step for the full sort when a cell takes
 //                      more than kMaxIncoming movers or holds more than
 //                      kMaxCellCount particles;
-//   k_resort_scatter   (sph_kernels.hip) slot -> new slot: cell start +
+//   k_resort_place     (sph_kernels.hip) slot -> new slot: cell start +
 //                      stayers of the cell with a smaller index (a stayer:
 //                      its rank among the old range's stayers; a mover: a
 //                      binary search of the old range) + listed movers in
 //                      with a smaller index; integrates the slot's state
 //                      again (the same arithmetic) and writes it there with
 //                      its index, scene and key.  On a full-sort step it
-//                      writes the radix sort's input instead;
-//   radix sort + k_resort_gather (sph_kernels.hip)  only on a full-sort step
-//                      (gated on the device, so the launch sequence never
-//                      changes and graphs replay it).
+//                      gathers from the radix sort's output instead;
+//   the radix sort     (sph_sort.hip) on the pairs the integrate kernel also
+//                      writes, only on a full-sort step: launched every
+//                      step before k_resort_place, its kernels return at
+//                      once otherwise (so the launch sequence never changes
+//                      and graphs replay it).
 // While the order is invalid (first step, after the learned surrogate or
 // the tuner rearranged the slots) every slot counts as a mover with no old
 // ranges, which is a counting sort by cell with the same result.
@@ -69,12 +71,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
     const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ in_next,
     const uint64_t* __restrict__ moved, uint32_t* __restrict__ new_start, uint32_t* __restrict__ new_end,
     uint32_t* __restrict__ counters, const uint32_t* __restrict__ tile_total, const uint32_t* __restrict__ tile_delta,
-    uint32_t* __restrict__ total_next, uint32_t* __restrict__ delta_next, uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ total_next, uint32_t* __restrict__ delta_next, uint32_t* __restrict__ valid_step,
+    uint32_t* __restrict__ err) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_total, s_excl;
     const uint32_t t = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     const bool ok = *valid != 0;
+    if (tile == 0 && t == 0) *valid_step = ok ? 1u : 0u;  // for k_resort_place, which sets valid
 
     // four consecutive cells per thread (16-byte loads; the arrays are
     // padded to a multiple of four cells)
@@ -162,9 +166,9 @@ inline size_t pad4(uint32_t cells) { return ((size_t)cells + 3) & ~(size_t)3; }
 
 size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells) {
     // mover counts x 2 (padded to four cells) | incoming lists | tile totals
-    // x 2 | tile deltas x 2 | valid, err | mover bits
+    // x 2 | tile deltas x 2 | valid, err, valid_step | mover bits
     return 2 * pad4(num_cells) * 4 + (size_t)num_cells * kMaxIncoming * 4 + 4 * (size_t)scan_tiles(num_cells) * 4 +
-           8 + ((size_t)n / 64 + 2) * 8;
+           16 + ((size_t)n / 64 + 2) * 8;
 }
 
 hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, void* scratch, hipStream_t s) {
@@ -186,7 +190,8 @@ hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, vo
     p += 4 * (size_t)plan.num_tiles * 4;
     plan.valid = reinterpret_cast<uint32_t*>(p);
     plan.err = reinterpret_cast<uint32_t*>(p + 4);
-    p += 8;
+    plan.valid_step = reinterpret_cast<uint32_t*>(p + 8);
+    p += 16;
     plan.moved = reinterpret_cast<uint64_t*>(p);
     p += ((size_t)n / 64 + 2) * 8;
     if (e == hipSuccess) e = hipMemsetAsync(q, 0, (size_t)(p - q), s);
@@ -202,7 +207,7 @@ hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPl
     hipLaunchKernelGGL(k_cell_scan, dim3(rp.num_tiles), dim3(kBlock), 0, s, c.num_cells, rp.valid, b.cell_start,
                        b.cell_end, rp.in_cnt[parity], rp.in_cnt[parity ^ 1], rp.moved, b.cell_start_alt,
                        b.cell_end_alt, b.counters, rp.tile_total[parity], rp.tile_delta[parity],
-                       rp.tile_total[parity ^ 1], rp.tile_delta[parity ^ 1], rp.err);
+                       rp.tile_total[parity ^ 1], rp.tile_delta[parity ^ 1], rp.valid_step, rp.err);
     return hipGetLastError();
 }
 

@@ -11,8 +11,9 @@
 // Structure (one read of the keys for all digit histograms, then one kernel
 // per pass):
 //   k_hist        per-block digit histograms of every pass in one read
-//                 (runs of equal digits in a wave counted by one atomic),
-//   k_hist_scan   sum over blocks + exclusive scan -> global digit offsets,
+//                 (runs of equal digits in a wave counted by one atomic);
+//                 the last block to finish sums them over the blocks and
+//                 scans -> global digit offsets (scan_pass),
 //   k_pass        per tile: stable local ranking with 64-lane ballots (peer
 //                 mask per digit, rank = popcount of lower peers), tile digit
 //                 counts published and prefixed across tiles with a
@@ -42,8 +43,9 @@ constexpr int kHistThreads = 1024;
 constexpr int kHistKeys = 16;                       // keys per thread per round
 constexpr int kHistTile = kHistThreads * kHistKeys;  // keys per block per round
 constexpr int kScanThreads = 1024;
-constexpr int kScanQ = kScanThreads / kRadix;        // threads per digit in k_hist_scan
-static_assert(kHistBlocks % kScanQ == 0 && kHistBlocks / kScanQ <= 16, "k_hist_scan: one round of loads");
+constexpr int kScanQ = kScanThreads / kRadix;        // threads per digit in scan_pass
+static_assert(kHistBlocks % kScanQ == 0 && kHistBlocks / kScanQ <= 16, "scan_pass: one round of loads");
+static_assert(kHistThreads == kScanThreads, "the last histogram block runs the scan");
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 constexpr uint64_t kFlagAgg = 1ull << 30;
@@ -77,6 +79,36 @@ __device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Digit totals of pass p over the histogram blocks (kScanQ threads per
+// digit, every load of a thread in flight at once), then an exclusive scan
+// into offsets; the whole block of kScanThreads takes part.
+__device__ __forceinline__ void scan_pass(const uint32_t* __restrict__ block_hist, uint32_t nblocks,
+                                          uint32_t* __restrict__ offsets, uint32_t p,
+                                          uint32_t (*part)[kRadix], uint32_t* s_wave) {
+    const int t = threadIdx.x % kRadix;
+    const int q = threadIdx.x / kRadix;
+    constexpr int kRows = kHistBlocks / kScanQ;
+    uint32_t v[kRows];
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) {
+        const uint32_t b = (uint32_t)(q + u * kScanQ);
+        v[u] = block_hist[((size_t)min(b, nblocks - 1) * kMaxPasses + p) * kRadix + t];
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) sum += (uint32_t)(q + u * kScanQ) < nblocks ? v[u] : 0u;
+    part[q][t] = sum;
+    __syncthreads();
+    uint32_t tot = 0;
+    if (q == 0) {
+#pragma unroll
+        for (int r = 0; r < kScanQ; ++r) tot += part[r][t];
+    }
+    // threads past the first kRadix add nothing and come after every digit
+    const uint32_t incl = block_incl_scan<kScanThreads>(tot, s_wave);
+    if (q == 0) offsets[p * kRadix + t] = incl - tot;
+}
+
 // Digit histograms of every pass in one read of the keys.  Each thread
 // loads kHistKeys keys before counting (the loads overlap), and within a
 // wave a run of consecutive lanes with the same digit is counted by one
@@ -84,6 +116,7 @@ __device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
 // step's order), so runs are long and the atomics rarely collide.
 __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                       uint32_t bits, uint32_t* __restrict__ block_hist,
+                                                      uint32_t* __restrict__ offsets, uint32_t* __restrict__ done,
                                                       const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0) return;
     __shared__ uint32_t hist[kMaxPasses][kRadix];
@@ -118,40 +151,22 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
     __syncthreads();
     for (int c = t; c < (int)passes * kRadix; c += kHistThreads)
         block_hist[((size_t)blockIdx.x * kMaxPasses + c / kRadix) * kRadix + c % kRadix] = hist[c / kRadix][c % kRadix];
-}
 
-// One block per pass: digit totals over the histogram blocks (kScanQ
-// threads per digit, every load of a thread in flight at once), then an
-// exclusive scan.
-__global__ __launch_bounds__(kScanThreads) void k_hist_scan(const uint32_t* __restrict__ block_hist,
-                                                           uint32_t nblocks, uint32_t* __restrict__ offsets,
-                                                           const uint32_t* __restrict__ gate) {
-    if (gate && *gate == 0) return;
+    // the last block to finish sums and scans the digits of every pass
     __shared__ uint32_t part[kScanQ][kRadix];
     __shared__ uint32_t s_wave[kScanThreads / kWave];
-    const int t = threadIdx.x % kRadix;
-    const int q = threadIdx.x / kRadix;
-    const int p = blockIdx.x;
-    constexpr int kRows = kHistBlocks / kScanQ;
-    uint32_t v[kRows];
-#pragma unroll
-    for (int u = 0; u < kRows; ++u) {
-        const uint32_t b = (uint32_t)(q + u * kScanQ);
-        v[u] = block_hist[((size_t)min(b, nblocks - 1) * kMaxPasses + p) * kRadix + t];
-    }
-    uint32_t sum = 0;
-#pragma unroll
-    for (int u = 0; u < kRows; ++u) sum += (uint32_t)(q + u * kScanQ) < nblocks ? v[u] : 0u;
-    part[q][t] = sum;
+    __shared__ bool s_last;
+    __threadfence();
     __syncthreads();
-    uint32_t tot = 0;
-    if (q == 0) {
-#pragma unroll
-        for (int r = 0; r < kScanQ; ++r) tot += part[r][t];
+    if (t == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (uint32_t p = 0; p < passes; ++p) {
+        scan_pass(block_hist, gridDim.x, offsets, p, part, s_wave);
+        __syncthreads();
     }
-    // threads past the first kRadix add nothing and come after every digit
-    const uint32_t incl = block_incl_scan<kScanThreads>(tot, s_wave);
-    if (q == 0) offsets[p * kRadix + t] = incl - tot;
+    if (t == 0) *done = 0u;  // ready for the next sort
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_pass(
@@ -278,7 +293,8 @@ uint32_t radix_num_tiles(uint32_t n) {
 }
 
 size_t radix_scratch_bytes(uint32_t n) {
-    // block histograms | digit offsets | status words | ticket | error word
+    // block histograms | digit offsets | status words | ticket | error word |
+    // finished histogram blocks
     return (size_t)kHistBlocks * kMaxPasses * kRadix * 4 + (size_t)kMaxPasses * kRadix * 4 +
            (size_t)radix_num_tiles(n) * kRadix * 8 + 64;
 }
@@ -295,7 +311,9 @@ hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipSt
     b += (size_t)plan.num_tiles * kRadix * 8;
     plan.ticket = reinterpret_cast<uint64_t*>(b);
     plan.err = reinterpret_cast<uint32_t*>(b + 8);
-    // status words start at epoch 0 / no flag; the ticket and error word at 0
+    plan.done = reinterpret_cast<uint32_t*>(b + 12);
+    // status words start at epoch 0 / no flag; the ticket, error word and
+    // histogram-block count at 0
     return hipMemsetAsync(plan.status, 0, (size_t)plan.num_tiles * kRadix * 8 + 64, stream);
 }
 
@@ -312,9 +330,8 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
     if (passes > (uint32_t)kMaxPasses) return hipErrorInvalidValue;
     if (passes > 0) {
         const uint32_t hb = std::min<uint32_t>(kHistBlocks, std::max<uint32_t>(1u, (n + kHistTile - 1) / kHistTile));
-        hipLaunchKernelGGL(k_hist, dim3(hb), dim3(kHistThreads), 0, stream, kin, n, bits, plan.block_hist, gate);
-        hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(kScanThreads), 0, stream, plan.block_hist, hb,
-                           plan.digit_off, gate);
+        hipLaunchKernelGGL(k_hist, dim3(hb), dim3(kHistThreads), 0, stream, kin, n, bits, plan.block_hist,
+                           plan.digit_off, plan.done, gate);
     }
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t shift = 8 * p;
