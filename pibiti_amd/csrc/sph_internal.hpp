@@ -171,7 +171,9 @@ struct LaunchCfg {
 uint32_t stats_chunks(const LaunchCfg& c);
 size_t stats_partials_count(const LaunchCfg& c);
 
-hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
+// The integration in place, keeping the slot order (the surrogate's steps
+// between sorts).
+hipError_t launch_integrate(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
 // The step's integration for the incremental re-sort: the new cell key of
 // each slot into nkey (slot order), the mover bits and per-cell mover counts
 // into the plan (every slot is a mover while the plan's order is invalid).
@@ -189,8 +191,6 @@ hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPl
 // Particle colours and dye after the force update: b holds the step's state
 // before the update (pos_cur, vel_cur, rhop, dvel, pid_cur).
 hipError_t launch_colour(const LaunchCfg& c, const StepBuffers& b, float4* colour, float* dye, hipStream_t s);
-hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
-                          const uint32_t* old_slot, hipStream_t s);
 hipError_t launch_density(const LaunchCfg& c, const StepBuffers& b, hipStream_t s);
 // Density plus each slot's neighbour list, group-major: entries 4g .. 4g + 3
 // of slot k are nbr[4 (g n + k) .. 4 (g n + k) + 3] (g < kNbrCap / 4), and

@@ -567,27 +567,19 @@ __device__ __forceinline__ uint32_t integrate_one(const sph_params& p, float dt,
     return h;
 }
 
+// The integration in place, for steps that keep their order (the learned
+// surrogate's steps between sorts).
 template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_integrate_hash(
-    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid,
-    uint32_t n, uint32_t num_cells, float4* __restrict__ pos, float4* __restrict__ vel,
-    const uint32_t* __restrict__ pid, uint32_t* __restrict__ key_pid,
-    uint32_t* __restrict__ val_pid, uint32_t* __restrict__ cell_start,
-    uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
+__global__ __launch_bounds__(kBlock) void k_integrate(sph_params one, const sph_params* __restrict__ tab,
+                                                      const uint32_t* __restrict__ sid, uint32_t n,
+                                                      float4* __restrict__ pos, float4* __restrict__ vel,
+                                                      uint32_t* __restrict__ counters,
+                                                      const float* __restrict__ dt_dev) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * blockDim.x;
-    // cellStart of the previous step is dead from here on: clear it.
-    for (uint32_t c = tid; c < num_cells; c += nthreads) cell_start[c] = kEmptyCell;
     if (tid >= n) return;
-
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
-    const uint32_t h = integrate_one(p, dt_dev[scene], pos, vel, tid, counters);
-    if (key_pid) {  // null: a step that keeps its order writes no sort input
-        const uint32_t id = pid[tid];
-        key_pid[id] = h;
-        val_pid[id] = tid;
-    }
+    (void)integrate_one(p, dt_dev[scene], pos, vel, tid, counters);
 }
 
 // counter[key] += d (mod 2^32) for every active lane with `on`, one atomic
@@ -740,36 +732,6 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
     // the cell's last slot counts a cell above the candidate cap
     const uint32_t end = new_end[c];
     if (dst + 1 == end && end - new_start[c] > p.max_par_in_cell) atomicAdd(&counters[kCntCapCells], 1u);
-}
-
-// ------------------------------------------------------------------ reorder
-template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_reorder(
-    sph_params one, const sph_params* __restrict__ tab, uint32_t n,
-    const uint32_t* __restrict__ skey, const uint32_t* __restrict__ old_slot,
-    const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
-    const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in,
-    float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out,
-    uint32_t* __restrict__ sid_out, uint32_t* __restrict__ cell_start,
-    uint32_t* __restrict__ cell_end, uint32_t* __restrict__ counters) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t s = old_slot[k];
-    pos_out[k] = pos_in[s];
-    vel_out[k] = vel_in[s];
-    pid_out[k] = pid_in[s];
-    uint32_t cap = one.max_par_in_cell;
-    if constexpr (kBatch) {
-        const uint32_t sc = sid_in[s];
-        sid_out[k] = sc;
-        cap = tab[sc].max_par_in_cell;
-    }
-    const uint32_t h = skey[k];
-    if (k == 0 || skey[k - 1] != h) cell_start[h] = k;
-    if (k + 1 == n || skey[k + 1] != h) {
-        cell_end[h] = k + 1;
-        if (k >= cap && skey[k - cap] == h) atomicAdd(&counters[kCntCapCells], 1u);
-    }
 }
 
 // --------------------------------------------------- candidate positions
@@ -1671,13 +1633,6 @@ inline uint32_t blocks_for(uint64_t n, uint32_t block) {
 
 // ------------------------------------------------------------- dispatchers
 template <bool kBatch>
-void integrate_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
-    hipLaunchKernelGGL((k_integrate_hash<kBatch>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
-                       c.one, c.tab, b.sid_cur, c.n, c.num_cells, b.pos_cur, b.vel_cur, b.pid_cur,
-                       b.key_pid, b.val_pid, b.cell_start, b.counters, b.dt_dev);
-}
-
-template <bool kBatch>
 hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
                              const RadixSortPlan& sort, hipStream_t s) {
     // the full sort runs only when the cell scan flagged the step
@@ -1691,15 +1646,6 @@ hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const Res
                        b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt, b.counters, rp.err,
                        b.dt_dev);
     return hipGetLastError();
-}
-
-template <bool kBatch>
-void reorder_impl(const LaunchCfg& c, const StepBuffers& b, const uint32_t* skey,
-                  const uint32_t* oslot, hipStream_t s) {
-    hipLaunchKernelGGL((k_reorder<kBatch>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, c.one,
-                       c.tab, c.n, skey, oslot, b.pos_cur, b.vel_cur, b.pid_cur, b.sid_cur,
-                       b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.cell_start, b.cell_end,
-                       b.counters);
 }
 
 // The pair-mask path (density writes the force's pair masks, the force
@@ -1814,11 +1760,14 @@ size_t stats_partials_count(const LaunchCfg& c) {
     return (size_t)stats_chunks(c) * c.nscenes * kPNum;
 }
 
-hipError_t launch_integrate_hash(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
+hipError_t launch_integrate(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
+    const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
     if (c.tab)
-        integrate_impl<true>(c, b, s);
+        hipLaunchKernelGGL((k_integrate<true>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur,
+                           b.counters, b.dt_dev);
     else
-        integrate_impl<false>(c, b, s);
+        hipLaunchKernelGGL((k_integrate<false>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.vel_cur, b.counters, b.dt_dev);
     return hipGetLastError();
 }
 
@@ -1839,15 +1788,6 @@ hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, cons
 hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
                         const RadixSortPlan& sort, hipStream_t s) {
     return c.tab ? resort_apply_impl<true>(c, b, rp, parity, sort, s) : resort_apply_impl<false>(c, b, rp, parity, sort, s);
-}
-
-hipError_t launch_reorder(const LaunchCfg& c, const StepBuffers& b, const uint32_t* sorted_key,
-                          const uint32_t* old_slot, hipStream_t s) {
-    if (c.tab)
-        reorder_impl<true>(c, b, sorted_key, old_slot, s);
-    else
-        reorder_impl<false>(c, b, sorted_key, old_slot, s);
-    return hipGetLastError();
 }
 
 hipError_t launch_density_lists(const LaunchCfg& c, const StepBuffers& b, uint32_t* nbr, uint32_t* nnb,

@@ -366,10 +366,7 @@ int enqueue_coarse(Engine& e, LearnState& L, const sph_model& m, float skin, boo
     } else {
         // a step between sorts keeps the slot order and the cell ranges
         // (and so the order stays valid for the next sort)
-        LaunchCfg ci = c;
-        ci.num_cells = 0;
-        b.key_pid = nullptr;
-        SPH_TRY(launch_integrate_hash(ci, b, e.stream));
+        SPH_TRY(launch_integrate(c, b, e.stream));
     }
     b = e.buffers();
     if (timed) SPH_TRY(hipEventRecord(e.ev[1], e.stream));
