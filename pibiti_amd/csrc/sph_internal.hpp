@@ -78,16 +78,24 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
 // padded to a multiple of four cells.  The result equals the
 // stable radix sort of (cell, index) exactly.  The full radix sort runs
 // instead (decided on the device) when a cell receives more than
-// kMaxIncoming movers or holds more than kMaxCellCount particles.
+// in_cap movers or holds more than kMaxCellCount particles.  in_cap is
+// kMaxIncoming, less where the grid is so large for its particles that the
+// incoming lists (in_cap words per cell) would pass kInListBudget (0: every
+// step with a mover takes the full sort).
 constexpr uint32_t kMaxIncoming = 32;
+constexpr uint64_t kInListBudgetMin = 64ull << 20;   // bytes
+constexpr uint64_t kInListBudgetPerParticle = 128;   // bytes
+constexpr uint32_t kDirectTiles = 2048;  // above this many scan tiles, k_tile_scan sums the tiles first
 constexpr uint32_t kScanTile = 1024;  // cells per block of the cell scan
 constexpr uint32_t kMaxCellCount = 4096;
 
 struct ResortPlan {
     uint32_t n = 0, num_cells = 0, num_tiles = 0;
+    uint32_t in_cap = 0;         // incoming-list entries per cell
+    uint32_t* tile_start = nullptr;  // first slot of each tile (only above kDirectTiles tiles)
     uint32_t* valid = nullptr;   // device word: slot order, skey and the current cell ranges agree
     uint32_t* in_cnt[2] = {nullptr, nullptr};  // movers into each cell; steps alternate (parity)
-    uint32_t* inlist = nullptr;  // original indices of the movers, kMaxIncoming per cell
+    uint32_t* inlist = nullptr;  // original indices of the movers, in_cap per cell
     uint64_t* moved = nullptr;   // one bit per slot: the cell changed this step
     uint32_t* tile_total[2] = {nullptr, nullptr};  // particles per tile of kScanTile cells in the last order
     uint32_t* tile_delta[2] = {nullptr, nullptr};  // movers in less movers out per tile this step

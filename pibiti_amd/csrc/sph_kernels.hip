@@ -599,13 +599,13 @@ __device__ __forceinline__ void wave_add(uint32_t* __restrict__ counter, uint32_
 // key (in slot order; the state is integrated again where k_resort_place
 // moves it).  A slot whose key changed (every slot while the order is not
 // valid) is a mover: its bit is set, its new cell's mover count raised and
-// its original index listed there (the first kMaxIncoming of a cell).
+// its original index listed there (the first in_cap of a cell).
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
     const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pid,
     const uint32_t* __restrict__ skey, uint32_t* __restrict__ nkey, const uint32_t* __restrict__ valid,
-    uint64_t* __restrict__ moved, uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ inlist,
+    uint64_t* __restrict__ moved, uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ inlist, uint32_t in_cap,
     uint32_t* __restrict__ tile_delta, uint32_t* __restrict__ key_pid, uint32_t* __restrict__ val_pid,
     uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     if ((threadIdx.x & 63) == 0) moved[tid >> 6] = bits;
     if (mv) {
         const uint32_t t = atomicAdd(&in_cnt[h], 1u);
-        if (t < kMaxIncoming) inlist[(size_t)h * kMaxIncoming + t] = pid[tid];
+        if (t < in_cap) inlist[(size_t)h * in_cap + t] = pid[tid];
     }
     // per scan tile: movers in less movers out (a move within a tile is none)
     const uint32_t tile_new = h / kScanTile, tile_old = h_old / kScanTile;
@@ -656,7 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
     const uint64_t* __restrict__ moved, const uint32_t* __restrict__ valid_step, uint32_t* __restrict__ valid,
     const uint32_t* __restrict__ old_start, const uint32_t* __restrict__ old_end,
     const uint32_t* __restrict__ new_start, const uint32_t* __restrict__ new_end,
-    const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist, const uint32_t* __restrict__ sorted_key,
+    const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist, uint32_t in_cap,
+    const uint32_t* __restrict__ sorted_key,
     const uint32_t* __restrict__ sorted_slot, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
     const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out, uint32_t* __restrict__ sid_out,
@@ -692,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
         // a stayer lies in its cell's old range
         if (!mv && (os == kEmptyCell || k < os || k >= oe)) bad = true;
         const uint32_t nin = in_cnt[c];
-        if (bad || nin > kMaxIncoming || end - base != stay + nin) {
+        if (bad || nin > in_cap || end - base != stay + nin) {
             atomicOr(err, 1u);
             return;
         }
@@ -711,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
             local = (lo - os) - movers_between(moved, os, lo);
         }
         uint32_t inr = 0;
-        const uint32_t* inc = inlist + (size_t)c * kMaxIncoming;
+        const uint32_t* inc = inlist + (size_t)c * in_cap;
         for (uint32_t q = 0; q < nin; ++q) inr += inc[q] < id ? 1u : 0u;
         dst = base + local + inr;
         if (dst >= end) {
@@ -1642,7 +1643,8 @@ hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const Res
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_resort_place<kBatch>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, c.one, c.tab,
                        c.n, b.key_tmp, rp.moved, rp.valid_step, rp.valid, b.cell_start, b.cell_end, b.cell_start_alt,
-                       b.cell_end_alt, rp.in_cnt[parity], rp.inlist, skey, oslot, b.pos_cur, b.vel_cur, b.pid_cur,
+                       b.cell_end_alt, rp.in_cnt[parity], rp.inlist, rp.in_cap, skey, oslot, b.pos_cur, b.vel_cur,
+                       b.pid_cur,
                        b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt, b.counters, rp.err,
                        b.dt_dev);
     return hipGetLastError();
@@ -1777,11 +1779,11 @@ hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, cons
     if (c.tab)
         hipLaunchKernelGGL((k_integrate_moves<true>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
                            b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
-                           rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
+                           rp.in_cap, rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
     else
         hipLaunchKernelGGL((k_integrate_moves<false>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
                            b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
-                           rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
+                           rp.in_cap, rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
     return hipGetLastError();
 }
 

@@ -11,17 +11,19 @@
 //
 //   k_integrate_moves  (sph_kernels.hip) the new key of each slot, mover
 //                      bits, movers counted into their new cells and their
-//                      original indices listed there (up to kMaxIncoming per
+//                      original indices listed there (up to in_cap per
 //                      cell), movers in less movers out per tile of
 //                      kScanTile cells; the integrated state is not stored;
 //   k_cell_scan        (here) per cell: stayers = old count - movers out
 //                      (the old range and the mover bits), new count =
 //                      stayers + movers in; exclusive scan over the tile's
 //                      cells, the tile's start summed from the earlier
-//                      tiles' counts of the last order and their deltas ->
+//                      tiles' counts of the last order and their deltas
+//                      (by k_tile_scan first on grids of more than
+//                      kDirectTiles tiles) ->
 //                      new cell ranges; clears the other step's counts;
 //                      flags the step for the full sort when a cell takes
-//                      more than kMaxIncoming movers or holds more than
+//                      more than in_cap movers or holds more than
 //                      kMaxCellCount particles;
 //   k_resort_place     (sph_kernels.hip) slot -> new slot: cell start +
 //                      stayers of the cell with a smaller index (a stayer:
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
     const uint64_t* __restrict__ moved, uint32_t* __restrict__ new_start, uint32_t* __restrict__ new_end,
     uint32_t* __restrict__ counters, const uint32_t* __restrict__ tile_total, const uint32_t* __restrict__ tile_delta,
     uint32_t* __restrict__ total_next, uint32_t* __restrict__ delta_next, uint32_t* __restrict__ valid_step,
-    uint32_t* __restrict__ err) {
+    uint32_t in_cap, const uint32_t* __restrict__ tile_start, uint32_t* __restrict__ err) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_total, s_excl;
     const uint32_t t = threadIdx.x;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
                 stay = old - movers_between(moved, ss[j], es[j]);
             }
             cnt[j] = stay + ins[j];
-            big = big || ins[j] > kMaxIncoming || cnt[j] > kMaxCellCount || old > kMaxCellCount;
+            big = big || ins[j] > in_cap || cnt[j] > kMaxCellCount || old > kMaxCellCount;
         }
     }
     if (__any(big) && (t & 63u) == 0) atomicAdd(&counters[kCntFullSort], 1u);
@@ -118,7 +120,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
     // tile's count of the previous order plus its movers in less its movers
     // out (no wait on other blocks)
     uint32_t pre = 0;
-    for (uint32_t q = t; q < tile; q += kBlock) pre += (ok ? tile_total[q] : 0u) + tile_delta[q];
+    if (tile_start) {
+        if (t == 0) pre = tile_start[tile];  // summed by k_tile_scan (large grids)
+    } else {
+        for (uint32_t q = t; q < tile; q += kBlock) pre += (ok ? tile_total[q] : 0u) + tile_delta[q];
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off);
     __syncthreads();  // s_wave of the scan is read; s_total written
@@ -155,39 +161,80 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
     }
 }
 
+// First slot of every tile (large grids): the exclusive prefix over tiles of
+// each tile's count in the last order plus its movers in less its movers
+// out, by one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_tile_scan(uint32_t num_tiles, const uint32_t* __restrict__ valid,
+                                                    const uint32_t* __restrict__ tile_total,
+                                                    const uint32_t* __restrict__ tile_delta,
+                                                    uint32_t* __restrict__ tile_start) {
+    __shared__ uint32_t s_wave[1024 / 64];
+    __shared__ uint32_t s_carry;
+    const bool ok = *valid != 0;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < num_tiles; base += 1024) {
+        const uint32_t q = base + threadIdx.x;
+        const uint32_t v = q < num_tiles ? (ok ? tile_total[q] : 0u) + tile_delta[q] : 0u;
+        const uint32_t incl = block_incl_scan(v, s_wave);
+        const uint32_t carry = s_carry;
+        if (q < num_tiles) tile_start[q] = carry + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) s_carry = carry + incl;
+        __syncthreads();
+    }
+}
+
 inline uint32_t scan_tiles(uint32_t num_cells) {
     const uint32_t t = (uint32_t)(((uint64_t)num_cells + kScanTile - 1) / kScanTile);
     return t == 0 ? 1 : t;
 }
 
-}  // namespace
-
 inline size_t pad4(uint32_t cells) { return ((size_t)cells + 3) & ~(size_t)3; }
+
+// Incoming-list entries per cell: kMaxIncoming, less when the lists of a
+// grid that large for its particles would pass the budget.
+inline uint32_t in_capacity(uint32_t n, uint32_t num_cells) {
+    uint64_t budget = (uint64_t)n * kInListBudgetPerParticle;
+    if (budget < kInListBudgetMin) budget = kInListBudgetMin;
+    const uint64_t cap = budget / ((uint64_t)num_cells * 4);
+    return cap < kMaxIncoming ? (uint32_t)cap : kMaxIncoming;
+}
+
+inline size_t tile_start_words(uint32_t num_cells) {
+    return scan_tiles(num_cells) > kDirectTiles ? scan_tiles(num_cells) : 0;
+}
+
+}  // namespace
 
 size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells) {
     // mover counts x 2 (padded to four cells) | incoming lists | tile totals
-    // x 2 | tile deltas x 2 | valid, err, valid_step | mover bits
-    return 2 * pad4(num_cells) * 4 + (size_t)num_cells * kMaxIncoming * 4 + 4 * (size_t)scan_tiles(num_cells) * 4 +
-           16 + ((size_t)n / 64 + 2) * 8;
+    // x 2 | tile deltas x 2 | tile starts (large grids) | valid, err,
+    // valid_step | mover bits
+    return 2 * pad4(num_cells) * 4 + (size_t)num_cells * in_capacity(n, num_cells) * 4 +
+           4 * (size_t)scan_tiles(num_cells) * 4 + tile_start_words(num_cells) * 4 + 16 + ((size_t)n / 64 + 2) * 8;
 }
 
 hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, void* scratch, hipStream_t s) {
     plan.n = n;
     plan.num_cells = num_cells;
     plan.num_tiles = scan_tiles(num_cells);
+    plan.in_cap = in_capacity(n, num_cells);
     char* p = static_cast<char*>(scratch);
     plan.in_cnt[0] = reinterpret_cast<uint32_t*>(p);
     plan.in_cnt[1] = plan.in_cnt[0] + pad4(num_cells);
     hipError_t e = hipMemsetAsync(p, 0, 2 * pad4(num_cells) * 4, s);
     p += 2 * pad4(num_cells) * 4;
     plan.inlist = reinterpret_cast<uint32_t*>(p);
-    p += (size_t)num_cells * kMaxIncoming * 4;
+    p += (size_t)num_cells * plan.in_cap * 4;
     char* q = p;
     for (int k = 0; k < 2; ++k) {
         plan.tile_total[k] = reinterpret_cast<uint32_t*>(p) + (size_t)k * plan.num_tiles;
         plan.tile_delta[k] = reinterpret_cast<uint32_t*>(p) + (size_t)(2 + k) * plan.num_tiles;
     }
     p += 4 * (size_t)plan.num_tiles * 4;
+    plan.tile_start = tile_start_words(num_cells) ? reinterpret_cast<uint32_t*>(p) : nullptr;
+    p += tile_start_words(num_cells) * 4;
     plan.valid = reinterpret_cast<uint32_t*>(p);
     plan.err = reinterpret_cast<uint32_t*>(p + 4);
     plan.valid_step = reinterpret_cast<uint32_t*>(p + 8);
@@ -204,10 +251,14 @@ hipError_t resort_invalidate(const ResortPlan& plan, hipStream_t s) {
 
 hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity, hipStream_t s) {
     if (rp.n != c.n || rp.num_cells != c.num_cells) return hipErrorInvalidValue;
+    if (rp.tile_start)
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, rp.num_tiles, rp.valid, rp.tile_total[parity],
+                           rp.tile_delta[parity], rp.tile_start);
     hipLaunchKernelGGL(k_cell_scan, dim3(rp.num_tiles), dim3(kBlock), 0, s, c.num_cells, rp.valid, b.cell_start,
                        b.cell_end, rp.in_cnt[parity], rp.in_cnt[parity ^ 1], rp.moved, b.cell_start_alt,
                        b.cell_end_alt, b.counters, rp.tile_total[parity], rp.tile_delta[parity],
-                       rp.tile_total[parity ^ 1], rp.tile_delta[parity ^ 1], rp.valid_step, rp.err);
+                       rp.tile_total[parity ^ 1], rp.tile_delta[parity ^ 1], rp.valid_step, rp.in_cap, rp.tile_start,
+                       rp.err);
     return hipGetLastError();
 }
 

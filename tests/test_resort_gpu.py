@@ -149,3 +149,42 @@ def test_upload_that_moves_many_particles():
     o = oracle.step_f32(p, x, v)
     assert_equal_oracle(g, o, "upload")
     assert full == 0
+
+
+def run_and_compare(p, pos, vel, steps, what):
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        kinds = []
+        for _ in range(steps):
+            s.update(1)
+            st = s.stats()
+            kinds.append(st["full_sort"])
+            assert st["sort_errors"] == 0
+        g = arrays(s)
+        assert_sorted_like_oracle(s, p)
+    o = oracle.run_f32(p, pos, vel, steps)
+    assert_equal_oracle(g, o, what)
+    return kinds
+
+
+def test_large_grid_scans_its_tiles_first():
+    """2.25M cells (more than 2,048 scan tiles): the tiles' starts come from
+    a scan of their own; the incoming lists shrink to the memory budget (7
+    per cell here), so the first step, 8 particles per cell, sorts fully."""
+    p, pos, vel, _ = scenes.dam_break(20, 20, 20, spacing=0.01, h=0.02, world_max=(3.0, 2.0, 3.0), dt=5e-5,
+                                      viscosity=0.1, vel_sigma=0.5, seed=26)
+    assert int(p.num_cells) == 150 * 100 * 150
+    print("case: 8000 particles in 2.25M cells, 6 steps", flush=True)
+    kinds = run_and_compare(p, pos, vel, 6, "large grid")
+    assert kinds[0] == 1 and kinds[1:] == [0] * 5, kinds
+
+
+def test_sparse_grid_without_incoming_lists():
+    """50.6M cells for 2,000 particles: no room for incoming lists, every
+    step with a particle changing cell takes the full sort."""
+    p, pos, vel, _ = scenes.dam_break(10, 10, 20, spacing=0.01, h=0.02, world_max=(7.4, 7.4, 7.4), dt=5e-5,
+                                      viscosity=0.1, vel_sigma=2.0, seed=27)
+    assert int(p.num_cells) == 370 ** 3
+    print("case: 2000 particles in 50.6M cells, 3 steps", flush=True)
+    kinds = run_and_compare(p, pos, vel, 3, "sparse grid")
+    assert kinds == [1, 1, 1], kinds
