@@ -752,7 +752,10 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
 struct PosGlobal {
     const float4* __restrict__ p;
     __device__ __forceinline__ int32_t offset(int) const { return 0; }
-    __device__ __forceinline__ float4 at(uint32_t j, int32_t) const { return p[j]; }
+    // a 32-bit byte offset (slots < 2^27): one base register plus the offset
+    __device__ __forceinline__ float4 at(uint32_t j, int32_t) const {
+        return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(p) + (j << 4));
+    }
 };
 
 struct PosStaged {
@@ -870,6 +873,10 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
     const float r2f = p.force_r2_max;
+    // The pair-mask path reads the step's position array, which is padded
+    // past its end: a group's loads past the cell's end stay in bounds and
+    // are masked out, so they need no clamp.
+    constexpr bool kUnclamped = kMasks && std::is_same<Src, PosGlobal>::value;
     uint32_t mword = 0;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
@@ -888,14 +895,20 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
             for (uint32_t j0 = s; j0 < e; j0 += 4) {
                 float4 xq[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) xq[k] = src.at(min(j0 + k, e - 1), soff);
+                for (int k = 0; k < 4; ++k) xq[k] = src.at(kUnclamped ? j0 + k : min(j0 + k, e - 1), soff);
+                // the group's candidates: slots below the cell's end, less the
+                // particle itself (i - j0 < 4 only in its own cell)
+                const uint32_t left = e - j0;
+                uint32_t lanes = left >= 4u ? 0xFu : (1u << left) - 1u;
+                if (i - j0 < 4u) lanes &= ~(1u << (i - j0));
+                uint32_t fbits = 0;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t j = j0 + k;
                     const float4 xj = xq[k];
                     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
                     const float r2 = rx * rx + ry * ry + rz * rz;
-                    const bool hit = (r2 < h2) && (j != i) && (j < e);
+                    const bool hit = (r2 < h2) && ((lanes >> k) & 1u);
                     const float c3 = h2 - r2;
                     const float wgt = (c3 * c3) * c3;
                     if (kPrec == SPH_PREC_KAHAN) {
@@ -905,8 +918,9 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                         cell = cell + (hit ? wgt : 0.0f);
                     }
                     if constexpr (kLists) lw.put(j, hit);
-                    if constexpr (kMasks) cmask |= ((r2 <= r2f) && (j != i) && (j < e) ? 1u : 0u) << (j - s);
+                    if constexpr (kMasks) fbits |= (r2 <= r2f ? 1u : 0u) << k;
                 }
+                if constexpr (kMasks) cmask |= (fbits & lanes) << (j0 - s);
                 if constexpr (kLists) lw.drain(lists, n, i);
             }
             if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
