@@ -891,38 +891,45 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
             float cell = 0.0f;
             const int32_t soff = src.offset(ord / 3);
             // candidates four at a time: the loads go out together (indices
-            // past the cell repeat its last slot and are masked out of the sums)
-            for (uint32_t j0 = s; j0 < e; j0 += 4) {
-                float4 xq[4];
+            // past the cell's end are masked out of the sums); the particle
+            // itself can only be a candidate of its own cell (ord 13), which
+            // takes the walk with the extra test
+            auto walk = [&](auto self_tag) {
+                constexpr bool kSelf = decltype(self_tag)::value;
+                for (uint32_t j0 = s; j0 < e; j0 += 4) {
+                    float4 xq[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) xq[k] = src.at(kUnclamped ? j0 + k : min(j0 + k, e - 1), soff);
-                // the group's candidates: slots below the cell's end, less the
-                // particle itself (i - j0 < 4 only in its own cell)
-                const uint32_t left = e - j0;
-                uint32_t lanes = left >= 4u ? 0xFu : (1u << left) - 1u;
-                if (i - j0 < 4u) lanes &= ~(1u << (i - j0));
-                uint32_t fbits = 0;
+                    for (int k = 0; k < 4; ++k) xq[k] = src.at(kUnclamped ? j0 + k : min(j0 + k, e - 1), soff);
+                    const uint32_t left = e - j0;
+                    const uint32_t self = kSelf ? i - j0 : 0u;
+                    uint32_t fbits = 0;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t j = j0 + k;
-                    const float4 xj = xq[k];
-                    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                    const float r2 = rx * rx + ry * ry + rz * rz;
-                    const bool hit = (r2 < h2) && ((lanes >> k) & 1u);
-                    const float c3 = h2 - r2;
-                    const float wgt = (c3 * c3) * c3;
-                    if (kPrec == SPH_PREC_KAHAN) {
-                        if (hit) ks.add(wgt);
-                    } else {
-                        // the partial sum is >= +0, so adding +0 for a miss is exact
-                        cell = cell + (hit ? wgt : 0.0f);
+                    for (int k = 0; k < 4; ++k) {
+                        const uint32_t j = j0 + k;
+                        const bool valid = left > (uint32_t)k && (!kSelf || self != (uint32_t)k);
+                        const float4 xj = xq[k];
+                        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        const bool hit = (r2 < h2) && valid;
+                        const float c3 = h2 - r2;
+                        const float wgt = (c3 * c3) * c3;
+                        if (kPrec == SPH_PREC_KAHAN) {
+                            if (hit) ks.add(wgt);
+                        } else {
+                            // the partial sum is >= +0, so adding +0 for a miss is exact
+                            cell = cell + (hit ? wgt : 0.0f);
+                        }
+                        if constexpr (kLists) lw.put(j, hit);
+                        if constexpr (kMasks) fbits |= ((r2 <= r2f) && valid ? 1u : 0u) << k;
                     }
-                    if constexpr (kLists) lw.put(j, hit);
-                    if constexpr (kMasks) fbits |= (r2 <= r2f ? 1u : 0u) << k;
+                    if constexpr (kMasks) cmask |= fbits << (j0 - s);
+                    if constexpr (kLists) lw.drain(lists, n, i);
                 }
-                if constexpr (kMasks) cmask |= (fbits & lanes) << (j0 - s);
-                if constexpr (kLists) lw.drain(lists, n, i);
-            }
+            };
+            if (ord == 13)
+                walk(std::true_type{});
+            else
+                walk(std::false_type{});
             if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
         }
         if constexpr (kMasks) {
