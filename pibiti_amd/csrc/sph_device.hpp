@@ -143,7 +143,8 @@ struct CellWalk {
         cap = p.max_par_in_cell;
     }
     // Slot range [s, e) of neighbour cell (dx, dy, dz), capped at max_par_in_cell;
-    // false if the cell is outside the grid, culled or empty.
+    // false if the cell is outside the grid, culled or empty (s and e are then
+    // meaningless).
     __device__ __forceinline__ bool open(int dx, int dy, int dz,
                                          const uint32_t* __restrict__ cell_start,
                                          const uint32_t* __restrict__ cell_end, uint32_t& s,
@@ -152,10 +153,10 @@ struct CellWalk {
         if (az.d2(dz) + ay.d2(dy) + ax.d2(dx) > cull2) return false;
         const uint32_t c = base + (uint32_t)(az.g + dz) * gyx + (uint32_t)(ay.g + dy) * gx +
                            (uint32_t)(ax.g + dx);
+        // both loads go out together (an empty cell's end is stale and unused)
         s = cell_start[c];
-        if (s == kEmptyCell) return false;
         e = min(cell_end[c], s + cap);
-        return true;
+        return s != kEmptyCell;
     }
     // Cell key of the particle's own cell.
     __device__ __forceinline__ uint32_t own_key() const {
