@@ -366,7 +366,7 @@ struct Engine {
         const bool keep = (flags & SPH_FLAG_KEEP_DVEL) != 0;
         if (timed) SPH_TRY(hipEventRecord(ev[0], stream));
         if (device_clock) SPH_TRY(launch_stats(c, buffers(), true, stream));
-        SPH_TRY(launch_fill_u32(counters, 0u, kCntNum, stream));
+        // (k_integrate_moves clears the step's counters)
         // the mover counts alternate with the cell tables
         SPH_TRY(launch_integrate_moves(c, buffers(), resort, cur_cell, key_tmp, stream));
         if (timed) SPH_TRY(hipEventRecord(ev[1], stream));

@@ -609,13 +609,16 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     uint32_t* __restrict__ tile_delta, uint32_t* __restrict__ key_pid, uint32_t* __restrict__ val_pid,
     uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    // the step's counters start here: every kernel that raises them runs later
+    if (tid < kCntNum) counters[tid] = 0u;
     if (tid >= n) return;
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, tid, scene);
     // the key only: k_resort_place integrates the state again as it moves it
+    // (and counts an out-of-grid cell there)
     float4 x4 = pos[tid];
     float4 v4 = vel[tid];
-    const uint32_t h = integrate_state(p, dt_dev[scene], x4, v4, counters);
+    const uint32_t h = integrate_state(p, dt_dev[scene], x4, v4, nullptr);
     const bool ok = *valid != 0;
     const uint32_t h_old = ok ? skey[tid] : kEmptyCell;
     const bool mv = !ok || h_old != h;
@@ -724,7 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
     const sph_params& p = scene_params<kBatch>(one, tab, sid_in, src, scene);
     float4 x4 = pos_in[src];
     float4 v4 = vel_in[src];
-    (void)integrate_state(p, dt_dev[scene], x4, v4, nullptr);
+    (void)integrate_state(p, dt_dev[scene], x4, v4, counters);
     pos_out[dst] = x4;
     vel_out[dst] = v4;
     pid_out[dst] = id;
