@@ -876,10 +876,10 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
     const float r2f = p.force_r2_max;
-    // The pair-mask path reads the step's position array, which is padded
-    // past its end: a group's loads past the cell's end stay in bounds and
-    // are masked out, so they need no clamp.
-    constexpr bool kUnclamped = kMasks && std::is_same<Src, PosGlobal>::value;
+    // Position arrays in global memory are padded past their end (dalloc):
+    // a group's loads past the cell's end stay in bounds and are masked out,
+    // so they need no clamp.  LDS-staged positions keep it.
+    constexpr bool kUnclamped = std::is_same<Src, PosGlobal>::value;
     uint32_t mword = 0;
     for (int ord = 0; ord < 27; ++ord) {
         uint32_t s, e;
