@@ -188,3 +188,28 @@ def test_sparse_grid_without_incoming_lists():
     print("case: 2000 particles in 50.6M cells, 3 steps", flush=True)
     kinds = run_and_compare(p, pos, vel, 3, "sparse grid")
     assert kinds == [1, 1, 1], kinds
+
+
+def test_batch_with_a_dense_scene():
+    """A batch of a dense scene (125 particles per cell) and two ordinary
+    ones: the dense scene's first step sends the whole batch through the
+    full sort, the later steps order incrementally; every scene equals its
+    own run and the oracle."""
+    pd, xd, vd, _ = dense(lattice=(12, 12, 12), seed=28)
+    pa, xa, va, _ = scenes.scaled_dam_break((10, 8, 10), seed=29, vel_sigma=1.0)
+    pb, xb, vb, _ = scenes.scaled_dam_break((12, 6, 9), seed=30)
+    sc = [(pd, xd, vd), (pa, xa, va), (pb, xb, vb)]
+    steps = 5
+    print("case: batch of a dense and two ordinary scenes,", steps, "steps", flush=True)
+    kinds = []
+    with sph.SphBatch([p for p, _, _ in sc]) as b:
+        for s, (p, x, v) in enumerate(sc):
+            b.reset(s, x, v)
+        for _ in range(steps):
+            b.update(1)
+            kinds.append(b.stats(0)["full_sort"])
+        got = [dict(pos=b.get_array(s, sph.SPH_ARRAY_POSITION), vel=b.get_array(s, sph.SPH_ARRAY_VELOCITY),
+                    dens=b.get_array(s, sph.SPH_ARRAY_DENSITY)) for s in range(len(sc))]
+    assert kinds == [1, 0, 0, 0, 0], kinds
+    for s, (p, x, v) in enumerate(sc):
+        assert_equal_oracle(got[s], oracle.run_f32(p, x, v, steps), f"scene {s}")
