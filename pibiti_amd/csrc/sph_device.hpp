@@ -158,6 +158,31 @@ struct CellWalk {
         e = min(cell_end[c], s + cap);
         return s != kEmptyCell;
     }
+    // The row of neighbour cells (dy, dz): whether it lies in the grid, its
+    // squared distance so far and the key of its dx = 0 cell, computed once
+    // for the row's three cells (the same values open() computes per cell).
+    struct Row {
+        bool ok;
+        float d2;
+        uint32_t key;
+    };
+    __device__ __forceinline__ Row row(int dy, int dz) const {
+        Row r;
+        r.ok = az.ok(dz) && ay.ok(dy);
+        r.d2 = az.d2(dz) + ay.d2(dy);
+        r.key = base + (uint32_t)(az.g + dz) * gyx + (uint32_t)(ay.g + dy) * gx + (uint32_t)ax.g;
+        return r;
+    }
+    // open() for cell dx of a row.
+    __device__ __forceinline__ bool open_in(const Row& r, int dx, const uint32_t* __restrict__ cell_start,
+                                            const uint32_t* __restrict__ cell_end, uint32_t& s, uint32_t& e) const {
+        if (!r.ok || !ax.ok(dx)) return false;
+        if (r.d2 + ax.d2(dx) > cull2) return false;
+        const uint32_t c = r.key + (uint32_t)dx;
+        s = cell_start[c];
+        e = min(cell_end[c], s + cap);
+        return s != kEmptyCell;
+    }
     // Cell key of the particle's own cell.
     __device__ __forceinline__ uint32_t own_key() const {
         return base + (uint32_t)az.g * gyx + (uint32_t)ay.g * gx + (uint32_t)ax.g;

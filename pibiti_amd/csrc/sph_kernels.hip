@@ -881,65 +881,75 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     // so they need no clamp.  LDS-staged positions keep it.
     constexpr bool kUnclamped = std::is_same<Src, PosGlobal>::value;
     uint32_t mword = 0;
-    for (int ord = 0; ord < 27; ++ord) {
-        uint32_t s, e;
-        const int dx = ord % 3 - 1, dy = (ord / 3) % 3 - 1, dz = ord / 9 - 1;
-        bool open;
-        if constexpr (kCells)
-            open = w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
-        else
-            open = w.open(dx, dy, dz, cell_start, cell_end, s, e);
-        uint32_t cmask = 0;
-        if (open) {
-            float cell = 0.0f;
-            const int32_t soff = src.offset(ord / 3);
-            // candidates four at a time: the loads go out together (indices
-            // past the cell's end are masked out of the sums); the particle
-            // itself can only be a candidate of its own cell (ord 13), which
-            // takes the walk with the extra test
-            auto walk = [&](auto self_tag) {
-                constexpr bool kSelf = decltype(self_tag)::value;
-                for (uint32_t j0 = s; j0 < e; j0 += 4) {
-                    float4 xq[4];
+    // the 27 cells in (dz, dy, dx) order; a row's grid test, distance and
+    // key are computed once for its three cells
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+        for (int dy = -1; dy <= 1; ++dy) {
+            const CellWalk::Row row = w.row(dy, dz);
+#pragma unroll 1
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int ord = 9 * (dz + 1) + 3 * (dy + 1) + (dx + 1);
+                uint32_t s, e;
+                bool open;
+                if constexpr (kCells)
+                    open = w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
+                else
+                    open = w.open_in(row, dx, cell_start, cell_end, s, e);
+                uint32_t cmask = 0;
+                if (open) {
+                    float cell = 0.0f;
+                    const int32_t soff = src.offset(ord / 3);
+                    // candidates four at a time: the loads go out together (indices
+                    // past the cell's end are masked out of the sums); the particle
+                    // itself can only be a candidate of its own cell (ord 13), which
+                    // takes the walk with the extra test
+                    auto walk = [&](auto self_tag) {
+                        constexpr bool kSelf = decltype(self_tag)::value;
+                        for (uint32_t j0 = s; j0 < e; j0 += 4) {
+                            float4 xq[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) xq[k] = src.at(kUnclamped ? j0 + k : min(j0 + k, e - 1), soff);
-                    const uint32_t left = e - j0;
-                    const uint32_t self = kSelf ? i - j0 : 0u;
-                    uint32_t fbits = 0;
+                            for (int k = 0; k < 4; ++k) xq[k] = src.at(kUnclamped ? j0 + k : min(j0 + k, e - 1), soff);
+                            const uint32_t left = e - j0;
+                            const uint32_t self = kSelf ? i - j0 : 0u;
+                            uint32_t fbits = 0;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const uint32_t j = j0 + k;
-                        const bool valid = left > (uint32_t)k && (!kSelf || self != (uint32_t)k);
-                        const float4 xj = xq[k];
-                        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                        const float r2 = rx * rx + ry * ry + rz * rz;
-                        const bool hit = (r2 < h2) && valid;
-                        const float c3 = h2 - r2;
-                        const float wgt = (c3 * c3) * c3;
-                        if (kPrec == SPH_PREC_KAHAN) {
-                            if (hit) ks.add(wgt);
-                        } else {
-                            // the partial sum is >= +0, so adding +0 for a miss is exact
-                            cell = cell + (hit ? wgt : 0.0f);
+                            for (int k = 0; k < 4; ++k) {
+                                const uint32_t j = j0 + k;
+                                const bool valid = left > (uint32_t)k && (!kSelf || self != (uint32_t)k);
+                                const float4 xj = xq[k];
+                                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                                const float r2 = rx * rx + ry * ry + rz * rz;
+                                const bool hit = (r2 < h2) && valid;
+                                const float c3 = h2 - r2;
+                                const float wgt = (c3 * c3) * c3;
+                                if (kPrec == SPH_PREC_KAHAN) {
+                                    if (hit) ks.add(wgt);
+                                } else {
+                                    // the partial sum is >= +0, so adding +0 for a miss is exact
+                                    cell = cell + (hit ? wgt : 0.0f);
+                                }
+                                if constexpr (kLists) lw.put(j, hit);
+                                if constexpr (kMasks) fbits |= ((r2 <= r2f) && valid ? 1u : 0u) << k;
+                            }
+                            if constexpr (kMasks) cmask |= fbits << (j0 - s);
+                            if constexpr (kLists) lw.drain(lists, n, i);
                         }
-                        if constexpr (kLists) lw.put(j, hit);
-                        if constexpr (kMasks) fbits |= ((r2 <= r2f) && valid ? 1u : 0u) << k;
-                    }
-                    if constexpr (kMasks) cmask |= fbits << (j0 - s);
-                    if constexpr (kLists) lw.drain(lists, n, i);
+                    };
+                    if (ord == 13)
+                        walk(std::true_type{});
+                    else
+                        walk(std::false_type{});
+                    if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
                 }
-            };
-            if (ord == 13)
-                walk(std::true_type{});
-            else
-                walk(std::false_type{});
-            if (kPrec != SPH_PREC_KAHAN) sum = sum + cell;
-        }
-        if constexpr (kMasks) {
-            if (ord & 1)
-                nbr[(uint32_t)(ord >> 1) * n + i] = mword | (cmask << 16);
-            else
-                mword = cmask;
+                if constexpr (kMasks) {
+                    if (ord & 1)
+                        nbr[(uint32_t)(ord >> 1) * n + i] = mword | (cmask << 16);
+                    else
+                        mword = cmask;
+                }
+            }
         }
     }
     if constexpr (kMasks) nbr[(kMaskWords - 1u) * n + i] = mword;
@@ -1813,7 +1823,8 @@ hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, cons
 
 hipError_t resort_apply(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity,
                         const RadixSortPlan& sort, hipStream_t s) {
-    return c.tab ? resort_apply_impl<true>(c, b, rp, parity, sort, s) : resort_apply_impl<false>(c, b, rp, parity, sort, s);
+    return c.tab ? resort_apply_impl<true>(c, b, rp, parity, sort, s)
+                 : resort_apply_impl<false>(c, b, rp, parity, sort, s);
 }
 
 hipError_t launch_density_lists(const LaunchCfg& c, const StepBuffers& b, uint32_t* nbr, uint32_t* nnb,
