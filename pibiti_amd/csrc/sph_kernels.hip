@@ -1128,54 +1128,64 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
     uint32_t mword = 0;
-    for (int ord = 0; ord < 27; ++ord) {
-        uint32_t s = 0, e = 0;
-        const int dx = ord % 3 - 1, dy = (ord / 3) % 3 - 1, dz = ord / 9 - 1;
-        bool open;
-        if constexpr (kCells)
-            open = active && w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
-        else
-            open = active && w.open(dx, dy, dz, cell_start, cell_end, s, e);
-        if constexpr (kMasks) {
-            if ((ord & 1) == 0) mword = masks[(uint32_t)(ord >> 1) * n + ii];
-            uint32_t m = open ? ((ord & 1) ? (mword >> 16) : (mword & 0xFFFFu)) : 0u;
-            if (__any(cnt + __popc(m) > kList)) {
-                force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
-                cnt = 0;
-            }
-            const uint32_t tag = (uint32_t)ord << kSlotBits;
-            while (m != 0u) {
-                list[cnt * kB + t] = (s + (uint32_t)__builtin_ctz(m)) | tag;
-                ++cnt;
-                m &= m - 1u;
-            }
-            continue;
-        }
-        // every candidate of this cell could pass: make room first
-        if (__any(cnt + (int)(e - s) > kList)) {
-            force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
-            cnt = 0;
-        }
-        if (!open) continue;
-        const uint32_t tag = (uint32_t)ord << kSlotBits;
-        const int32_t soff = src.offset(ord / 3);
-        // candidates four at a time (see density_one)
-        for (uint32_t j0 = s; j0 < e; j0 += 4) {
-            float4 xq[4];
+    // the 27 cells in (dz, dy, dx) order; a row's grid test, distance and
+    // key are computed once for its three cells
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+        for (int dy = -1; dy <= 1; ++dy) {
+            const CellWalk::Row row = w.row(dy, dz);
+#pragma unroll 1
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int ord = 9 * (dz + 1) + 3 * (dy + 1) + (dx + 1);
+                uint32_t s = 0, e = 0;
+                bool open;
+                if constexpr (kCells)
+                    open = active && w.open_staged(dx, dy, dz, cv->t, cv->cells, xbase, s, e);
+                else
+                    open = active && w.open_in(row, dx, cell_start, cell_end, s, e);
+                if constexpr (kMasks) {
+                    if ((ord & 1) == 0) mword = masks[(uint32_t)(ord >> 1) * n + ii];
+                    uint32_t m = open ? ((ord & 1) ? (mword >> 16) : (mword & 0xFFFFu)) : 0u;
+                    if (__any(cnt + __popc(m) > kList)) {
+                        force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
+                        cnt = 0;
+                    }
+                    const uint32_t tag = (uint32_t)ord << kSlotBits;
+                    while (m != 0u) {
+                        list[cnt * kB + t] = (s + (uint32_t)__builtin_ctz(m)) | tag;
+                        ++cnt;
+                        m &= m - 1u;
+                    }
+                    continue;
+                }
+                // every candidate of this cell could pass: make room first
+                if (__any(cnt + (int)(e - s) > kList)) {
+                    force_drain<kPrec, kB>(p, list, t, cnt, xi, vi, ri, src, vel, rhop, a);
+                    cnt = 0;
+                }
+                if (!open) continue;
+                const uint32_t tag = (uint32_t)ord << kSlotBits;
+                const int32_t soff = src.offset(ord / 3);
+                // candidates four at a time (see density_one)
+                for (uint32_t j0 = s; j0 < e; j0 += 4) {
+                    float4 xq[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) xq[k] = src.at(min(j0 + k, e - 1), soff);
+                    for (int k = 0; k < 4; ++k) xq[k] = src.at(min(j0 + k, e - 1), soff);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t j = j0 + k;
-                const float4 xj = xq[k];
-                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max.  The
-                // store is unconditional, also for the repeated slots past the
-                // cell's end (cnt <= list size holds for the whole cell, and the
-                // list has one spare row for a full list); only a hit advances cnt.
-                list[cnt * kB + t] = j | tag;
-                cnt += (r2 <= r2max && j != i && j < e) ? 1 : 0;
+                    for (int k = 0; k < 4; ++k) {
+                        const uint32_t j = j0 + k;
+                        const float4 xj = xq[k];
+                        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        // max(minDist, sqrtf(r2)) < h  <=>  r2 <= force_r2_max.  The
+                        // store is unconditional, also for the repeated slots past the
+                        // cell's end (cnt <= list size holds for the whole cell, and the
+                        // list has one spare row for a full list); only a hit advances cnt.
+                        list[cnt * kB + t] = j | tag;
+                        cnt += (r2 <= r2max && j != i && j < e) ? 1 : 0;
+                    }
+                }
             }
         }
     }
