@@ -1032,6 +1032,9 @@ __global__ __launch_bounds__(kBlock) void k_density_persistent(
 // could overflow some lane's list.  The list size (entries per lane) trades
 // LDS per wave (occupancy) against drains at partial lane occupancy.
 constexpr int kForceListDefault = 24;
+// The pair-mask path appends no candidate tests, only passing pairs: a
+// shorter list (more blocks per CU) is a little faster there.
+constexpr int kForceListMasks = 16;
 constexpr uint32_t kSlotBits = 27;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
 
@@ -1760,7 +1763,7 @@ template <bool kBatch, int kPrec>
 void force_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float4* dvel,
                 hipStream_t s) {
     if (use_masks(c, b)) {
-        hipLaunchKernelGGL((k_force_ordered<kBatch, kPrec, kForceListDefault, 1, true>), dim3(blocks_for(c.n, kBlock)),
+        hipLaunchKernelGGL((k_force_ordered<kBatch, kPrec, kForceListMasks, 1, true>), dim3(blocks_for(c.n, kBlock)),
                            dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
                            b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.masks);
         return;
