@@ -15,10 +15,11 @@ reported (max and RMS):
 FP64 appears on the GPU only in the global reductions (mass, kinetic and
 potential energy, max speed) that feed the CFL time step; those are checked
 against a host float64 reduction of the read-back state.
-The report is printed and, when a gpurun_out/ directory exists, written to
-gpurun_out/precision_4m.json.
+The report is printed and, when the environment names a directory in
+PIBITI_REPORT_DIR, also written there as precision_4m.json.
 """
 import json
+import os
 from pathlib import Path
 
 import numpy as np
@@ -28,8 +29,6 @@ import oracle
 from pibiti_amd import scenes, sph
 
 pytestmark = pytest.mark.gpu
-
-ROOT = Path(__file__).resolve().parent.parent
 
 
 def field_errors(g, d, o32, p):
@@ -102,6 +101,6 @@ def test_precision_study_4m_colliding_blocks():
     # compensation can only help the sums it compensates
     assert report["kahan"]["rho_relative_bulk"]["rms"] <= report["reference"]["rho_relative_bulk"]["rms"] * 1.05
     print("precision report:", json.dumps(report), flush=True)
-    out_dir = ROOT / "gpurun_out"
-    if out_dir.is_dir():
-        (out_dir / "precision_4m.json").write_text(json.dumps(report, indent=1))
+    out_dir = os.environ.get("PIBITI_REPORT_DIR")
+    if out_dir and Path(out_dir).is_dir():
+        (Path(out_dir) / "precision_4m.json").write_text(json.dumps(report, indent=1))
