@@ -37,7 +37,11 @@ typedef enum sph_status {
     SPH_ERR_DEVICE = 5,           /* a HIP runtime call failed                 */
     SPH_ERR_NOT_INITIALISED = 6,  /* step/readback before the state was set    */
     SPH_ERR_UNSUPPORTED = 7,      /* feature not built (e.g. other bnd types)  */
-    SPH_ERR_CAPACITY = 8          /* size above a fixed capacity               */
+    SPH_ERR_CAPACITY = 8,         /* size above a fixed capacity               */
+    SPH_ERR_ORDER = 9             /* a step's particle order could not be built
+                                     (results invalid): reported by the next
+                                     synchronising call; the state must be
+                                     uploaded again before the next step     */
 } sph_status;
 
 /* Boundary types: the reference's BndType values (source/CUDA/Params.cuh:23).
@@ -317,7 +321,10 @@ typedef struct sph_stats {
     uint32_t cap_cells;         /* cells holding more than max_par_in_cell   */
     uint32_t out_of_grid;       /* particles whose cell had to be clamped    */
     uint32_t max_cell_count;    /* largest particle count of one cell        */
-    uint32_t sort_errors;       /* nonzero: a sort look-back wait hit its bound (results invalid) */
+    uint32_t sort_errors;       /* steps since the last sph_get_stats whose incremental order
+                                   failed a consistency check and that were re-sorted in full
+                                   instead (their results are valid); a failure that could not
+                                   be repaired is SPH_ERR_ORDER instead */
     uint32_t full_sort;         /* 1: the last step ordered the slots with the full radix sort,
                                    0: incrementally from the previous order (same order)      */
     uint32_t reserved[5];
@@ -546,6 +553,41 @@ int sph_batch_autotune(sph_batch* batch, const char* cache_path, uint32_t tune_o
                        float* best_ms);
 int sph_batch_info(sph_batch* batch, uint32_t* nscenes, uint32_t* total_particles,
                    uint32_t* total_cells, uint32_t* key_bits);
+
+/* ------------------------------------------------------------------------ */
+/* The reference's RadixSort(KeyValuePair* pData0, KeyValuePair* pData1,
+ * uint elements, uint bits) (source/CUDA/radixsort.cuh:30-32, driver
+ * source/CUDA/radixsort_kernel.cu:445-472): a stable LSD sort of n pairs
+ * by the low `bits` bits of the key, 8 bits per pass.  pairs holds n
+ * (uint32 key, uint32 value) records interleaved, the KeyValuePair layout,
+ * and receives the sorted pairs; temp (n records) is the ping-pong buffer.
+ * Both are device pointers on the current HIP device; stream is a
+ * hipStream_t (NULL: the null stream).  Unlike the reference, the result is
+ * in pairs for every key width (the reference leaves it there for its 16-
+ * and 32-bit calls), the last pass of a width that is not a multiple of 8
+ * takes only the remaining bits (the reference's passes take whole bytes,
+ * so it orders by the low 8 ceil(bits / 8) bits; the same for keys below
+ * 2^bits and for its 16- and 32-bit calls), and the call waits for the
+ * sort: it returns
+ * SPH_ERR_ORDER if a device-side wait of the sort hit its bound (pairs then
+ * hold no valid order).  Scratch is allocated and freed by the call.      */
+#define SPH_SORT_MAX (1u << 27)
+int sph_radix_sort(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, void* stream);
+
+
+/* ------------------------------------------------------------------------ */
+/* Testing hook (no counterpart in the reference): perturbs one device word
+ * of the step's order state between steps, so that tests can drive the
+ * consistency checks from the host while every kernel runs normally.
+ *   SPH_DEBUG_MOVER_COUNT: adds value to cell `index`'s mover count of the
+ *     next step (the step then counts that cell wrong);
+ *   SPH_DEBUG_CELL_END: adds value to cell `index`'s end in the current
+ *     cell ranges (the next step's old ranges);
+ *   SPH_DEBUG_ORDER_ERROR: marks the current order as failed, as a step
+ *     whose order could not be built does (index and value unused).
+ * Synchronous.                                                             */
+enum { SPH_DEBUG_MOVER_COUNT = 1, SPH_DEBUG_CELL_END = 2, SPH_DEBUG_ORDER_ERROR = 3 };
+int sph_debug_poke(sph_ctx* ctx, int what, uint32_t index, uint32_t value);
 
 /* ------------------------------------------------------------------------ */
 /* Self-test of the BF16 matrix-core operand layouts the learned surrogate

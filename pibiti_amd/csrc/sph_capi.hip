@@ -95,9 +95,12 @@ int sph_get_device_view(sph_ctx* c, sph_device_view* out) {
 
 int sph_synchronize(sph_ctx* c) {
     if (!c) return SPH_ERR_INVALID_ARGUMENT;
-    DeviceGuard guard(c->eng.device);
-    SPH_TRY(hipStreamSynchronize(c->eng.stream));
-    return SPH_OK;
+    return c->eng.synchronize();
+}
+
+int sph_debug_poke(sph_ctx* c, int what, uint32_t index, uint32_t value) {
+    if (!c) return SPH_ERR_INVALID_ARGUMENT;
+    return c->eng.debug_poke(what, index, value);
 }
 
 int sph_set_timing(sph_ctx* c, int enable) {
@@ -201,9 +204,7 @@ int sph_batch_get_stats(sph_batch* b, uint32_t scene, sph_stats* out) {
 
 int sph_batch_synchronize(sph_batch* b) {
     if (!b) return SPH_ERR_INVALID_ARGUMENT;
-    DeviceGuard guard(b->eng.device);
-    SPH_TRY(hipStreamSynchronize(b->eng.stream));
-    return SPH_OK;
+    return b->eng.synchronize();
 }
 
 int sph_batch_set_tune(sph_batch* b, const uint32_t tune[4]) {

@@ -137,6 +137,7 @@ struct Engine {
     const uint32_t* last_skey = nullptr;
     std::vector<uint8_t> have_pos, have_vel;  // per scene: full upload seen
     bool stepped = false;
+    bool order_failed = false;  // a step's order could not be built: the state must be uploaded again
     int last_kind = 0;          // what produced the current state: 0 upload, 1 plain step, 2 surrogate step
     uint64_t steps = 0;
     double host_clock = 0.0;
@@ -301,6 +302,8 @@ struct Engine {
         if (e == hipSuccess) e = radix_plan_init(sort, (uint32_t)N, sort_scratch, stream);
         if (e == hipSuccess) e = hipMalloc(&resort_scratch, resort_scratch_bytes((uint32_t)N, (uint32_t)NC));
         if (e == hipSuccess) e = resort_plan_init(resort, (uint32_t)N, (uint32_t)NC, resort_scratch, stream);
+        // one error word for the sort and the re-sort
+        sort.err = resort.err;
         for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_scene_off, scene_off.data(), (nscenes + 1) * sizeof(uint32_t),
@@ -471,6 +474,68 @@ struct Engine {
         return SPH_OK;
     }
 
+    // The error word of the step's order, read at every synchronising call.
+    // A failed order is reported once as SPH_ERR_ORDER; the state then
+    // counts as not uploaded (steps refuse to run, reads report the error)
+    // until both arrays of every scene are uploaded in full again.
+    int check_order() {
+        DeviceGuard guard(device);
+        uint32_t err = 0;
+        SPH_TRY(hipMemcpyAsync(&err, resort.err, sizeof(err), hipMemcpyDeviceToHost, stream));
+        SPH_TRY(hipStreamSynchronize(stream));
+        if (err & kErrInvalid) {
+            SPH_TRY(launch_fill_u32(resort.err, 0u, 1, stream));
+            SPH_TRY(resort_invalidate(resort, stream));
+            // the slot -> index map may no longer be a permutation: back to
+            // the identity, so that the uploads that must follow place every
+            // particle
+            SPH_TRY(launch_iota_u32(pid[cur_pid], cfg.n, stream));
+            if (batch) SPH_TRY(launch_scene_ids(cfg.n, cfg.nscenes, d_scene_off, pid[cur_pid], sid[cur_pid], stream));
+            SPH_TRY(hipStreamSynchronize(stream));
+            order_failed = true;
+            std::fill(have_pos.begin(), have_pos.end(), 0);
+            std::fill(have_vel.begin(), have_vel.end(), 0);
+        }
+        return order_failed ? SPH_ERR_ORDER : SPH_OK;
+    }
+
+    int synchronize() {
+        DeviceGuard guard(device);
+        SPH_TRY(hipStreamSynchronize(stream));
+        return check_order();
+    }
+
+    // Testing hook: perturb one word of the order state (sph_debug_poke).
+    int debug_poke(int what, uint32_t index, uint32_t value) {
+        DeviceGuard guard(device);
+        uint32_t* word = nullptr;
+        bool add = true;
+        switch (what) {
+            case SPH_DEBUG_MOVER_COUNT:
+                if (index >= cfg.num_cells) return SPH_ERR_INVALID_ARGUMENT;
+                word = resort.in_cnt[cur_cell] + index;  // the next step counts its movers here
+                break;
+            case SPH_DEBUG_CELL_END:
+                if (index >= cfg.num_cells) return SPH_ERR_INVALID_ARGUMENT;
+                word = cell_end[cur_cell] + index;
+                break;
+            case SPH_DEBUG_ORDER_ERROR:
+                word = resort.err;
+                value = kErrInvalid;
+                add = false;
+                break;
+            default:
+                return SPH_ERR_INVALID_ARGUMENT;
+        }
+        uint32_t v = 0;
+        SPH_TRY(hipMemcpyAsync(&v, word, sizeof(v), hipMemcpyDeviceToHost, stream));
+        SPH_TRY(hipStreamSynchronize(stream));
+        v = add ? v + value : (v | value);
+        SPH_TRY(hipMemcpyAsync(word, &v, sizeof(v), hipMemcpyHostToDevice, stream));
+        SPH_TRY(hipStreamSynchronize(stream));
+        return SPH_OK;
+    }
+
     int set_array(uint32_t scene, int which, const float* data, uint32_t first, uint32_t count) {
         if (!data || scene >= cfg.nscenes) return SPH_ERR_INVALID_ARGUMENT;
         if (which != SPH_ARRAY_POSITION && which != SPH_ARRAY_VELOCITY) return SPH_ERR_INVALID_ARGUMENT;
@@ -485,12 +550,14 @@ struct Engine {
                                             count, dst, stream));
         SPH_TRY(hipStreamSynchronize(stream));
         if (first == 0 && count == ns) (which == SPH_ARRAY_POSITION ? have_pos : have_vel)[scene] = 1;
+        if (order_failed && ready()) order_failed = false;
         last_kind = 0;
         return SPH_OK;
     }
 
     int get_array(uint32_t scene, int which, float* out) {
         if (!out || scene >= cfg.nscenes) return SPH_ERR_INVALID_ARGUMENT;
+        SPH_TRY_ST(check_order());
         DeviceGuard guard(device);
         const uint32_t n = cfg.n;
         const uint32_t* p = pid[cur_pid];
@@ -541,6 +608,7 @@ struct Engine {
 
     int get_stats(uint32_t scene, sph_stats* out) {
         if (!out || scene >= cfg.nscenes) return SPH_ERR_INVALID_ARGUMENT;
+        SPH_TRY_ST(check_order());
         DeviceGuard guard(device);
         std::memset(out, 0, sizeof(*out));
         const uint32_t ns = cfg.nscenes;
@@ -556,13 +624,13 @@ struct Engine {
         SPH_TRY(launch_stats(c, buffers(), false, stream));
         SPH_TRY(launch_fill_u32(cell_max, 0u, 1, stream));
         if (stepped) SPH_TRY(launch_cell_stats(c, buffers(), cell_max, stream));
-        uint32_t cnt[kCntNum], cmax = 0, sort_err = 0, resort_err = 0;
+        uint32_t cnt[kCntNum], cmax = 0, repairs = 0;
         SPH_TRY(hipMemcpyAsync(st.data(), stats, st.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipMemcpyAsync(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipMemcpyAsync(&cmax, cell_max, sizeof(cmax), hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipMemcpyAsync(&sort_err, sort.err, sizeof(sort_err), hipMemcpyDeviceToHost, stream));
-        SPH_TRY(hipMemcpyAsync(&resort_err, resort.err, sizeof(resort_err), hipMemcpyDeviceToHost, stream));
+        SPH_TRY(hipMemcpyAsync(&repairs, resort.repairs, sizeof(repairs), hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipStreamSynchronize(stream));
+        SPH_TRY(launch_fill_u32(resort.repairs, 0u, 1, stream));
         // the next step's time step (as the next step will compute it) for reporting
         const double dt_next = st[(size_t)scene * kNumStats + kSDt];
         for (uint32_t s = 0; s < ns; ++s)
@@ -581,13 +649,14 @@ struct Engine {
         out->cap_cells = cnt[kCntCapCells];
         out->out_of_grid = cnt[kCntOutOfGrid];
         out->max_cell_count = cmax;
-        out->sort_errors = sort_err | resort_err;
+        out->sort_errors = repairs;
         out->full_sort = cnt[kCntFullSort] != 0;
         return SPH_OK;
     }
 
     int get_grid(uint32_t* sorted_hash, uint32_t* perm, uint32_t* cs, uint32_t* ce) {
         if (!stepped) return SPH_ERR_NOT_INITIALISED;
+        SPH_TRY_ST(check_order());
         DeviceGuard guard(device);
         const size_t n = cfg.n, ncell = cfg.num_cells;
         if (sorted_hash) SPH_TRY(hipMemcpyAsync(sorted_hash, last_skey, n * 4, hipMemcpyDeviceToHost, stream));

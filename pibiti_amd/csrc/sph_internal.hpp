@@ -100,8 +100,22 @@ struct ResortPlan {
     uint32_t* tile_total[2] = {nullptr, nullptr};  // particles per tile of kScanTile cells in the last order
     uint32_t* tile_delta[2] = {nullptr, nullptr};  // movers in less movers out per tile this step
     uint32_t* valid_step = nullptr;  // this step's copy of valid (the last kernel of a step sets valid)
-    uint32_t* err = nullptr;     // set when the counts disagree (results invalid)
+    // Error word, shared with the radix sort's plan: kErrInvalid when a
+    // step's order could not be built (results invalid; sticky until the
+    // host reads and clears it).
+    uint32_t* err = nullptr;
+    // Repair flags of the current step: kRepairInputs from the integrate
+    // kernel (a slot outside its cell's old range), kRepairCounts from the
+    // cell scan (a tile's count disagrees two ways).  Either sends the step
+    // through the full sort with the ranges taken from the sorted keys; the
+    // place kernel clears the flags, leaves the order invalid for the next
+    // step and counts the step in `repairs`.
+    uint32_t* repair = nullptr;
+    uint32_t* repairs = nullptr;  // steps repaired so far (cleared when the host reads it)
 };
+constexpr uint32_t kErrInvalid = 1u;
+constexpr uint32_t kRepairInputs = 1u;
+constexpr uint32_t kRepairCounts = 2u;
 size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells);
 // Carves the plan out of `scratch`, clears it (not graph-capturable) and
 // marks the order invalid.

@@ -607,7 +607,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     const uint32_t* __restrict__ skey, uint32_t* __restrict__ nkey, const uint32_t* __restrict__ valid,
     uint64_t* __restrict__ moved, uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ inlist, uint32_t in_cap,
     uint32_t* __restrict__ tile_delta, uint32_t* __restrict__ key_pid, uint32_t* __restrict__ val_pid,
-    uint32_t* __restrict__ counters, const float* __restrict__ dt_dev) {
+    uint32_t* __restrict__ counters, const float* __restrict__ dt_dev, uint32_t num_cells,
+    const uint32_t* __restrict__ old_start, const uint32_t* __restrict__ old_end, const uint32_t* __restrict__ err,
+    uint32_t* __restrict__ repair) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     // the step's counters start here: every kernel that raises them runs later
     if (tid < kCntNum) counters[tid] = 0u;
@@ -619,8 +621,22 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     float4 x4 = pos[tid];
     float4 v4 = vel[tid];
     const uint32_t h = integrate_state(p, dt_dev[scene], x4, v4, nullptr);
-    const bool ok = *valid != 0;
-    const uint32_t h_old = ok ? skey[tid] : kEmptyCell;
+    // an order left behind by a failed step is never built upon
+    const bool ok = *valid != 0 && (*err & kErrInvalid) == 0;
+    uint32_t h_old = ok ? skey[tid] : kEmptyCell;
+    if (ok) {
+        // the slot must lie in its old cell's range; otherwise the step's
+        // counts cannot be trusted and it takes the full sort
+        bool in_range = h_old < num_cells;
+        if (in_range) {
+            const uint32_t os = old_start[h_old], oe = old_end[h_old];
+            in_range = os != kEmptyCell && tid >= os && tid < oe;
+        }
+        if (!in_range) {
+            atomicOr(repair, kRepairInputs);
+            h_old = kEmptyCell;
+        }
+    }
     const bool mv = !ok || h_old != h;
     nkey[tid] = h;
     // the radix sort's input, for a step that turns out to need it
@@ -638,7 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
     const uint32_t tile_new = h / kScanTile, tile_old = h_old / kScanTile;
     const bool cross = mv && (!ok || tile_new != tile_old);
     wave_add(tile_delta, tile_new, 1u, cross);
-    wave_add(tile_delta, tile_old, ~0u, cross && ok);
+    wave_add(tile_delta, tile_old, ~0u, cross && ok && h_old != kEmptyCell);
 }
 
 // ----------------------------------------------- incremental re-sort: apply
@@ -658,23 +674,35 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
     sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ nkey,
     const uint64_t* __restrict__ moved, const uint32_t* __restrict__ valid_step, uint32_t* __restrict__ valid,
     const uint32_t* __restrict__ old_start, const uint32_t* __restrict__ old_end,
-    const uint32_t* __restrict__ new_start, const uint32_t* __restrict__ new_end,
+    uint32_t* __restrict__ new_start, uint32_t* __restrict__ new_end,
     const uint32_t* __restrict__ in_cnt, const uint32_t* __restrict__ inlist, uint32_t in_cap,
     const uint32_t* __restrict__ sorted_key,
     const uint32_t* __restrict__ sorted_slot, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
     const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out, uint32_t* __restrict__ sid_out,
     uint32_t* __restrict__ skey_out, uint32_t* __restrict__ counters, uint32_t* __restrict__ err,
-    const float* __restrict__ dt_dev) {
+    const float* __restrict__ dt_dev, uint32_t* __restrict__ repair, uint32_t* __restrict__ repairs) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *valid = 1u;
+    if (k == 0) {
+        // a repaired step's tile totals may be off: the next step starts
+        // its order from scratch
+        const uint32_t r = *repair;
+        *valid = r ? 0u : 1u;
+        *repair = 0u;
+        if (r) atomicAdd(repairs, 1u);
+    }
     if (k >= n) return;
+    const bool full = counters[kCntFullSort] != 0;
     uint32_t src, dst, c, id;
-    if (counters[kCntFullSort]) {
+    if (full) {
         src = sorted_slot[k];
         dst = k;
         c = sorted_key[k];
         id = pid_in[src];
+        // the cell ranges from the sorted keys (the same values the cell scan
+        // wrote, except on a repaired step)
+        if (k == 0 || sorted_key[k - 1] != c) new_start[c] = k;
+        if (k + 1 == n || sorted_key[k + 1] != c) new_end[c] = k + 1;
     } else {
         src = k;
         c = nkey[k];
@@ -697,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
         if (!mv && (os == kEmptyCell || k < os || k >= oe)) bad = true;
         const uint32_t nin = in_cnt[c];
         if (bad || nin > in_cap || end - base != stay + nin) {
-            atomicOr(err, 1u);
+            atomicOr(err, kErrInvalid);
             return;
         }
         uint32_t local = 0;
@@ -719,7 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
         for (uint32_t q = 0; q < nin; ++q) inr += inc[q] < id ? 1u : 0u;
         dst = base + local + inr;
         if (dst >= end) {
-            atomicOr(err, 1u);
+            atomicOr(err, kErrInvalid);
             return;
         }
     }
@@ -734,8 +762,14 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
     skey_out[dst] = c;
     if constexpr (kBatch) sid_out[dst] = scene;
     // the cell's last slot counts a cell above the candidate cap
-    const uint32_t end = new_end[c];
-    if (dst + 1 == end && end - new_start[c] > p.max_par_in_cell) atomicAdd(&counters[kCntCapCells], 1u);
+    if (full) {
+        const uint32_t cap = p.max_par_in_cell;
+        if ((k + 1 == n || sorted_key[k + 1] != c) && k >= cap && sorted_key[k - cap] == c)
+            atomicAdd(&counters[kCntCapCells], 1u);
+    } else {
+        const uint32_t end = new_end[c];
+        if (dst + 1 == end && end - new_start[c] > p.max_par_in_cell) atomicAdd(&counters[kCntCapCells], 1u);
+    }
 }
 
 // --------------------------------------------------- candidate positions
@@ -1693,7 +1727,7 @@ hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const Res
                        b.cell_end_alt, rp.in_cnt[parity], rp.inlist, rp.in_cap, skey, oslot, b.pos_cur, b.vel_cur,
                        b.pid_cur,
                        b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt, b.counters, rp.err,
-                       b.dt_dev);
+                       b.dt_dev, rp.repair, rp.repairs);
     return hipGetLastError();
 }
 
@@ -1826,11 +1860,13 @@ hipError_t launch_integrate_moves(const LaunchCfg& c, const StepBuffers& b, cons
     if (c.tab)
         hipLaunchKernelGGL((k_integrate_moves<true>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
                            b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
-                           rp.in_cap, rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
+                           rp.in_cap, rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev,
+                           c.num_cells, b.cell_start, b.cell_end, rp.err, rp.repair);
     else
         hipLaunchKernelGGL((k_integrate_moves<false>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
                            b.vel_cur, b.pid_cur, b.skey, nkey, rp.valid, rp.moved, rp.in_cnt[parity], rp.inlist,
-                           rp.in_cap, rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev);
+                           rp.in_cap, rp.tile_delta[parity], b.key_pid, b.val_pid, b.counters, b.dt_dev,
+                           c.num_cells, b.cell_start, b.cell_end, rp.err, rp.repair);
     return hipGetLastError();
 }
 

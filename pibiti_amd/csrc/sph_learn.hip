@@ -751,7 +751,9 @@ int sph_batch_learn_get_window(sph_batch* b, uint32_t scene, int which, float* o
 int sph_step_learned(sph_ctx* c, sph_model* m, uint32_t ncoarse, uint32_t factor, float tol, float skin,
                      sph_learned_report* out) {
     if (!c) return SPH_ERR_INVALID_ARGUMENT;
-    return step_learned(c->eng, m, ncoarse, factor, tol, skin, out);
+    const int st = step_learned(c->eng, m, ncoarse, factor, tol, skin, out);
+    if (st != SPH_OK) return st;
+    return c->eng.check_order();
 }
 
 }  // extern "C"

@@ -24,7 +24,8 @@
 //                      new cell ranges; clears the other step's counts;
 //                      flags the step for the full sort when a cell takes
 //                      more than in_cap movers or holds more than
-//                      kMaxCellCount particles;
+//                      kMaxCellCount particles, and when the step's inputs
+//                      fail a consistency check (below);
 //   k_resort_place     (sph_kernels.hip) slot -> new slot: cell start +
 //                      stayers of the cell with a smaller index (a stayer:
 //                      its rank among the old range's stayers; a mover: a
@@ -41,6 +42,19 @@
 // While the order is invalid (first step, after the learned surrogate or
 // the tuner rearranged the slots) every slot counts as a mover with no old
 // ranges, which is a counting sort by cell with the same result.
+//
+// Consistency checks.  The order is only built upon while its inputs agree:
+// the integrate kernel checks that every slot lies in its cell's old range,
+// and the cell scan counts each tile two ways (the last order's tile total
+// plus the step's deltas, against its own per-cell counts).  A failed check
+// sends the step through the full sort, whose place kernel writes every
+// occupied cell's range from the sorted keys (the failing tiles' cells are
+// written empty by the scan), so the step's results are those of a clean
+// sort; the order is then rebuilt from scratch on the next step.  An
+// inconsistency that only the place kernel sees (or a sort look-back that
+// hits its bound) cannot be repaired within the step: it sets kErrInvalid
+// in the error word, which the host reports at its next synchronising call
+// and which keeps later steps from building on the order.
 #include "sph_device.hpp"
 
 namespace sph {
@@ -74,13 +88,20 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
     const uint64_t* __restrict__ moved, uint32_t* __restrict__ new_start, uint32_t* __restrict__ new_end,
     uint32_t* __restrict__ counters, const uint32_t* __restrict__ tile_total, const uint32_t* __restrict__ tile_delta,
     uint32_t* __restrict__ total_next, uint32_t* __restrict__ delta_next, uint32_t* __restrict__ valid_step,
-    uint32_t in_cap, const uint32_t* __restrict__ tile_start, uint32_t* __restrict__ err) {
+    uint32_t in_cap, const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ err,
+    uint32_t* __restrict__ repair, uint32_t n) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_total, s_excl;
+    __shared__ uint32_t s_bad;
     const uint32_t t = threadIdx.x;
     const uint32_t tile = blockIdx.x;
-    const bool ok = *valid != 0;
+    // an order left behind by a failed step is never built upon
+    const bool ok = *valid != 0 && (*err & kErrInvalid) == 0;
+    // the integrate kernel found a slot outside its cell's old range: the
+    // counts below cannot be trusted, the step takes the full sort
+    const bool rep_in = (*repair & kRepairInputs) != 0;
     if (tile == 0 && t == 0) *valid_step = ok ? 1u : 0u;  // for k_resort_place, which sets valid
+    if (t == 0) s_bad = rep_in ? 1u : 0u;
 
     // four consecutive cells per thread (16-byte loads; the arrays are
     // padded to a multiple of four cells)
@@ -104,13 +125,18 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
             if (c0 + (uint32_t)j >= num_cells) break;
             uint32_t stay = 0, old = 0;
             if (ss[j] != kEmptyCell) {
-                old = es[j] - ss[j];
-                stay = old - movers_between(moved, ss[j], es[j]);
+                // an old range outside the slots is counted as empty (and
+                // the tile's two-way count below then disagrees)
+                if (es[j] > ss[j] && es[j] <= n) {
+                    old = es[j] - ss[j];
+                    stay = old - movers_between(moved, ss[j], es[j]);
+                }
             }
             cnt[j] = stay + ins[j];
             big = big || ins[j] > in_cap || cnt[j] > kMaxCellCount || old > kMaxCellCount;
         }
     }
+    big = big || rep_in;
     if (__any(big) && (t & 63u) == 0) atomicAdd(&counters[kCntFullSort], 1u);
     const uint32_t sum = cnt[0] + cnt[1] + cnt[2] + cnt[3];
     const uint32_t incl = block_incl_scan(sum, s_wave);
@@ -134,18 +160,25 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
         uint32_t e = 0;
         for (int q = 0; q < kBlock / 64; ++q) e += s_wave[q];
         s_excl = e;
-        // the tile's own count two ways
-        if ((ok ? tile_total[tile] : 0u) + tile_delta[tile] != s_total) atomicOr(err, 1u);
+        // the tile's own count two ways: on a disagreement the step takes the
+        // full sort, and the tile's cells are written empty here (the place
+        // kernel writes every occupied cell's range from the sorted keys)
+        if ((ok ? tile_total[tile] : 0u) + tile_delta[tile] != s_total && !rep_in) {
+            s_bad = 1u;
+            atomicOr(repair, kRepairCounts);
+            atomicAdd(&counters[kCntFullSort], 1u);
+        }
         total_next[tile] = s_total;
         delta_next[tile] = 0;
     }
     __syncthreads();
     if (c0 < num_cells) {
+        const bool bad = s_bad != 0;
         uint32_t s = s_excl + incl - sum;
         uint32_t st[kCellItems], en[kCellItems];
 #pragma unroll
         for (int j = 0; j < kCellItems; ++j) {
-            st[j] = cnt[j] ? s : kEmptyCell;
+            st[j] = cnt[j] && !bad ? s : kEmptyCell;
             s += cnt[j];
             en[j] = s;
         }
@@ -165,12 +198,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(
 // each tile's count in the last order plus its movers in less its movers
 // out, by one block of 1024 threads.
 __global__ __launch_bounds__(1024) void k_tile_scan(uint32_t num_tiles, const uint32_t* __restrict__ valid,
+                                                    const uint32_t* __restrict__ err,
                                                     const uint32_t* __restrict__ tile_total,
                                                     const uint32_t* __restrict__ tile_delta,
                                                     uint32_t* __restrict__ tile_start) {
     __shared__ uint32_t s_wave[1024 / 64];
     __shared__ uint32_t s_carry;
-    const bool ok = *valid != 0;
+    const bool ok = *valid != 0 && (*err & kErrInvalid) == 0;
     if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
     for (uint32_t base = 0; base < num_tiles; base += 1024) {
@@ -210,9 +244,9 @@ inline size_t tile_start_words(uint32_t num_cells) {
 size_t resort_scratch_bytes(uint32_t n, uint32_t num_cells) {
     // mover counts x 2 (padded to four cells) | incoming lists | tile totals
     // x 2 | tile deltas x 2 | tile starts (large grids) | valid, err,
-    // valid_step | mover bits
+    // valid_step, repair, repairs (32 bytes) | mover bits
     return 2 * pad4(num_cells) * 4 + (size_t)num_cells * in_capacity(n, num_cells) * 4 +
-           4 * (size_t)scan_tiles(num_cells) * 4 + tile_start_words(num_cells) * 4 + 16 + ((size_t)n / 64 + 2) * 8;
+           4 * (size_t)scan_tiles(num_cells) * 4 + tile_start_words(num_cells) * 4 + 32 + ((size_t)n / 64 + 2) * 8;
 }
 
 hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, void* scratch, hipStream_t s) {
@@ -238,7 +272,9 @@ hipError_t resort_plan_init(ResortPlan& plan, uint32_t n, uint32_t num_cells, vo
     plan.valid = reinterpret_cast<uint32_t*>(p);
     plan.err = reinterpret_cast<uint32_t*>(p + 4);
     plan.valid_step = reinterpret_cast<uint32_t*>(p + 8);
-    p += 16;
+    plan.repair = reinterpret_cast<uint32_t*>(p + 12);
+    plan.repairs = reinterpret_cast<uint32_t*>(p + 16);
+    p += 32;
     plan.moved = reinterpret_cast<uint64_t*>(p);
     p += ((size_t)n / 64 + 2) * 8;
     if (e == hipSuccess) e = hipMemsetAsync(q, 0, (size_t)(p - q), s);
@@ -252,13 +288,13 @@ hipError_t resort_invalidate(const ResortPlan& plan, hipStream_t s) {
 hipError_t resort_order(const LaunchCfg& c, const StepBuffers& b, const ResortPlan& rp, int parity, hipStream_t s) {
     if (rp.n != c.n || rp.num_cells != c.num_cells) return hipErrorInvalidValue;
     if (rp.tile_start)
-        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, rp.num_tiles, rp.valid, rp.tile_total[parity],
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, rp.num_tiles, rp.valid, rp.err, rp.tile_total[parity],
                            rp.tile_delta[parity], rp.tile_start);
     hipLaunchKernelGGL(k_cell_scan, dim3(rp.num_tiles), dim3(kBlock), 0, s, c.num_cells, rp.valid, b.cell_start,
                        b.cell_end, rp.in_cnt[parity], rp.in_cnt[parity ^ 1], rp.moved, b.cell_start_alt,
                        b.cell_end_alt, b.counters, rp.tile_total[parity], rp.tile_delta[parity],
                        rp.tile_total[parity ^ 1], rp.tile_delta[parity ^ 1], rp.valid_step, rp.in_cap, rp.tile_start,
-                       rp.err);
+                       rp.err, rp.repair, c.n);
     return hipGetLastError();
 }
 

@@ -139,6 +139,7 @@ const char* sph_strerror(int status) {
         case SPH_ERR_NOT_INITIALISED: return "particle state not initialised";
         case SPH_ERR_UNSUPPORTED: return "unsupported feature";
         case SPH_ERR_CAPACITY: return "size exceeds a fixed capacity";
+        case SPH_ERR_ORDER: return "a step's particle order could not be built (results invalid; upload the state again)";
         default: return "unknown status";
     }
 }

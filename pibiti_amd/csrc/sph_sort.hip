@@ -114,6 +114,8 @@ __device__ __forceinline__ void scan_pass(const uint32_t* __restrict__ block_his
 // wave a run of consecutive lanes with the same digit is counted by one
 // LDS atomic of its first lane: the keys arrive nearly sorted (the previous
 // step's order), so runs are long and the atomics rarely collide.
+// kStride: 1 for a key array, 2 for (key, value) pairs (KeyValuePair layout).
+template <int kStride>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                       uint32_t bits, uint32_t* __restrict__ block_hist,
                                                       uint32_t* __restrict__ offsets, uint32_t* __restrict__ done,
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
 #pragma unroll
         for (int it = 0; it < kHistKeys; ++it) {
             const uint64_t e = base + (uint64_t)it * kHistThreads + t;
-            k[it] = keys[e < n ? e : n - 1];
+            k[it] = keys[(e < n ? e : n - 1) * kStride];
         }
 #pragma unroll
         for (int it = 0; it < kHistKeys; ++it) {
@@ -139,7 +141,9 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
             const bool valid = base + (uint64_t)it * kHistThreads + t < n;
             const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
             for (uint32_t p = 0; p < passes; ++p) {
-                const uint32_t d = (k[it] >> (8 * p)) & 0xFFu;
+                // the pass's digit: its bits of the low `bits` (k_pass masks the same way)
+                const uint32_t pb = bits - 8 * p < 8 ? bits - 8 * p : 8;
+                const uint32_t d = (k[it] >> (8 * p)) & ((1u << pb) - 1u);
                 const uint32_t dprev = __shfl_up(d, 1);
                 const bool head = valid && (lane == 0 || dprev != d);
                 const uint64_t later = __ballot(head) & after_me;
@@ -169,6 +173,10 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
     if (t == 0) *done = 0u;  // ready for the next sort
 }
 
+// kPairsIn / kPairsOut: the input / output is one array of (key, value)
+// pairs (keys_in / keys_out; vals_in / vals_out unused) instead of a key
+// array and a value array.
+template <bool kPairsIn, bool kPairsOut>
 __global__ __launch_bounds__(kSortBlock) void k_pass(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
@@ -204,8 +212,17 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(
     for (int it = 0; it < kItems; ++it) {
         const uint64_t e = wbase + (uint64_t)it * kWave + lane;
         const bool valid = e < n;
-        const uint32_t key = valid ? keys_in[e] : 0u;
-        const uint32_t val = valid ? vals_in[e] : 0u;
+        uint32_t key = 0u, val = 0u;
+        if (valid) {
+            if constexpr (kPairsIn) {
+                const uint2 kv = reinterpret_cast<const uint2*>(keys_in)[e];
+                key = kv.x;
+                val = kv.y;
+            } else {
+                key = keys_in[e];
+                val = vals_in[e];
+            }
+        }
         const uint32_t d = (key >> shift) & mask;
         uint64_t peers = __ballot(valid);
         for (uint32_t bit = 0; bit < pass_bits; ++bit) {
@@ -280,8 +297,12 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(
         const uint32_t key = skeys[i];
         const uint32_t d = (key >> shift) & mask;
         const uint32_t g = gbase[d] + (i - dig_off[d]);
-        keys_out[g] = key;
-        vals_out[g] = svals[i];
+        if constexpr (kPairsOut) {
+            reinterpret_cast<uint2*>(keys_out)[g] = make_uint2(key, svals[i]);
+        } else {
+            keys_out[g] = key;
+            vals_out[g] = svals[i];
+        }
     }
 }
 
@@ -292,14 +313,16 @@ uint32_t radix_num_tiles(uint32_t n) {
     return tiles == 0 ? 1 : tiles;
 }
 
-size_t radix_scratch_bytes(uint32_t n) {
-    // block histograms | digit offsets | status words | ticket | error word |
-    // finished histogram blocks
-    return (size_t)kHistBlocks * kMaxPasses * kRadix * 4 + (size_t)kMaxPasses * kRadix * 4 +
-           (size_t)radix_num_tiles(n) * kRadix * 8 + 64;
-}
+namespace {
 
-hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipStream_t stream) {
+// Scratch layout: block histograms | digit offsets | ticket (8 bytes), error
+// word, finished histogram blocks (64 bytes in all) | status words (one per
+// tile and digit).
+constexpr size_t kHistBytes = (size_t)kHistBlocks * kMaxPasses * kRadix * 4 + (size_t)kMaxPasses * kRadix * 4;
+
+constexpr size_t scratch_bytes_for_tiles(uint64_t tiles) { return kHistBytes + 64 + (size_t)tiles * kRadix * 8; }
+
+void carve(RadixSortPlan& plan, uint32_t n, void* scratch) {
     plan.n = n;
     plan.num_tiles = radix_num_tiles(n);
     char* b = static_cast<char*>(scratch);
@@ -307,14 +330,22 @@ hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipSt
     b += (size_t)kHistBlocks * kMaxPasses * kRadix * 4;
     plan.digit_off = reinterpret_cast<uint32_t*>(b);
     b += (size_t)kMaxPasses * kRadix * 4;
-    plan.status = reinterpret_cast<uint64_t*>(b);
-    b += (size_t)plan.num_tiles * kRadix * 8;
     plan.ticket = reinterpret_cast<uint64_t*>(b);
     plan.err = reinterpret_cast<uint32_t*>(b + 8);
     plan.done = reinterpret_cast<uint32_t*>(b + 12);
-    // status words start at epoch 0 / no flag; the ticket, error word and
-    // histogram-block count at 0
-    return hipMemsetAsync(plan.status, 0, (size_t)plan.num_tiles * kRadix * 8 + 64, stream);
+    b += 64;
+    plan.status = reinterpret_cast<uint64_t*>(b);
+}
+
+}  // namespace
+
+size_t radix_scratch_bytes(uint32_t n) { return scratch_bytes_for_tiles(radix_num_tiles(n)); }
+
+hipError_t radix_plan_init(RadixSortPlan& plan, uint32_t n, void* scratch, hipStream_t stream) {
+    carve(plan, n, scratch);
+    // the ticket, error word and histogram-block count at 0; status words at
+    // epoch 0 / no flag (one contiguous range)
+    return hipMemsetAsync(plan.ticket, 0, 64 + (size_t)plan.num_tiles * kRadix * 8, stream);
 }
 
 hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* keys0,
@@ -330,13 +361,13 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
     if (passes > (uint32_t)kMaxPasses) return hipErrorInvalidValue;
     if (passes > 0) {
         const uint32_t hb = std::min<uint32_t>(kHistBlocks, std::max<uint32_t>(1u, (n + kHistTile - 1) / kHistTile));
-        hipLaunchKernelGGL(k_hist, dim3(hb), dim3(kHistThreads), 0, stream, kin, n, bits, plan.block_hist,
+        hipLaunchKernelGGL(k_hist<1>, dim3(hb), dim3(kHistThreads), 0, stream, kin, n, bits, plan.block_hist,
                            plan.digit_off, plan.done, gate);
     }
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t shift = 8 * p;
         const uint32_t pb = (bits - shift) < 8 ? (bits - shift) : 8;
-        hipLaunchKernelGGL(k_pass, dim3(plan.num_tiles), dim3(kSortBlock), 0, stream, kin, vin, kout,
+        hipLaunchKernelGGL((k_pass<false, false>), dim3(plan.num_tiles), dim3(kSortBlock), 0, stream, kin, vin, kout,
                            vout, n, shift, pb, plan.digit_off + p * kRadix, plan.status, plan.ticket,
                            plan.num_tiles, plan.err, gate);
         uint32_t* tk = kin;
@@ -351,4 +382,103 @@ hipError_t radix_sort_pairs(const RadixSortPlan& plan, uint32_t bits, uint32_t* 
     return hipGetLastError();
 }
 
+// Stable LSD sort of plan.n (key, value) pairs in place in `pairs` (the
+// KeyValuePair layout: key, value interleaved) with `temp` (plan.n pairs) as
+// the ping-pong buffer: the first pass reads the pairs and writes a key
+// array and a value array into the other buffer, the middle passes sort
+// those arrays, the last pass writes pairs again (into `pairs`, or into
+// `temp` and a copy back after an odd number of passes).  The plan's
+// look-back state is cleared first with fill kernels, so a plan carved from
+// any scratch of radix_scratch_bytes(plan.n) bytes works whatever its last
+// use was.
+hipError_t radix_sort_kv(const RadixSortPlan& plan, uint32_t bits, uint32_t* pairs, uint32_t* temp,
+                         hipStream_t stream) {
+    const uint32_t n = plan.n;
+    const uint32_t passes = (bits + 7) / 8;
+    if (passes > (uint32_t)kMaxPasses) return hipErrorInvalidValue;
+    if (passes == 0 || n == 0) return hipSuccess;
+    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(plan.ticket), 0u,
+                                   (64 + (size_t)plan.num_tiles * kRadix * 8) / 4, stream);
+    if (e != hipSuccess) return e;
+    const uint32_t hb = std::min<uint32_t>(kHistBlocks, std::max<uint32_t>(1u, (n + kHistTile - 1) / kHistTile));
+    hipLaunchKernelGGL(k_hist<2>, dim3(hb), dim3(kHistThreads), 0, stream, pairs, n, bits, plan.block_hist,
+                       plan.digit_off, plan.done, nullptr);
+    uint32_t* src = pairs;
+    uint32_t* dst = temp;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t shift = 8 * p;
+        const uint32_t pb = (bits - shift) < 8 ? (bits - shift) : 8;
+        const bool first = p == 0, last = p + 1 == passes;
+        const dim3 grid(plan.num_tiles), block(kSortBlock);
+        const uint32_t* off = plan.digit_off + p * kRadix;
+        if (first && last)
+            hipLaunchKernelGGL((k_pass<true, true>), grid, block, 0, stream, src, src + n, dst, dst + n, n, shift, pb,
+                               off, plan.status, plan.ticket, plan.num_tiles, plan.err, nullptr);
+        else if (first)
+            hipLaunchKernelGGL((k_pass<true, false>), grid, block, 0, stream, src, src + n, dst, dst + n, n, shift,
+                               pb, off, plan.status, plan.ticket, plan.num_tiles, plan.err, nullptr);
+        else if (last)
+            hipLaunchKernelGGL((k_pass<false, true>), grid, block, 0, stream, src, src + n, dst, dst + n, n, shift,
+                               pb, off, plan.status, plan.ticket, plan.num_tiles, plan.err, nullptr);
+        else
+            hipLaunchKernelGGL((k_pass<false, false>), grid, block, 0, stream, src, src + n, dst, dst + n, n, shift,
+                               pb, off, plan.status, plan.ticket, plan.num_tiles, plan.err, nullptr);
+        std::swap(src, dst);
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess && src != pairs)
+        e = hipMemcpyAsync(pairs, src, (size_t)n * 8, hipMemcpyDeviceToDevice, stream);
+    return e;
+}
+
 }  // namespace sph
+
+// --------------------------------------------------------------------------
+// The reference's RadixSort seam (source/CUDA/radixsort.cuh:30-32) as a C ABI
+// entry point that allocates its scratch for the call.
+namespace {
+
+using sph::RadixSortPlan;
+
+int status_of(hipError_t e) {
+    if (e == hipSuccess) return SPH_OK;
+    if (e == hipErrorOutOfMemory) return SPH_ERR_OUT_OF_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SPH_ERR_NO_DEVICE;
+    return SPH_ERR_DEVICE;
+}
+
+int check_sort_args(const uint32_t* pairs, const uint32_t* temp, uint32_t bits) {
+    if (!pairs || !temp || bits > 32 || pairs == temp) return SPH_ERR_INVALID_ARGUMENT;
+    return SPH_OK;
+}
+
+// Sort, wait for it, and report a look-back wait that hit its bound.
+int sort_and_wait(RadixSortPlan& plan, uint32_t bits, uint32_t* pairs, uint32_t* temp, hipStream_t s) {
+    hipError_t e = sph::radix_sort_kv(plan, bits, pairs, temp, s);
+    uint32_t err = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&err, plan.err, sizeof(err), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return status_of(e);
+    return err ? SPH_ERR_ORDER : SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_radix_sort(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, void* stream) {
+    int st = check_sort_args(pairs, temp, bits);
+    if (st != SPH_OK) return st;
+    if (n > SPH_SORT_MAX) return SPH_ERR_CAPACITY;
+    if (n == 0 || bits == 0) return SPH_OK;
+    void* buf = nullptr;
+    hipError_t e = hipMalloc(&buf, sph::radix_scratch_bytes(n));
+    if (e != hipSuccess) return status_of(e);
+    RadixSortPlan plan;
+    sph::carve(plan, n, buf);
+    st = sort_and_wait(plan, bits, pairs, temp, static_cast<hipStream_t>(stream));
+    (void)hipFree(buf);
+    return st;
+}
+
+}  // extern "C"

@@ -16,6 +16,10 @@ LIB_PATH = _PKG / "libpibiti_sph.so"
 
 # status codes (sph_status)
 SPH_OK = 0
+SPH_ERR_NOT_INITIALISED = 6
+SPH_ERR_ORDER = 9
+# sph_debug_poke targets
+SPH_DEBUG_MOVER_COUNT, SPH_DEBUG_CELL_END, SPH_DEBUG_ORDER_ERROR = 1, 2, 3
 # enums
 SPH_EOS_LINEAR, SPH_EOS_TAIT = 0, 1
 SPH_PREC_REFERENCE, SPH_PREC_KAHAN = 0, 1
@@ -326,6 +330,8 @@ SIGNATURES = {
     "sph_get_grid": (C.c_int, [_P, _up, _up, _up, _up]),
     "sph_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "sph_synchronize": (C.c_int, [_P]),
+    "sph_debug_poke": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32]),
+    "sph_radix_sort": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
     "sph_get_device_view": (C.c_int, [_P, C.POINTER(DeviceView)]),
     "sph_set_timing": (C.c_int, [_P, C.c_int]),
     "sph_get_timing": (C.c_int, [_P, _fp]),
@@ -548,6 +554,10 @@ class SphSystem:
 
     def synchronize(self) -> None:
         _check(lib().sph_synchronize(self._ctx), "sph_synchronize")
+
+    def debug_poke(self, what: int, index: int = 0, value: int = 0) -> None:
+        """Testing hook: perturb one word of the order state (sph_debug_poke)."""
+        _check(lib().sph_debug_poke(self._ctx, what, index, value & 0xFFFFFFFF), "sph_debug_poke")
 
     def get_grid(self) -> dict:
         n, nc = self.n, int(self.params.num_cells)
