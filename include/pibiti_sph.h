@@ -76,8 +76,14 @@ enum { SPH_EOS_LINEAR = 0, SPH_EOS_TAIT = 1 };
  *   loop structure of computeDensityD/computeForceD
  *   (source/CUDA/Kernel_Cell.cui:142-199, source/CUDA/System.cu:228-250).
  * KAHAN: one compensated FP32 accumulator per sum over the same candidate
- *   order (extension).                                                      */
-enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1 };
+ *   order (extension).
+ * FAST: integer results exact (keys, order, cell ranges, neighbour sets
+ *   with the cap and both predicates) and every float within 1e-5 of the
+ *   reference arithmetic and of FP64 (condition-scaled) instead of bit for
+ *   bit: one running sum per particle, the force over pair lists the density
+ *   kernel writes, hardware square root and reciprocal, fused multiply-adds
+ *   (extension; max_par_in_cell <= 64).                                    */
+enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1, SPH_PREC_FAST = 2 };
 
 /* Arrays exchanged with the host.  POSITION/VELOCITY keep the reference's
  * 0/1 flag values of cSPH::setArray (source/SPH/SPH_Util.cpp:59-71) and are
@@ -86,8 +92,9 @@ enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1 };
  * TARGET is the window-mean pair-force acceleration of a kept training
  * window (float4, w = 0; sph_learn_get_window only); NEIGHBOURS is the
  * number of neighbours within h the last surrogate step listed for each
- * particle (uint32 per particle, in the float buffer's words; sph_get_array
- * after sph_step_learned only).
+ * particle, or after a plain step in SPH_PREC_FAST the number of force
+ * pairs (r2 <= force_r2_max) the force kernel used (uint32 per particle, in
+ * the float buffer's words).
  * All arrays are indexed by the ORIGINAL particle index.                   */
 enum {
     SPH_ARRAY_POSITION = 0,
@@ -537,6 +544,11 @@ int sph_batch_set_params(sph_batch* batch, uint32_t scene, const sph_params* par
 int sph_batch_set_array(sph_batch* batch, uint32_t scene, int which, const float* data,
                         uint32_t first, uint32_t count);
 int sph_batch_get_array(sph_batch* batch, uint32_t scene, int which, float* out);
+/* sph_get_grid of one scene, in scene-local keys, indices and slots: the
+ * scene's sorted keys less its first global key, its permutation (sorted
+ * slot -> scene-local original index) and its cells' ranges.             */
+int sph_batch_get_grid(sph_batch* batch, uint32_t scene, uint32_t* sorted_hash, uint32_t* perm,
+                       uint32_t* cell_start, uint32_t* cell_end);
 int sph_batch_step(sph_batch* batch, uint32_t nsteps);
 int sph_batch_capture(sph_batch* batch, uint32_t nsteps);
 int sph_batch_replay(sph_batch* batch, uint32_t times);
@@ -573,7 +585,6 @@ int sph_batch_info(sph_batch* batch, uint32_t* nscenes, uint32_t* total_particle
  * hold no valid order).  Scratch is allocated and freed by the call.      */
 #define SPH_SORT_MAX (1u << 27)
 int sph_radix_sort(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, void* stream);
-
 
 /* ------------------------------------------------------------------------ */
 /* Testing hook (no counterpart in the reference): perturbs one device word

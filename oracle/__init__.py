@@ -20,8 +20,13 @@ Parity pinning status (what checks each oracle function; all in tests/):
   force ........... two-particle closed form, antisymmetry (sum of m*dv == 0),
                     Galilean invariance, clamp cases (min_dist, min_dens)
   step_f64 ........ FP64 twin, pinned by the same closed forms at float64
-  golden .......... tests/golden/*.npz fixtures produced by this oracle are
-                    regression vectors, not independent pins.
+  neighbour_counts  brute-force pair counts of the same predicates
+                    (test_oracle_closed_forms.py), and cap truncation on a
+                    crowded cell counted by hand
+  grid_of ......... numpy's stable lexsort of the hashes; ranges against a
+                    brute-force scan of the sorted keys
+The reference ships no golden vectors or fixtures for this path, so there
+are none under tests/golden.
 """
 from __future__ import annotations
 
@@ -71,6 +76,7 @@ def lib() -> C.CDLL:
         h.oracle_hash_f32.argtypes = [C.c_void_p, C.c_uint32, _fp, _up]
         h.oracle_density_f32.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _fp, _up, _up, _fp, _fp]
         h.oracle_density_bruteforce_f64.argtypes = [C.c_void_p, C.c_uint32, _fp, _dp, _up]
+        h.oracle_neighbour_counts_f32.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _fp, _up, _up, _up]
         _lib = h
     return _lib
 
@@ -178,6 +184,37 @@ def density_f32(params, spos4, cell_start, cell_end, prec: int = 0):
     pres = np.empty(n, np.float32)
     lib().oracle_density_f32(_pp(params), prec, n, _f(sp), _u(cs), _u(ce), _f(dens), _f(pres))
     return dens, pres
+
+
+def neighbour_counts_f32(params, spos4, cell_start, cell_end, pred: int = 0):
+    """Per sorted slot: candidates of the capped 27-cell walk that pass the
+    density predicate (pred 0: r2 < h2) or the force predicate (pred 1)."""
+    sp = np.ascontiguousarray(spos4, np.float32).reshape(-1, 4)
+    n = len(sp)
+    out = np.empty(n, np.uint32)
+    lib().oracle_neighbour_counts_f32(_pp(params), int(pred), n, _f(sp),
+                                      _u(np.ascontiguousarray(cell_start, np.uint32)),
+                                      _u(np.ascontiguousarray(cell_end, np.uint32)), _u(out))
+    return out
+
+
+def grid_of(params, pos4):
+    """The stable (cell, index) order of positions given in original order:
+    keys, permutation (sorted slot -> original index), sorted positions and
+    the cell ranges (empty cells 0xFFFFFFFF)."""
+    pos = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+    h, _ = hash_f32(params, pos)
+    perm = np.lexsort((np.arange(len(h)), h)).astype(np.uint32)
+    sh = h[perm]
+    nc = int(params.num_cells)
+    counts = np.bincount(sh, minlength=nc)
+    start = np.full(nc, 0xFFFFFFFF, np.uint32)
+    end = np.full(nc, 0xFFFFFFFF, np.uint32)
+    occ = np.flatnonzero(counts)
+    first = np.searchsorted(sh, occ)
+    start[occ] = first
+    end[occ] = first + counts[occ]
+    return dict(hash=sh, perm=perm, spos=pos[perm], cell_start=start, cell_end=end)
 
 
 def density_bruteforce_f64(params, pos4):

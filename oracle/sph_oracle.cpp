@@ -1007,6 +1007,44 @@ void oracle_density_f32(const sph_params* p, int prec, uint32_t n, const float* 
     }
 }
 
+// Neighbour counts of the density walk (computeDensityD's loops,
+// source/CUDA/Kernel_Cell.cui:142-199): for each sorted slot the candidates
+// j != i of the 27 cells, at most max_par_in_cell per cell, with
+// r2 < h2 (pred 0, the density predicate) or r2 <= force_r2_max (pred 1, the
+// force predicate of compForcePair, Kernel_Cell.cui:214-218, as one
+// comparison).  counts in sorted order.
+void oracle_neighbour_counts_f32(const sph_params* p, int pred, uint32_t n, const float* spos4,
+                                 const uint32_t* cell_start, const uint32_t* cell_end, uint32_t* counts) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const float xi[3] = {spos4[4 * i], spos4[4 * i + 1], spos4[4 * i + 2]};
+        int g[3];
+        grid_pos<float>(*p, xi, g, nullptr);
+        uint32_t cnt = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int cx = g[0] + dx, cy = g[1] + dy, cz = g[2] + dz;
+                    if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)p->grid_size[0] || cy >= (int)p->grid_size[1] ||
+                        cz >= (int)p->grid_size[2])
+                        continue;
+                    const uint32_t c = grid_hash(*p, cx, cy, cz);
+                    const uint32_t s = cell_start[c];
+                    if (s == kEmpty) continue;
+                    const uint32_t e = std::min(cell_end[c], s + p->max_par_in_cell);
+                    for (uint32_t j = s; j < e; ++j) {
+                        if (j == i) continue;
+                        const float rx = xi[0] - spos4[4 * j], ry = xi[1] - spos4[4 * j + 1],
+                                    rz = xi[2] - spos4[4 * j + 2];
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        if (pred == 0 ? r2 < p->h2 : r2 <= p->force_r2_max) ++cnt;
+                    }
+                }
+        counts[i] = cnt;
+    }
+}
+
 // Brute-force O(n^2) density in double: all pairs j != i with r2 < h^2,
 // no grid.  The reference for neighbour-set completeness when cell >= h and
 // no cell exceeds max_par_in_cell.

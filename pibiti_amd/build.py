@@ -20,7 +20,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libpibiti_sph.so"
 RUN_BIN = PKG / "_build" / "sph_run"  # C++ host driver (pibiti_amd/host/sph_run.cpp)
 
-HIP_SOURCES = ["sph_kernels.hip", "sph_sort.hip", "sph_resort.hip", "sph_skin.hip", "sph_capi.hip", "sph_mlp.hip", "sph_learn.hip"]
+HIP_SOURCES = ["sph_kernels.hip", "sph_fast.hip", "sph_sort.hip", "sph_resort.hip", "sph_skin.hip", "sph_capi.hip",
+               "sph_mlp.hip", "sph_learn.hip"]
 CPP_SOURCES = ["sph_scene.cpp", "sph_xml.cpp", "sph_inject.cpp"]
 
 # -packed-fp32-ops keeps the compiler from emitting packed fp32 ops.

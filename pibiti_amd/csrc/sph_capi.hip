@@ -70,7 +70,7 @@ int sph_step(sph_ctx* c, uint32_t nsteps) {
 int sph_get_grid(sph_ctx* c, uint32_t* sorted_hash, uint32_t* perm, uint32_t* cell_start,
                  uint32_t* cell_end) {
     if (!c) return SPH_ERR_INVALID_ARGUMENT;
-    return c->eng.get_grid(sorted_hash, perm, cell_start, cell_end);
+    return c->eng.get_grid(0, sorted_hash, perm, cell_start, cell_end);
 }
 
 int sph_get_stats(sph_ctx* c, sph_stats* out) {
@@ -180,6 +180,12 @@ int sph_batch_set_array(sph_batch* b, uint32_t scene, int which, const float* da
 int sph_batch_get_array(sph_batch* b, uint32_t scene, int which, float* out) {
     if (!b) return SPH_ERR_INVALID_ARGUMENT;
     return b->eng.get_array(scene, which, out);
+}
+
+int sph_batch_get_grid(sph_batch* b, uint32_t scene, uint32_t* sorted_hash, uint32_t* perm, uint32_t* cell_start,
+                       uint32_t* cell_end) {
+    if (!b) return SPH_ERR_INVALID_ARGUMENT;
+    return b->eng.get_grid(scene, sorted_hash, perm, cell_start, cell_end);
 }
 
 int sph_batch_step(sph_batch* b, uint32_t nsteps) {

@@ -73,8 +73,9 @@ inline int validate(const sph_params* p) {
     if (!p) return SPH_ERR_INVALID_ARGUMENT;
     if (p->num_particles == 0 || p->num_particles > kMaxParticles || p->num_cells == 0 ||
         p->max_par_in_cell == 0 || !(p->h > 0.0f) || !(p->time_step > 0.0f) ||
-        p->eos > SPH_EOS_TAIT || p->precision > SPH_PREC_KAHAN || !(p->t_end >= 0.0f))
+        p->eos > SPH_EOS_TAIT || p->precision > SPH_PREC_FAST || !(p->t_end >= 0.0f))
         return SPH_ERR_INVALID_PARAMS;
+    if (p->precision == SPH_PREC_FAST && p->max_par_in_cell > kFastMaxCap) return SPH_ERR_UNSUPPORTED;
     if (check_shape(*p) != SPH_OK) return SPH_ERR_INVALID_PARAMS;
     if ((uint64_t)p->grid_size[0] * p->grid_size[1] * p->grid_size[2] != p->num_cells ||
         p->grid_size_yx != p->grid_size[0] * p->grid_size[1])
@@ -115,6 +116,9 @@ struct Engine {
     float2* rhop = nullptr;
     float4* dvel = nullptr;
     uint32_t* masks = nullptr;  // force pair masks (kMaskWords per slot) while cells hold <= 16 candidates
+    uint4* fast_nbr = nullptr;  // SPH_PREC_FAST: force pair lists (fast_groups uint4 per slot, group-major)
+    uint32_t* fast_nnb = nullptr;
+    uint32_t fast_groups = 0;
     float4* colour = nullptr;   // per sorted slot, scenes with clr_type != NONE
     float* dye = nullptr;       // per ORIGINAL index (it follows the particle)
     uint32_t *key_pid = nullptr, *val_pid = nullptr, *key_tmp = nullptr, *val_tmp = nullptr;
@@ -204,7 +208,7 @@ struct Engine {
             (void)hipFree(cell_end[k]);
             (void)hipFree(skey_buf[k]);
         }
-        void* ptrs[] = {rhop,     dvel,     masks,    colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
+        void* ptrs[] = {rhop,     dvel,     masks,    fast_nbr, fast_nnb, colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
                         counters, cell_max, work,    partials,
                         stats,    dt_dev,   staging,  sort_scratch, resort_scratch, d_tab, d_scene_off};
         for (void* q : ptrs) (void)hipFree(q);
@@ -329,6 +333,7 @@ struct Engine {
         if (e == hipSuccess && dvel)
             e = launch_fill_u32(reinterpret_cast<uint32_t*>(dvel), 0u, 4 * N, stream);
         if (e == hipSuccess) e = ensure_colour();
+        if (e == hipSuccess) e = ensure_fast();
         if (e == hipSuccess) {
             std::vector<float> dts(nscenes);
             for (uint32_t s = 0; s < nscenes; ++s) dts[s] = scenes[s].time_step;
@@ -353,6 +358,20 @@ struct Engine {
         if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(colour), 0u, 4 * N, stream);
         if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dye), 0u, N, stream);
         if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dvel), 0u, 4 * N, stream);
+        return e;
+    }
+
+    // FAST precision: the force pair lists, sized for the largest cap.
+    hipError_t ensure_fast() {
+        if (cfg.precision != SPH_PREC_FAST) return hipSuccess;
+        const uint32_t g = fast_list_groups(cfg.max_cap);
+        if (fast_nbr && g <= fast_groups) return hipSuccess;
+        (void)hipFree(fast_nbr);
+        fast_nbr = nullptr;
+        fast_groups = 0;
+        hipError_t e = dalloc(&fast_nbr, (size_t)g * cfg.n);
+        if (e == hipSuccess && !fast_nnb) e = dalloc(&fast_nnb, (size_t)cfg.n);
+        if (e == hipSuccess) fast_groups = g;
         return e;
     }
 
@@ -383,10 +402,18 @@ struct Engine {
         cur_key ^= 1;
         last_skey = skey_buf[cur_key];
         if (timed) SPH_TRY(hipEventRecord(ev[3], stream));
-        SPH_TRY(launch_density(c, buffers(), stream));
+        const bool fast = cfg.precision == SPH_PREC_FAST;
+        if (fast)
+            SPH_TRY(launch_density_fast(c, buffers(), fast_nbr, fast_nnb, stream));
+        else
+            SPH_TRY(launch_density(c, buffers(), stream));
         if (timed) SPH_TRY(hipEventRecord(ev[4], stream));
         StepBuffers b = buffers();
-        SPH_TRY(launch_force(c, b, b.vel_alt, keep || cfg.colour, stream));
+        if (fast)
+            SPH_TRY(launch_force_fast(c, b, fast_nbr, fast_nnb, b.vel_alt, keep || cfg.colour, stream));
+        else
+            SPH_TRY(launch_force(c, b, b.vel_alt, keep || cfg.colour, stream));
+        if (fast && cfg.physics) SPH_TRY(launch_extra_terms(c, b, b.vel_alt, keep || cfg.colour, stream));
         if (cfg.colour) SPH_TRY(launch_colour(c, b, colour, dye, stream));
         cur_vel ^= 1;
         last_kind = 1;
@@ -562,6 +589,7 @@ struct Engine {
         const uint32_t n = cfg.n;
         const uint32_t* p = pid[cur_pid];
         int rec = 4;
+        bool pair_counts = false;  // FAST lists: the count word holds entries | pairs << 16
         switch (which) {
             case SPH_ARRAY_POSITION:
                 SPH_TRY(launch_scatter_to_original(n, p, pos[cur_pos], staging, 4, 4, stream));
@@ -592,8 +620,14 @@ struct Engine {
                 rec = 1;
                 break;
             case SPH_ARRAY_NEIGHBOURS:
-                if (last_kind != 2) return SPH_ERR_UNSUPPORTED;
-                SPH_TRY_ST(learn_neighbour_counts(*this));
+                if (last_kind == 1 && cfg.precision == SPH_PREC_FAST && fast_nnb) {
+                    // the last plain step's force pairs per particle
+                    SPH_TRY(launch_scatter_to_original(n, p, fast_nnb, staging, 1, 1, stream));
+                    pair_counts = true;
+                } else {
+                    if (last_kind != 2) return SPH_ERR_UNSUPPORTED;
+                    SPH_TRY_ST(learn_neighbour_counts(*this));
+                }
                 rec = 1;
                 break;
             default:
@@ -603,6 +637,10 @@ struct Engine {
         SPH_TRY(hipMemcpyAsync(out, src, (size_t)scenes[scene].num_particles * rec * sizeof(float),
                                hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipStreamSynchronize(stream));
+        if (pair_counts) {
+            uint32_t* w = reinterpret_cast<uint32_t*>(out);
+            for (uint32_t k = 0; k < scenes[scene].num_particles; ++k) w[k] >>= 16;
+        }
         return SPH_OK;
     }
 
@@ -654,27 +692,44 @@ struct Engine {
         return SPH_OK;
     }
 
-    int get_grid(uint32_t* sorted_hash, uint32_t* perm, uint32_t* cs, uint32_t* ce) {
+    // The grid of one scene: its slots are one contiguous run of the sorted
+    // order (its cell keys are a range of their own), reported in scene-local
+    // keys, indices and slots.
+    int get_grid(uint32_t scene, uint32_t* sorted_hash, uint32_t* perm, uint32_t* cs, uint32_t* ce) {
+        if (scene >= cfg.nscenes) return SPH_ERR_INVALID_ARGUMENT;
         if (!stepped) return SPH_ERR_NOT_INITIALISED;
         SPH_TRY_ST(check_order());
         DeviceGuard guard(device);
-        const size_t n = cfg.n, ncell = cfg.num_cells;
-        if (sorted_hash) SPH_TRY(hipMemcpyAsync(sorted_hash, last_skey, n * 4, hipMemcpyDeviceToHost, stream));
-        if (perm) SPH_TRY(hipMemcpyAsync(perm, pid[cur_pid], n * 4, hipMemcpyDeviceToHost, stream));
+        const uint32_t a = scene_off[scene];
+        const size_t n = scenes[scene].num_particles, ncell = scenes[scene].num_cells;
+        const uint32_t base = scenes[scene].cell_base;
+        if (sorted_hash)
+            SPH_TRY(hipMemcpyAsync(sorted_hash, last_skey + a, n * 4, hipMemcpyDeviceToHost, stream));
+        if (perm) SPH_TRY(hipMemcpyAsync(perm, pid[cur_pid] + a, n * 4, hipMemcpyDeviceToHost, stream));
         std::vector<uint32_t> tmp;
         uint32_t* csp = cs;
         if (ce && !cs) {
             tmp.resize(ncell);
             csp = tmp.data();
         }
-        if (csp) SPH_TRY(hipMemcpyAsync(csp, cell_start[cur_cell], ncell * 4, hipMemcpyDeviceToHost, stream));
-        if (ce) SPH_TRY(hipMemcpyAsync(ce, cell_end[cur_cell], ncell * 4, hipMemcpyDeviceToHost, stream));
+        if (csp)
+            SPH_TRY(hipMemcpyAsync(csp, cell_start[cur_cell] + base, ncell * 4, hipMemcpyDeviceToHost, stream));
+        if (ce) SPH_TRY(hipMemcpyAsync(ce, cell_end[cur_cell] + base, ncell * 4, hipMemcpyDeviceToHost, stream));
         SPH_TRY(hipStreamSynchronize(stream));
+        if (sorted_hash && base)
+            for (size_t k = 0; k < n; ++k) sorted_hash[k] -= base;
+        if (perm && a)
+            for (size_t k = 0; k < n; ++k) perm[k] -= a;
         // cellEnd is only maintained for occupied cells; report empty ones as
         // the reference marks an empty cellStart.
-        if (ce)
-            for (size_t k = 0; k < ncell; ++k)
-                if (csp[k] == kEmptyCell) ce[k] = kEmptyCell;
+        for (size_t k = 0; k < ncell && csp; ++k) {
+            if (csp[k] == kEmptyCell) {
+                if (ce) ce[k] = kEmptyCell;
+            } else {
+                if (ce) ce[k] -= a;
+                csp[k] -= a;
+            }
+        }
         return SPH_OK;
     }
 
@@ -719,6 +774,7 @@ struct Engine {
         cfg.physics = 0;
         for (const auto& sc : scenes) cfg.physics |= sc.physics;
         SPH_TRY(ensure_colour());
+        SPH_TRY(ensure_fast());
         if (batch)
             SPH_TRY(hipMemcpyAsync(d_tab + scene, &cur, sizeof(sph_params), hipMemcpyHostToDevice, stream));
         SPH_TRY(hipMemcpyAsync(dt_dev + scene, &cur.time_step, sizeof(float), hipMemcpyHostToDevice, stream));
@@ -841,6 +897,7 @@ struct Engine {
         // 2 = the pair-mask path (force lists filled from the density
         // kernel's masks), else the list size of the candidate-testing force
         const uint32_t lists[] = {2, 16, 24, 32};
+        const bool fast = cfg.precision == SPH_PREC_FAST;  // one launch shape: time it once
         // gather grid and variant: per-lane loads in XCD order, cell tables in
         // LDS, or a persistent grid
         const uint32_t shapes[][2] = {{0, 0}, {0, 2}, {2 * cus, 0}, {4 * cus, 0}, {8 * cus, 0}};
@@ -849,10 +906,11 @@ struct Engine {
         const uint32_t saved_tune[4] = {cfg.tune[0], cfg.tune[1], cfg.tune[2], cfg.tune[3]};
         int st = SPH_OK;
         for (uint32_t li : lists) {
-            if (li == 2 ? (cfg.max_cap > kMaskCap || !masks) : cfg.max_cap > li) continue;
+            if (fast && li != lists[0]) break;
+            if (!fast && (li == 2 ? (cfg.max_cap > kMaskCap || !masks) : cfg.max_cap > li)) continue;
             for (const auto& sh : shapes) {
                 if (e != hipSuccess || st != SPH_OK) break;
-                if (li == 2 && (sh[0] != 0 || sh[1] != 0)) continue;  // one launch shape
+                if ((li == 2 || fast) && (sh[0] != 0 || sh[1] != 0)) continue;  // one launch shape
                 const uint32_t g = sh[0];
                 const uint32_t cand[4] = {li, g, 0, sh[1]};
                 set_tune(cand);

@@ -1,0 +1,215 @@
+// sph_fast.hip - the density and force gathers of the FAST precision mode.
+//
+// The same step as the reference-order kernels (sph_kernels.hip), under the
+// float contract of the task rather than bit-exactness: integer outputs stay
+// exact (cell keys, order, cell ranges, and the neighbour sets with the
+// per-cell cap and both predicates, r2 < h2 for the density and
+// r2 <= force_r2_max for the force), every float is within 1e-5 of the FP32
+// restatement of the reference and of the FP64 twin, condition-scaled.  That
+// frees the arithmetic:
+//   density  one running sum per particle instead of per-cell partial sums
+//            added in cell order (source/CUDA/Kernel_Cell.cui:142-199);
+//            the exact candidate walk and predicates are unchanged, and the
+//            walk writes each particle's force pairs to a list;
+//   force    the listed pairs only (no cell walk, no candidate tests), with
+//            the hardware square root and reciprocal, fused multiply-adds,
+//            d12 = min(minDens, (1/rho_i)(1/rho_j)) and the pressure term
+//            multiplied by 1/r, where the reference divides
+//            (compForcePair, source/CUDA/Kernel_Cell.cui:210-226; the
+//            velocity update of System.cu:228-250).
+//
+// Pair lists.  The density walk tests candidates four at a time (four
+// consecutive slots of one cell); a group with at least one force pair
+// leaves one list entry, (first slot << 4) | 4-bit mask of the pairs, so the
+// walk spends one LDS write per group rather than per pair.  Entries are
+// group-major (entries 4g .. 4g + 3 of slot i in the uint4 at g n + i) and
+// there is room for every group the capped walk can produce (27 cells of at
+// most ceil(max_par_in_cell / 4) groups), so the lists never overflow.  The
+// count word of a slot holds the entries (low 16 bits) and the pairs (high
+// 16 bits).
+#include "sph_device.hpp"
+
+namespace sph {
+namespace {
+
+using namespace dev;
+
+constexpr int kBlock = 256;
+constexpr uint32_t kRing = 4;  // ring entries per lane (a group adds at most one)
+
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Density + EOS and the force pair list of one slot per thread.
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const sph_params* __restrict__ tab,
+                                                         const uint32_t* __restrict__ sid, uint32_t n,
+                                                         const float4* __restrict__ pos,
+                                                         const uint32_t* __restrict__ cell_start,
+                                                         const uint32_t* __restrict__ cell_end,
+                                                         float2* __restrict__ rhop, uint4* __restrict__ nbr,
+                                                         uint32_t* __restrict__ nnb) {
+    __shared__ uint32_t ring[kRing * kBlock];
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
+    const float4 xi = pos[i];
+    CellWalk w;
+    w.init(p, xi.x, xi.y, xi.z);
+    const float h2 = p.h2, r2f = p.force_r2_max;
+    uint32_t* const slot0 = ring + threadIdx.x;
+    uint32_t count = 0, flushed = 0, pairs = 0;
+    float sum = 0.0f;
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+        for (int dy = -1; dy <= 1; ++dy) {
+            const CellWalk::Row row = w.row(dy, dz);
+#pragma unroll 1
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint32_t s, e;
+                if (!w.open_in(row, dx, cell_start, cell_end, s, e)) continue;
+                // the particle itself is a candidate of its own cell only
+                const uint32_t self = (dx | dy | dz) == 0 ? i : 0xFFFFFFFFu;
+                for (uint32_t j0 = s; j0 < e; j0 += 4) {
+                    // four loads in flight; position arrays are padded past
+                    // their end, and indices past the cell's end are masked
+                    float4 xq[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        xq[k] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pos) +
+                                                                 ((j0 + (uint32_t)k) << 4));
+                    uint32_t bits = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const uint32_t j = j0 + (uint32_t)k;
+                        const bool valid = j < e && j != self;
+                        // the predicates need the reference's r2 exactly
+                        const float rx = xi.x - xq[k].x, ry = xi.y - xq[k].y, rz = xi.z - xq[k].z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        const float c = h2 - r2;
+                        sum += (valid && r2 < h2) ? (c * c) * c : 0.0f;
+                        bits |= (valid && r2 <= r2f) ? (1u << k) : 0u;
+                    }
+                    // one entry per group with a pair; the written slot is
+                    // only kept when the count advances
+                    slot0[(count & (kRing - 1u)) * kBlock] = (j0 << 4) | bits;
+                    count += bits ? 1u : 0u;
+                    pairs += (uint32_t)__popc(bits);
+                    if (count - flushed == kRing) {
+                        nbr[(size_t)(flushed >> 2) * n + i] =
+                            make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
+                        flushed += kRing;
+                    }
+                }
+            }
+        }
+    }
+    if (count > flushed)
+        nbr[(size_t)(flushed >> 2) * n + i] = make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
+    nnb[i] = count | (pairs << 16);
+    const float rho = (sum * p.poly6_kern) * p.particle_mass;
+    rhop[i] = make_float2(rho, eos(p, rho));
+}
+
+// Force + velocity update of one slot per thread over its listed pairs.
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_force_fast(sph_params one, const sph_params* __restrict__ tab,
+                                                       const uint32_t* __restrict__ sid, uint32_t n,
+                                                       const float4* __restrict__ pos,
+                                                       const float4* __restrict__ vel,
+                                                       const float2* __restrict__ rhop,
+                                                       const uint4* __restrict__ nbr,
+                                                       const uint32_t* __restrict__ nnb,
+                                                       float4* __restrict__ vel_out, float4* __restrict__ dvel,
+                                                       const float* __restrict__ dt_dev) {
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
+    const float4 xi = pos[i];
+    const float4 vi = vel[i];
+    const float2 ri = rhop[i];
+    const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
+    const float vterm = p.lap_kern * p.viscosity;
+    const float inv_ri = rcp_hw(ri.x);
+    const uint32_t word = nnb[i];
+    const uint32_t entries = word & 0xFFFFu, npairs = word >> 16;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    auto pair = [&](uint32_t j) {
+        const float4 xj = pos[j];
+        const float4 vj = vel[j];
+        const float2 rj = rhop[j];
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+        const float r = fmaxf(min_dist, sqrt_hw(r2));
+        const float d12 = fminf(min_dens, inv_ri * rcp_hw(rj.x));
+        const float cc = h - r;
+        const float pterm = (cc * spiky) * (ri.y + rj.y) * rcp_hw(r);
+        const float cd = cc * d12;
+        ax = __builtin_fmaf(__builtin_fmaf(pterm, rx, vterm * (vj.x - vi.x)), cd, ax);
+        ay = __builtin_fmaf(__builtin_fmaf(pterm, ry, vterm * (vj.y - vi.y)), cd, ay);
+        az = __builtin_fmaf(__builtin_fmaf(pterm, rz, vterm * (vj.z - vi.z)), cd, az);
+    };
+    // one pair per iteration, whatever entry it comes from, so that the
+    // lanes of a wave run about as many iterations as their pair counts
+    uint4 buf = make_uint4(0, 0, 0, 0);
+    uint32_t cur = 0, next = 0;
+    for (uint32_t t = 0; t < npairs; ++t) {
+        if ((cur & 15u) == 0u) {
+            if ((next & 3u) == 0u) buf = nbr[(size_t)(next >> 2) * n + i];
+            const uint32_t q = next & 3u;
+            cur = q == 0u ? buf.x : (q == 1u ? buf.y : (q == 2u ? buf.z : buf.w));
+            ++next;
+        }
+        const uint32_t k = (uint32_t)__builtin_ctz(cur);  // the lowest set bit is in the mask
+        cur &= cur - 1u;
+        pair((cur >> 4) + k);
+    }
+    (void)entries;
+    const float mdt = p.particle_mass * dt_dev[scene];
+    ax *= mdt;
+    ay *= mdt;
+    az *= mdt;
+    if (p.physics) {
+        // the pair increment only: k_extra_terms adds the scene's other terms
+        vel_out[i] = make_float4(ax, ay, az, 0.0f);
+        return;
+    }
+    vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
+    if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+inline uint32_t blocks_for(uint32_t n) { return n == 0 ? 1u : (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+uint32_t fast_list_groups(uint32_t max_par_in_cell) { return (27u * ((max_par_in_cell + 3u) / 4u) + 3u) / 4u; }
+
+hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, uint4* nbr, uint32_t* nnb,
+                               hipStream_t s) {
+    const dim3 grid(blocks_for(c.n)), block(kBlock);
+    if (c.tab)
+        hipLaunchKernelGGL(k_density_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.cell_start, b.cell_end, b.rhop, nbr, nnb);
+    else
+        hipLaunchKernelGGL(k_density_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.cell_start, b.cell_end, b.rhop, nbr, nnb);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_fast(const LaunchCfg& c, const StepBuffers& b, const uint4* nbr, const uint32_t* nnb,
+                             float4* vel_out, bool keep_dvel, hipStream_t s) {
+    float4* dvel = keep_dvel ? b.dvel : nullptr;
+    const dim3 grid(blocks_for(c.n)), block(kBlock);
+    if (c.tab)
+        hipLaunchKernelGGL(k_force_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.vel_cur, b.rhop, nbr, nnb, vel_out, dvel, b.dt_dev);
+    else
+        hipLaunchKernelGGL(k_force_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
+                           b.vel_cur, b.rhop, nbr, nnb, vel_out, dvel, b.dt_dev);
+    return hipGetLastError();
+}
+
+}  // namespace sph

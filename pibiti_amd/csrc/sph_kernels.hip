@@ -1944,15 +1944,20 @@ hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_ou
         else
             force_impl<false, SPH_PREC_REFERENCE>(c, b, vel_out, dvel, s);
     }
-    if (c.physics) {
-        const dim3 grid(blocks_for(c.n, kBlock));
-        if (c.tab)
-            hipLaunchKernelGGL(k_extra_terms<true>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n,
-                               b.pos_cur, b.vel_cur, vel_out, dvel, b.dt_dev);
-        else
-            hipLaunchKernelGGL(k_extra_terms<false>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n,
-                               b.pos_cur, b.vel_cur, vel_out, dvel, b.dt_dev);
-    }
+    if (c.physics) return launch_extra_terms(c, b, vel_out, keep_dvel, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_extra_terms(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, bool keep_dvel,
+                              hipStream_t s) {
+    float4* dvel = keep_dvel ? b.dvel : nullptr;
+    const dim3 grid(blocks_for(c.n, kBlock));
+    if (c.tab)
+        hipLaunchKernelGGL(k_extra_terms<true>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n,
+                           b.pos_cur, b.vel_cur, vel_out, dvel, b.dt_dev);
+    else
+        hipLaunchKernelGGL(k_extra_terms<false>, grid, dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n,
+                           b.pos_cur, b.vel_cur, vel_out, dvel, b.dt_dev);
     return hipGetLastError();
 }
 

@@ -244,7 +244,7 @@ int sph_params_from_scene(const sph_scene* s, sph_params* p) {
     if (!s || !p) return SPH_ERR_INVALID_ARGUMENT;
     if (s->num_particles == 0 || !(s->h > 0.0f) || !(s->cell_size > 0.0f) ||
         !(s->time_step > 0.0f) || !(s->rest_density > 0.0f) || !(s->particle_r > 0.0f) ||
-        s->max_par_in_cell == 0 || s->eos > SPH_EOS_TAIT || s->precision > SPH_PREC_KAHAN ||
+        s->max_par_in_cell == 0 || s->eos > SPH_EOS_TAIT || s->precision > SPH_PREC_FAST ||
         !(s->cfl >= 0.0f) || !(s->stiffness >= 0.0f))
         return SPH_ERR_INVALID_PARAMS;
     for (int c = 0; c < 3; ++c)

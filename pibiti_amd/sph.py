@@ -22,7 +22,7 @@ SPH_ERR_ORDER = 9
 SPH_DEBUG_MOVER_COUNT, SPH_DEBUG_CELL_END, SPH_DEBUG_ORDER_ERROR = 1, 2, 3
 # enums
 SPH_EOS_LINEAR, SPH_EOS_TAIT = 0, 1
-SPH_PREC_REFERENCE, SPH_PREC_KAHAN = 0, 1
+SPH_PREC_REFERENCE, SPH_PREC_KAHAN, SPH_PREC_FAST = 0, 1, 2
 (SPH_ARRAY_POSITION, SPH_ARRAY_VELOCITY, SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_DVEL,
  SPH_ARRAY_TARGET, SPH_ARRAY_NEIGHBOURS, SPH_ARRAY_COLOR, SPH_ARRAY_DYE) = range(9)
 _SCALAR_ARRAYS = (SPH_ARRAY_DENSITY, SPH_ARRAY_PRESSURE, SPH_ARRAY_NEIGHBOURS, SPH_ARRAY_DYE)
@@ -344,6 +344,7 @@ SIGNATURES = {
     "sph_batch_set_params": (C.c_int, [_P, C.c_uint32, C.POINTER(Params)]),
     "sph_batch_set_array": (C.c_int, [_P, C.c_uint32, C.c_int, _fp, C.c_uint32, C.c_uint32]),
     "sph_batch_get_array": (C.c_int, [_P, C.c_uint32, C.c_int, _fp]),
+    "sph_batch_get_grid": (C.c_int, [_P, C.c_uint32, _up, _up, _up, _up]),
     "sph_batch_step": (C.c_int, [_P, C.c_uint32]),
     "sph_batch_capture": (C.c_int, [_P, C.c_uint32]),
     "sph_batch_replay": (C.c_int, [_P, C.c_uint32]),
@@ -752,6 +753,16 @@ class SphBatch:
         shape = (n,) if which in _SCALAR_ARRAYS else (n, 4)
         out = np.empty(shape, np.float32)
         _check(lib().sph_batch_get_array(self._b, scene, which, _f(out)), "sph_batch_get_array")
+        return out
+
+    def get_grid(self, scene: int) -> dict:
+        """One scene's grid in scene-local keys, indices and slots."""
+        p = self.params(scene)
+        n, nc = int(p.num_particles), int(p.num_cells)
+        out = {"sorted_hash": np.empty(n, np.uint32), "perm": np.empty(n, np.uint32),
+               "cell_start": np.empty(nc, np.uint32), "cell_end": np.empty(nc, np.uint32)}
+        _check(lib().sph_batch_get_grid(self._b, scene, _u(out["sorted_hash"]), _u(out["perm"]),
+                                        _u(out["cell_start"]), _u(out["cell_end"])), "sph_batch_get_grid")
         return out
 
     def update(self, nsteps: int = 1) -> None:

@@ -230,6 +230,40 @@ def test_cap_truncates_each_cell_to_first_sorted_particles():
     np.testing.assert_allclose(o["dens"][4], k * (w(4, 0) + w(4, 1)), rtol=1e-12)
 
 
+def test_neighbour_counts_match_brute_force_pairs():
+    """neighbour_counts_f32 (the density walk's passing candidates) against
+    all pairs within h counted by brute force in float32, on a scene with no
+    cell over the cap; the force predicate counts the same pairs up to the
+    few at r ~ h where the two predicates differ."""
+    from oracle import mlp_oracle as mo
+    p, pos, vel, _ = scenes.dam_break(9, 8, 9, spacing=0.02, h=0.04, world_max=(1, 1, 1),
+                                      dt=1e-4, viscosity=0.1, seed=6, gravity=(0.0, 0.0, 0.0))
+    pos[:, :3] += np.float32(0.3)
+    g = oracle.grid_of(p, pos)
+    occ = g["cell_start"] != EMPTY
+    assert (g["cell_end"][occ] - g["cell_start"][occ]).max() <= p.max_par_in_cell
+    cnt = oracle.neighbour_counts_f32(p, g["spos"], g["cell_start"], g["cell_end"], pred=0)
+    pi, _ = mo.pairs_bruteforce(pos, p.h)
+    brute = np.bincount(pi, minlength=len(pos))
+    assert np.array_equal(cnt, brute[g["perm"]])
+    fc = oracle.neighbour_counts_f32(p, g["spos"], g["cell_start"], g["cell_end"], pred=1)
+    assert np.abs(fc.astype(np.int64) - cnt).max() <= 1 and cnt.max() > 20
+
+
+def test_neighbour_counts_cap_truncation_by_hand():
+    """Five particles in one cell with cap 2: each sees only slots 0 and 1
+    (the two lowest indices), so slots 0 and 1 count one neighbour each and
+    slots 2 to 4 count two."""
+    p = make_params(5, h=0.04, cap=2, soft=0.0, hard=0.0)
+    pts = [[0.485, 0.485, 0.485], [0.49, 0.49, 0.49], [0.495, 0.495, 0.495], [0.50, 0.50, 0.50],
+           [0.515, 0.485, 0.485]]
+    pos, _ = state(pts)
+    g = oracle.grid_of(p, pos)
+    assert np.array_equal(g["perm"], np.arange(5))
+    cnt = oracle.neighbour_counts_f32(p, g["spos"], g["cell_start"], g["cell_end"])
+    assert cnt.tolist() == [1, 1, 2, 2, 2]
+
+
 @pytest.mark.parametrize("eos", [0, 1])
 def test_eos_closed_form(eos):
     p, pos, vel, _ = scenes.dam_break(8, 8, 8, spacing=0.02, h=0.04, world_max=(1, 1, 1), dt=1e-4,
