@@ -586,6 +586,17 @@ int sph_batch_info(sph_batch* batch, uint32_t* nscenes, uint32_t* total_particle
 #define SPH_SORT_MAX (1u << 27)
 int sph_radix_sort(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, void* stream);
 
+/* sph_radix_sort without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate, such as tight serving loops
+ * and harnesses that forbid hipMalloc.  Its scratch is a fixed array in
+ * device memory, one per GPU, kept until the process ends (about 0.8 MB);
+ * n above SPH_SORT_NOALLOC_MAX is SPH_ERR_CAPACITY.  Every call on a GPU
+ * uses that GPU's array, so calls on one GPU are serialised by a lock in
+ * the library and each waits for its sort: they do not overlap within a
+ * process.                                                                 */
+#define SPH_SORT_NOALLOC_MAX (1u << 20)
+int sph_radix_sort_noalloc(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Testing hook (no counterpart in the reference): perturbs one device word
  * of the step's order state between steps, so that tests can drive the

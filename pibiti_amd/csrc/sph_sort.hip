@@ -25,6 +25,7 @@
 // carry the launch number, so nothing needs clearing between launches and
 // replays of a captured graph stay correct.
 #include <algorithm>
+#include <mutex>
 
 #include "sph_internal.hpp"
 
@@ -434,11 +435,25 @@ hipError_t radix_sort_kv(const RadixSortPlan& plan, uint32_t bits, uint32_t* pai
 }  // namespace sph
 
 // --------------------------------------------------------------------------
-// The reference's RadixSort seam (source/CUDA/radixsort.cuh:30-32) as a C ABI
-// entry point that allocates its scratch for the call.
+// The reference's RadixSort seam (source/CUDA/radixsort.cuh:30-32) as C ABI
+// entry points: sph_radix_sort allocates its scratch for the call,
+// sph_radix_sort_noalloc uses a fixed device array instead.
 namespace {
 
 using sph::RadixSortPlan;
+
+// Largest sort the fixed array serves: the reference's 1M-particle scenes.
+constexpr uint32_t kNoAllocMaxPairs = SPH_SORT_NOALLOC_MAX;
+constexpr size_t kNoAllocBytes = sph::scratch_bytes_for_tiles((kNoAllocMaxPairs + sph::kTile - 1) / sph::kTile);
+constexpr int kMaxDevices = 64;
+
+// The allocation-free sort's scratch: one copy per GPU, held until the
+// process ends (about 0.8 MB).  Host code uses the per-device address from
+// hipGetSymbolAddress (noalloc_ptr), never this name.
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex noalloc_mu;
+void* noalloc_ptr[kMaxDevices] = {};
 
 int status_of(hipError_t e) {
     if (e == hipSuccess) return SPH_OK;
@@ -479,6 +494,26 @@ int sph_radix_sort(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, v
     st = sort_and_wait(plan, bits, pairs, temp, static_cast<hipStream_t>(stream));
     (void)hipFree(buf);
     return st;
+}
+
+int sph_radix_sort_noalloc(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, void* stream) {
+    int st = check_sort_args(pairs, temp, bits);
+    if (st != SPH_OK) return st;
+    if (n > kNoAllocMaxPairs) return SPH_ERR_CAPACITY;
+    if (n == 0 || bits == 0) return SPH_OK;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SPH_ERR_NO_DEVICE;
+    // one call per device at a time: the array is shared by every call on it
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return e != hipSuccess ? status_of(e) : SPH_ERR_DEVICE;
+        noalloc_ptr[dev] = ptr;
+    }
+    RadixSortPlan plan;
+    sph::carve(plan, n, noalloc_ptr[dev]);
+    return sort_and_wait(plan, bits, pairs, temp, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

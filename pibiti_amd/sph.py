@@ -332,6 +332,7 @@ SIGNATURES = {
     "sph_synchronize": (C.c_int, [_P]),
     "sph_debug_poke": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32]),
     "sph_radix_sort": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
+    "sph_radix_sort_noalloc": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
     "sph_get_device_view": (C.c_int, [_P, C.POINTER(DeviceView)]),
     "sph_set_timing": (C.c_int, [_P, C.c_int]),
     "sph_get_timing": (C.c_int, [_P, _fp]),
