@@ -31,13 +31,16 @@ are none under tests/golden.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 _DIR = Path(__file__).resolve().parent
-LIB_PATH = _DIR / "liboracle.so"
+# PIBITI_ORACLE_LIB names another build of the same source (the sanitizer
+# build of `make -C oracle asan`).
+LIB_PATH = Path(os.environ.get("PIBITI_ORACLE_LIB", str(_DIR / "liboracle.so")))
 _lib = None
 
 _fp = C.POINTER(C.c_float)

@@ -7,12 +7,18 @@ the library is missing, every entry point raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "libpibiti_sph.so"
+# PIBITI_SPH_HOST_LIB names a build of the library's host-only C++ (the
+# sanitizer build of `make -C oracle asan`) for the CPU tests of the scene
+# constants, scene files and injection; its entry points are the host-only
+# ones, the device entry points stay unbound.
+_HOST_LIB = os.environ.get("PIBITI_SPH_HOST_LIB")
+LIB_PATH = Path(_HOST_LIB) if _HOST_LIB else _PKG / "libpibiti_sph.so"
 
 # status codes (sph_status)
 SPH_OK = 0
@@ -384,7 +390,9 @@ def lib() -> C.CDLL:
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pibiti_amd.build`")
         h = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
-            fn = getattr(h, name)
+            fn = getattr(h, name, None) if _HOST_LIB else getattr(h, name)
+            if fn is None:
+                continue
             fn.restype = res
             fn.argtypes = args
         _lib_handle = h
