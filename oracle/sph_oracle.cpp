@@ -1010,9 +1010,9 @@ void oracle_density_f32(const sph_params* p, int prec, uint32_t n, const float* 
 // Neighbour counts of the density walk (computeDensityD's loops,
 // source/CUDA/Kernel_Cell.cui:142-199): for each sorted slot the candidates
 // j != i of the 27 cells, at most max_par_in_cell per cell, with
-// r2 < h2 (pred 0, the density predicate) or r2 <= force_r2_max (pred 1, the
-// force predicate of compForcePair, Kernel_Cell.cui:214-218, as one
-// comparison).  counts in sorted order.
+// r2 < h2 (pred 0, the density predicate) or max(minDist, sqrtf(r2)) < h
+// (pred 1, the force predicate of compForcePair, Kernel_Cell.cui:214-218).
+// counts in sorted order.
 void oracle_neighbour_counts_f32(const sph_params* p, int pred, uint32_t n, const float* spos4,
                                  const uint32_t* cell_start, const uint32_t* cell_end, uint32_t* counts) {
 #pragma omp parallel for schedule(dynamic, 256)
@@ -1038,7 +1038,10 @@ void oracle_neighbour_counts_f32(const sph_params* p, int pred, uint32_t n, cons
                         const float rx = xi[0] - spos4[4 * j], ry = xi[1] - spos4[4 * j + 1],
                                     rz = xi[2] - spos4[4 * j + 2];
                         const float r2 = rx * rx + ry * ry + rz * rz;
-                        if (pred == 0 ? r2 < p->h2 : r2 <= p->force_r2_max) ++cnt;
+                        // the force predicate as compForcePair evaluates it (a NaN r2
+                        // passes: max(minDist, NaN) is minDist)
+                        const bool pass = pred == 0 ? r2 < p->h2 : std::fmax(p->min_dist, std::sqrt(r2)) < p->h;
+                        if (pass) ++cnt;
                     }
                 }
         counts[i] = cnt;

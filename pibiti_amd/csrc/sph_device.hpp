@@ -85,6 +85,16 @@ __device__ __forceinline__ float sqrt_ieee(float x) {
     return sqrt_rn_normal(x);
 }
 
+// The force predicate of compForcePair (source/CUDA/Kernel_Cell.cui:214-218),
+// max(minDist, sqrtf(r2)) < h, as one comparison with force_r2_max (the
+// largest float r2 that passes; -1 when minDist >= h): r2 <= force_r2_max,
+// and also a NaN r2 while minDist < h, because max(minDist, NaN) is minDist
+// there (a particle at a NaN position is everyone's force pair in its 27
+// cells, as in the reference).
+__device__ __forceinline__ bool force_pair(float r2, float force_r2_max) {
+    return !(r2 > force_r2_max) && force_r2_max >= 0.0f;
+}
+
 struct KahanF {
     float s, c;
     __device__ __forceinline__ void add(float v) {
@@ -98,11 +108,14 @@ struct KahanF {
 // ------------------------------------------------------- neighbour-cell walk
 // The 27 cells around a particle's cell in the reference's (dz, dy, dx)
 // order (source/CUDA/Kernel_Cell.cui:188-192).  A cell whose box is farther
-// than h from the particle is skipped: none of its particles can pass
-// r2 < h2 (or the force predicate, which is stricter), so its partial sum
-// would be +0 and adding +0 leaves the total unchanged.  The test is
-// conservative: each face distance is shrunk by a fuzz covering the
+// than h from the particle is skipped: none of its particles at a finite
+// position can pass r2 < h2 (or the force predicate, which is stricter), so
+// its partial sum would be +0 and adding +0 leaves the total unchanged.  The
+// test is conservative: each face distance is shrunk by a fuzz covering the
 // rounding of cell assignment, and the bound carries a 1e-5 relative margin.
+// A particle at a NaN position passes the force predicate from anywhere in
+// its 27 cells (force_pair), so a step whose integration produced one
+// (counter kCntNaN) walks all 27 cells (those inside the grid) unculled.
 struct AxisWalk {
     int g;           // own cell coordinate
     int gmax;        // grid size - 1
@@ -125,6 +138,12 @@ struct AxisWalk {
     __device__ __forceinline__ bool ok(int d) const { return g + d >= 0 && g + d <= gmax; }
 };
 
+// Whether the step's walks may cull cells: no particle sits at a NaN
+// position (the integration counts them in counters[kCntNaN]).
+__device__ __forceinline__ bool walk_culled(const uint32_t* __restrict__ counters) {
+    return counters[kCntNaN] == 0u;
+}
+
 struct CellWalk {
     AxisWalk ax, ay, az;
     float cull2;        // squared cull radius
@@ -132,11 +151,12 @@ struct CellWalk {
     uint32_t gyx, gx;   // key strides
     uint32_t cap;       // max_par_in_cell
 
-    __device__ __forceinline__ void init(const sph_params& p, float x, float y, float z) {
+    // cull = false: no cell of the 27 is skipped for its distance.
+    __device__ __forceinline__ void init(const sph_params& p, float x, float y, float z, bool cull) {
         ax.init(x, p.world_min[0], p.world_max[0], p.cell_size[0], p.grid_size[0]);
         ay.init(y, p.world_min[1], p.world_max[1], p.cell_size[1], p.grid_size[1]);
         az.init(z, p.world_min[2], p.world_max[2], p.cell_size[2], p.grid_size[2]);
-        cull2 = p.h2 * 1.00001f;
+        cull2 = cull ? p.h2 * 1.00001f : __builtin_huge_valf();
         base = p.cell_base;
         gyx = p.grid_size_yx;
         gx = p.grid_size[0];

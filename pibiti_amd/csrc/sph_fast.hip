@@ -48,7 +48,8 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
                                                          const uint32_t* __restrict__ cell_start,
                                                          const uint32_t* __restrict__ cell_end,
                                                          float2* __restrict__ rhop, uint4* __restrict__ nbr,
-                                                         uint32_t* __restrict__ nnb) {
+                                                         uint32_t* __restrict__ nnb,
+                                                         const uint32_t* __restrict__ counters) {
     __shared__ uint32_t ring[kRing * kBlock];
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -56,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
     const float4 xi = pos[i];
     CellWalk w;
-    w.init(p, xi.x, xi.y, xi.z);
+    w.init(p, xi.x, xi.y, xi.z, walk_culled(counters));
     const float h2 = p.h2, r2f = p.force_r2_max;
     uint32_t* const slot0 = ring + threadIdx.x;
     uint32_t count = 0, flushed = 0, pairs = 0;
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
                         const float r2 = rx * rx + ry * ry + rz * rz;
                         const float c = h2 - r2;
                         sum += (valid && r2 < h2) ? (c * c) * c : 0.0f;
-                        bits |= (valid && r2 <= r2f) ? (1u << k) : 0u;
+                        bits |= (valid && force_pair(r2, r2f)) ? (1u << k) : 0u;
                     }
                     // one entry per group with a pair; the written slot is
                     // only kept when the count advances
@@ -113,18 +114,42 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     rhop[i] = make_float2(rho, eos(p, rho));
 }
 
-// Force + velocity update of one slot per thread over its listed pairs.
+// ------------------------------------------------------------------ force
+constexpr int kFT = 256;  // slots per force block
+
+// The pair term of the force update, accumulated into (ax, ay, az).
+__device__ __forceinline__ void pair_term(const float4& xi, const float4& vi, float inv_ri, float pi,
+                                          const float4& xj, const float4& vj, float rhoj, float pj, float h,
+                                          float spiky, float vterm, float min_dist, float min_dens, float& ax,
+                                          float& ay, float& az) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+    const float r = fmaxf(min_dist, sqrt_hw(r2));
+    const float d12 = fminf(min_dens, inv_ri * rcp_hw(rhoj));
+    const float cc = h - r;
+    const float pterm = (cc * spiky) * (pi + pj) * rcp_hw(r);
+    const float cd = cc * d12;
+    ax = __builtin_fmaf(__builtin_fmaf(pterm, rx, vterm * (vj.x - vi.x)), cd, ax);
+    ay = __builtin_fmaf(__builtin_fmaf(pterm, ry, vterm * (vj.y - vi.y)), cd, ay);
+    az = __builtin_fmaf(__builtin_fmaf(pterm, rz, vterm * (vj.z - vi.z)), cd, az);
+}
+
+// Force + velocity update of one slot per thread over its listed pairs,
+// four pairs per iteration: the pairs are decoded first, so that their
+// twelve loads go out together.  A lane's pairs come in the walk's order
+// (ascending slots of the same 27 cells for the lanes of one cell), which
+// keeps the lanes of a wave on nearby slots.
 template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_force_fast(sph_params one, const sph_params* __restrict__ tab,
-                                                       const uint32_t* __restrict__ sid, uint32_t n,
-                                                       const float4* __restrict__ pos,
-                                                       const float4* __restrict__ vel,
-                                                       const float2* __restrict__ rhop,
-                                                       const uint4* __restrict__ nbr,
-                                                       const uint32_t* __restrict__ nnb,
-                                                       float4* __restrict__ vel_out, float4* __restrict__ dvel,
-                                                       const float* __restrict__ dt_dev) {
-    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kFT) void k_force_fast(sph_params one, const sph_params* __restrict__ tab,
+                                                    const uint32_t* __restrict__ sid, uint32_t n,
+                                                    const float4* __restrict__ pos,
+                                                    const float4* __restrict__ vel,
+                                                    const float2* __restrict__ rhop,
+                                                    const uint4* __restrict__ nbr,
+                                                    const uint32_t* __restrict__ nnb,
+                                                    float4* __restrict__ vel_out, float4* __restrict__ dvel,
+                                                    const float* __restrict__ dt_dev) {
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kFT + threadIdx.x;
     if (i >= n) return;
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
@@ -134,29 +159,12 @@ __global__ __launch_bounds__(kBlock) void k_force_fast(sph_params one, const sph
     const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
     const float vterm = p.lap_kern * p.viscosity;
     const float inv_ri = rcp_hw(ri.x);
-    const uint32_t word = nnb[i];
-    const uint32_t entries = word & 0xFFFFu, npairs = word >> 16;
+    const uint32_t npairs = nnb[i] >> 16;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    auto pair = [&](uint32_t j) {
-        const float4 xj = pos[j];
-        const float4 vj = vel[j];
-        const float2 rj = rhop[j];
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-        const float r = fmaxf(min_dist, sqrt_hw(r2));
-        const float d12 = fminf(min_dens, inv_ri * rcp_hw(rj.x));
-        const float cc = h - r;
-        const float pterm = (cc * spiky) * (ri.y + rj.y) * rcp_hw(r);
-        const float cd = cc * d12;
-        ax = __builtin_fmaf(__builtin_fmaf(pterm, rx, vterm * (vj.x - vi.x)), cd, ax);
-        ay = __builtin_fmaf(__builtin_fmaf(pterm, ry, vterm * (vj.y - vi.y)), cd, ay);
-        az = __builtin_fmaf(__builtin_fmaf(pterm, rz, vterm * (vj.z - vi.z)), cd, az);
-    };
-    // one pair per iteration, whatever entry it comes from, so that the
-    // lanes of a wave run about as many iterations as their pair counts
     uint4 buf = make_uint4(0, 0, 0, 0);
     uint32_t cur = 0, next = 0;
-    for (uint32_t t = 0; t < npairs; ++t) {
+    // the next pair's slot (entries hold a group's first slot and pair mask)
+    auto next_pair = [&]() {
         if ((cur & 15u) == 0u) {
             if ((next & 3u) == 0u) buf = nbr[(size_t)(next >> 2) * n + i];
             const uint32_t q = next & 3u;
@@ -165,9 +173,33 @@ __global__ __launch_bounds__(kBlock) void k_force_fast(sph_params one, const sph
         }
         const uint32_t k = (uint32_t)__builtin_ctz(cur);  // the lowest set bit is in the mask
         cur &= cur - 1u;
-        pair((cur >> 4) + k);
+        return (cur >> 4) + k;
+    };
+    for (uint32_t t = 0; t < npairs; t += 4) {
+        uint32_t j[4];
+        j[0] = next_pair();
+#pragma unroll
+        for (int k = 1; k < 4; ++k) j[k] = t + (uint32_t)k < npairs ? next_pair() : j[0];
+        float4 xq[4], vq[4];
+        float2 rq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            xq[k] = pos[j[k]];
+            vq[k] = vel[j[k]];
+            rq[k] = rhop[j[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float bx = ax, by = ay, bz = az;
+            pair_term(xi, vi, inv_ri, ri.y, xq[k], vq[k], rq[k].x, rq[k].y, h, spiky, vterm, min_dist, min_dens, bx,
+                      by, bz);
+            // past the last pair (a repeat of the first of the four): nothing added
+            const bool on = t + (uint32_t)k < npairs;
+            ax = on ? bx : ax;
+            ay = on ? by : ay;
+            az = on ? bz : az;
+        }
     }
-    (void)entries;
     const float mdt = p.particle_mass * dt_dev[scene];
     ax *= mdt;
     ay *= mdt;
@@ -192,17 +224,18 @@ hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, uint4* 
     const dim3 grid(blocks_for(c.n)), block(kBlock);
     if (c.tab)
         hipLaunchKernelGGL(k_density_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.cell_start, b.cell_end, b.rhop, nbr, nnb);
+                           b.cell_start, b.cell_end, b.rhop, nbr, nnb, b.counters);
     else
         hipLaunchKernelGGL(k_density_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.cell_start, b.cell_end, b.rhop, nbr, nnb);
+                           b.cell_start, b.cell_end, b.rhop, nbr, nnb, b.counters);
     return hipGetLastError();
 }
 
 hipError_t launch_force_fast(const LaunchCfg& c, const StepBuffers& b, const uint4* nbr, const uint32_t* nnb,
                              float4* vel_out, bool keep_dvel, hipStream_t s) {
+    if (c.n == 0) return hipErrorInvalidValue;
     float4* dvel = keep_dvel ? b.dvel : nullptr;
-    const dim3 grid(blocks_for(c.n)), block(kBlock);
+    const dim3 grid((c.n + kFT - 1) / kFT), block(kFT);
     if (c.tab)
         hipLaunchKernelGGL(k_force_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
                            b.vel_cur, b.rhop, nbr, nnb, vel_out, dvel, b.dt_dev);

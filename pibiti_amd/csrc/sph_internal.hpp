@@ -23,7 +23,8 @@ enum Counter : int {
     kCntPairs = 3,        // listed neighbour pairs (min(count, kNbrCap) summed)
     kCntSkinMoved = 4,    // particles farther than the skin from their position at the last sort
     kCntFullSort = 5,     // nonzero: this step's order comes from the full radix sort
-    kCntNum = 6
+    kCntNaN = 6,          // particles with a NaN coordinate (the walks then skip no cell)
+    kCntNum = 7
 };
 
 // Per-scene FP64 diagnostics: stats[scene * kNumStats + k].

@@ -552,6 +552,8 @@ __device__ __forceinline__ uint32_t integrate_state(const sph_params& p, float d
     const int gy = cell_coord(y, p.world_min[1], p.cell_size[1], p.grid_size[1], out);
     const int gz = cell_coord(z, p.world_min[2], p.cell_size[2], p.grid_size[2], out);
     if (out && counters) atomicAdd(&counters[kCntOutOfGrid], 1u);
+    if (counters && (__builtin_isnan(x) || __builtin_isnan(y) || __builtin_isnan(z)))
+        atomicAdd(&counters[kCntNaN], 1u);
     return grid_hash(p, gx, gy, gz);
 }
 
@@ -865,6 +867,8 @@ __device__ bool stage_block(Stage<kCap>& S, uint32_t b0, uint32_t count, const s
 }
 
 // ------------------------------------------------------------------ density
+// counters: the step's counters, never null (the walk reads kCntNaN).
+//
 // With kLists the density walk also records each particle's neighbours
 // (the slots that pass r2 < h2, j != i, in walk order) in the group-major
 // lists (ListRing), and their count in nnb[i]; a count above kNbrCap
@@ -899,7 +903,7 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
         src = *src_in;
     }
     CellWalk w;
-    w.init(p, xi.x, xi.y, xi.z);
+    w.init(p, xi.x, xi.y, xi.z, walk_culled(counters));
     const float h2 = p.h2;
     float sum = 0.0f;
     KahanF ks{0.0f, 0.0f};
@@ -965,7 +969,7 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                                     cell = cell + (hit ? wgt : 0.0f);
                                 }
                                 if constexpr (kLists) lw.put(j, hit);
-                                if constexpr (kMasks) fbits |= ((r2 <= r2f) && valid ? 1u : 0u) << k;
+                                if constexpr (kMasks) fbits |= (force_pair(r2, r2f) && valid ? 1u : 0u) << k;
                             }
                             if constexpr (kMasks) cmask |= fbits << (j0 - s);
                             if constexpr (kLists) lw.drain(lists, n, i);
@@ -1006,10 +1010,10 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params one, const sph_pa
                                                     const float4* __restrict__ pos,
                                                     const uint32_t* __restrict__ cell_start,
                                                     const uint32_t* __restrict__ cell_end,
-                                                    float2* __restrict__ rhop) {
+                                                    float2* __restrict__ rhop, uint32_t* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    density_one<kBatch, kPrec>(i, one, tab, sid, pos, cell_start, cell_end, rhop);
+    density_one<kBatch, kPrec>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n, nullptr, nullptr, counters);
 }
 
 // Persistent form: a fixed grid pulls 256-slot tiles from a ticket counter.
@@ -1038,14 +1042,17 @@ template <bool kBatch, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_density_persistent(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
     const float4* __restrict__ pos, const uint32_t* __restrict__ cell_start,
-    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ work) {
+    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ work,
+    uint32_t* __restrict__ counters) {
     __shared__ uint32_t s_tile;
     const uint32_t ntiles = (n + kBlock - 1) / kBlock;
     for (;;) {
         const uint32_t tile = next_tile(work, &s_tile);
         if (tile >= ntiles) break;
         const uint32_t i = tile * kBlock + threadIdx.x;
-        if (i < n) density_one<kBatch, kPrec>(i, one, tab, sid, pos, cell_start, cell_end, rhop);
+        if (i < n)
+            density_one<kBatch, kPrec>(i, one, tab, sid, pos, cell_start, cell_end, rhop, n, nullptr, nullptr,
+                                       counters);
     }
     retire_block(work);
 }
@@ -1139,7 +1146,8 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
                                           const uint32_t* __restrict__ cell_start,
                                           const uint32_t* __restrict__ cell_end,
                                           float4* __restrict__ vel_out, float4* __restrict__ dvel,
-                                          const float* __restrict__ dt_dev, const Src* src_in = nullptr,
+                                          const float* __restrict__ dt_dev,
+                                          const uint32_t* __restrict__ counters, const Src* src_in = nullptr,
                                           const CellView* cv = nullptr, const uint32_t* __restrict__ masks = nullptr,
                                           uint32_t n = 0) {
     const int t = threadIdx.x;
@@ -1156,7 +1164,7 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
     const float4 vi = vel[ii];
     const float2 ri = rhop[ii];
     CellWalk w;
-    w.init(p, xi.x, xi.y, xi.z);
+    w.init(p, xi.x, xi.y, xi.z, walk_culled(counters));
     const float r2max = p.force_r2_max;
 
     ForceAcc a{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f},
@@ -1220,7 +1228,7 @@ __device__ __forceinline__ void force_one(uint32_t i, bool active, uint32_t* lis
                         // cell's end (cnt <= list size holds for the whole cell, and the
                         // list has one spare row for a full list); only a hit advances cnt.
                         list[cnt * kB + t] = j | tag;
-                        cnt += (r2 <= r2max && j != i && j < e) ? 1 : 0;
+                        cnt += (force_pair(r2, r2max) && j != i && j < e) ? 1 : 0;
                     }
                 }
             }
@@ -1258,11 +1266,12 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev,
+    const uint32_t* __restrict__ counters) {
     __shared__ uint32_t list[(kList + 1) * kBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
-                                    cell_end, vel_out, dvel, dt_dev);
+                                    cell_end, vel_out, dvel, dt_dev, counters);
 }
 
 template <bool kBatch, int kPrec, int kList>
@@ -1271,7 +1280,8 @@ __global__ __launch_bounds__(kBlock) void k_force_persistent(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev, uint32_t* __restrict__ work) {
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev,
+    const uint32_t* __restrict__ counters, uint32_t* __restrict__ work) {
     __shared__ uint32_t list[(kList + 1) * kBlock];
     __shared__ uint32_t s_tile;
     const uint32_t ntiles = (n + kBlock - 1) / kBlock;
@@ -1280,7 +1290,7 @@ __global__ __launch_bounds__(kBlock) void k_force_persistent(
         if (tile >= ntiles) break;
         const uint32_t i = tile * kBlock + threadIdx.x;
         force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
-                                        cell_end, vel_out, dvel, dt_dev);
+                                        cell_end, vel_out, dvel, dt_dev, counters);
     }
     retire_block(work);
 }
@@ -1331,11 +1341,12 @@ template <bool kBatch, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_density_masks(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
     const float4* __restrict__ pos, const uint32_t* __restrict__ cell_start,
-    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ masks) {
+    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint32_t* __restrict__ masks,
+    uint32_t* __restrict__ counters) {
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     density_one<kBatch, kPrec, false, PosGlobal, false, true>(i, one, tab, sid, pos, cell_start, cell_end, rhop,
-                                                             n, masks);
+                                                             n, masks, nullptr, counters);
 }
 
 // Cell-table stage: the (start, end) pairs of the block's 9 neighbour rows
@@ -1418,7 +1429,8 @@ __global__ __launch_bounds__(kBlock) void k_force_cells(
     const float4* __restrict__ pos, const uint32_t* __restrict__ skey, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev,
+    const uint32_t* __restrict__ counters) {
     __shared__ uint32_t list[(kList + 1) * kBlock];
     __shared__ CellStage S;
     const uint32_t b0 = xcd_tile(blockIdx.x, gridDim.x) * kBlock;
@@ -1428,11 +1440,11 @@ __global__ __launch_bounds__(kBlock) void k_force_cells(
     const uint32_t i = b0 + threadIdx.x;
     if (staged)
         force_one<kBatch, kPrec, kList, kBlock, PosGlobal, true>(i, i < n, list, one, tab, sid, pos, vel, rhop,
-                                                                 cell_start, cell_end, vel_out, dvel, dt_dev,
+                                                                 cell_start, cell_end, vel_out, dvel, dt_dev, counters,
                                                                  nullptr, &cv);
     else
         force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end,
-                                        vel_out, dvel, dt_dev);
+                                        vel_out, dvel, dt_dev, counters);
 }
 
 template <bool kBatch, int kPrec, int kList, int kOrder, bool kMasks = false>
@@ -1441,16 +1453,17 @@ __global__ __launch_bounds__(kBlock) void k_force_ordered(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev, const uint32_t* __restrict__ masks = nullptr) {
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev,
+    const uint32_t* __restrict__ counters, const uint32_t* __restrict__ masks = nullptr) {
     __shared__ uint32_t list[(kList + 1) * kBlock];
     const uint32_t i = (kOrder ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock + threadIdx.x;
     if constexpr (kMasks)
         force_one<kBatch, kPrec, kList, kBlock, PosGlobal, false, true>(
-            i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end, vel_out, dvel, dt_dev, nullptr,
+            i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end, vel_out, dvel, dt_dev, counters, nullptr,
             nullptr, masks, n);
     else
         force_one<kBatch, kPrec, kList>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start, cell_end,
-                                        vel_out, dvel, dt_dev);
+                                        vel_out, dvel, dt_dev, counters);
 }
 
 template <bool kBatch, int kPrec, int kList>
@@ -1459,7 +1472,8 @@ __global__ __launch_bounds__(kForceTile) void k_force_tiled(
     const float4* __restrict__ pos, const uint32_t* __restrict__ skey, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev,
+    const uint32_t* __restrict__ counters) {
     __shared__ uint32_t list[(kList + 1) * kForceTile];
     __shared__ Stage<kForceStage> S;
     const uint32_t b0 = xcd_tile(blockIdx.x, gridDim.x) * kForceTile;
@@ -1469,11 +1483,11 @@ __global__ __launch_bounds__(kForceTile) void k_force_tiled(
     if (staged) {
         const PosStaged src{S.pos, S.off};
         force_one<kBatch, kPrec, kList, kForceTile, PosStaged>(i, i < n, list, one, tab, sid, pos, vel, rhop,
-                                                               cell_start, cell_end, vel_out, dvel, dt_dev,
+                                                               cell_start, cell_end, vel_out, dvel, dt_dev, counters,
                                                                &src);
     } else {
         force_one<kBatch, kPrec, kList, kForceTile>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
-                                                    cell_end, vel_out, dvel, dt_dev);
+                                                    cell_end, vel_out, dvel, dt_dev, counters);
     }
 }
 
@@ -1485,7 +1499,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float2* __restrict__ rhop, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_end, float4* __restrict__ vel_out,
-    float4* __restrict__ dvel, const float* __restrict__ dt_dev) {
+    float4* __restrict__ dvel, const float* __restrict__ dt_dev,
+    const uint32_t* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t scene;
@@ -1494,7 +1509,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
     const float4 vi = vel[i];
     const float2 ri = rhop[i];
     CellWalk w;
-    w.init(p, xi.x, xi.y, xi.z);
+    w.init(p, xi.x, xi.y, xi.z, walk_culled(counters));
     const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
     const float vterm = p.lap_kern * p.viscosity;
     const float r2max = p.force_r2_max;
@@ -1510,7 +1525,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(
             const float4 xj = pos[j];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
             const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 <= r2max) {
+            if (force_pair(r2, r2max)) {
                 const float4 vj = vel[j];
                 const float2 rj = rhop[j];
                 const float r = fmaxf(min_dist, sqrt_ieee(r2));
@@ -1744,7 +1759,7 @@ void density_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
     const uint32_t grid = c.tune[1];
     if (use_masks(c, b))
         hipLaunchKernelGGL((k_density_masks<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
-                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop, b.masks);
+                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop, b.masks, b.counters);
     else if (!grid && c.tune[3] == 2 && b.skey)
         hipLaunchKernelGGL((k_density_cells<kBatch, kPrec, false>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock),
                            0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.cell_start, b.cell_end,
@@ -1760,11 +1775,11 @@ void density_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
     else if (grid)
         hipLaunchKernelGGL((k_density_persistent<kBatch, kPrec>), dim3(grid), dim3(kBlock), 0, s,
                            c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop,
-                           b.work);
+                           b.work, b.counters);
     else
         hipLaunchKernelGGL((k_density<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0,
                            s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end,
-                           b.rhop);
+                           b.rhop, b.counters);
 }
 
 template <bool kBatch, int kPrec, int kList>
@@ -1774,23 +1789,23 @@ void force_list_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, 
     if (!grid && c.tune[3] == 2 && b.skey)
         hipLaunchKernelGGL((k_force_cells<kBatch, kPrec, kList>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0,
                            s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.vel_cur, b.rhop, b.cell_start,
-                           b.cell_end, vel_out, dvel, b.dt_dev);
+                           b.cell_end, vel_out, dvel, b.dt_dev, b.counters);
     else if (!grid && c.tune[3] == 1 && b.skey)
         hipLaunchKernelGGL((k_force_tiled<kBatch, kPrec, kList>), dim3(blocks_for(c.n, kForceTile)),
                            dim3(kForceTile), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.skey, b.vel_cur,
-                           b.rhop, b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+                           b.rhop, b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.counters);
     else if (!grid && c.tune[3] == 0)
         hipLaunchKernelGGL((k_force_ordered<kBatch, kPrec, kList, 1>), dim3(blocks_for(c.n, kBlock)),
                            dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
-                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.counters);
     else if (grid)
         hipLaunchKernelGGL((k_force_persistent<kBatch, kPrec, kList>), dim3(grid), dim3(kBlock), 0, s,
                            c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop, b.cell_start,
-                           b.cell_end, vel_out, dvel, b.dt_dev, b.work + 2);
+                           b.cell_end, vel_out, dvel, b.dt_dev, b.counters, b.work + 2);
     else
         hipLaunchKernelGGL((k_force<kBatch, kPrec, kList>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock),
                            0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
-                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.counters);
 }
 
 template <bool kBatch, int kPrec>
@@ -1799,7 +1814,7 @@ void force_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float
     if (use_masks(c, b)) {
         hipLaunchKernelGGL((k_force_ordered<kBatch, kPrec, kForceListMasks, 1, true>), dim3(blocks_for(c.n, kBlock)),
                            dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
-                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.masks);
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.counters, b.masks);
         return;
     }
     uint32_t list = c.tune[0] ? c.tune[0] : (uint32_t)kForceListDefault;
@@ -1810,7 +1825,7 @@ void force_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float
         // cell could overflow any neighbour list
         hipLaunchKernelGGL((k_force_fused<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock),
                            0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,
-                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev);
+                           b.cell_start, b.cell_end, vel_out, dvel, b.dt_dev, b.counters);
         return;
     }
     switch (list) {
