@@ -118,6 +118,7 @@ struct Engine {
     uint32_t* masks = nullptr;  // force pair masks (kMaskWords per slot) while cells hold <= 16 candidates
     uint4* fast_nbr = nullptr;  // SPH_PREC_FAST: force pair lists (fast_groups uint4 per slot, group-major)
     uint32_t* fast_nnb = nullptr;
+    float4* fast_rec = nullptr;  // SPH_PREC_FAST: force records, 2 per slot
     uint32_t fast_groups = 0;
     float4* colour = nullptr;   // per sorted slot, scenes with clr_type != NONE
     float* dye = nullptr;       // per ORIGINAL index (it follows the particle)
@@ -208,7 +209,7 @@ struct Engine {
             (void)hipFree(cell_end[k]);
             (void)hipFree(skey_buf[k]);
         }
-        void* ptrs[] = {rhop,     dvel,     masks,    fast_nbr, fast_nnb, colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
+        void* ptrs[] = {rhop,     dvel,     masks,    fast_nbr, fast_nnb, fast_rec, colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
                         counters, cell_max, work,    partials,
                         stats,    dt_dev,   staging,  sort_scratch, resort_scratch, d_tab, d_scene_off};
         for (void* q : ptrs) (void)hipFree(q);
@@ -371,6 +372,7 @@ struct Engine {
         fast_groups = 0;
         hipError_t e = dalloc(&fast_nbr, (size_t)g * cfg.n);
         if (e == hipSuccess && !fast_nnb) e = dalloc(&fast_nnb, (size_t)cfg.n);
+        if (e == hipSuccess && !fast_rec) e = dalloc(&fast_rec, 2 * (size_t)cfg.n);
         if (e == hipSuccess) fast_groups = g;
         return e;
     }
@@ -404,13 +406,13 @@ struct Engine {
         if (timed) SPH_TRY(hipEventRecord(ev[3], stream));
         const bool fast = cfg.precision == SPH_PREC_FAST;
         if (fast)
-            SPH_TRY(launch_density_fast(c, buffers(), fast_nbr, fast_nnb, stream));
+            SPH_TRY(launch_density_fast(c, buffers(), fast_rec, fast_nbr, fast_nnb, stream));
         else
             SPH_TRY(launch_density(c, buffers(), stream));
         if (timed) SPH_TRY(hipEventRecord(ev[4], stream));
         StepBuffers b = buffers();
         if (fast)
-            SPH_TRY(launch_force_fast(c, b, fast_nbr, fast_nnb, b.vel_alt, keep || cfg.colour, stream));
+            SPH_TRY(launch_force_fast(c, b, fast_rec, fast_nbr, fast_nnb, b.vel_alt, keep || cfg.colour, stream));
         else
             SPH_TRY(launch_force(c, b, b.vel_alt, keep || cfg.colour, stream));
         if (fast && cfg.physics) SPH_TRY(launch_extra_terms(c, b, b.vel_alt, keep || cfg.colour, stream));

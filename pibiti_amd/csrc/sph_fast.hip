@@ -40,15 +40,31 @@ constexpr uint32_t kRing = 4;  // ring entries per lane (a group adds at most on
 __device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Density + EOS and the force pair list of one slot per thread.
+// Density + EOS, the force pair list and the force record of one slot per
+// thread.
+//
+// The walk goes by rows: the neighbour cells (dx = -1, 0, 1) of one (dy, dz)
+// have consecutive keys, so their slots form one run, and the candidates of
+// a row are that run, four consecutive slots at a time.  A run breaks only
+// after a cell capped at max_par_in_cell (its slots past the cap are not
+// candidates); the slots before the break are walked as a run of their own.
+// The lanes of one cell then walk the same runs in step and load the same
+// slots, and a row costs one loop instead of three.  Culling drops a side
+// cell of a row whose box is farther than h (the middle cell of a row that
+// is kept is never farther than the row).
+//
+// The force record of slot i is two float4: (x, y, z, 1/rho) and
+// (vx, vy, vz, p), everything the force reads of a particle in one aligned
+// 32-byte block.
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const sph_params* __restrict__ tab,
                                                          const uint32_t* __restrict__ sid, uint32_t n,
                                                          const float4* __restrict__ pos,
+                                                         const float4* __restrict__ vel,
                                                          const uint32_t* __restrict__ cell_start,
                                                          const uint32_t* __restrict__ cell_end,
-                                                         float2* __restrict__ rhop, uint4* __restrict__ nbr,
-                                                         uint32_t* __restrict__ nnb,
+                                                         float2* __restrict__ rhop, float4* __restrict__ rec,
+                                                         uint4* __restrict__ nbr, uint32_t* __restrict__ nnb,
                                                          const uint32_t* __restrict__ counters) {
     __shared__ uint32_t ring[kRing * kBlock];
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -62,72 +78,104 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     uint32_t* const slot0 = ring + threadIdx.x;
     uint32_t count = 0, flushed = 0, pairs = 0;
     float sum = 0.0f;
+    // The candidates [s, e) of one run, four at a time: a candidate past the
+    // run's end (and, in the particle's own row, the particle itself) takes
+    // r2 = +inf, which fails both predicates.  The density weight
+    // max(h2 - r2, 0)^3 is the reference's (r2 < h2 ? (h2 - r2)^3 : 0),
+    // also for a NaN r2 (max gives 0).
+    auto walk_run = [&](uint32_t s, uint32_t e, auto self_row) {
+        constexpr bool kSelfRow = decltype(self_row)::value;
+        for (uint32_t j0 = s; j0 < e; j0 += 4) {
+            // four loads in flight; position arrays are padded past their end
+            float4 xq[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                xq[k] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pos) +
+                                                         ((j0 + (uint32_t)k) << 4));
+            const uint32_t left = e - j0;
+            uint32_t bits = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                // the predicates need the reference's r2 exactly
+                const float rx = xi.x - xq[k].x, ry = xi.y - xq[k].y, rz = xi.z - xq[k].z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                bool valid = left > (uint32_t)k;
+                if constexpr (kSelfRow) valid = valid && j0 + (uint32_t)k != i;
+                const float r2v = valid ? r2 : __builtin_huge_valf();
+                const float c = fmaxf(h2 - r2v, 0.0f);
+                sum = __builtin_fmaf(c * c, c, sum);
+                bits |= force_pair(r2v, r2f) ? (1u << k) : 0u;
+            }
+            // one entry per group with a pair; the written slot is only
+            // kept when the count advances
+            slot0[(count & (kRing - 1u)) * kBlock] = (j0 << 4) | bits;
+            count += bits ? 1u : 0u;
+            pairs += (uint32_t)__popc(bits);
+            if (count - flushed == kRing) {
+                nbr[(size_t)(flushed >> 2) * n + i] =
+                    make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
+                flushed += kRing;
+            }
+        }
+    };
 #pragma unroll 1
     for (int dz = -1; dz <= 1; ++dz) {
 #pragma unroll 1
         for (int dy = -1; dy <= 1; ++dy) {
             const CellWalk::Row row = w.row(dy, dz);
+            if (!row.ok) continue;
+            const bool own = (dy | dz) == 0;
+            uint32_t rs = 0, re = 0;  // the run so far (empty: rs == re)
 #pragma unroll 1
             for (int dx = -1; dx <= 1; ++dx) {
                 uint32_t s, e;
                 if (!w.open_in(row, dx, cell_start, cell_end, s, e)) continue;
-                // the particle itself is a candidate of its own cell only
-                const uint32_t self = (dx | dy | dz) == 0 ? i : 0xFFFFFFFFu;
-                for (uint32_t j0 = s; j0 < e; j0 += 4) {
-                    // four loads in flight; position arrays are padded past
-                    // their end, and indices past the cell's end are masked
-                    float4 xq[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        xq[k] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pos) +
-                                                                 ((j0 + (uint32_t)k) << 4));
-                    uint32_t bits = 0;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const uint32_t j = j0 + (uint32_t)k;
-                        const bool valid = j < e && j != self;
-                        // the predicates need the reference's r2 exactly
-                        const float rx = xi.x - xq[k].x, ry = xi.y - xq[k].y, rz = xi.z - xq[k].z;
-                        const float r2 = rx * rx + ry * ry + rz * rz;
-                        const float c = h2 - r2;
-                        sum += (valid && r2 < h2) ? (c * c) * c : 0.0f;
-                        bits |= (valid && force_pair(r2, r2f)) ? (1u << k) : 0u;
-                    }
-                    // one entry per group with a pair; the written slot is
-                    // only kept when the count advances
-                    slot0[(count & (kRing - 1u)) * kBlock] = (j0 << 4) | bits;
-                    count += bits ? 1u : 0u;
-                    pairs += (uint32_t)__popc(bits);
-                    if (count - flushed == kRing) {
-                        nbr[(size_t)(flushed >> 2) * n + i] =
-                            make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
-                        flushed += kRing;
-                    }
+                if (s != re) {
+                    // a break: a capped cell before (or the row's first open cell)
+                    if (own)
+                        walk_run(rs, re, std::true_type{});
+                    else
+                        walk_run(rs, re, std::false_type{});
+                    rs = s;
                 }
+                re = e;
             }
+            if (own)
+                walk_run(rs, re, std::true_type{});
+            else
+                walk_run(rs, re, std::false_type{});
         }
     }
     if (count > flushed)
         nbr[(size_t)(flushed >> 2) * n + i] = make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
     nnb[i] = count | (pairs << 16);
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
-    rhop[i] = make_float2(rho, eos(p, rho));
+    const float pr = eos(p, rho);
+    rhop[i] = make_float2(rho, pr);
+    const float4 vi = vel[i];
+    rec[2 * (size_t)i] = make_float4(xi.x, xi.y, xi.z, rcp_hw(rho));
+    rec[2 * (size_t)i + 1] = make_float4(vi.x, vi.y, vi.z, pr);
 }
 
 // ------------------------------------------------------------------ force
 constexpr int kFT = 256;  // slots per force block
 
-// The pair term of the force update, accumulated into (ax, ay, az).
-__device__ __forceinline__ void pair_term(const float4& xi, const float4& vi, float inv_ri, float pi,
-                                          const float4& xj, const float4& vj, float rhoj, float pj, float h,
-                                          float spiky, float vterm, float min_dist, float min_dens, float& ax,
-                                          float& ay, float& az) {
+// The pair term of the force update from the force records of i (xi, vi)
+// and j (xj, vj), accumulated into (ax, ay, az).  1/r and r come from one
+// reciprocal square root: r = max(minDist, r2 / sqrt(r2)) and
+// 1/r = min(1/minDist, 1/sqrt(r2)), which also give r = minDist for r2 = 0
+// and for a NaN r2, as max(minDist, sqrt(r2)) does.
+__device__ __forceinline__ void pair_term(const float4& xi, const float4& vi, const float4& xj, const float4& vj,
+                                          float h, float spiky, float vterm, float min_dist, float inv_min_dist,
+                                          float min_dens, float& ax, float& ay, float& az) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-    const float r = fmaxf(min_dist, sqrt_hw(r2));
-    const float d12 = fminf(min_dens, inv_ri * rcp_hw(rhoj));
+    const float rs = __builtin_amdgcn_rsqf(r2);
+    const float r = fmaxf(min_dist, r2 * rs);
+    const float inv_r = fminf(inv_min_dist, rs);
+    const float d12 = fminf(min_dens, xi.w * xj.w);
     const float cc = h - r;
-    const float pterm = (cc * spiky) * (pi + pj) * rcp_hw(r);
+    const float pterm = (cc * spiky) * (vi.w + vj.w) * inv_r;
     const float cd = cc * d12;
     ax = __builtin_fmaf(__builtin_fmaf(pterm, rx, vterm * (vj.x - vi.x)), cd, ax);
     ay = __builtin_fmaf(__builtin_fmaf(pterm, ry, vterm * (vj.y - vi.y)), cd, ay);
@@ -136,15 +184,14 @@ __device__ __forceinline__ void pair_term(const float4& xi, const float4& vi, fl
 
 // Force + velocity update of one slot per thread over its listed pairs,
 // four pairs per iteration: the pairs are decoded first, so that their
-// twelve loads go out together.  A lane's pairs come in the walk's order
-// (ascending slots of the same 27 cells for the lanes of one cell), which
-// keeps the lanes of a wave on nearby slots.
+// eight loads go out together.  A lane's pairs come in the walk's order
+// (ascending slots of the same rows for the lanes of one cell), which keeps
+// the lanes of a wave on nearby records.
 template <bool kBatch>
 __global__ __launch_bounds__(kFT) void k_force_fast(sph_params one, const sph_params* __restrict__ tab,
                                                     const uint32_t* __restrict__ sid, uint32_t n,
-                                                    const float4* __restrict__ pos,
                                                     const float4* __restrict__ vel,
-                                                    const float2* __restrict__ rhop,
+                                                    const float4* __restrict__ rec,
                                                     const uint4* __restrict__ nbr,
                                                     const uint32_t* __restrict__ nnb,
                                                     float4* __restrict__ vel_out, float4* __restrict__ dvel,
@@ -153,12 +200,11 @@ __global__ __launch_bounds__(kFT) void k_force_fast(sph_params one, const sph_pa
     if (i >= n) return;
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
-    const float4 xi = pos[i];
-    const float4 vi = vel[i];
-    const float2 ri = rhop[i];
+    const float4 xi = rec[2 * (size_t)i];
+    const float4 vi = rec[2 * (size_t)i + 1];
     const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
+    const float inv_min_dist = 1.0f / min_dist;
     const float vterm = p.lap_kern * p.viscosity;
-    const float inv_ri = rcp_hw(ri.x);
     const uint32_t npairs = nnb[i] >> 16;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     uint4 buf = make_uint4(0, 0, 0, 0);
@@ -181,18 +227,15 @@ __global__ __launch_bounds__(kFT) void k_force_fast(sph_params one, const sph_pa
 #pragma unroll
         for (int k = 1; k < 4; ++k) j[k] = t + (uint32_t)k < npairs ? next_pair() : j[0];
         float4 xq[4], vq[4];
-        float2 rq[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            xq[k] = pos[j[k]];
-            vq[k] = vel[j[k]];
-            rq[k] = rhop[j[k]];
+            xq[k] = rec[2 * (size_t)j[k]];
+            vq[k] = rec[2 * (size_t)j[k] + 1];
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             float bx = ax, by = ay, bz = az;
-            pair_term(xi, vi, inv_ri, ri.y, xq[k], vq[k], rq[k].x, rq[k].y, h, spiky, vterm, min_dist, min_dens, bx,
-                      by, bz);
+            pair_term(xi, vi, xq[k], vq[k], h, spiky, vterm, min_dist, inv_min_dist, min_dens, bx, by, bz);
             // past the last pair (a repeat of the first of the four): nothing added
             const bool on = t + (uint32_t)k < npairs;
             ax = on ? bx : ax;
@@ -209,7 +252,8 @@ __global__ __launch_bounds__(kFT) void k_force_fast(sph_params one, const sph_pa
         vel_out[i] = make_float4(ax, ay, az, 0.0f);
         return;
     }
-    vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vi.w + 0.0f);
+    // w from the state (the record's w holds the pressure)
+    vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vel[i].w + 0.0f);
     if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
 
@@ -219,29 +263,29 @@ inline uint32_t blocks_for(uint32_t n) { return n == 0 ? 1u : (n + kBlock - 1) /
 
 uint32_t fast_list_groups(uint32_t max_par_in_cell) { return (27u * ((max_par_in_cell + 3u) / 4u) + 3u) / 4u; }
 
-hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, uint4* nbr, uint32_t* nnb,
+hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, float4* rec, uint4* nbr, uint32_t* nnb,
                                hipStream_t s) {
     const dim3 grid(blocks_for(c.n)), block(kBlock);
     if (c.tab)
         hipLaunchKernelGGL(k_density_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.cell_start, b.cell_end, b.rhop, nbr, nnb, b.counters);
+                           b.vel_cur, b.cell_start, b.cell_end, b.rhop, rec, nbr, nnb, b.counters);
     else
         hipLaunchKernelGGL(k_density_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.cell_start, b.cell_end, b.rhop, nbr, nnb, b.counters);
+                           b.vel_cur, b.cell_start, b.cell_end, b.rhop, rec, nbr, nnb, b.counters);
     return hipGetLastError();
 }
 
-hipError_t launch_force_fast(const LaunchCfg& c, const StepBuffers& b, const uint4* nbr, const uint32_t* nnb,
-                             float4* vel_out, bool keep_dvel, hipStream_t s) {
+hipError_t launch_force_fast(const LaunchCfg& c, const StepBuffers& b, const float4* rec, const uint4* nbr,
+                             const uint32_t* nnb, float4* vel_out, bool keep_dvel, hipStream_t s) {
     if (c.n == 0) return hipErrorInvalidValue;
     float4* dvel = keep_dvel ? b.dvel : nullptr;
     const dim3 grid((c.n + kFT - 1) / kFT), block(kFT);
     if (c.tab)
-        hipLaunchKernelGGL(k_force_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.vel_cur, b.rhop, nbr, nnb, vel_out, dvel, b.dt_dev);
+        hipLaunchKernelGGL(k_force_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.vel_cur, rec,
+                           nbr, nnb, vel_out, dvel, b.dt_dev);
     else
-        hipLaunchKernelGGL(k_force_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.vel_cur, b.rhop, nbr, nnb, vel_out, dvel, b.dt_dev);
+        hipLaunchKernelGGL(k_force_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.vel_cur, rec,
+                           nbr, nnb, vel_out, dvel, b.dt_dev);
     return hipGetLastError();
 }
 
