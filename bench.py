@@ -53,6 +53,11 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # dense
 
 
+def progress(msg: str) -> None:
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def pair_flops(pairs: float, n: int) -> float:
     """Algorithmic FLOPs of one surrogate pass: both pair layers for every
     listed pair over the 9 real edge inputs (9 -> 64 -> 64, 2 FLOPs per
@@ -70,8 +75,9 @@ def learned_section(args, p, snap_pos, snap_vel) -> dict:
     n = int(p.num_particles)
     rho0 = float(p.rest_density)
     factor = 4
-    fine = 1000
+    fine = args.learned_dt
     coarse = fine // factor
+    progress(f"learned: training ({args.train_steps} steps, {args.train_iters} updates per window)")
     model = sph.Model(learn.default_norm(p), seed=1)
     hist = learn.train(model, scenes_count=200, steps=args.train_steps, stride=8, iters=args.train_iters,
                        lr=args.train_lr, lr_end=1e-4)
@@ -92,6 +98,7 @@ def learned_section(args, p, snap_pos, snap_vel) -> dict:
         return {"rms": float(np.sqrt(np.mean(d * d))), "max": float(d.max())}
 
     # the plain run at dt: reference trajectory and throughput baseline
+    progress(f"learned: plain run over {fine} dt")
     with fresh() as c:
         c.synchronize()
         t0 = time.perf_counter()
@@ -109,6 +116,7 @@ def learned_section(args, p, snap_pos, snap_vel) -> dict:
         c.update(1)
         one = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_ref1)
     # the whole workload, with the 1 % drift fallback
+    progress(f"learned: surrogate run, {coarse} steps of {factor} dt")
     with fresh() as c:
         # zero steps: allocates the surrogate's buffers outside the timed region
         c.step_learned(model, 0, factor=factor, tol=args.tol, skin=args.skin)
@@ -119,6 +127,7 @@ def learned_section(args, p, snap_pos, snap_vel) -> dict:
         secs = time.perf_counter() - t0
         run = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_plain_end)
     # the plain step at factor x dt over the same workload
+    progress("learned: plain step at 4 dt, zero network, kernel timings")
     big = q.copy()
     big.time_step = factor * float(p.time_step)
     with fresh() as c:
@@ -385,6 +394,8 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-modes", action="store_true", help="skip the second precision mode")
+    ap.add_argument("--learned-dt", type=int, default=1000,
+                    help="length of the surrogate workload in plain time steps (a multiple of 4)")
     ap.add_argument("--train-steps", type=int, default=1600)
     ap.add_argument("--train-iters", type=int, default=32)
     ap.add_argument("--train-lr", type=float, default=3e-3)
@@ -403,6 +414,7 @@ def main() -> int:
               "(the multi-GPU decomposition is a design, not a build)", file=sys.stderr)
         return 2
 
+    progress(f"plain step, {args.prec} mode: {args.warmup} warm-up and {args.steps} timed steps")
     head = plain_run(args.prec, args)
     p, n, ms, st, info = head["p"], head["n"], head["ms"], head["stats"], head["info"]
     ab = algorithmic_bytes(p)
@@ -434,6 +446,7 @@ def main() -> int:
     modes = {args.prec: mode_summary(head)}
     if not args.no_modes:
         other = "reference" if args.prec != "reference" else "fast"
+        progress(f"plain step, {other} mode")
         modes[other] = mode_summary(plain_run(other, args))
     modes["note"] = ("fast: integer results exact, floats within 1e-5 of the reference arithmetic and of FP64 "
                      "(condition-scaled); reference: bit-exact with the FP32 restatement of the reference")
@@ -475,8 +488,10 @@ def main() -> int:
     if not args.no_learned and args.warmup > 0:
         out["learned"] = learned_section(args, p, head["snap_pos"], head["snap_vel"])
     if not args.no_batch:
+        progress("batch of 64 scenes")
         out["batch"] = batch_section()
     if not args.no_cpu:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget, p, head["pos"], head["vel"])
     if args.copy_calibration:
         copy_calibration()
