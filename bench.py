@@ -200,14 +200,15 @@ def learned_section(args, p, snap_pos, snap_vel) -> dict:
     }
 
 
-def batch_section() -> dict:
+def batch_section(prec_name: str) -> dict:
     """The batched workload: 64 independent dam breaks with particle counts
     log-uniform in [2k, 200k] (seeded), boxes scaled to each count, advanced
-    as one particle array.  The autotuner picks the launch shape on the live
-    state, 10 steps are captured in a hipGraph, and 10 simulated ms per scene
-    (100 steps of 1e-4 s) are timed as 10 graph replays."""
+    as one particle array in the given precision mode.  The autotuner picks
+    the launch shape on the live state, 10 steps are captured in a hipGraph,
+    and 10 simulated ms per scene (100 steps of 1e-4 s) are timed as 10
+    graph replays."""
     counts = scenes.batch_counts(64)
-    sc = [scenes.batch_scene(c, i) for i, c in enumerate(counts)]
+    sc = [scenes.batch_scene(c, i, precision=PREC[prec_name]) for i, c in enumerate(counts)]
     params = [q for q, _, _, _ in sc]
     total = sum(int(q.num_particles) for q in params)
     with sph.SphBatch(params) as b:
@@ -226,7 +227,7 @@ def batch_section() -> dict:
     return {
         "workload": "64 dam breaks, particle counts log-uniform in [2k, 200k] (seed 5), spacing 0.02, "
                     "h 0.04, dt 1e-4, 10 simulated ms per scene as 10 replays of a 10-step hipGraph",
-        "scenes": len(sc), "particles": total, "cells": info["cells"], "steps": 100,
+        "precision": prec_name, "scenes": len(sc), "particles": total, "cells": info["cells"], "steps": 100,
         "value": total * 100 / secs, "unit": "particle-updates/s", "ms_per_step": secs * 1e3 / 100,
         "tune": tune, "autotuned_ms_per_step": round(tuned_ms, 5),
         "min_scene": min(counts), "max_scene": max(counts),
@@ -488,8 +489,13 @@ def main() -> int:
     if not args.no_learned and args.warmup > 0:
         out["learned"] = learned_section(args, p, head["snap_pos"], head["snap_vel"])
     if not args.no_batch:
-        progress("batch of 64 scenes")
-        out["batch"] = batch_section()
+        progress(f"batch of 64 scenes, {args.prec} mode")
+        out["batch"] = batch_section(args.prec)
+        other = "reference" if args.prec == "fast" else "fast"
+        if not args.no_modes:
+            progress(f"batch of 64 scenes, {other} mode")
+            out["batch"]["other_mode"] = {k: v for k, v in batch_section(other).items()
+                                          if k in ("precision", "value", "ms_per_step", "tune")}
     if not args.no_cpu:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget, p, head["pos"], head["vel"])
