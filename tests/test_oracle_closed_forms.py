@@ -350,3 +350,55 @@ def test_f32_reference_vs_f64_within_scaled_tolerance():
     assert scaled_err(k["dens"], d["dens"], np.abs(d["dens"])) < 1e-5
     # skip_integrate keeps the float inputs: the FP64 positions are the FP32 ones
     assert np.array_equal(d["pos"], f["pos"].astype(np.float64))
+
+
+def test_nan_position_is_a_force_pair_of_its_27_cells():
+    """The reference's force predicate is max(minDist, |r|) < h
+    (Kernel_Cell.cui:214-216), and the max of minDist and NaN is minDist: a
+    particle at a NaN position is a force pair of every particle whose 27
+    cells hold its (clamped) cell, at any distance, and of no one else.  It
+    passes no density test (NaN < h2 is false).  Pinned on the oracle against
+    cell coordinates decoded from its keys: the force-pair counts rise by
+    one exactly there, the density counts do not change, and one step turns
+    exactly those particles' velocities NaN."""
+    p, pos, vel, _ = scenes.dam_break(9, 8, 9, spacing=0.02, h=0.04, world_max=(1, 1, 1),
+                                      dt=1e-4, viscosity=0.1, seed=6, gravity=(0.0, 0.0, 0.0))
+    pos[:, 1:3] += np.float32(0.3)
+    n = len(pos)
+    gx, gy = int(p.grid_size[0]), int(p.grid_size[1])
+    h_clean, _ = oracle.hash_f32(p, pos)
+    bad = int(np.flatnonzero((h_clean % gx) == 1)[7])  # a particle of the second cell column in x
+    x = pos.copy()
+    x[bad, 0] = np.nan
+    keys, oob = oracle.hash_f32(p, x)
+    assert oob == 1 and keys[bad] % gx == 0  # a NaN coordinate clamps to cell 0
+    c = np.stack([keys % gx, (keys // gx) % gy, keys // (gx * gy)], 1).astype(np.int64)
+    near = np.all(np.abs(c - c[bad]) <= 1, axis=1)
+    near[bad] = False
+    others = np.arange(n) != bad
+    assert 5 < near.sum() < n // 4
+
+    def counts(positions, pred):
+        g = oracle.grid_of(p, positions)
+        occ = g["cell_start"] != EMPTY
+        assert (g["cell_end"][occ] - g["cell_start"][occ]).max() < p.max_par_in_cell
+        cnt = oracle.neighbour_counts_f32(p, g["spos"], g["cell_start"], g["cell_end"], pred=pred)
+        out = np.empty(len(positions), np.int64)
+        out[g["perm"]] = cnt
+        return out
+
+    keep = np.delete(np.arange(n), bad)
+    for pred, rise in ((1, near), (0, np.zeros(n, bool))):
+        with_bad = counts(x, pred)
+        without = np.zeros(n, np.int64)
+        without[keep] = counts(x[keep], pred)
+        assert np.array_equal(with_bad[others] - without[others], rise[others].astype(np.int64)), pred
+    # the particle itself: every candidate of its 27 cells passes the force test, none the density test
+    assert counts(x, 1)[bad] == near.sum() and counts(x, 0)[bad] == 0
+
+    v = vel.copy()
+    v[:, :3] = 0.0
+    o = oracle.step_f32(p, x, v)
+    nanv = np.isnan(o["vel"][:, :3]).any(axis=1)
+    assert nanv[bad]
+    assert np.array_equal(nanv[others], near[others])
