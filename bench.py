@@ -144,6 +144,10 @@ def learned_section(args, p, snap_pos, snap_vel) -> dict:
     with fresh() as c:
         c.step_learned(zero, coarse, factor=factor, tol=1e9, skin=args.skin)
         run_zero = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_plain_end)
+    with fresh() as c:
+        c.step_learned(zero, 1, factor=factor, tol=1e9, skin=args.skin)
+        c.update(1)
+        one_zero = err(c.get_array(sph.SPH_ARRAY_DENSITY), rho_ref1)
     zero.close()
     # kernel times (HIP events) over a few more surrogate steps
     with fresh() as c:
@@ -181,7 +185,7 @@ def learned_section(args, p, snap_pos, snap_vel) -> dict:
                                "sim_time_speedup": round((fine / big_secs) / (fine / plain_secs), 3),
                                "note": "the plain step at 4 x dt over the same workload: the method without "
                                        "the network"},
-        "zero_network": {"density_error_workload": run_zero,
+        "zero_network": {"density_error_workload": run_zero, "density_error_one_step": one_zero,
                          "note": "the surrogate step with every weight zero: no pair force at all"},
         "meets_1pct": run["rms"] <= 0.01,
         "beats_plain_at_factor_dt": run["rms"] < run_plain_big["rms"] and speedup > (fine / big_secs) / (fine / plain_secs),
