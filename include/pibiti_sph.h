@@ -611,6 +611,12 @@ int sph_radix_sort_noalloc(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t
 enum { SPH_DEBUG_MOVER_COUNT = 1, SPH_DEBUG_CELL_END = 2, SPH_DEBUG_ORDER_ERROR = 3 };
 int sph_debug_poke(sph_ctx* ctx, int what, uint32_t index, uint32_t value);
 
+/* Testing hook (no counterpart in the reference): makes the library's
+ * after-th device allocation from now on fail with SPH_ERR_OUT_OF_MEMORY,
+ * as a full device would (after = 1: the next one); 0 disarms it.  It fires
+ * once and disarms itself.  Process-wide.                                  */
+int sph_debug_fail_alloc(uint32_t after);
+
 /* ------------------------------------------------------------------------ */
 /* Self-test of the BF16 matrix-core operand layouts the learned surrogate
  * uses (no counterpart in the reference).  variant 0: A 32x16, B 16x32,

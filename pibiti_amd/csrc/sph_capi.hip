@@ -1,10 +1,29 @@
 // sph_capi.hip - extern "C" entry points of include/pibiti_sph.h for single
 // scenes and batches; the device context is sph::Engine (sph_engine.hpp).
+#include <atomic>
 #include <cstring>
 
 #include "sph_engine.hpp"
 
 using namespace sph;
+
+namespace sph {
+namespace {
+// sph_debug_fail_alloc: the allocation that takes the count from 1 to 0 fails.
+std::atomic<uint32_t> fail_alloc_countdown{0};
+}  // namespace
+
+hipError_t device_malloc(void** ptr, size_t bytes) {
+    uint32_t k = fail_alloc_countdown.load();
+    while (k > 0 && !fail_alloc_countdown.compare_exchange_weak(k, k - 1)) {
+    }
+    if (k == 1) {
+        *ptr = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(ptr, bytes);
+}
+}  // namespace sph
 
 extern "C" {
 
@@ -101,6 +120,11 @@ int sph_synchronize(sph_ctx* c) {
 int sph_debug_poke(sph_ctx* c, int what, uint32_t index, uint32_t value) {
     if (!c) return SPH_ERR_INVALID_ARGUMENT;
     return c->eng.debug_poke(what, index, value);
+}
+
+int sph_debug_fail_alloc(uint32_t after) {
+    fail_alloc_countdown.store(after);
+    return SPH_OK;
 }
 
 int sph_set_timing(sph_ctx* c, int enable) {

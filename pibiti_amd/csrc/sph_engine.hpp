@@ -63,7 +63,7 @@ inline int to_status(hipError_t e) {
 // of four position loads may run up to three records past the last slot.
 template <typename T>
 inline hipError_t dalloc(T** ptr, size_t count) {
-    return hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 64);
+    return device_malloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 64);
 }
 
 // Slot ranges of the sort's 27-bit slot field bound the particle count.
@@ -303,9 +303,9 @@ struct Engine {
         if (e == hipSuccess) e = dalloc(&staging, N);
         if (e == hipSuccess) e = dalloc(&d_scene_off, (size_t)nscenes + 1);
         if (e == hipSuccess && as_batch) e = dalloc(&d_tab, (size_t)nscenes);
-        if (e == hipSuccess) e = hipMalloc(&sort_scratch, radix_scratch_bytes((uint32_t)N));
+        if (e == hipSuccess) e = device_malloc(&sort_scratch, radix_scratch_bytes((uint32_t)N));
         if (e == hipSuccess) e = radix_plan_init(sort, (uint32_t)N, sort_scratch, stream);
-        if (e == hipSuccess) e = hipMalloc(&resort_scratch, resort_scratch_bytes((uint32_t)N, (uint32_t)NC));
+        if (e == hipSuccess) e = device_malloc(&resort_scratch, resort_scratch_bytes((uint32_t)N, (uint32_t)NC));
         if (e == hipSuccess) e = resort_plan_init(resort, (uint32_t)N, (uint32_t)NC, resort_scratch, stream);
         // one error word for the sort and the re-sort
         sort.err = resort.err;
@@ -351,10 +351,10 @@ struct Engine {
         cfg.colour = 0;
         for (const auto& sc : scenes)
             if (sc.clr_type != SPH_CLR_NONE) cfg.colour = 1;
-        if (!cfg.colour || colour) return hipSuccess;
+        if (!cfg.colour || (colour && dye && dvel)) return hipSuccess;
         const size_t N = cfg.n;
-        hipError_t e = dalloc(&colour, N);
-        if (e == hipSuccess) e = dalloc(&dye, N);
+        hipError_t e = colour ? hipSuccess : dalloc(&colour, N);
+        if (e == hipSuccess && !dye) e = dalloc(&dye, N);
         if (e == hipSuccess && !dvel) e = dalloc(&dvel, N);
         if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(colour), 0u, 4 * N, stream);
         if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dye), 0u, N, stream);
@@ -429,6 +429,15 @@ struct Engine {
         return true;
     }
 
+    // The buffers the configured step needs exist: a lazy allocation that
+    // failed (set_params to FAST or to a colour mode under a full device)
+    // leaves the context unable to step until a later call allocates them.
+    bool buffers_ok() const {
+        if (cfg.precision == SPH_PREC_FAST && (!fast_nbr || !fast_nnb || !fast_rec)) return false;
+        if (cfg.colour && (!colour || !dye || !dvel)) return false;
+        return true;
+    }
+
     void after_steps(uint32_t k) {
         steps += k;
         if (!device_clock) host_clock += (double)scenes[0].time_step * k;
@@ -437,6 +446,7 @@ struct Engine {
 
     int step(uint32_t nsteps) {
         if (!ready()) return SPH_ERR_NOT_INITIALISED;
+        if (!buffers_ok()) return SPH_ERR_OUT_OF_MEMORY;
         DeviceGuard guard(device);
         for (uint32_t s = 0; s < nsteps; ++s) {
             const bool tm = timing && s + 1 == nsteps;
@@ -462,6 +472,7 @@ struct Engine {
     int capture(uint32_t nsteps) {
         if (nsteps == 0 || (nsteps & 1u)) return SPH_ERR_INVALID_ARGUMENT;
         if (!ready()) return SPH_ERR_NOT_INITIALISED;
+        if (!buffers_ok()) return SPH_ERR_OUT_OF_MEMORY;
         // a recording replays fixed parameters: no turning rotor or moving wave
         if (changes_per_step(scenes[0])) return SPH_ERR_UNSUPPORTED;
         DeviceGuard guard(device);
@@ -835,6 +846,7 @@ struct Engine {
     // identical results, so the choice only moves time.
     int autotune(const char* cache_path, uint32_t out[4], float* best_ms_out) {
         if (!ready()) return SPH_ERR_NOT_INITIALISED;
+        if (!buffers_ok()) return SPH_ERR_OUT_OF_MEMORY;
         const std::string key = tune_key();
         uint32_t t[4] = {0, 0, 0, 0};
         float ms = 0.0f;

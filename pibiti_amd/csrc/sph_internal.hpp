@@ -261,6 +261,9 @@ hipError_t launch_scatter_to_original(uint32_t n, const uint32_t* pid, const voi
 hipError_t launch_update_from_original(uint32_t n, const uint32_t* pid, const float4* staging,
                                        uint32_t first, uint32_t count, float4* dst, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* dst, uint32_t value, size_t count, hipStream_t s);
+// Every device allocation of the library: hipMalloc, or hipErrorOutOfMemory
+// when the allocation-failure hook (sph_debug_fail_alloc) fires.
+hipError_t device_malloc(void** ptr, size_t bytes);
 hipError_t launch_iota_u32(uint32_t* dst, uint32_t n, hipStream_t s);
 // sid[k] = scene of original index pid[k]; scene_off holds the first
 // original index of each scene (scenes are contiguous in original order).

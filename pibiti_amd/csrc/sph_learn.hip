@@ -8,6 +8,7 @@
 // ones are summed per original particle index.  Gradients and Adam updates
 // run on the device against that window; nothing leaves HBM except the loss.
 #include <cmath>
+#include <initializer_list>
 
 #include "sph_engine.hpp"
 #include "sph_mlp.hpp"
@@ -101,6 +102,15 @@ LearnState* learn_state(Engine& e) {
     return e.learn;
 }
 
+// Frees a group of buffers whose allocation failed part way, so that a
+// later call starts the group again from nothing.
+void drop_all(std::initializer_list<void**> ptrs) {
+    for (void** q : ptrs) {
+        (void)hipFree(*q);
+        *q = nullptr;
+    }
+}
+
 hipError_t ensure_window(LearnState& L, bool batch) {
     if (L.pos) return hipSuccess;
     const size_t N = L.n;
@@ -112,6 +122,9 @@ hipError_t ensure_window(LearnState& L, bool batch) {
     if (e == hipSuccess) e = dalloc(&L.nbr, N * kNbrCap);
     if (e == hipSuccess) e = dalloc(&L.nnb, N);
     if (e == hipSuccess) e = dalloc(&L.acc, N);
+    if (e != hipSuccess)
+        drop_all({(void**)&L.pos, (void**)&L.vel, (void**)&L.rhop, (void**)&L.pid, (void**)&L.sid, (void**)&L.nbr,
+                  (void**)&L.nnb, (void**)&L.acc});
     return e;
 }
 
@@ -128,7 +141,11 @@ hipError_t ensure_train(LearnState& L) {
     if (e == hipSuccess) e = dalloc(&t.chunks, (size_t)mlp::kRedChunks * (mlp::kNumParams + 1));
     if (e == hipSuccess) e = dalloc(&t.grad, (size_t)mlp::kNumParams);
     if (e == hipSuccess) e = dalloc(&t.loss, (size_t)1);
-    if (e == hipSuccess) L.train_ready = true;
+    if (e == hipSuccess)
+        L.train_ready = true;
+    else
+        drop_all({(void**)&t.h, (void**)&t.out, (void**)&t.dh, (void**)&t.part_pair, (void**)&t.part_node,
+                  (void**)&t.loss_part, (void**)&t.chunks, (void**)&t.grad, (void**)&t.loss});
     return e;
 }
 
@@ -149,7 +166,20 @@ hipError_t ensure_step(LearnState& L, uint32_t nscenes) {
         e = hipHostMalloc(reinterpret_cast<void**>(&L.host_cnt),
                           2 * kStepCounters * kMaxCheck * sizeof(uint32_t));
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&L.done[k], hipEventDisableTiming);
-    if (e == hipSuccess) L.step_ready = true;
+    if (e == hipSuccess) {
+        L.step_ready = true;
+        return e;
+    }
+    drop_all({(void**)&L.lnbr, (void**)&L.lnnb, (void**)&L.spos, (void**)&L.svel, (void**)&L.spid,
+              (void**)&L.dev_part, (void**)&L.dev, (void**)&L.ovf, (void**)&L.dt_coarse});
+    if (L.host) (void)hipHostFree(L.host);
+    if (L.host_cnt) (void)hipHostFree(L.host_cnt);
+    L.host = nullptr;
+    L.host_cnt = nullptr;
+    for (hipEvent_t& ev : L.done) {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+    }
     return e;
 }
 
@@ -438,6 +468,7 @@ int step_learned(Engine& e, sph_model* m, uint32_t ncoarse, uint32_t factor, flo
         e.scenes[0].clr_type != SPH_CLR_NONE)
         return SPH_ERR_UNSUPPORTED;
     if (!e.ready() || e.last_kind == 0) return SPH_ERR_NOT_INITIALISED;
+    if (!e.buffers_ok()) return SPH_ERR_OUT_OF_MEMORY;
     LearnState* L = learn_state(e);
     if (!L) return SPH_ERR_OUT_OF_MEMORY;
     DeviceGuard guard(e.device);
@@ -619,12 +650,12 @@ int model_create(int device, const sph_mlp_norm* norm, uint64_t seed, sph_model*
     for (int k = 0; k < mlp::kOut * mlp::kW3Cols; ++k) w[mlp::kOffW3 + k] = (float)(s3 * normal());
     DeviceGuard guard(device);
     const size_t bytes = mlp::kNumParams * sizeof(float);
-    hipError_t e = hipMalloc(&m->params, bytes);
-    if (e == hipSuccess) e = hipMalloc(&m->adam_m, bytes);
-    if (e == hipSuccess) e = hipMalloc(&m->adam_v, bytes);
-    if (e == hipSuccess) e = hipMalloc(&m->pk.w1f, mlp_w1f_bytes());
-    if (e == hipSuccess) e = hipMalloc(&m->pk.w2f, mlp_w2f_bytes());
-    if (e == hipSuccess) e = hipMalloc(&m->pk.w2tf, mlp_w2f_bytes());
+    hipError_t e = device_malloc((void**)&m->params, bytes);
+    if (e == hipSuccess) e = device_malloc((void**)&m->adam_m, bytes);
+    if (e == hipSuccess) e = device_malloc((void**)&m->adam_v, bytes);
+    if (e == hipSuccess) e = device_malloc((void**)&m->pk.w1f, mlp_w1f_bytes());
+    if (e == hipSuccess) e = device_malloc((void**)&m->pk.w2f, mlp_w2f_bytes());
+    if (e == hipSuccess) e = device_malloc((void**)&m->pk.w2tf, mlp_w2f_bytes());
     if (e == hipSuccess) e = hipMemcpy(m->params, w.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(m->adam_m, 0, bytes);
     if (e == hipSuccess) e = hipMemset(m->adam_v, 0, bytes);

@@ -956,9 +956,9 @@ int sph_mfma_selftest(int variant, const uint16_t* a, const uint16_t* b, float* 
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SPH_ERR_NO_DEVICE;
     const size_t na = 512, nb = 512, nc = variant == 0 ? 1024 : 256;
     void *da = nullptr, *db = nullptr, *dc = nullptr;
-    hipError_t e = hipMalloc(&da, na * 2);
-    if (e == hipSuccess) e = hipMalloc(&db, nb * 2);
-    if (e == hipSuccess) e = hipMalloc(&dc, nc * 4);
+    hipError_t e = sph::device_malloc(&da, na * 2);
+    if (e == hipSuccess) e = sph::device_malloc(&db, nb * 2);
+    if (e == hipSuccess) e = sph::device_malloc(&dc, nc * 4);
     if (e == hipSuccess) e = hipMemcpy(da, a, na * 2, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(db, b, nb * 2, hipMemcpyHostToDevice);
     if (e == hipSuccess) {

@@ -487,7 +487,7 @@ int sph_radix_sort(uint32_t* pairs, uint32_t* temp, uint32_t n, uint32_t bits, v
     if (n > SPH_SORT_MAX) return SPH_ERR_CAPACITY;
     if (n == 0 || bits == 0) return SPH_OK;
     void* buf = nullptr;
-    hipError_t e = hipMalloc(&buf, sph::radix_scratch_bytes(n));
+    hipError_t e = sph::device_malloc(&buf, sph::radix_scratch_bytes(n));
     if (e != hipSuccess) return status_of(e);
     RadixSortPlan plan;
     sph::carve(plan, n, buf);

@@ -22,6 +22,7 @@ LIB_PATH = Path(_HOST_LIB) if _HOST_LIB else _PKG / "libpibiti_sph.so"
 
 # status codes (sph_status)
 SPH_OK = 0
+SPH_ERR_OUT_OF_MEMORY = 4
 SPH_ERR_NOT_INITIALISED = 6
 SPH_ERR_ORDER = 9
 # sph_debug_poke targets
@@ -337,6 +338,7 @@ SIGNATURES = {
     "sph_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "sph_synchronize": (C.c_int, [_P]),
     "sph_debug_poke": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32]),
+    "sph_debug_fail_alloc": (C.c_int, [C.c_uint32]),
     "sph_radix_sort": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
     "sph_radix_sort_noalloc": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
     "sph_get_device_view": (C.c_int, [_P, C.POINTER(DeviceView)]),
@@ -500,6 +502,12 @@ def scene_emit(info: SceneInfo, emitter: int) -> tuple[np.ndarray, np.ndarray]:
     n = C.c_uint32()
     _check(lib().sph_scene_emit(C.byref(info), emitter, _f(pos), _f(vel), C.byref(n)), "sph_scene_emit")
     return pos[:n.value].copy(), vel[:n.value].copy()
+
+
+def debug_fail_alloc(after: int) -> None:
+    """Testing hook: the after-th device allocation of the library from now
+    on fails with SPH_ERR_OUT_OF_MEMORY (0 disarms; sph_debug_fail_alloc)."""
+    _check(lib().sph_debug_fail_alloc(int(after)), "sph_debug_fail_alloc")
 
 
 class SphSystem:

@@ -1,0 +1,121 @@
+"""Device allocation failures, forced from the host (sph_debug_fail_alloc).
+
+Every device allocation a context makes is made to fail in turn, one per
+attempt: creation, the first upload and the lazily allocated buffers of the
+first steps (FAST pair lists and records, kept increments).  Each failure
+must come back as SPH_ERR_OUT_OF_MEMORY from the call that allocates, the
+context must release what it holds (device free memory returns to where it
+was), and the library must stay usable: once the hook is past the last
+allocation, the run equals a run without any failure bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import DeviceArray
+from pibiti_amd import scenes, sph
+
+pytestmark = pytest.mark.gpu
+
+ARRAYS = (sph.SPH_ARRAY_POSITION, sph.SPH_ARRAY_VELOCITY, sph.SPH_ARRAY_DENSITY)
+
+
+def free_bytes() -> int:
+    h = DeviceArray.hip()
+    h.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    free, total = C.c_size_t(), C.c_size_t()
+    assert h.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def single(p, pos, vel):
+    with sph.SphSystem(p, keep_dvel=True) as s:
+        s.reset(pos, vel)
+        s.update(3)
+        return [s.get_array(k) for k in ARRAYS]
+
+
+def batch(sc):
+    with sph.SphBatch([q for q, _, _, _ in sc]) as b:
+        for k, (_, x, v, _) in enumerate(sc):
+            b.reset(k, x, v)
+        b.update(3)
+        return [b.get_array(k, w) for k in range(len(sc)) for w in ARRAYS]
+
+
+def fail_each_allocation(run):
+    want = run()
+    base = free_bytes()
+    failures = 0
+    for k in range(1, 400):
+        sph.debug_fail_alloc(k)
+        try:
+            got = run()
+        except sph.SphError as e:
+            assert e.status == sph.SPH_ERR_OUT_OF_MEMORY, (k, str(e))
+            failures += 1
+            # 16 MiB of slack: the runtime's own pools move a little
+            assert abs(free_bytes() - base) < (16 << 20), (k, free_bytes() - base)
+            continue
+        finally:
+            sph.debug_fail_alloc(0)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b), k
+        return failures
+    pytest.fail("the hook never passed the last allocation")
+
+
+@pytest.mark.parametrize("prec", [sph.SPH_PREC_REFERENCE, sph.SPH_PREC_FAST])
+def test_each_allocation_of_a_context_can_fail(prec):
+    p, pos, vel, _ = scenes.cpu_dam_break_15k(precision=prec)
+    print("case: allocation failures, 15k, precision", prec, flush=True)
+    failures = fail_each_allocation(lambda: single(p, pos, vel))
+    print("allocations that failed in turn:", failures, flush=True)
+    assert failures > 10
+
+
+def test_each_allocation_of_a_batch_can_fail():
+    sc = [scenes.scaled_dam_break(l, seed=60 + k, vel_sigma=0.5) for k, l in enumerate([(8, 6, 8), (10, 5, 7)])]
+    print("case: allocation failures, batch of 2", flush=True)
+    failures = fail_each_allocation(lambda: batch(sc))
+    assert failures > 10
+
+
+def test_failed_lazy_allocation_refuses_steps_until_retried():
+    """Switching a running context to FAST with a colour mode allocates the
+    pair lists, records and colour buffers then.  When one of those
+    allocations fails, set_params reports it, the context refuses to step
+    (rather than launch kernels on missing buffers), and a second
+    set_params allocates what is missing: the run then equals one that
+    never failed."""
+    p, pos, vel, _ = scenes.cpu_dam_break_15k()
+    q = p.copy()
+    q.precision = sph.SPH_PREC_FAST
+    q.clr_type = sph.SPH_CLR_VEL
+
+    def run(fail_at):
+        with sph.SphSystem(p) as s:
+            s.reset(pos, vel)
+            s.update(2)
+            if fail_at:
+                sph.debug_fail_alloc(fail_at)
+                try:
+                    with pytest.raises(sph.SphError) as ex:
+                        s.set_params(q)
+                    assert ex.value.status == sph.SPH_ERR_OUT_OF_MEMORY
+                finally:
+                    sph.debug_fail_alloc(0)
+                with pytest.raises(sph.SphError) as ex:
+                    s.update(1)
+                assert ex.value.status == sph.SPH_ERR_OUT_OF_MEMORY
+            s.set_params(q)
+            s.update(2)
+            return [s.get_array(k) for k in ARRAYS + (sph.SPH_ARRAY_COLOR,)]
+
+    print("case: lazy allocation failures (FAST + colour after 2 steps)", flush=True)
+    want = run(0)
+    for k in (1, 2, 3, 4, 5, 6):
+        got = run(k)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b), k
