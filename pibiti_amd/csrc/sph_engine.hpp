@@ -59,11 +59,11 @@ inline int to_status(hipError_t e) {
         if (st_ != SPH_OK) return st_;   \
     } while (0)
 
-// Device arrays carry 64 bytes past their end: the density kernel's group
-// of four position loads may run up to three records past the last slot.
+// Device arrays carry 128 bytes past their end: the density kernels' groups
+// of position loads (at most eight float4) may run past the last slot.
 template <typename T>
 inline hipError_t dalloc(T** ptr, size_t count) {
-    return device_malloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 64);
+    return device_malloc(reinterpret_cast<void**>(ptr), count * sizeof(T) + 128);
 }
 
 // Slot ranges of the sort's 27-bit slot field bound the particle count.

@@ -35,7 +35,8 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;
-constexpr uint32_t kRing = 4;  // ring entries per lane (a group adds at most one)
+constexpr uint32_t kRing = 4;  // ring entries per lane (a group of four adds at most one)
+constexpr int kGroup = 4;      // candidates per iteration of the density walk
 
 __device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -83,19 +84,30 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     // r2 = +inf, which fails both predicates.  The density weight
     // max(h2 - r2, 0)^3 is the reference's (r2 < h2 ? (h2 - r2)^3 : 0),
     // also for a NaN r2 (max gives 0).
+    // one list entry per four consecutive candidates with a pair; the
+    // written slot is only kept when the count advances
+    auto append = [&](uint32_t j0, uint32_t bits) {
+        slot0[(count & (kRing - 1u)) * kBlock] = (j0 << 4) | bits;
+        count += bits ? 1u : 0u;
+        pairs += (uint32_t)__popc(bits);
+        if (count - flushed == kRing) {
+            nbr[(size_t)(flushed >> 2) * n + i] =
+                make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
+            flushed += kRing;
+        }
+    };
     auto walk_run = [&](uint32_t s, uint32_t e, auto self_row) {
         constexpr bool kSelfRow = decltype(self_row)::value;
-        for (uint32_t j0 = s; j0 < e; j0 += 4) {
-            // four loads in flight; position arrays are padded past their end
-            float4 xq[4];
+        const float4* q = pos + s;
+        for (uint32_t j0 = s; j0 < e; j0 += kGroup, q += kGroup) {
+            // kGroup loads in flight; position arrays are padded past their end
+            float4 xq[kGroup];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                xq[k] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pos) +
-                                                         ((j0 + (uint32_t)k) << 4));
+            for (int k = 0; k < kGroup; ++k) xq[k] = q[k];
             const uint32_t left = e - j0;
             uint32_t bits = 0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < kGroup; ++k) {
                 // the predicates need the reference's r2 exactly
                 const float rx = xi.x - xq[k].x, ry = xi.y - xq[k].y, rz = xi.z - xq[k].z;
                 const float r2 = rx * rx + ry * ry + rz * rz;
@@ -106,16 +118,8 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
                 sum = __builtin_fmaf(c * c, c, sum);
                 bits |= force_pair(r2v, r2f) ? (1u << k) : 0u;
             }
-            // one entry per group with a pair; the written slot is only
-            // kept when the count advances
-            slot0[(count & (kRing - 1u)) * kBlock] = (j0 << 4) | bits;
-            count += bits ? 1u : 0u;
-            pairs += (uint32_t)__popc(bits);
-            if (count - flushed == kRing) {
-                nbr[(size_t)(flushed >> 2) * n + i] =
-                    make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
-                flushed += kRing;
-            }
+#pragma unroll
+            for (int g = 0; g < kGroup / 4; ++g) append(j0 + 4u * (uint32_t)g, (bits >> (4 * g)) & 15u);
         }
     };
 #pragma unroll 1
