@@ -186,6 +186,10 @@ struct Engine {
         b.stats = stats;
         b.dt_dev = dt_dev;
         b.masks = cfg.max_cap <= kMaskCap ? masks : nullptr;
+        const bool lists = fast_nbr && fast_nnb && fast_rec && fast_list_groups(cfg.max_cap) <= fast_groups;
+        b.pairs = lists ? fast_nbr : nullptr;
+        b.pair_cnt = lists ? fast_nnb : nullptr;
+        b.rec = lists ? fast_rec : nullptr;
         return b;
     }
 
@@ -362,9 +366,11 @@ struct Engine {
         return e;
     }
 
-    // FAST precision: the force pair lists, sized for the largest cap.
+    // The force pair lists and records (FAST precision, and the reference
+    // order's default gathers), sized for the largest cap; none above
+    // kFastMaxCap (the reference order then takes other gathers).
     hipError_t ensure_fast() {
-        if (cfg.precision != SPH_PREC_FAST) return hipSuccess;
+        if (cfg.max_cap > kFastMaxCap) return hipSuccess;
         const uint32_t g = fast_list_groups(cfg.max_cap);
         if (fast_nbr && g <= fast_groups) return hipSuccess;
         (void)hipFree(fast_nbr);
@@ -433,7 +439,7 @@ struct Engine {
     // failed (set_params to FAST or to a colour mode under a full device)
     // leaves the context unable to step until a later call allocates them.
     bool buffers_ok() const {
-        if (cfg.precision == SPH_PREC_FAST && (!fast_nbr || !fast_nnb || !fast_rec)) return false;
+        if (cfg.max_cap <= kFastMaxCap && (!fast_nbr || !fast_nnb || !fast_rec)) return false;
         if (cfg.colour && (!colour || !dye || !dvel)) return false;
         return true;
     }
@@ -908,9 +914,11 @@ struct Engine {
             return r;
         };
         e = snap(true);
-        // 2 = the pair-mask path (force lists filled from the density
-        // kernel's masks), else the list size of the candidate-testing force
-        const uint32_t lists[] = {2, 16, 24, 32};
+        // 3 = the pair-entry path (the force walks the density kernel's
+        // pair entries), 2 = the pair-mask path (force lists filled from the
+        // density kernel's masks), else the list size of the
+        // candidate-testing force
+        const uint32_t lists[] = {3, 2, 16, 24, 32};
         const bool fast = cfg.precision == SPH_PREC_FAST;  // one launch shape: time it once
         // gather grid and variant: per-lane loads in XCD order, cell tables in
         // LDS, or a persistent grid
@@ -921,10 +929,12 @@ struct Engine {
         int st = SPH_OK;
         for (uint32_t li : lists) {
             if (fast && li != lists[0]) break;
-            if (!fast && (li == 2 ? (cfg.max_cap > kMaskCap || !masks) : cfg.max_cap > li)) continue;
+            if (!fast && li == 3 && (cfg.max_cap > kFastMaxCap || !fast_nbr)) continue;
+            if (!fast && li == 2 && (cfg.max_cap > kMaskCap || !masks)) continue;
+            if (!fast && li > 3 && cfg.max_cap > li) continue;
             for (const auto& sh : shapes) {
                 if (e != hipSuccess || st != SPH_OK) break;
-                if ((li == 2 || fast) && (sh[0] != 0 || sh[1] != 0)) continue;  // one launch shape
+                if ((li <= 3 || fast) && (sh[0] != 0 || sh[1] != 0)) continue;  // one launch shape
                 const uint32_t g = sh[0];
                 const uint32_t cand[4] = {li, g, 0, sh[1]};
                 set_tune(cand);

@@ -153,12 +153,22 @@ struct StepBuffers {
     double* stats;        // kNumStats doubles per scene
     float* dt_dev;        // time step of each scene
     uint32_t* masks;      // force pair masks of the density kernel (kMaskWords per slot), or null
+    uint4* pairs;         // pair-entry lists (group-major), or null: see kPairSlotShift
+    uint32_t* pair_cnt;   // entries | pairs << 16 per slot
+    float4* rec;          // force records, 2 float4 per slot
 };
 
 // Force pair masks: 27 cells x 16 bits in 14 words per slot (word w of slot
 // i at masks[w n + i]), usable while every cell holds at most 16 candidates.
 constexpr uint32_t kMaskWords = 14;
 constexpr uint32_t kMaskCap = 16;
+// Pair entries from the reference-order density walk to its force kernel:
+// (first slot << 5) | (new cell << 4) | 4-bit mask of the force pairs among
+// four consecutive candidates of one cell, in walk order; the new-cell flag
+// marks a cell's first entry.  Group-major uint4 lists as in FAST mode.
+constexpr uint32_t kPairSlotShift = 5;
+constexpr uint32_t kPairNewCell = 1u << 4;
+constexpr uint32_t kPairRing = 4;  // entries per lane in the density kernel's LDS ring
 
 // What a launch covers: one scene (params by value) or a batch of scenes
 // (device parameter table + per-slot scene ids + static slot ranges).

@@ -881,7 +881,7 @@ __device__ bool stage_block(Stage<kCap>& S, uint32_t b0, uint32_t count, const s
 // Masks of cells 2w and 2w+1 share word w (low, high half): kMaskWords
 // words per slot, word w of slot i at nbr[w n + i].
 template <bool kBatch, int kPrec, bool kLists = false, typename Src = PosGlobal, bool kCells = false,
-          bool kMasks = false>
+          bool kMasks = false, bool kEntries = false>
 __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& one,
                                             const sph_params* __restrict__ tab,
                                             const uint32_t* __restrict__ sid,
@@ -892,7 +892,9 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                                             uint32_t n = 0, uint32_t* __restrict__ nbr = nullptr,
                                             uint32_t* __restrict__ nnb = nullptr,
                                             uint32_t* __restrict__ counters = nullptr,
-                                            const Src* src_in = nullptr, const CellView* cv = nullptr) {
+                                            const Src* src_in = nullptr, const CellView* cv = nullptr,
+                                            float4* __restrict__ rec = nullptr,
+                                            const float4* __restrict__ vel = nullptr) {
     uint32_t scene;
     const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
     const float4 xi = pos[i];
@@ -910,6 +912,11 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     __shared__ uint32_t ring[kLists ? kRingSlots * kRingLanes : 1];
     ListRing lw;
     if constexpr (kLists) lw.init(ring);
+    // kEntries: the pair entries pass through a four-entry ring per lane
+    __shared__ uint32_t ering[kEntries ? kPairRing * kBlock : 1];
+    uint32_t* const eslot = ering + threadIdx.x;
+    uint32_t ecount = 0, eflushed = 0, epairs = 0;
+    uint4* const entries = reinterpret_cast<uint4*>(nbr);
     uint4* const lists = reinterpret_cast<uint4*>(nbr);
     int xbase = 0;
     if constexpr (kCells) xbase = (int)(w.own_key() - cv->kmin) + 1;
@@ -939,6 +946,7 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                 if (open) {
                     float cell = 0.0f;
                     const int32_t soff = src.offset(ord / 3);
+                    uint32_t fresh = kPairNewCell;  // the cell's first entry opens its partial sum
                     // candidates four at a time: the loads go out together (indices
                     // past the cell's end are masked out of the sums); the particle
                     // itself can only be a candidate of its own cell (ord 13), which
@@ -969,9 +977,24 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                                     cell = cell + (hit ? wgt : 0.0f);
                                 }
                                 if constexpr (kLists) lw.put(j, hit);
-                                if constexpr (kMasks) fbits |= (force_pair(r2, r2f) && valid ? 1u : 0u) << k;
+                                if constexpr (kMasks || kEntries)
+                                    fbits |= (force_pair(r2, r2f) && valid ? 1u : 0u) << k;
                             }
                             if constexpr (kMasks) cmask |= fbits << (j0 - s);
+                            if constexpr (kEntries) {
+                                // the written slot is only kept when the count advances
+                                eslot[(ecount & (kPairRing - 1u)) * kBlock] = (j0 << kPairSlotShift) | fresh | fbits;
+                                if (fbits) {
+                                    ++ecount;
+                                    epairs += (uint32_t)__popc(fbits);
+                                    fresh = 0;
+                                }
+                                if (ecount - eflushed == kPairRing) {
+                                    entries[(size_t)(eflushed >> 2) * n + i] =
+                                        make_uint4(eslot[0], eslot[kBlock], eslot[2 * kBlock], eslot[3 * kBlock]);
+                                    eflushed += kPairRing;
+                                }
+                            }
                             if constexpr (kLists) lw.drain(lists, n, i);
                         }
                     };
@@ -993,7 +1016,17 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     if constexpr (kMasks) nbr[(kMaskWords - 1u) * n + i] = mword;
     if (kPrec == SPH_PREC_KAHAN) sum = ks.s;
     const float rho = (sum * p.poly6_kern) * p.particle_mass;
-    rhop[i] = make_float2(rho, eos(p, rho));
+    const float pres = eos(p, rho);
+    rhop[i] = make_float2(rho, pres);
+    if constexpr (kEntries) {
+        if (ecount > eflushed)
+            entries[(size_t)(eflushed >> 2) * n + i] =
+                make_uint4(eslot[0], eslot[kBlock], eslot[2 * kBlock], eslot[3 * kBlock]);
+        nnb[i] = ecount | (epairs << 16);
+        const float4 vi = vel[i];
+        rec[2 * (size_t)i] = make_float4(xi.x, xi.y, xi.z, rho);
+        rec[2 * (size_t)i + 1] = make_float4(vi.x, vi.y, vi.z, pres);
+    }
     if constexpr (kLists) {
         lw.finish(lists, n, i);
         nnb[i] = lw.count;
@@ -1349,6 +1382,21 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(
                                                              n, masks, nullptr, counters);
 }
 
+// Density plus the force kernel's pair entries and force records
+// (density_one with kEntries), one slot per lane in XCD-ordered tiles.
+template <bool kBatch, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_density_entries(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint4* __restrict__ entries,
+    uint32_t* __restrict__ nnb, float4* __restrict__ rec, uint32_t* __restrict__ counters) {
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    density_one<kBatch, kPrec, false, PosGlobal, false, false, true>(
+        i, one, tab, sid, pos, cell_start, cell_end, rhop, n, reinterpret_cast<uint32_t*>(entries), nnb, counters,
+        nullptr, nullptr, rec, vel);
+}
+
 // Cell-table stage: the (start, end) pairs of the block's 9 neighbour rows
 // (the key ranges of stage_block), so that opening a neighbour cell reads
 // LDS instead of two dependent global loads.
@@ -1489,6 +1537,127 @@ __global__ __launch_bounds__(kForceTile) void k_force_tiled(
         force_one<kBatch, kPrec, kList, kForceTile>(i, i < n, list, one, tab, sid, pos, vel, rhop, cell_start,
                                                     cell_end, vel_out, dvel, dt_dev, counters);
     }
+}
+
+// Force over the density kernel's pair entries (k_density_entries), one
+// slot per lane: each lane walks only its own pairs, in the reference's
+// order (cells in walk order, slots ascending), with the reference's
+// arithmetic and per-cell partial sums (a new-cell entry adds the open
+// partial sum to the total first), so the result equals force_one's bit for
+// bit.  Four pairs per iteration: they are decoded first, so that their
+// eight record loads go out together.  A record is (x, y, z, rho) and
+// (vx, vy, vz, p), one aligned 32-byte block.
+template <bool kBatch, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_force_entries(
+    sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
+    const float4* __restrict__ vel, const float4* __restrict__ rec, const uint4* __restrict__ entries,
+    const uint32_t* __restrict__ nnb, float4* __restrict__ vel_out, float4* __restrict__ dvel,
+    const float* __restrict__ dt_dev) {
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t scene;
+    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
+    const float4 xi = rec[2 * (size_t)i];       // (x, y, z, rho)
+    const float4 vi = rec[2 * (size_t)i + 1];   // (vx, vy, vz, p)
+    const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
+    const float vterm = p.lap_kern * p.viscosity;
+    const uint32_t npairs = nnb[i] >> 16;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;  // total
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;  // the current cell's partial sum
+    KahanF kx{0.0f, 0.0f}, ky{0.0f, 0.0f}, kz{0.0f, 0.0f};
+    uint4 buf = make_uint4(0, 0, 0, 0);
+    uint32_t bits = 0, jb = 0, next = 0, fresh = 0;
+    // the next pair's slot, and whether it opens a cell
+    auto next_pair = [&](bool& opens) {
+        opens = false;
+        if (bits == 0u) {
+            if ((next & 3u) == 0u) buf = entries[(size_t)(next >> 2) * n + i];
+            const uint32_t q = next & 3u;
+            const uint32_t e = q == 0u ? buf.x : (q == 1u ? buf.y : (q == 2u ? buf.z : buf.w));
+            ++next;
+            bits = e & 15u;
+            fresh = e & kPairNewCell;
+            jb = e >> kPairSlotShift;
+        }
+        opens = fresh != 0u;
+        fresh = 0u;
+        const uint32_t k = (uint32_t)__builtin_ctz(bits);
+        bits &= bits - 1u;
+        return jb + k;
+    };
+    for (uint32_t t = 0; t < npairs; t += 4) {
+        uint32_t j[4];
+        bool opens[4];
+        j[0] = next_pair(opens[0]);
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            opens[k] = false;
+            j[k] = t + (uint32_t)k < npairs ? next_pair(opens[k]) : j[0];
+        }
+        float4 xq[4], vq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            xq[k] = rec[2 * (size_t)j[k]];
+            vq[k] = rec[2 * (size_t)j[k] + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            // past the last pair (a repeat of the first of the four): nothing added
+            const bool on = t + (uint32_t)k < npairs;
+            const float4 xj = xq[k], vj = vq[k];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float r = fmaxf(min_dist, sqrt_ieee(r2));
+            const float d12 = fminf(min_dens, 1.0f / (xi.w * xj.w));
+            const float cc = h - r;
+            const float pterm = ((cc * spiky) * (vi.w + vj.w)) / r;
+            const float cd = cc * d12;
+            const float gxv = (pterm * rx + vterm * (vj.x - vi.x)) * cd;
+            const float gyv = (pterm * ry + vterm * (vj.y - vi.y)) * cd;
+            const float gzv = (pterm * rz + vterm * (vj.z - vi.z)) * cd;
+            if (kPrec == SPH_PREC_KAHAN) {
+                if (on) {
+                    kx.add(gxv);
+                    ky.add(gyv);
+                    kz.add(gzv);
+                }
+            } else {
+                if (on && opens[k]) {
+                    ax = ax + fx;
+                    ay = ay + fy;
+                    az = az + fz;
+                    fx = fy = fz = 0.0f;
+                }
+                if (on) {
+                    fx = fx + gxv;
+                    fy = fy + gyv;
+                    fz = fz + gzv;
+                }
+            }
+        }
+    }
+    if (kPrec == SPH_PREC_KAHAN) {
+        ax = kx.s;
+        ay = ky.s;
+        az = kz.s;
+    } else {
+        ax = ax + fx;
+        ay = ay + fy;
+        az = az + fz;
+    }
+    const float mdt = p.particle_mass * dt_dev[scene];
+    ax = ax * mdt;
+    ay = ay * mdt;
+    az = az * mdt;
+    if (p.physics) {
+        // the pair increment only: k_extra_terms adds the scene's other
+        // terms and finishes the update
+        vel_out[i] = make_float4(ax, ay, az, 0.0f);
+        return;
+    }
+    const float4 v = vel[i];
+    vel_out[i] = make_float4(v.x + ax, v.y + ay, v.z + az, v.w + 0.0f);
+    if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
 
 // Force kernel for max_par_in_cell above the list capacity: the pair body
@@ -1749,15 +1918,27 @@ hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const Res
 // The pair-mask path (density writes the force's pair masks, the force
 // fills its lists from them): tune[0] == 2, and the default launch
 // (tune all 0) whenever cells hold at most 16 candidates.
+// tune[0]: 0 = default (pair entries where the lists exist, else pair
+// masks), 1 = fused force, 2 = pair masks, 3 = pair entries, 16-48 = the
+// candidate-testing force's list size.
+bool use_entries(const LaunchCfg& c, const StepBuffers& b) {
+    const bool asked = c.tune[0] == 3 || (c.tune[0] == 0 && c.tune[1] == 0 && c.tune[3] == 0);
+    return asked && b.pairs && c.max_cap <= kFastMaxCap;
+}
+
 bool use_masks(const LaunchCfg& c, const StepBuffers& b) {
     const bool asked = c.tune[0] == 2 || (c.tune[0] == 0 && c.tune[1] == 0 && c.tune[3] == 0);
-    return asked && b.masks && c.max_cap <= kMaskCap;
+    return asked && !use_entries(c, b) && b.masks && c.max_cap <= kMaskCap;
 }
 
 template <bool kBatch, int kPrec>
 void density_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
     const uint32_t grid = c.tune[1];
-    if (use_masks(c, b))
+    if (use_entries(c, b))
+        hipLaunchKernelGGL((k_density_entries<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
+                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.cell_start, b.cell_end, b.rhop,
+                           b.pairs, b.pair_cnt, b.rec, b.counters);
+    else if (use_masks(c, b))
         hipLaunchKernelGGL((k_density_masks<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
                            c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop, b.masks, b.counters);
     else if (!grid && c.tune[3] == 2 && b.skey)
@@ -1811,6 +1992,11 @@ void force_list_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, 
 template <bool kBatch, int kPrec>
 void force_impl(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, float4* dvel,
                 hipStream_t s) {
+    if (use_entries(c, b)) {
+        hipLaunchKernelGGL((k_force_entries<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s, c.one,
+                           c.tab, b.sid_cur, c.n, b.vel_cur, b.rec, b.pairs, b.pair_cnt, vel_out, dvel, b.dt_dev);
+        return;
+    }
     if (use_masks(c, b)) {
         hipLaunchKernelGGL((k_force_ordered<kBatch, kPrec, kForceListMasks, 1, true>), dim3(blocks_for(c.n, kBlock)),
                            dim3(kBlock), 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.rhop,

@@ -83,14 +83,17 @@ def test_each_allocation_of_a_batch_can_fail():
 
 
 def test_failed_lazy_allocation_refuses_steps_until_retried():
-    """Switching a running context to FAST with a colour mode allocates the
-    pair lists, records and colour buffers then.  When one of those
-    allocations fails, set_params reports it, the context refuses to step
+    """A context made with max_par_in_cell above 64 has no pair lists.
+    Switching it to FAST with a cap of 16 and a colour mode allocates the
+    pair lists, records and colour buffers then (six allocations).  When
+    one of them fails, set_params reports it, the context refuses to step
     (rather than launch kernels on missing buffers), and a second
     set_params allocates what is missing: the run then equals one that
     never failed."""
     p, pos, vel, _ = scenes.cpu_dam_break_15k()
+    p.max_par_in_cell = 80
     q = p.copy()
+    q.max_par_in_cell = 16
     q.precision = sph.SPH_PREC_FAST
     q.clr_type = sph.SPH_CLR_VEL
 

@@ -95,8 +95,8 @@ def test_single_scene_graph_replay_and_persistent_grid():
         assert np.array_equal(ref[k], got[k]), k
 
 
-@pytest.mark.parametrize("tune", [(2, 0), (0, 0), (1, 0), (24, 0), (32, 256), (16, 1024), (16, 0, 0, 1), (24, 0, 0, 2),
-                                  (16, 0, 0, 3), (48, 0, 0, 1)])
+@pytest.mark.parametrize("tune", [(3, 0), (2, 0), (0, 0), (1, 0), (24, 0), (32, 256), (16, 1024), (16, 0, 0, 1),
+                                  (24, 0, 0, 2), (16, 0, 0, 3), (48, 0, 0, 1)])
 def test_launch_variants_identical(tune):
     sc = small_batch([4000, 9000], seed=11)
     base = None
@@ -166,7 +166,7 @@ def test_autotune_restores_state_and_caches(tmp_path):
         before = [read_scene(b, s)["pos"] for s in range(len(sc))]
         tune, ms = b.autotune(str(cache))
         after = [read_scene(b, s)["pos"] for s in range(len(sc))]
-        assert tune[0] in (2, 16, 24, 32) and ms > 0
+        assert tune[0] in (3, 2, 16, 24, 32) and ms > 0
         for s in range(len(sc)):
             assert np.array_equal(before[s], after[s])
         assert b.stats(0)["steps"] == 2
