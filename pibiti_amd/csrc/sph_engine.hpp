@@ -116,9 +116,9 @@ struct Engine {
     float2* rhop = nullptr;
     float4* dvel = nullptr;
     uint32_t* masks = nullptr;  // force pair masks (kMaskWords per slot) while cells hold <= 16 candidates
-    uint4* fast_nbr = nullptr;  // SPH_PREC_FAST: force pair lists (fast_groups uint4 per slot, group-major)
+    uint4* fast_nbr = nullptr;  // force pair entries (fast_groups uint4 per slot, group-major), max_cap <= 64
     uint32_t* fast_nnb = nullptr;
-    float4* fast_rec = nullptr;  // SPH_PREC_FAST: force records, 2 per slot
+    float4* fast_rec = nullptr;  // force records, 2 per slot
     uint32_t fast_groups = 0;
     float4* colour = nullptr;   // per sorted slot, scenes with clr_type != NONE
     float* dye = nullptr;       // per ORIGINAL index (it follows the particle)
@@ -417,11 +417,8 @@ struct Engine {
             SPH_TRY(launch_density(c, buffers(), stream));
         if (timed) SPH_TRY(hipEventRecord(ev[4], stream));
         StepBuffers b = buffers();
-        if (fast)
-            SPH_TRY(launch_force_fast(c, b, fast_rec, fast_nbr, fast_nnb, b.vel_alt, keep || cfg.colour, stream));
-        else
-            SPH_TRY(launch_force(c, b, b.vel_alt, keep || cfg.colour, stream));
-        if (fast && cfg.physics) SPH_TRY(launch_extra_terms(c, b, b.vel_alt, keep || cfg.colour, stream));
+        // (the FAST force walks k_density_fast's pair entries)
+        SPH_TRY(launch_force(c, b, b.vel_alt, keep || cfg.colour, stream));
         if (cfg.colour) SPH_TRY(launch_colour(c, b, colour, dye, stream));
         cur_vel ^= 1;
         last_kind = 1;

@@ -1,4 +1,4 @@
-// sph_fast.hip - the density and force gathers of the FAST precision mode.
+// sph_fast.hip - the density gather of the FAST precision mode.
 //
 // The same step as the reference-order kernels (sph_kernels.hip), under the
 // float contract of the task rather than bit-exactness: integer outputs stay
@@ -6,22 +6,17 @@
 // per-cell cap and both predicates, r2 < h2 for the density and
 // r2 <= force_r2_max for the force), every float is within 1e-5 of the FP32
 // restatement of the reference and of the FP64 twin, condition-scaled.  That
-// frees the arithmetic:
-//   density  one running sum per particle instead of per-cell partial sums
-//            added in cell order (source/CUDA/Kernel_Cell.cui:142-199);
-//            the exact candidate walk and predicates are unchanged, and the
-//            walk writes each particle's force pairs to a list;
-//   force    the listed pairs only (no cell walk, no candidate tests), with
-//            the hardware square root and reciprocal, fused multiply-adds,
-//            d12 = min(minDens, (1/rho_i)(1/rho_j)) and the pressure term
-//            multiplied by 1/r, where the reference divides
-//            (compForcePair, source/CUDA/Kernel_Cell.cui:210-226; the
-//            velocity update of System.cu:228-250).
+// frees the arithmetic: the density is one running sum per particle with
+// fused multiply-adds instead of per-cell partial sums added in cell order
+// (source/CUDA/Kernel_Cell.cui:142-199); the candidate walk and predicates
+// are the reference's.  The force is the reference order's pair-entry
+// kernel (k_force_entries, sph_kernels.hip) with the fast arithmetic.
 //
 // Pair lists.  The density walk tests candidates four at a time (four
-// consecutive slots of one cell); a group with at least one force pair
-// leaves one list entry, (first slot << 4) | 4-bit mask of the pairs, so the
-// walk spends one LDS write per group rather than per pair.  Entries are
+// consecutive slots of one row's run); a group with at least one force pair
+// leaves one list entry, (first slot << kPairSlotShift) | 4-bit mask of the
+// pairs (no new-cell flag: FAST keeps no per-cell partial sums), so the walk
+// spends one LDS write per group rather than per pair.  Entries are
 // group-major (entries 4g .. 4g + 3 of slot i in the uint4 at g n + i) and
 // there is room for every group the capped walk can produce (27 cells of at
 // most ceil(max_par_in_cell / 4) groups), so the lists never overflow.  The
@@ -38,9 +33,6 @@ constexpr int kBlock = 256;
 constexpr uint32_t kRing = 4;  // ring entries per lane (a group of four adds at most one)
 constexpr int kGroup = 4;      // candidates per iteration of the density walk
 
-__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
-
 // Density + EOS, the force pair list and the force record of one slot per
 // thread.
 //
@@ -54,7 +46,7 @@ __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrt
 // cell of a row whose box is farther than h (the middle cell of a row that
 // is kept is never farther than the row).
 //
-// The force record of slot i is two float4: (x, y, z, 1/rho) and
+// The force record of slot i is two float4: (x, y, z, rho) and
 // (vx, vy, vz, p), everything the force reads of a particle in one aligned
 // 32-byte block.
 template <bool kBatch>
@@ -87,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     // one list entry per four consecutive candidates with a pair; the
     // written slot is only kept when the count advances
     auto append = [&](uint32_t j0, uint32_t bits) {
-        slot0[(count & (kRing - 1u)) * kBlock] = (j0 << 4) | bits;
+        slot0[(count & (kRing - 1u)) * kBlock] = (j0 << kPairSlotShift) | bits;
         count += bits ? 1u : 0u;
         pairs += (uint32_t)__popc(bits);
         if (count - flushed == kRing) {
@@ -157,108 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     const float pr = eos(p, rho);
     rhop[i] = make_float2(rho, pr);
     const float4 vi = vel[i];
-    rec[2 * (size_t)i] = make_float4(xi.x, xi.y, xi.z, rcp_hw(rho));
+    rec[2 * (size_t)i] = make_float4(xi.x, xi.y, xi.z, rho);
     rec[2 * (size_t)i + 1] = make_float4(vi.x, vi.y, vi.z, pr);
-}
-
-// ------------------------------------------------------------------ force
-constexpr int kFT = 256;  // slots per force block
-
-// The pair term of the force update from the force records of i (xi, vi)
-// and j (xj, vj), accumulated into (ax, ay, az).  1/r and r come from one
-// reciprocal square root: r = max(minDist, r2 / sqrt(r2)) and
-// 1/r = min(1/minDist, 1/sqrt(r2)), which also give r = minDist for r2 = 0
-// and for a NaN r2, as max(minDist, sqrt(r2)) does.
-__device__ __forceinline__ void pair_term(const float4& xi, const float4& vi, const float4& xj, const float4& vj,
-                                          float h, float spiky, float vterm, float min_dist, float inv_min_dist,
-                                          float min_dens, float& ax, float& ay, float& az) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-    const float rs = __builtin_amdgcn_rsqf(r2);
-    const float r = fmaxf(min_dist, r2 * rs);
-    const float inv_r = fminf(inv_min_dist, rs);
-    const float d12 = fminf(min_dens, xi.w * xj.w);
-    const float cc = h - r;
-    const float pterm = (cc * spiky) * (vi.w + vj.w) * inv_r;
-    const float cd = cc * d12;
-    ax = __builtin_fmaf(__builtin_fmaf(pterm, rx, vterm * (vj.x - vi.x)), cd, ax);
-    ay = __builtin_fmaf(__builtin_fmaf(pterm, ry, vterm * (vj.y - vi.y)), cd, ay);
-    az = __builtin_fmaf(__builtin_fmaf(pterm, rz, vterm * (vj.z - vi.z)), cd, az);
-}
-
-// Force + velocity update of one slot per thread over its listed pairs,
-// four pairs per iteration: the pairs are decoded first, so that their
-// eight loads go out together.  A lane's pairs come in the walk's order
-// (ascending slots of the same rows for the lanes of one cell), which keeps
-// the lanes of a wave on nearby records.
-template <bool kBatch>
-__global__ __launch_bounds__(kFT) void k_force_fast(sph_params one, const sph_params* __restrict__ tab,
-                                                    const uint32_t* __restrict__ sid, uint32_t n,
-                                                    const float4* __restrict__ vel,
-                                                    const float4* __restrict__ rec,
-                                                    const uint4* __restrict__ nbr,
-                                                    const uint32_t* __restrict__ nnb,
-                                                    float4* __restrict__ vel_out, float4* __restrict__ dvel,
-                                                    const float* __restrict__ dt_dev) {
-    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kFT + threadIdx.x;
-    if (i >= n) return;
-    uint32_t scene;
-    const sph_params& p = scene_params<kBatch>(one, tab, sid, i, scene);
-    const float4 xi = rec[2 * (size_t)i];
-    const float4 vi = rec[2 * (size_t)i + 1];
-    const float h = p.h, spiky = p.spiky_kern, min_dist = p.min_dist, min_dens = p.min_dens;
-    const float inv_min_dist = 1.0f / min_dist;
-    const float vterm = p.lap_kern * p.viscosity;
-    const uint32_t npairs = nnb[i] >> 16;
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    uint4 buf = make_uint4(0, 0, 0, 0);
-    uint32_t cur = 0, next = 0;
-    // the next pair's slot (entries hold a group's first slot and pair mask)
-    auto next_pair = [&]() {
-        if ((cur & 15u) == 0u) {
-            if ((next & 3u) == 0u) buf = nbr[(size_t)(next >> 2) * n + i];
-            const uint32_t q = next & 3u;
-            cur = q == 0u ? buf.x : (q == 1u ? buf.y : (q == 2u ? buf.z : buf.w));
-            ++next;
-        }
-        const uint32_t k = (uint32_t)__builtin_ctz(cur);  // the lowest set bit is in the mask
-        cur &= cur - 1u;
-        return (cur >> 4) + k;
-    };
-    for (uint32_t t = 0; t < npairs; t += 4) {
-        uint32_t j[4];
-        j[0] = next_pair();
-#pragma unroll
-        for (int k = 1; k < 4; ++k) j[k] = t + (uint32_t)k < npairs ? next_pair() : j[0];
-        float4 xq[4], vq[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            xq[k] = rec[2 * (size_t)j[k]];
-            vq[k] = rec[2 * (size_t)j[k] + 1];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float bx = ax, by = ay, bz = az;
-            pair_term(xi, vi, xq[k], vq[k], h, spiky, vterm, min_dist, inv_min_dist, min_dens, bx, by, bz);
-            // past the last pair (a repeat of the first of the four): nothing added
-            const bool on = t + (uint32_t)k < npairs;
-            ax = on ? bx : ax;
-            ay = on ? by : ay;
-            az = on ? bz : az;
-        }
-    }
-    const float mdt = p.particle_mass * dt_dev[scene];
-    ax *= mdt;
-    ay *= mdt;
-    az *= mdt;
-    if (p.physics) {
-        // the pair increment only: k_extra_terms adds the scene's other terms
-        vel_out[i] = make_float4(ax, ay, az, 0.0f);
-        return;
-    }
-    // w from the state (the record's w holds the pressure)
-    vel_out[i] = make_float4(vi.x + ax, vi.y + ay, vi.z + az, vel[i].w + 0.0f);
-    if (dvel) dvel[i] = make_float4(ax, ay, az, 0.0f);
 }
 
 inline uint32_t blocks_for(uint32_t n) { return n == 0 ? 1u : (n + kBlock - 1) / kBlock; }
@@ -279,18 +171,5 @@ hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, float4*
     return hipGetLastError();
 }
 
-hipError_t launch_force_fast(const LaunchCfg& c, const StepBuffers& b, const float4* rec, const uint4* nbr,
-                             const uint32_t* nnb, float4* vel_out, bool keep_dvel, hipStream_t s) {
-    if (c.n == 0) return hipErrorInvalidValue;
-    float4* dvel = keep_dvel ? b.dvel : nullptr;
-    const dim3 grid((c.n + kFT - 1) / kFT), block(kFT);
-    if (c.tab)
-        hipLaunchKernelGGL(k_force_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.vel_cur, rec,
-                           nbr, nnb, vel_out, dvel, b.dt_dev);
-    else
-        hipLaunchKernelGGL(k_force_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.vel_cur, rec,
-                           nbr, nnb, vel_out, dvel, b.dt_dev);
-    return hipGetLastError();
-}
 
 }  // namespace sph

@@ -246,15 +246,13 @@ hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_ou
 hipError_t launch_extra_terms(const LaunchCfg& c, const StepBuffers& b, float4* vel_out, bool keep_dvel,
                               hipStream_t s);
 // SPH_PREC_FAST (sph_fast.hip): density + EOS writing each slot's force
-// pairs (group-major uint4 lists of fast_list_groups(max_par_in_cell)
-// groups, counts in nnb) and force record (rec: 2 float4 per slot), then the
-// force over those lists and records.
+// pair entries (group-major uint4 lists of fast_list_groups(max_par_in_cell)
+// groups, counts in nnb) and force record (rec: 2 float4 per slot);
+// launch_force then walks those entries with the fast arithmetic.
 constexpr uint32_t kFastMaxCap = 64;
 uint32_t fast_list_groups(uint32_t max_par_in_cell);
 hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, float4* rec, uint4* nbr, uint32_t* nnb,
                                hipStream_t s);
-hipError_t launch_force_fast(const LaunchCfg& c, const StepBuffers& b, const float4* rec, const uint4* nbr,
-                             const uint32_t* nnb, float4* vel_out, bool keep_dvel, hipStream_t s);
 // FP64 per-scene reductions of (mass, kinetic, potential, max speed) over
 // pos_cur/vel_cur, then per-scene finalisation: stats, the next time step
 // (fixed, CFL, clipped at t_end) into dt_dev, and the clock when advance_clock.

@@ -1539,14 +1539,16 @@ __global__ __launch_bounds__(kForceTile) void k_force_tiled(
     }
 }
 
-// Force over the density kernel's pair entries (k_density_entries), one
-// slot per lane: each lane walks only its own pairs, in the reference's
-// order (cells in walk order, slots ascending), with the reference's
-// arithmetic and per-cell partial sums (a new-cell entry adds the open
+// Force over the density kernel's pair entries (k_density_entries, or
+// k_density_fast in FAST mode), one slot per lane: each lane walks only its
+// own pairs, in the reference's order (cells in walk order, slots
+// ascending), with the reference's pair arithmetic.  In the reference order
+// (kPrec 0) with per-cell partial sums (a new-cell entry adds the open
 // partial sum to the total first), so the result equals force_one's bit for
-// bit.  Four pairs per iteration: they are decoded first, so that their
-// eight record loads go out together.  A record is (x, y, z, rho) and
-// (vx, vy, vz, p), one aligned 32-byte block.
+// bit; Kahan: compensated sums; FAST: the entries carry no new-cell flags,
+// so one running sum.  Four pairs per iteration: they are decoded first, so
+// that their eight record loads go out together.  A record is (x, y, z, rho)
+// and (vx, vy, vz, p), one aligned 32-byte block.
 template <bool kBatch, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_force_entries(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
@@ -2134,7 +2136,17 @@ hipError_t launch_force(const LaunchCfg& c, const StepBuffers& b, float4* vel_ou
                         hipStream_t s) {
     float4* dvel = keep_dvel ? b.dvel : nullptr;
     const bool kahan = c.precision == SPH_PREC_KAHAN;
-    if (c.tab) {
+    if (c.precision == SPH_PREC_FAST) {
+        // the pair entries and records of k_density_fast
+        if (!b.pairs || !b.pair_cnt || !b.rec) return hipErrorInvalidValue;
+        const dim3 grid(blocks_for(c.n, kBlock)), block(kBlock);
+        if (c.tab)
+            hipLaunchKernelGGL((k_force_entries<true, SPH_PREC_FAST>), grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n,
+                               b.vel_cur, b.rec, b.pairs, b.pair_cnt, vel_out, dvel, b.dt_dev);
+        else
+            hipLaunchKernelGGL((k_force_entries<false, SPH_PREC_FAST>), grid, block, 0, s, c.one, c.tab, b.sid_cur,
+                               c.n, b.vel_cur, b.rec, b.pairs, b.pair_cnt, vel_out, dvel, b.dt_dev);
+    } else if (c.tab) {
         if (kahan)
             force_impl<true, SPH_PREC_KAHAN>(c, b, vel_out, dvel, s);
         else
