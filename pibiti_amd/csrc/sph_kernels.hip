@@ -1549,6 +1549,8 @@ __global__ __launch_bounds__(kForceTile) void k_force_tiled(
 // so one running sum.  Four pairs per iteration: they are decoded first, so
 // that their eight record loads go out together.  A record is (x, y, z, rho)
 // and (vx, vy, vz, p), one aligned 32-byte block.
+constexpr int kForcePairs = 4;  // pairs decoded and loaded per iteration
+
 template <bool kBatch, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_force_entries(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
@@ -1587,24 +1589,24 @@ __global__ __launch_bounds__(kBlock) void k_force_entries(
         bits &= bits - 1u;
         return jb + k;
     };
-    for (uint32_t t = 0; t < npairs; t += 4) {
-        uint32_t j[4];
-        bool opens[4];
+    for (uint32_t t = 0; t < npairs; t += kForcePairs) {
+        uint32_t j[kForcePairs];
+        bool opens[kForcePairs];
         j[0] = next_pair(opens[0]);
 #pragma unroll
-        for (int k = 1; k < 4; ++k) {
+        for (int k = 1; k < kForcePairs; ++k) {
             opens[k] = false;
             j[k] = t + (uint32_t)k < npairs ? next_pair(opens[k]) : j[0];
         }
-        float4 xq[4], vq[4];
+        float4 xq[kForcePairs], vq[kForcePairs];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kForcePairs; ++k) {
             xq[k] = rec[2 * (size_t)j[k]];
             vq[k] = rec[2 * (size_t)j[k] + 1];
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            // past the last pair (a repeat of the first of the four): nothing added
+        for (int k = 0; k < kForcePairs; ++k) {
+            // past the last pair (a repeat of the batch's first): nothing added
             const bool on = t + (uint32_t)k < npairs;
             const float4 xj = xq[k], vj = vq[k];
             const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
