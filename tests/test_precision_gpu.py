@@ -14,7 +14,9 @@ reported (max and RMS):
   x     relative to |x64| after the whole step (input: the same FP32 state)
 FP64 appears on the GPU only in the global reductions (mass, kinetic and
 potential energy, max speed) that feed the CFL time step; those are checked
-against a host float64 reduction of the read-back state.
+against a host float64 reduction of the read-back state, and a last step
+with the CFL rule active (cfl 0.025: half the fixed step) is checked
+against the oracle's step at the time step the rule picks.
 The report is printed and, when the environment names a directory in
 PIBITI_REPORT_DIR, also written there as precision_4m.json.
 """
@@ -100,6 +102,28 @@ def test_precision_study_4m_colliding_blocks():
             assert errs[f]["max"] < 1e-5, (name, f, errs[f])
     # compensation can only help the sums it compensates
     assert report["kahan"]["rho_relative_bulk"]["rms"] <= report["reference"]["rho_relative_bulk"]["rms"] * 1.05
+
+    # the CFL rule at this scale: the FP64 max speed of the 4M state picks the
+    # step (half the fixed one here), and the step taken with it equals the
+    # FP32 oracle's step at that time step bit for bit
+    q = p.copy()
+    q.cfl = 0.025
+    vmax = float(np.sqrt((vv * vv).sum(1)).max())
+    dt_expect = min(float(q.time_step), float(q.cfl) * float(q.h) / (float(q.sound_speed) + vmax))
+    assert dt_expect < 0.75 * float(q.time_step)  # the rule is active
+    print("case: 4M one CFL step", flush=True)
+    with sph.SphSystem(q) as s:
+        s.reset(x0, v0)
+        assert abs(s.stats()["dt_cfl"] - dt_expect) <= 1e-15
+        s.update(1)
+        xc = s.get_array(sph.SPH_ARRAY_POSITION)
+        vc = s.get_array(sph.SPH_ARRAY_VELOCITY)
+        assert abs(s.stats()["sim_time"] - float(np.float32(dt_expect))) < 1e-15
+    r = p.copy()
+    r.time_step = np.float32(dt_expect)
+    o = oracle.step_f32(r, x0, v0)
+    assert np.array_equal(xc, o["pos"]) and np.array_equal(vc, o["vel"])
+    report["cfl"] = {"cfl": float(q.cfl), "max_speed": vmax, "dt": dt_expect, "fixed_dt": float(q.time_step)}
     print("precision report:", json.dumps(report), flush=True)
     out_dir = os.environ.get("PIBITI_REPORT_DIR")
     if out_dir and Path(out_dir).is_dir():
