@@ -80,9 +80,9 @@ enum { SPH_EOS_LINEAR = 0, SPH_EOS_TAIT = 1 };
  * FAST: integer results exact (keys, order, cell ranges, neighbour sets
  *   with the cap and both predicates) and every float within 1e-5 of the
  *   reference arithmetic and of FP64 (condition-scaled) instead of bit for
- *   bit: one running sum per particle, the force over pair lists the density
- *   kernel writes, hardware square root and reciprocal, fused multiply-adds
- *   (extension; max_par_in_cell <= 64).                                    */
+ *   bit: the density as one running sum per particle with fused
+ *   multiply-adds, the force as one running sum over the pair entries the
+ *   density kernel writes (extension; max_par_in_cell <= 64).              */
 enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1, SPH_PREC_FAST = 2 };
 
 /* Arrays exchanged with the host.  POSITION/VELOCITY keep the reference's
