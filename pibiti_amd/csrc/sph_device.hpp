@@ -288,5 +288,28 @@ struct CellView {
     uint32_t kmin;    // smallest own-cell key of the block
 };
 
+// Four floats at a 4-byte aligned address: a 16-byte global load needs only
+// dword alignment.  Four candidates of the packed position array (three
+// floats a slot) are three of them, where the float4 array takes four; the
+// vector memory unit's cost goes by the wave's load instructions, not by
+// their width.
+struct alignas(4) F4 {
+    float x, y, z, w;
+};
+
+// The positions of slots j0 .. j0 + 3 from the packed array (padded past its
+// end, so the loads need no clamp).
+__device__ __forceinline__ void packed_group(const float* __restrict__ xyz, uint32_t j0, float (&cx)[4],
+                                             float (&cy)[4], float (&cz)[4]) {
+    const F4* q = reinterpret_cast<const F4*>(xyz + 3 * (size_t)j0);
+    const F4 a = q[0];  // x0 y0 z0 x1
+    const F4 b = q[1];  // y1 z1 x2 y2
+    const F4 d = q[2];  // z2 x3 y3 z3
+    cx[0] = a.x, cy[0] = a.y, cz[0] = a.z;
+    cx[1] = a.w, cy[1] = b.x, cz[1] = b.y;
+    cx[2] = b.z, cy[2] = b.w, cz[2] = d.x;
+    cx[3] = d.y, cy[3] = d.z, cz[3] = d.w;
+}
+
 }  // namespace dev
 }  // namespace sph

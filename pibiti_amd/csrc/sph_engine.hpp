@@ -119,6 +119,7 @@ struct Engine {
     uint4* fast_nbr = nullptr;  // force pair entries (fast_groups uint4 per slot, group-major), max_cap <= 64
     uint32_t* fast_nnb = nullptr;
     float4* fast_rec = nullptr;  // force records, 2 per slot
+    float* fast_xyz = nullptr;   // packed positions, 3 floats per slot
     uint32_t fast_groups = 0;
     float4* colour = nullptr;   // per sorted slot, scenes with clr_type != NONE
     float* dye = nullptr;       // per ORIGINAL index (it follows the particle)
@@ -190,6 +191,8 @@ struct Engine {
         b.pairs = lists ? fast_nbr : nullptr;
         b.pair_cnt = lists ? fast_nnb : nullptr;
         b.rec = lists ? fast_rec : nullptr;
+        // the place kernel packs the positions for the pair-entry densities
+        b.xyz = lists && fast_xyz ? fast_xyz : nullptr;
         return b;
     }
 
@@ -213,7 +216,7 @@ struct Engine {
             (void)hipFree(cell_end[k]);
             (void)hipFree(skey_buf[k]);
         }
-        void* ptrs[] = {rhop,     dvel,     masks,    fast_nbr, fast_nnb, fast_rec, colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
+        void* ptrs[] = {rhop,     dvel,     masks,    fast_nbr, fast_nnb, fast_rec, fast_xyz, colour,   dye,      key_pid,  val_pid,    key_tmp, val_tmp,
                         counters, cell_max, work,    partials,
                         stats,    dt_dev,   staging,  sort_scratch, resort_scratch, d_tab, d_scene_off};
         for (void* q : ptrs) (void)hipFree(q);
@@ -379,6 +382,7 @@ struct Engine {
         hipError_t e = dalloc(&fast_nbr, (size_t)g * cfg.n);
         if (e == hipSuccess && !fast_nnb) e = dalloc(&fast_nnb, (size_t)cfg.n);
         if (e == hipSuccess && !fast_rec) e = dalloc(&fast_rec, 2 * (size_t)cfg.n);
+        if (e == hipSuccess && !fast_xyz) e = dalloc(&fast_xyz, 3 * (size_t)cfg.n);
         if (e == hipSuccess) fast_groups = g;
         return e;
     }
@@ -436,7 +440,7 @@ struct Engine {
     // failed (set_params to FAST or to a colour mode under a full device)
     // leaves the context unable to step until a later call allocates them.
     bool buffers_ok() const {
-        if (cfg.max_cap <= kFastMaxCap && (!fast_nbr || !fast_nnb || !fast_rec)) return false;
+        if (cfg.max_cap <= kFastMaxCap && (!fast_nbr || !fast_nnb || !fast_rec || !fast_xyz)) return false;
         if (cfg.colour && (!colour || !dye || !dvel)) return false;
         return true;
     }

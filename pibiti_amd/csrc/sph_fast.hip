@@ -53,6 +53,7 @@ template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const sph_params* __restrict__ tab,
                                                          const uint32_t* __restrict__ sid, uint32_t n,
                                                          const float4* __restrict__ pos,
+                                                         const float* __restrict__ xyz,
                                                          const float4* __restrict__ vel,
                                                          const uint32_t* __restrict__ cell_start,
                                                          const uint32_t* __restrict__ cell_end,
@@ -68,14 +69,10 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     CellWalk w;
     w.init(p, xi.x, xi.y, xi.z, walk_culled(counters));
     const float h2 = p.h2, r2f = p.force_r2_max;
+    const float inv_h2 = 1.0f / h2;
     uint32_t* const slot0 = ring + threadIdx.x;
     uint32_t count = 0, flushed = 0, pairs = 0;
-    float sum = 0.0f;
-    // The candidates [s, e) of one run, four at a time: a candidate past the
-    // run's end (and, in the particle's own row, the particle itself) takes
-    // r2 = +inf, which fails both predicates.  The density weight
-    // max(h2 - r2, 0)^3 is the reference's (r2 < h2 ? (h2 - r2)^3 : 0),
-    // also for a NaN r2 (max gives 0).
+    float sum = 0.0f;  // of (1 - r2 / h2)^3; times h2^3 below
     // one list entry per four consecutive candidates with a pair; the
     // written slot is only kept when the count advances
     auto append = [&](uint32_t j0, uint32_t bits) {
@@ -88,30 +85,46 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
             flushed += kRing;
         }
     };
-    auto walk_run = [&](uint32_t s, uint32_t e, auto self_row) {
-        constexpr bool kSelfRow = decltype(self_row)::value;
-        const float4* q = pos + s;
-        for (uint32_t j0 = s; j0 < e; j0 += kGroup, q += kGroup) {
-            // kGroup loads in flight; position arrays are padded past their end
-            float4 xq[kGroup];
+    // The candidates j0 .. j0 + 3; in the tail group of a run only the
+    // first `left` of them (the others take r2 = +inf, which fails both
+    // predicates).  The predicates use the reference's r2 exactly.  The
+    // density weight is (1 - r2 / h2)^3 clamped to [0, 1], one fused
+    // multiply-add with an output clamp: 0 for r2 >= h2 and for a NaN r2,
+    // as the reference's (r2 < h2 ? (h2 - r2)^3 : 0) is.
+    // The candidates' positions come from the packed array (packed_group:
+    // three loads for four candidates); those loads are what bounds this
+    // walk.
+    auto group = [&](uint32_t j0, uint32_t left, auto tail) {
+        constexpr bool kTail = decltype(tail)::value;
+        float cx[kGroup], cy[kGroup], cz[kGroup];
+        packed_group(xyz, j0, cx, cy, cz);
+        uint32_t bits = 0;
 #pragma unroll
-            for (int k = 0; k < kGroup; ++k) xq[k] = q[k];
-            const uint32_t left = e - j0;
-            uint32_t bits = 0;
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-                // the predicates need the reference's r2 exactly
-                const float rx = xi.x - xq[k].x, ry = xi.y - xq[k].y, rz = xi.z - xq[k].z;
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                bool valid = left > (uint32_t)k;
-                if constexpr (kSelfRow) valid = valid && j0 + (uint32_t)k != i;
-                const float r2v = valid ? r2 : __builtin_huge_valf();
-                const float c = fmaxf(h2 - r2v, 0.0f);
-                sum = __builtin_fmaf(c * c, c, sum);
-                bits |= force_pair(r2v, r2f) ? (1u << k) : 0u;
-            }
-#pragma unroll
-            for (int g = 0; g < kGroup / 4; ++g) append(j0 + 4u * (uint32_t)g, (bits >> (4 * g)) & 15u);
+        for (int k = 0; k < kGroup; ++k) {
+            const float rx = xi.x - cx[k], ry = xi.y - cy[k], rz = xi.z - cz[k];
+            float r2 = rx * rx + ry * ry + rz * rz;
+            if constexpr (kTail)
+                if (k > 0) r2 = left > (uint32_t)k ? r2 : __builtin_huge_valf();
+            const float c = fminf(fmaxf(__builtin_fmaf(r2, -inv_h2, 1.0f), 0.0f), 1.0f);
+            sum = __builtin_fmaf(c * c, c, sum);
+            bits |= force_pair(r2, r2f) ? (1u << k) : 0u;
+        }
+        append(j0, bits);
+    };
+    // the candidates [s, e): full groups, then one tail group
+    auto walk_run = [&](uint32_t s, uint32_t e) {
+        uint32_t left = e - s;
+        for (; left >= (uint32_t)kGroup; left -= kGroup, s += kGroup) group(s, kGroup, std::false_type{});
+        if (left) group(s, left, std::true_type{});
+    };
+    // a run of the particle's own row without the particle itself (it is in
+    // the run unless its cell is capped before it)
+    auto walk_own = [&](uint32_t s, uint32_t e) {
+        if (i >= s && i < e) {
+            walk_run(s, i);
+            walk_run(i + 1, e);
+        } else {
+            walk_run(s, e);
         }
     };
 #pragma unroll 1
@@ -129,23 +142,23 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
                 if (s != re) {
                     // a break: a capped cell before (or the row's first open cell)
                     if (own)
-                        walk_run(rs, re, std::true_type{});
+                        walk_own(rs, re);
                     else
-                        walk_run(rs, re, std::false_type{});
+                        walk_run(rs, re);
                     rs = s;
                 }
                 re = e;
             }
             if (own)
-                walk_run(rs, re, std::true_type{});
+                walk_own(rs, re);
             else
-                walk_run(rs, re, std::false_type{});
+                walk_run(rs, re);
         }
     }
     if (count > flushed)
         nbr[(size_t)(flushed >> 2) * n + i] = make_uint4(slot0[0], slot0[kBlock], slot0[2 * kBlock], slot0[3 * kBlock]);
     nnb[i] = count | (pairs << 16);
-    const float rho = (sum * p.poly6_kern) * p.particle_mass;
+    const float rho = (sum * (((p.poly6_kern * h2) * h2) * h2)) * p.particle_mass;
     const float pr = eos(p, rho);
     rhop[i] = make_float2(rho, pr);
     const float4 vi = vel[i];
@@ -161,13 +174,14 @@ uint32_t fast_list_groups(uint32_t max_par_in_cell) { return (27u * ((max_par_in
 
 hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, float4* rec, uint4* nbr, uint32_t* nnb,
                                hipStream_t s) {
+    if (!b.xyz) return hipErrorInvalidValue;  // the place kernel packs the positions only when FAST is set
     const dim3 grid(blocks_for(c.n)), block(kBlock);
     if (c.tab)
         hipLaunchKernelGGL(k_density_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.vel_cur, b.cell_start, b.cell_end, b.rhop, rec, nbr, nnb, b.counters);
+                           b.xyz, b.vel_cur, b.cell_start, b.cell_end, b.rhop, rec, nbr, nnb, b.counters);
     else
         hipLaunchKernelGGL(k_density_fast<false>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,
-                           b.vel_cur, b.cell_start, b.cell_end, b.rhop, rec, nbr, nnb, b.counters);
+                           b.xyz, b.vel_cur, b.cell_start, b.cell_end, b.rhop, rec, nbr, nnb, b.counters);
     return hipGetLastError();
 }
 

@@ -156,6 +156,7 @@ struct StepBuffers {
     uint4* pairs;         // pair-entry lists (group-major), or null: see kPairSlotShift
     uint32_t* pair_cnt;   // entries | pairs << 16 per slot
     float4* rec;          // force records, 2 float4 per slot
+    float* xyz;           // x, y, z per slot, packed (the pair-entry densities' candidates), or null
 };
 
 // Force pair masks: 27 cells x 16 bits in 14 words per slot (word w of slot

@@ -670,7 +670,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate_moves(
 //     cell's listed movers in with a smaller index;
 //   full sort: new slot k takes the sorted pair's old slot.
 // Marks the new order valid; the step's own validity was copied to
-// valid_step by the cell scan.
+// valid_step by the cell scan.  xyz_out (null unless the pair-entry gathers
+// are in use) takes the positions packed, three floats per slot, which the
+// pair-entry densities read their candidates from.
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_resort_place(
     sph_params one, const sph_params* __restrict__ tab, uint32_t n, const uint32_t* __restrict__ nkey,
@@ -683,7 +685,8 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
     const uint32_t* __restrict__ pid_in, const uint32_t* __restrict__ sid_in, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ pid_out, uint32_t* __restrict__ sid_out,
     uint32_t* __restrict__ skey_out, uint32_t* __restrict__ counters, uint32_t* __restrict__ err,
-    const float* __restrict__ dt_dev, uint32_t* __restrict__ repair, uint32_t* __restrict__ repairs) {
+    const float* __restrict__ dt_dev, uint32_t* __restrict__ repair, uint32_t* __restrict__ repairs,
+    float* __restrict__ xyz_out) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         // a repaired step's tile totals may be off: the next step starts
@@ -762,6 +765,11 @@ __global__ __launch_bounds__(kBlock) void k_resort_place(
     vel_out[dst] = v4;
     pid_out[dst] = id;
     skey_out[dst] = c;
+    if (xyz_out) {
+        xyz_out[3 * (size_t)dst] = x4.x;
+        xyz_out[3 * (size_t)dst + 1] = x4.y;
+        xyz_out[3 * (size_t)dst + 2] = x4.z;
+    }
     if constexpr (kBatch) sid_out[dst] = scene;
     // the cell's last slot counts a cell above the candidate cap
     if (full) {
@@ -795,6 +803,13 @@ struct PosGlobal {
     __device__ __forceinline__ float4 at(uint32_t j, int32_t) const {
         return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(p) + (j << 4));
     }
+};
+
+// The packed array (three floats a slot), read four candidates at a time
+// (packed_group); the walk's own positions still come from the float4 array.
+struct PosPacked {
+    const float* __restrict__ xyz;
+    __device__ __forceinline__ int32_t offset(int) const { return 0; }
 };
 
 struct PosStaged {
@@ -924,7 +939,8 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
     // Position arrays in global memory are padded past their end (dalloc):
     // a group's loads past the cell's end stay in bounds and are masked out,
     // so they need no clamp.  LDS-staged positions keep it.
-    constexpr bool kUnclamped = std::is_same<Src, PosGlobal>::value;
+    constexpr bool kPacked = std::is_same<Src, PosPacked>::value;
+    constexpr bool kUnclamped = std::is_same<Src, PosGlobal>::value || kPacked;
     uint32_t mword = 0;
     // the 27 cells in (dz, dy, dx) order; a row's grid test, distance and
     // key are computed once for its three cells
@@ -954,9 +970,16 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                     auto walk = [&](auto self_tag) {
                         constexpr bool kSelf = decltype(self_tag)::value;
                         for (uint32_t j0 = s; j0 < e; j0 += 4) {
-                            float4 xq[4];
+                            float cx[4], cy[4], cz[4];
+                            if constexpr (kPacked) {
+                                packed_group(src.xyz, j0, cx, cy, cz);
+                            } else {
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) xq[k] = src.at(kUnclamped ? j0 + k : min(j0 + k, e - 1), soff);
+                                for (int k = 0; k < 4; ++k) {
+                                    const float4 xq = src.at(kUnclamped ? j0 + k : min(j0 + k, e - 1), soff);
+                                    cx[k] = xq.x, cy[k] = xq.y, cz[k] = xq.z;
+                                }
+                            }
                             const uint32_t left = e - j0;
                             const uint32_t self = kSelf ? i - j0 : 0u;
                             uint32_t fbits = 0;
@@ -964,8 +987,7 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                             for (int k = 0; k < 4; ++k) {
                                 const uint32_t j = j0 + k;
                                 const bool valid = left > (uint32_t)k && (!kSelf || self != (uint32_t)k);
-                                const float4 xj = xq[k];
-                                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                                const float rx = xi.x - cx[k], ry = xi.y - cy[k], rz = xi.z - cz[k];
                                 const float r2 = rx * rx + ry * ry + rz * rz;
                                 const bool hit = (r2 < h2) && valid;
                                 const float c3 = h2 - r2;
@@ -1387,14 +1409,16 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(
 template <bool kBatch, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_density_entries(
     sph_params one, const sph_params* __restrict__ tab, const uint32_t* __restrict__ sid, uint32_t n,
-    const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ cell_start,
-    const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop, uint4* __restrict__ entries,
-    uint32_t* __restrict__ nnb, float4* __restrict__ rec, uint32_t* __restrict__ counters) {
+    const float4* __restrict__ pos, const float* __restrict__ xyz, const float4* __restrict__ vel,
+    const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end, float2* __restrict__ rhop,
+    uint4* __restrict__ entries, uint32_t* __restrict__ nnb, float4* __restrict__ rec,
+    uint32_t* __restrict__ counters) {
     const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    density_one<kBatch, kPrec, false, PosGlobal, false, false, true>(
+    const PosPacked src{xyz};
+    density_one<kBatch, kPrec, false, PosPacked, false, false, true>(
         i, one, tab, sid, pos, cell_start, cell_end, rhop, n, reinterpret_cast<uint32_t*>(entries), nnb, counters,
-        nullptr, nullptr, rec, vel);
+        &src, nullptr, rec, vel);
 }
 
 // Cell-table stage: the (start, end) pairs of the block's 9 neighbour rows
@@ -1915,7 +1939,7 @@ hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const Res
                        b.cell_end_alt, rp.in_cnt[parity], rp.inlist, rp.in_cap, skey, oslot, b.pos_cur, b.vel_cur,
                        b.pid_cur,
                        b.sid_cur, b.pos_alt, b.vel_alt, b.pid_alt, b.sid_alt, b.skey_alt, b.counters, rp.err,
-                       b.dt_dev, rp.repair, rp.repairs);
+                       b.dt_dev, rp.repair, rp.repairs, b.xyz);
     return hipGetLastError();
 }
 
@@ -1927,7 +1951,7 @@ hipError_t resort_apply_impl(const LaunchCfg& c, const StepBuffers& b, const Res
 // candidate-testing force's list size.
 bool use_entries(const LaunchCfg& c, const StepBuffers& b) {
     const bool asked = c.tune[0] == 3 || (c.tune[0] == 0 && c.tune[1] == 0 && c.tune[3] == 0);
-    return asked && b.pairs && c.max_cap <= kFastMaxCap;
+    return asked && b.pairs && b.xyz && c.max_cap <= kFastMaxCap;
 }
 
 bool use_masks(const LaunchCfg& c, const StepBuffers& b) {
@@ -1940,8 +1964,8 @@ void density_impl(const LaunchCfg& c, const StepBuffers& b, hipStream_t s) {
     const uint32_t grid = c.tune[1];
     if (use_entries(c, b))
         hipLaunchKernelGGL((k_density_entries<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
-                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.vel_cur, b.cell_start, b.cell_end, b.rhop,
-                           b.pairs, b.pair_cnt, b.rec, b.counters);
+                           c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.xyz, b.vel_cur, b.cell_start, b.cell_end,
+                           b.rhop, b.pairs, b.pair_cnt, b.rec, b.counters);
     else if (use_masks(c, b))
         hipLaunchKernelGGL((k_density_masks<kBatch, kPrec>), dim3(blocks_for(c.n, kBlock)), dim3(kBlock), 0, s,
                            c.one, c.tab, b.sid_cur, c.n, b.pos_cur, b.cell_start, b.cell_end, b.rhop, b.masks, b.counters);
