@@ -297,11 +297,12 @@ struct alignas(4) F4 {
     float x, y, z, w;
 };
 
-// The positions of slots j0 .. j0 + 3 from the packed array (padded past its
-// end, so the loads need no clamp).
-__device__ __forceinline__ void packed_group(const float* __restrict__ xyz, uint32_t j0, float (&cx)[4],
-                                             float (&cy)[4], float (&cz)[4]) {
-    const F4* q = reinterpret_cast<const F4*>(xyz + 3 * (size_t)j0);
+// The positions of four consecutive slots from the packed array, q = the
+// first one's x (the array is padded past its end, so the loads need no
+// clamp).
+__device__ __forceinline__ void packed_group(const float* __restrict__ xq, float (&cx)[4], float (&cy)[4],
+                                             float (&cz)[4]) {
+    const F4* q = reinterpret_cast<const F4*>(xq);
     const F4 a = q[0];  // x0 y0 z0 x1
     const F4 b = q[1];  // y1 z1 x2 y2
     const F4 d = q[2];  // z2 x3 y3 z3

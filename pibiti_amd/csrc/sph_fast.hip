@@ -75,8 +75,8 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     float sum = 0.0f;  // of (1 - r2 / h2)^3; times h2^3 below
     // one list entry per four consecutive candidates with a pair; the
     // written slot is only kept when the count advances
-    auto append = [&](uint32_t j0, uint32_t bits) {
-        slot0[(count & (kRing - 1u)) * kBlock] = (j0 << kPairSlotShift) | bits;
+    auto append = [&](uint32_t tag, uint32_t bits) {
+        slot0[(count & (kRing - 1u)) * kBlock] = tag | bits;
         count += bits ? 1u : 0u;
         pairs += (uint32_t)__popc(bits);
         if (count - flushed == kRing) {
@@ -94,10 +94,10 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
     // The candidates' positions come from the packed array (packed_group:
     // three loads for four candidates); those loads are what bounds this
     // walk.
-    auto group = [&](uint32_t j0, uint32_t left, auto tail) {
+    auto group = [&](const float* q, uint32_t tag, uint32_t left, auto tail) {
         constexpr bool kTail = decltype(tail)::value;
         float cx[kGroup], cy[kGroup], cz[kGroup];
-        packed_group(xyz, j0, cx, cy, cz);
+        packed_group(q, cx, cy, cz);
         uint32_t bits = 0;
 #pragma unroll
         for (int k = 0; k < kGroup; ++k) {
@@ -109,13 +109,16 @@ __global__ __launch_bounds__(kBlock) void k_density_fast(sph_params one, const s
             sum = __builtin_fmaf(c * c, c, sum);
             bits |= force_pair(r2, r2f) ? (1u << k) : 0u;
         }
-        append(j0, bits);
+        append(tag, bits);
     };
     // the candidates [s, e): full groups, then one tail group
+    // (tag: the group's first slot, shifted into place for its entry)
     auto walk_run = [&](uint32_t s, uint32_t e) {
-        uint32_t left = e - s;
-        for (; left >= (uint32_t)kGroup; left -= kGroup, s += kGroup) group(s, kGroup, std::false_type{});
-        if (left) group(s, left, std::true_type{});
+        const float* q = xyz + 3 * (size_t)s;
+        uint32_t tag = s << kPairSlotShift, left = e - s;
+        for (; left >= (uint32_t)kGroup; left -= kGroup, q += 3 * kGroup, tag += kGroup << kPairSlotShift)
+            group(q, tag, kGroup, std::false_type{});
+        if (left) group(q, tag, left, std::true_type{});
     };
     // a run of the particle's own row without the particle itself (it is in
     // the run unless its cell is capped before it)
@@ -174,7 +177,7 @@ uint32_t fast_list_groups(uint32_t max_par_in_cell) { return (27u * ((max_par_in
 
 hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, float4* rec, uint4* nbr, uint32_t* nnb,
                                hipStream_t s) {
-    if (!b.xyz) return hipErrorInvalidValue;  // the place kernel packs the positions only when FAST is set
+    if (!b.xyz) return hipErrorInvalidValue;  // the place kernel packs the positions
     const dim3 grid(blocks_for(c.n)), block(kBlock);
     if (c.tab)
         hipLaunchKernelGGL(k_density_fast<true>, grid, block, 0, s, c.one, c.tab, b.sid_cur, c.n, b.pos_cur,

@@ -969,10 +969,13 @@ __device__ __forceinline__ uint32_t density_one(uint32_t i, const sph_params& on
                     // takes the walk with the extra test
                     auto walk = [&](auto self_tag) {
                         constexpr bool kSelf = decltype(self_tag)::value;
+                        const float* xqp = nullptr;  // kPacked: slot j0's x
+                        if constexpr (kPacked) xqp = src.xyz + 3 * (size_t)s;
                         for (uint32_t j0 = s; j0 < e; j0 += 4) {
                             float cx[4], cy[4], cz[4];
                             if constexpr (kPacked) {
-                                packed_group(src.xyz, j0, cx, cy, cz);
+                                packed_group(xqp, cx, cy, cz);
+                                xqp += 12;
                             } else {
 #pragma unroll
                                 for (int k = 0; k < 4; ++k) {
