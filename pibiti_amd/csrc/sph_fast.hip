@@ -10,7 +10,8 @@
 // fused multiply-adds instead of per-cell partial sums added in cell order
 // (source/CUDA/Kernel_Cell.cui:142-199); the candidate walk and predicates
 // are the reference's.  The force is the reference order's pair-entry
-// kernel (k_force_entries, sph_kernels.hip) with the fast arithmetic.
+// kernel (k_force_entries, sph_kernels.hip) with one running sum instead of
+// per-cell partial sums.
 //
 // Pair lists.  The density walk tests candidates four at a time (four
 // consecutive slots of one row's run); a group with at least one force pair
@@ -38,11 +39,13 @@ constexpr int kGroup = 4;      // candidates per iteration of the density walk
 //
 // The walk goes by rows: the neighbour cells (dx = -1, 0, 1) of one (dy, dz)
 // have consecutive keys, so their slots form one run, and the candidates of
-// a row are that run, four consecutive slots at a time.  A run breaks only
-// after a cell capped at max_par_in_cell (its slots past the cap are not
-// candidates); the slots before the break are walked as a run of their own.
-// The lanes of one cell then walk the same runs in step and load the same
-// slots, and a row costs one loop instead of three.  Culling drops a side
+// a row are that run, four consecutive slots at a time (three 16-byte loads
+// from the packed positions).  A run breaks only after a cell capped at
+// max_par_in_cell (its slots past the cap are not candidates); the slots
+// before the break are walked as a run of their own, and the own row's run
+// is split at the particle itself.  The lanes of one cell then walk the same
+// runs in step (in the own row, nearly) and load the same cache lines, and a
+// row costs one loop instead of three.  Culling drops a side
 // cell of a row whose box is farther than h (the middle cell of a row that
 // is kept is never farther than the row).
 //
