@@ -24,20 +24,29 @@ TOL = 1e-5
 FAST = sph.SPH_PREC_FAST
 
 
+def read_state(s):
+    return dict(pos=s.get_array(sph.SPH_ARRAY_POSITION), vel=s.get_array(sph.SPH_ARRAY_VELOCITY),
+                dens=s.get_array(sph.SPH_ARRAY_DENSITY), pres=s.get_array(sph.SPH_ARRAY_PRESSURE),
+                dvel=s.get_array(sph.SPH_ARRAY_DVEL), nnb=s.get_array(sph.SPH_ARRAY_NEIGHBOURS),
+                grid=s.get_grid(), stats=s.stats())
+
+
 def fast_step(p, pos, vel):
     q = p.copy()
     q.precision = FAST
     with sph.SphSystem(q, keep_dvel=True) as s:
         s.reset(pos, vel)
         s.update(1)
-        return dict(pos=s.get_array(sph.SPH_ARRAY_POSITION), vel=s.get_array(sph.SPH_ARRAY_VELOCITY),
-                    dens=s.get_array(sph.SPH_ARRAY_DENSITY), pres=s.get_array(sph.SPH_ARRAY_PRESSURE),
-                    dvel=s.get_array(sph.SPH_ARRAY_DVEL), nnb=s.get_array(sph.SPH_ARRAY_NEIGHBOURS),
-                    grid=s.get_grid(), stats=s.stats())
+        return read_state(s)
 
 
 def check_fast_step(p, pos, vel, what):
-    g = fast_step(p, pos, vel)
+    return compare_fast(fast_step(p, pos, vel), p, pos, vel, what)
+
+
+def compare_fast(g, p, pos, vel, what):
+    """g: the state after one FAST step from (pos, vel), checked against the
+    oracle's step from the same state under the FAST contract."""
     o = oracle.step_f32(p, pos, vel)
     gr = g["grid"]
     for k in ("sorted_hash", "perm", "cell_start", "cell_end"):
@@ -101,6 +110,61 @@ def test_fast_single_step_1m_with_crowded_cells():
     o = oracle.step_f32(p, x, v)
     assert o["counters"][0] >= 2  # cells above the cap
     check_fast_step(p, x, v, "1M crowded")
+
+
+def test_fast_partial_upload_between_steps():
+    """Host uploads between FAST steps (the emitters' path): velocities of 300
+    particles and positions of 100 replaced after three steps.  The next step
+    starts from the uploaded state, so its order, packed candidate positions
+    and walk see the new positions: the FAST contract against the oracle's
+    step from that state (the reference-mode twin is
+    test_gpu_parity.py::test_partial_upload_between_steps)."""
+    print("case: FAST partial upload between steps", flush=True)
+    p, pos, vel, _ = scenes.scaled_dam_break((20, 12, 20), seed=4)
+    q = p.copy()
+    q.precision = FAST
+    with sph.SphSystem(q, keep_dvel=True) as s:
+        s.reset(pos, vel)
+        s.update(3)
+        x = s.get_array(sph.SPH_ARRAY_POSITION)
+        v = s.get_array(sph.SPH_ARRAY_VELOCITY)
+        newv = v.copy()
+        newv[1000:1300, :3] = np.float32([0.0, -0.5, 0.1])
+        s.set_array(sph.SPH_ARRAY_VELOCITY, newv[1000:1300], first=1000)
+        newx = x.copy()
+        newx[2000:2100, 1] += np.float32(0.05)
+        s.set_array(sph.SPH_ARRAY_POSITION, newx[2000:2100], first=2000)
+        s.update(1)
+        g = read_state(s)
+    compare_fast(g, p, newx, newv, "FAST partial upload")
+
+
+def test_fast_after_reference_steps():
+    """A context switched from the reference order to FAST between steps
+    (sph_set_params): the FAST step after the switch meets the FAST contract
+    from the state the reference steps left, and switching back gives the
+    reference step bit for bit."""
+    print("case: reference steps, then FAST, then reference again", flush=True)
+    p, pos, vel, _ = scenes.scaled_dam_break((20, 12, 20), seed=8)
+    with sph.SphSystem(p, keep_dvel=True) as s:
+        s.reset(pos, vel)
+        s.update(2)
+        x = s.get_array(sph.SPH_ARRAY_POSITION)
+        v = s.get_array(sph.SPH_ARRAY_VELOCITY)
+        q = p.copy()
+        q.precision = FAST
+        s.set_params(q)
+        s.update(1)
+        g = read_state(s)
+        s.set_params(p)
+        s.update(1)
+        # (neighbour counts are a FAST-mode output: not read here)
+        r = {k: s.get_array(a) for k, a in (("pos", sph.SPH_ARRAY_POSITION), ("vel", sph.SPH_ARRAY_VELOCITY),
+                                            ("dens", sph.SPH_ARRAY_DENSITY), ("pres", sph.SPH_ARRAY_PRESSURE))}
+    compare_fast(g, p, x, v, "FAST after reference steps")
+    o = oracle.step_f32(p, g["pos"], g["vel"])
+    for k in ("pos", "vel", "dens", "pres"):
+        assert np.array_equal(r[k], o[k]), f"reference step after FAST: {k}"
 
 
 def test_fast_batch_equals_scenes_alone_and_graph_replays():
