@@ -176,3 +176,47 @@ def test_graph_replay_of_a_still_shape_and_no_capture_of_a_moving_wave():
         s.reset(pos, vel)
         with pytest.raises(sph.SphError):
             s.capture(2)
+
+
+FAST_CASES = ["accelerators", "collider", "height_map_xz", "propeller_pair", "pump_s_outlet", "rotor_pump_z",
+              "sphere", "wave_moving"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", FAST_CASES)
+def test_shape_fast_step(case):
+    """One FAST step of the shape, rotor, height-map, collider and
+    accelerator scenes against the FP32 oracle's step under the FAST
+    contract: keys, order, cell ranges, positions and force-pair counts
+    exact; density and pressure within 1e-5 of the oracle, scaled by their
+    conditions; velocities (the pair force in FAST arithmetic plus the
+    scene's other terms in the reference's) within 1e-5 of the largest
+    speed."""
+    from conftest import scaled_err
+    kw = dict(CASES[case], prec=sph.SPH_PREC_FAST)
+    p, pos, vel = shape_scene(**kw)
+    print(f"case: FAST {case}, n = {p.num_particles}, one step", flush=True)
+    with sph.SphSystem(p, keep_dvel=True) as s:
+        s.reset(pos, vel)
+        s.update(1)
+        g = dict(pos=s.get_array(sph.SPH_ARRAY_POSITION), vel=s.get_array(sph.SPH_ARRAY_VELOCITY),
+                 dens=s.get_array(sph.SPH_ARRAY_DENSITY), pres=s.get_array(sph.SPH_ARRAY_PRESSURE),
+                 nnb=s.get_array(sph.SPH_ARRAY_NEIGHBOURS), grid=s.get_grid())
+    q = p.copy()
+    q.precision = sph.SPH_PREC_REFERENCE
+    o = oracle.step_f32(q, pos, vel)
+    for k in ("sorted_hash", "perm", "cell_start", "cell_end"):
+        assert np.array_equal(g["grid"][k], o[k]), f"{case} {k}"
+    assert np.array_equal(g["pos"], o["pos"]), f"{case} positions"
+    cnt = oracle.neighbour_counts_f32(q, o["pos"][o["perm"]], o["cell_start"], o["cell_end"], pred=1)
+    nnb = np.empty_like(cnt)
+    nnb[o["perm"]] = cnt
+    assert np.array_equal(g["nnb"], nnb), f"{case} force pair counts"
+    d = oracle.kernels_f64_on(o, q)
+    errs = dict(dens=scaled_err(g["dens"], o["dens"], d["dens_scale"]),
+                pres=scaled_err(g["pres"], o["pres"], d["pres_scale"]),
+                vel=float(np.abs(g["vel"][:, :3].astype(np.float64) - o["vel"][:, :3]).max()
+                          / np.abs(o["vel"][:, :3]).max()))
+    print(case, "scaled errors:", errs, flush=True)
+    for k, e in errs.items():
+        assert e < 1e-5, f"{case} {k}: {e}"
