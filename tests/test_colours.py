@@ -188,6 +188,46 @@ def test_colours_bit_exact_on_gpu(case):
         assert o["dye"].max() > 0
 
 
+FAST_CASES = [(sph.SPH_CLR_VEL, 0, sph.SPH_DYE_BOX, 0), (sph.SPH_CLR_DENS_ACC, 1, sph.SPH_DYE_SPHERE, 0),
+              (sph.SPH_CLR_ACCEL, 1, 0, 0)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", FAST_CASES)
+def test_colours_fast_mode_one_step(case):
+    """The colour pass after a FAST step: the colours follow the step's
+    velocities, densities and force increments, which FAST keeps within
+    1e-5 of the reference, so the colours agree within 1e-4 (absolute,
+    on components in [0, 1]); positions and the dye, which follows them,
+    are exact."""
+    clr, hue, dye_type, clear = case
+    rng = np.random.default_rng(3)
+    g = np.stack(np.meshgrid(*[np.arange(-0.1, 0.1, 0.01)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    n = len(g)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = g + rng.uniform(-0.001, 0.001, g.shape)
+    pos[:, 3] = 1
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.normal(0, 1.0, (n, 3))
+    p = params(n, clr, hue=hue, dye_type=dye_type, dye_clear=clear, dye_pos=[0.02, -0.03, 0.0],
+               dye_size=[0.03, 0.04, 0.05], dye_fade=20.0, time_step=2e-4, particle_mass=1e-3,
+               collider=1, coll_pos=[0.05, 0.0, 0.0], coll_r=0.02)
+    q = p.copy()
+    q.precision = sph.SPH_PREC_FAST
+    print("case: FAST colour", case, "n =", n, flush=True)
+    with sph.SphSystem(q) as s:
+        s.reset(pos, vel)
+        s.update(1)
+        got = dict(pos=s.get_array(sph.SPH_ARRAY_POSITION), colour=s.get_array(sph.SPH_ARRAY_COLOR),
+                   dye=s.get_array(sph.SPH_ARRAY_DYE))
+    o = oracle.run_f32_colour(p, pos, vel, 1)
+    assert np.array_equal(got["pos"], o["pos"])
+    assert np.array_equal(got["dye"], o["dye"])
+    err = float(np.abs(got["colour"].astype(np.float64) - o["colour"]).max())
+    print(case, "colour max abs diff", err, flush=True)
+    assert err < 1e-4, f"{case}: colour differs by {err}"
+
+
 def hip_runtime():
     import ctypes as C
     h = C.CDLL("libamdhip64.so")
