@@ -220,3 +220,28 @@ def test_shape_fast_step(case):
     print(case, "scaled errors:", errs, flush=True)
     for k, e in errs.items():
         assert e < 1e-5, f"{case} {k}: {e}"
+
+
+@pytest.mark.gpu
+def test_fast_shape_scenes_in_one_batch_equal_each_alone():
+    """FAST is deterministic per particle, so a batch of still shapes with a
+    collider and accelerators gives, after several steps, exactly what each
+    scene gives alone in FAST mode."""
+    names = ["sphere", "cylinder_z", "collider", "accelerators"]
+    scenes = [shape_scene(**dict(CASES[k], prec=sph.SPH_PREC_FAST)) for k in names]
+    ps = [sc[0] for sc in scenes]
+    print("case: FAST batch of", names, flush=True)
+    with sph.SphBatch(ps) as b:
+        for i, (p, pos, vel) in enumerate(scenes):
+            b.reset(i, pos, vel)
+        b.update(STEPS)
+        got = [(b.get_array(i, sph.SPH_ARRAY_POSITION), b.get_array(i, sph.SPH_ARRAY_VELOCITY),
+                b.get_array(i, sph.SPH_ARRAY_DENSITY)) for i in range(len(scenes))]
+    for i, (p, pos, vel) in enumerate(scenes):
+        with sph.SphSystem(p) as s:
+            s.reset(pos, vel)
+            s.update(STEPS)
+            alone = (s.get_array(sph.SPH_ARRAY_POSITION), s.get_array(sph.SPH_ARRAY_VELOCITY),
+                     s.get_array(sph.SPH_ARRAY_DENSITY))
+        for k, (a, c) in enumerate(zip(got[i], alone)):
+            assert np.array_equal(a, c), f"{names[i]} array {k}"
