@@ -4,7 +4,8 @@ The blocks are advanced on the GPU until they interpenetrate, then one step
 is taken in both accumulation modes (reference order and Kahan-compensated
 FP32).  Each is checked bit for bit against the FP32 oracle, and its
 per-field error against the FP64 oracle on the same post-integrate state is
-reported (max and RMS):
+reported (max and RMS); so is a FAST step's (integer results exact, fields
+within the same bounds):
   rho   against S_rho = m Poly6 sum_j [c^3 + 3 c^2 (h2 + r2)] (the sum's error
         scale: c = h2 - r2 cancels near h), and plainly relative to rho64
         over the bulk fluid (rho64 >= rho0/2) and over all particles
@@ -102,6 +103,32 @@ def test_precision_study_4m_colliding_blocks():
             assert errs[f]["max"] < 1e-5, (name, f, errs[f])
     # compensation can only help the sums it compensates
     assert report["kahan"]["rho_relative_bulk"]["rms"] <= report["reference"]["rho_relative_bulk"]["rms"] * 1.05
+
+    # FAST on the same state: integer results exact (keys, order, cell
+    # ranges, force-pair counts; positions after the step), the fields under
+    # the same 1e-5 bounds against FP64
+    q = p.copy()
+    q.precision = sph.SPH_PREC_FAST
+    print("case: 4M one step, precision fast", flush=True)
+    with sph.SphSystem(q, keep_dvel=True) as s:
+        s.reset(x0, v0)
+        s.update(1)
+        g = {k: s.get_array(i) for i, k in enumerate(["pos", "vel", "dens", "pres", "dvel"])}
+        nnb = s.get_array(sph.SPH_ARRAY_NEIGHBOURS)
+        grid = s.get_grid()
+    o = oracle.step_f32(p, x0, v0)
+    for k in ("sorted_hash", "perm", "cell_start", "cell_end"):
+        assert np.array_equal(grid[k], o[k]), f"fast {k}"
+    assert np.array_equal(g["pos"], o["pos"]), "fast positions"
+    cnt = oracle.neighbour_counts_f32(p, o["pos"][o["perm"]], o["cell_start"], o["cell_end"], pred=1)
+    want = np.empty_like(cnt)
+    want[o["perm"]] = cnt
+    assert np.array_equal(nnb, want), "fast force pair counts"
+    d = oracle.kernels_f64_on(o, p)
+    errs = field_errors(g, d, o, p)
+    report["fast"] = errs
+    for f in ("rho", "rho_relative_bulk", "p", "dv_scaled"):
+        assert errs[f]["max"] < 1e-5, ("fast", f, errs[f])
 
     # the CFL rule at this scale: the FP64 max speed of the 4M state picks the
     # step (half the fixed one here), and the step taken with it equals the
