@@ -92,9 +92,11 @@ enum { SPH_PREC_REFERENCE = 0, SPH_PREC_KAHAN = 1, SPH_PREC_FAST = 2 };
  * TARGET is the window-mean pair-force acceleration of a kept training
  * window (float4, w = 0; sph_learn_get_window only); NEIGHBOURS is the
  * number of neighbours within h the last surrogate step listed for each
- * particle, or after a plain step in SPH_PREC_FAST the number of force
- * pairs (r2 <= force_r2_max) the force kernel used (uint32 per particle, in
- * the float buffer's words).
+ * particle, or after a plain step whose gathers used the pair entries
+ * (SPH_PREC_FAST always; the reference and Kahan orders by default while
+ * max_par_in_cell <= 64) the number of force pairs (r2 <= force_r2_max)
+ * the force kernel used (uint32 per particle, in the float buffer's words);
+ * SPH_ERR_UNSUPPORTED after other steps.
  * All arrays are indexed by the ORIGINAL particle index.                   */
 enum {
     SPH_ARRAY_POSITION = 0,

@@ -145,6 +145,7 @@ struct Engine {
     bool stepped = false;
     bool order_failed = false;  // a step's order could not be built: the state must be uploaded again
     int last_kind = 0;          // what produced the current state: 0 upload, 1 plain step, 2 surrogate step
+    bool last_pair_counts = false;  // the last plain step's gathers used the pair entries (fast_nnb holds its counts)
     uint64_t steps = 0;
     double host_clock = 0.0;
     bool device_clock = false;  // clock and time step advanced on the device each step
@@ -415,6 +416,7 @@ struct Engine {
         last_skey = skey_buf[cur_key];
         if (timed) SPH_TRY(hipEventRecord(ev[3], stream));
         const bool fast = cfg.precision == SPH_PREC_FAST;
+        last_pair_counts = fast || pair_entries_in_use(c, buffers());
         if (fast)
             SPH_TRY(launch_density_fast(c, buffers(), fast_rec, fast_nbr, fast_nnb, stream));
         else
@@ -640,7 +642,7 @@ struct Engine {
                 rec = 1;
                 break;
             case SPH_ARRAY_NEIGHBOURS:
-                if (last_kind == 1 && cfg.precision == SPH_PREC_FAST && fast_nnb) {
+                if (last_kind == 1 && last_pair_counts && fast_nnb) {
                     // the last plain step's force pairs per particle
                     SPH_TRY(launch_scatter_to_original(n, p, fast_nnb, staging, 1, 1, stream));
                     pair_counts = true;

@@ -252,6 +252,11 @@ hipError_t launch_extra_terms(const LaunchCfg& c, const StepBuffers& b, float4* 
 // launch_force then walks those entries with the fast arithmetic.
 constexpr uint32_t kFastMaxCap = 64;
 uint32_t fast_list_groups(uint32_t max_par_in_cell);
+// Whether the step's gathers go through the pair entries (then each slot's
+// count word holds its force pairs): FAST always, the other orders while
+// the lists exist and the tune setting does not pick another path.
+bool pair_entries_in_use(const LaunchCfg& c, const StepBuffers& b);
+
 hipError_t launch_density_fast(const LaunchCfg& c, const StepBuffers& b, float4* rec, uint4* nbr, uint32_t* nnb,
                                hipStream_t s);
 // FP64 per-scene reductions of (mass, kinetic, potential, max speed) over

@@ -2064,6 +2064,8 @@ void stats_impl(const LaunchCfg& c, const StepBuffers& b, bool advance, hipStrea
 
 }  // namespace
 
+bool pair_entries_in_use(const LaunchCfg& c, const StepBuffers& b) { return use_entries(c, b); }
+
 uint32_t stats_chunks(const LaunchCfg& c) {
     uint32_t ch = (c.max_scene_n + 2047) / 2048;
     if (ch < 1) ch = 1;

@@ -158,13 +158,15 @@ def test_fast_after_reference_steps():
         g = read_state(s)
         s.set_params(p)
         s.update(1)
-        # (neighbour counts are a FAST-mode output: not read here)
-        r = {k: s.get_array(a) for k, a in (("pos", sph.SPH_ARRAY_POSITION), ("vel", sph.SPH_ARRAY_VELOCITY),
-                                            ("dens", sph.SPH_ARRAY_DENSITY), ("pres", sph.SPH_ARRAY_PRESSURE))}
+        r = read_state(s)
     compare_fast(g, p, x, v, "FAST after reference steps")
     o = oracle.step_f32(p, g["pos"], g["vel"])
     for k in ("pos", "vel", "dens", "pres"):
         assert np.array_equal(r[k], o[k]), f"reference step after FAST: {k}"
+    cnt = oracle.neighbour_counts_f32(p, o["pos"][o["perm"]], o["cell_start"], o["cell_end"], pred=1)
+    want = np.empty_like(cnt)
+    want[o["perm"]] = cnt
+    assert np.array_equal(r["nnb"], want), "reference step after FAST: force pair counts"
 
 
 def test_fast_batch_equals_scenes_alone_and_graph_replays():

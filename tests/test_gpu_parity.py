@@ -81,6 +81,33 @@ def test_single_step_15k_bit_exact(prec):
     assert_close_f64(g, o, p, pos, vel)
 
 
+@pytest.mark.parametrize("prec", [sph.SPH_PREC_REFERENCE, sph.SPH_PREC_KAHAN])
+def test_force_pair_counts_in_the_reference_orders(prec):
+    """After a plain step on the pair entries (the default while
+    max_par_in_cell <= 64), SPH_ARRAY_NEIGHBOURS holds each particle's force
+    pairs in these orders too: the oracle's count over the capped 27-cell
+    walk with r2 <= force_r2_max, exactly.  After a step on the pair-mask
+    gathers (tune 2) there are no counts: SPH_ERR_UNSUPPORTED."""
+    p, pos, vel, _ = scenes.cpu_dam_break_15k(precision=prec)
+    print("case: force pair counts, prec", prec, flush=True)
+    with sph.SphSystem(p) as s:
+        s.reset(pos, vel)
+        s.update(1)
+        nnb = s.get_array(sph.SPH_ARRAY_NEIGHBOURS)
+    o = oracle.step_f32(p, pos, vel, prec=prec)
+    cnt = oracle.neighbour_counts_f32(p, o["pos"][o["perm"]], o["cell_start"], o["cell_end"], pred=1)
+    want = np.empty_like(cnt)
+    want[o["perm"]] = cnt
+    assert np.array_equal(nnb, want)
+    assert want.min() >= 1
+    with sph.SphBatch([p]) as b:
+        b.reset(0, pos, vel)
+        b.set_tune([2, 0, 0, 0])
+        b.update(1)
+        with pytest.raises(sph.SphError):
+            b.get_array(0, sph.SPH_ARRAY_NEIGHBOURS)
+
+
 def test_100_steps_15k_bit_exact():
     p, pos, vel, _ = scenes.cpu_dam_break_15k()
     print("case: 15k dam break, 100 steps", flush=True)
