@@ -33,7 +33,7 @@ pytestmark = pytest.mark.gpu
 TOL_F64 = 1e-5
 
 
-def gpu_run(p, pos, vel, nsteps=1, keep_dvel=True):
+def gpu_run(p, pos, vel, nsteps=1, keep_dvel=True, pairs=False):
     with sph.SphSystem(p, keep_dvel=keep_dvel) as s:
         s.reset(pos, vel)
         s.update(nsteps)
@@ -42,7 +42,17 @@ def gpu_run(p, pos, vel, nsteps=1, keep_dvel=True):
                    grid=s.get_grid(), stats=s.stats())
         if keep_dvel:
             out["dvel"] = s.get_array(sph.SPH_ARRAY_DVEL)
+        if pairs:
+            out["nnb"] = s.get_array(sph.SPH_ARRAY_NEIGHBOURS)
     return out
+
+
+def oracle_pair_counts(p, o):
+    """Force pairs per particle (original order) of an oracle step."""
+    cnt = oracle.neighbour_counts_f32(p, o["pos"][o["perm"]], o["cell_start"], o["cell_end"], pred=1)
+    want = np.empty_like(cnt)
+    want[o["perm"]] = cnt
+    return want
 
 
 def assert_step_equal(g, o, what=""):
@@ -95,9 +105,7 @@ def test_force_pair_counts_in_the_reference_orders(prec):
         s.update(1)
         nnb = s.get_array(sph.SPH_ARRAY_NEIGHBOURS)
     o = oracle.step_f32(p, pos, vel, prec=prec)
-    cnt = oracle.neighbour_counts_f32(p, o["pos"][o["perm"]], o["cell_start"], o["cell_end"], pred=1)
-    want = np.empty_like(cnt)
-    want[o["perm"]] = cnt
+    want = oracle_pair_counts(p, o)
     assert np.array_equal(nnb, want)
     assert want.min() >= 1
     with sph.SphBatch([p]) as b:
@@ -123,9 +131,10 @@ def test_single_step_1m_bit_exact():
     """The 1,048,576-particle scene at full size, one step."""
     p, pos, vel, info = scenes.gpu_dam_break_1m()
     print("case: 1M dam break single step", info["grid"], flush=True)
-    g = gpu_run(p, pos, vel)
+    g = gpu_run(p, pos, vel, pairs=True)
     o = oracle.step_f32(p, pos, vel)
     assert_step_equal(g, o, "1M")
+    assert np.array_equal(g["nnb"], oracle_pair_counts(p, o)), "1M force pair counts"
     assert_close_f64(g, o, p, pos, vel)
 
 
